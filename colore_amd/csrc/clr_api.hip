@@ -1,0 +1,291 @@
+// clr_api.hip -- context management and the C ABI entry points.
+#include "clr_internal.h"
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+bool clr_fft_size_supported(int N);
+
+int clr_fail(clr_ctx *c, int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int clr_ensure(clr_ctx *c, void **p, size_t *have, size_t bytes) {
+  if (*have >= bytes && *p) return CLR_OK;
+  if (*p) {
+    CLR_HIP(c, hipStreamSynchronize(c->stream));
+    hipFree(*p);
+  }
+  *p = nullptr;
+  *have = 0;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return clr_fail(c, CLR_ERR_OOM, "device allocation of %zu bytes failed", bytes);
+  }
+  *have = bytes;
+  return CLR_OK;
+}
+
+static void free_all(clr_ctx *c) {
+  auto f = [](void *p) { if (p) hipFree(p); };
+  f(c->dens); f(c->npot); f(c->bg); f(c->pk); f(c->tw); f(c->amp); f(c->red); f(c->red_out);
+  f(c->rowcnt); f(c->inj); f(c->hp_vec); f(c->misc);
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < CLR_NPOP_MAX; i++) { f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); }
+  for (int i = 0; i < CLR_NPOP_MAX; i++) { f(c->cat[i].pos); f(c->cat[i].ipix); }
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+}
+
+extern "C" {
+
+int clr_version(void) { return 100; }
+
+const char *clr_last_error(const clr_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int clr_create(clr_ctx **out, int n_grid, double l_box, const double pos_obs[3], int device) {
+  if (!out) return CLR_ERR_ARG;
+  *out = nullptr;
+  clr_ctx *c = new clr_ctx();
+  if (!clr_fft_size_supported(n_grid) || !(l_box > 0)) {
+    clr_fail(c, CLR_ERR_ARG, "unsupported grid n_grid=%d l_box=%g", n_grid, l_box);
+    *out = c;
+    return CLR_ERR_ARG;
+  }
+  c->device = device;
+  c->N = n_grid;
+  c->NH = n_grid / 2 + 1;
+  c->ngx = 2 * c->NH;
+  c->L = l_box;
+  c->dx = l_box / n_grid;
+  for (int i = 0; i < 3; i++) c->obs[i] = pos_obs ? pos_obs[i] : 0.5 * l_box;
+  *out = c;
+  CLR_HIP(c, hipSetDevice(device));
+  CLR_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  CLR_HIP(c, hipEventCreate(&c->ev0));
+  CLR_HIP(c, hipEventCreate(&c->ev1));
+  c->grid_floats = (size_t)n_grid * n_grid * c->ngx;
+  if (hipMalloc((void **)&c->dens, c->grid_floats * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&c->npot, c->grid_floats * sizeof(float)) != hipSuccess) {
+    (void)hipGetLastError();
+    return clr_fail(c, CLR_ERR_OOM, "cannot allocate 2 grids of %zu floats", c->grid_floats);
+  }
+  // twiddles exp(+2 pi i n / N), FP64-rounded
+  std::vector<float2> tw(n_grid);
+  for (int n = 0; n < n_grid; n++) {
+    const double a = 2 * M_PI * (double)n / n_grid;
+    tw[n] = make_float2((float)cos(a), (float)sin(a));
+  }
+  CLR_HIP(c, hipMalloc((void **)&c->tw, n_grid * sizeof(float2)));
+  CLR_HIP(c, hipMemcpy(c->tw, tw.data(), n_grid * sizeof(float2), hipMemcpyHostToDevice));
+  return CLR_OK;
+}
+
+void clr_destroy(clr_ctx *ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_all(ctx);
+  delete ctx;
+}
+
+long long clr_device_bytes(const clr_ctx *c) {
+  if (!c) return 0;
+  long long b = 2LL * (long long)c->grid_floats * 4 + c->amp_n + c->red_n + c->red_out_n + c->rows_n + c->inj_n +
+                c->hp_n + c->misc_n;
+  for (int i = 0; i < CLR_NPOP_MAX; i++) b += c->cat[i].cap * 20;
+  return b;
+}
+
+int clr_upload_tables(clr_ctx *c, const clr_tables *t) {
+  if (!c || !t) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  if (t->na < 2 || !t->r_arr_r2z || !t->z_arr_r2z || !t->growth_d_arr || !t->growth_v_arr ||
+      !t->growth_pd_arr || !t->ihub_arr)
+    return clr_fail(c, CLR_ERR_ARG, "incomplete background tables");
+  if (t->numk < 2 || !t->logkarr || !t->pkarr) return clr_fail(c, CLR_ERR_ARG, "incomplete P(k) table");
+  const int na = t->na;
+  if (c->bg && c->na != na) { hipFree(c->bg); c->bg = nullptr; }
+  if (!c->bg) CLR_HIP(c, hipMalloc((void **)&c->bg, 6 * (size_t)na * sizeof(double)));
+  const double *src[6] = {t->r_arr_r2z, t->z_arr_r2z, t->growth_d_arr, t->growth_v_arr, t->growth_pd_arr,
+                          t->ihub_arr};
+  for (int i = 0; i < 6; i++)
+    CLR_HIP(c, hipMemcpy(c->bg + (size_t)i * na, src[i], na * sizeof(double), hipMemcpyHostToDevice));
+  c->na = na;
+  c->glob_idr = t->glob_idr;
+  c->r_last = t->r_arr_r2z[na - 1];
+  if (c->pk && c->numk != t->numk) { hipFree(c->pk); c->pk = nullptr; }
+  if (!c->pk) CLR_HIP(c, hipMalloc((void **)&c->pk, 2 * (size_t)t->numk * sizeof(double)));
+  CLR_HIP(c, hipMemcpy(c->pk, t->logkarr, t->numk * sizeof(double), hipMemcpyHostToDevice));
+  CLR_HIP(c, hipMemcpy(c->pk + t->numk, t->pkarr, t->numk * sizeof(double), hipMemcpyHostToDevice));
+  c->numk = t->numk;
+  c->logkmin = t->logkmin;
+  c->logkmax = t->logkmax;
+  c->idlogk = t->idlogk;
+  c->n_scal = t->n_scal;
+  c->fgrowth_0 = t->fgrowth_0;
+  c->hubble_0 = t->hubble_0;
+  c->omega_m = t->omega_m;
+  c->prefac_lensing = t->prefac_lensing;
+  c->r_max = t->r_max;
+  c->tables_set = 1;
+  return CLR_OK;
+}
+
+int clr_set_tracer(clr_ctx *c, int kind, int ipop, int bias_model, const double *amp_arr, const double *bz_arr) {
+  if (!c) return CLR_ERR_ARG;
+  if (kind < 0 || kind > 2 || ipop < 0 || ipop >= CLR_NPOP_MAX)
+    return clr_fail(c, CLR_ERR_ARG, "bad tracer kind %d / index %d", kind, ipop);
+  if (bias_model < 0 || bias_model > 2) return clr_fail(c, CLR_ERR_ARG, "bad bias model %d", bias_model);
+  if (!c->tables_set) return clr_fail(c, CLR_ERR_STATE, "clr_upload_tables must be called first");
+  if (!amp_arr || !bz_arr) return clr_fail(c, CLR_ERR_ARG, "missing tracer tables");
+  if (ipop > c->n_tr[kind]) return clr_fail(c, CLR_ERR_ARG, "tracers must be set in order (next is %d)", c->n_tr[kind]);
+  CLR_HIP(c, hipSetDevice(c->device));
+  TracerDev &T = c->tr[kind][ipop];
+  const size_t b = (size_t)c->na * sizeof(double);
+  if (!T.amp) CLR_HIP(c, hipMalloc((void **)&T.amp, b));
+  if (!T.bz) CLR_HIP(c, hipMalloc((void **)&T.bz, b));
+  if (!T.norm) CLR_HIP(c, hipMalloc((void **)&T.norm, b));
+  CLR_HIP(c, hipMemcpy(T.amp, amp_arr, b, hipMemcpyHostToDevice));
+  CLR_HIP(c, hipMemcpy(T.bz, bz_arr, b, hipMemcpyHostToDevice));
+  T.bz0 = bz_arr[0];
+  T.model = bias_model;
+  T.set = 1;
+  T.norm_set = 0;
+  if (ipop == c->n_tr[kind]) c->n_tr[kind]++;
+  return CLR_OK;
+}
+
+int clr_gaussian_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_potential, int rng_mode,
+                        const uint32_t *injected, double *sigma2_out) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_fields(c, seed, r_smooth, smooth_potential, rng_mode, injected, sigma2_out);
+}
+
+int clr_density(clr_ctx *c, int dens_type, double sigma2) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_density(c, dens_type, sigma2);
+}
+
+int clr_shell_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, int *nz_out,
+                    unsigned long long *counts, double *zsum, double *sums) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_norms(c, lognormal_fused, sigma2, cap, nz_out, counts, zsum, sums);
+}
+
+int clr_get_norm_table(clr_ctx *c, int kind, int ipop, double *norm_arr, double *norm0, double *normf) {
+  if (!c) return CLR_ERR_ARG;
+  if (kind < 0 || kind > 2 || ipop < 0 || ipop >= c->n_tr[kind] || !c->tr[kind][ipop].norm_set)
+    return clr_fail(c, CLR_ERR_STATE, "no norm table for tracer %d/%d", kind, ipop);
+  CLR_HIP(c, hipSetDevice(c->device));
+  TracerDev &T = c->tr[kind][ipop];
+  if (norm_arr) CLR_HIP(c, hipMemcpy(norm_arr, T.norm, (size_t)c->na * sizeof(double), hipMemcpyDeviceToHost));
+  if (norm0) *norm0 = T.norm0;
+  if (normf) *normf = T.normf;
+  return CLR_OK;
+}
+
+int clr_set_norm_table(clr_ctx *c, int kind, int ipop, const double *norm_arr, double norm0, double normf) {
+  if (!c || !norm_arr) return CLR_ERR_ARG;
+  if (kind < 0 || kind > 2 || ipop < 0 || ipop >= c->n_tr[kind] || !c->tr[kind][ipop].set)
+    return clr_fail(c, CLR_ERR_STATE, "tracer %d/%d not set", kind, ipop);
+  CLR_HIP(c, hipSetDevice(c->device));
+  TracerDev &T = c->tr[kind][ipop];
+  CLR_HIP(c, hipMemcpy(T.norm, norm_arr, (size_t)c->na * sizeof(double), hipMemcpyHostToDevice));
+  T.norm0 = norm0;
+  T.normf = normf;
+  T.norm_set = 1;
+  return CLR_OK;
+}
+
+int clr_sample_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc_out) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_sources(c, ipop, seed, nside_base, nsrc_out);
+}
+
+int clr_fetch_sources(clr_ctx *c, int ipop, float *pos4, int32_t *ipix, long long n) {
+  if (!c) return CLR_ERR_ARG;
+  if (ipop < 0 || ipop >= CLR_NPOP_MAX) return clr_fail(c, CLR_ERR_ARG, "bad population %d", ipop);
+  CatalogDev &C = c->cat[ipop];
+  if (n != C.n) return clr_fail(c, CLR_ERR_ARG, "population %d has %lld sources, not %lld", ipop, C.n, n);
+  if (n == 0) return CLR_OK;
+  CLR_HIP(c, hipSetDevice(c->device));
+  if (pos4) CLR_HIP(c, hipMemcpy(pos4, C.pos, (size_t)n * 4 * sizeof(float), hipMemcpyDeviceToHost));
+  if (ipix) CLR_HIP(c, hipMemcpy(ipix, C.ipix, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return CLR_OK;
+}
+
+int clr_source_properties(clr_ctx *c, int ipop, float *radecz, long long n) {
+  if (!c || !radecz) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_source_props(c, ipop, radecz, n);
+}
+
+int clr_kappa(clr_ctx *c, int nside, int nplanes, const double *r_src, float *out) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_los(c, 0, nside, nplanes, r_src, out);
+}
+
+int clr_isw(clr_ctx *c, int nside, int nplanes, const double *r_src, float *out) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_los(c, 1, nside, nplanes, r_src, out);
+}
+
+int clr_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf, float *data,
+             int32_t *nadd) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_imap(c, ipop, nside, nr, r0, rf, data, nadd);
+}
+
+int clr_fetch_grid(clr_ctx *c, int which, float *host) {
+  if (!c || !host) return CLR_ERR_ARG;
+  if (which != CLR_GRID_DENS && which != CLR_GRID_NPOT) return clr_fail(c, CLR_ERR_ARG, "bad grid %d", which);
+  CLR_HIP(c, hipSetDevice(c->device));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  CLR_HIP(c, hipMemcpy(host, which == CLR_GRID_DENS ? c->dens : c->npot, c->grid_floats * sizeof(float),
+                       hipMemcpyDeviceToHost));
+  return CLR_OK;
+}
+
+int clr_upload_grid(clr_ctx *c, int which, const float *host) {
+  if (!c || !host) return CLR_ERR_ARG;
+  if (which != CLR_GRID_DENS && which != CLR_GRID_NPOT) return clr_fail(c, CLR_ERR_ARG, "bad grid %d", which);
+  CLR_HIP(c, hipSetDevice(c->device));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  CLR_HIP(c, hipMemcpy(which == CLR_GRID_DENS ? c->dens : c->npot, host, c->grid_floats * sizeof(float),
+                       hipMemcpyHostToDevice));
+  return CLR_OK;
+}
+
+int clr_declare_grids(clr_ctx *c, double sigma2, int density_transformed) {
+  if (!c) return CLR_ERR_ARG;
+  c->have_fields = 1;
+  c->sigma2 = sigma2;
+  c->density_done = density_transformed ? 1 : 0;
+  c->norms_done = 0;
+  return CLR_OK;
+}
+
+int clr_stage_times(const clr_ctx *c, double *ms, int n) {
+  if (!c || !ms) return CLR_ERR_ARG;
+  for (int i = 0; i < n && i < ST_COUNT; i++) ms[i] = c->stage_ms[i];
+  return ST_COUNT;
+}
+
+}  // extern "C"

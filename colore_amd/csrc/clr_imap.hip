@@ -1,0 +1,204 @@
+// clr_imap.hip -- imap_set_cartesian_single (imap.c:135-245) on gfx950.
+//
+// One thread per grid cell inside [r0_min - 20, rf_max + 20] with T(r) > 0:
+// temperature T(r) bias_model(delta, b(r)) norm(r), RSD shift
+// v_r V1(r)/H(r) (imap.c:199-200), then nsub^3 sub-points at
+// x + (i+1/2) dx/nsub (imap.c:212-218); each sub-point whose shifted radius
+// falls in a shell adds T and 1 to its RING pixel (he_ang2pix,
+// healpix_extra.c:141-172) with device atomics, as the reference's
+// omp atomic does.  The shell of a radius is found by binary search over the
+// sorted shells; for contiguous shells this is the shell the reference's walk
+// (get_r_index_imap, imap.c:76-103) returns.
+#include "clr_internal.h"
+#include <algorithm>
+#include <math.h>
+
+using namespace clr;
+
+namespace {
+
+constexpr int kMaxShells = 1024;
+
+struct ImapArgs {
+  const float *dens, *phi;
+  const double *r_arr, *tz, *bz, *norm, *v1, *ih;
+  double idr, r_last, bz0, norm0, normf, v1_0, v1_last, ih_0, ih_last;
+  double dx, ox, oy, oz, rmin, rmax, factor_vel, idx;
+  int N, ngx, model, nr;
+  int64_t nside, npix;
+};
+
+// shell index: -1 below the first shell, nr above the last, -2 in a gap
+__device__ inline int shell_of(const double *__restrict__ r0, const double *__restrict__ rf, int nr, double r) {
+  if (r < r0[0]) return -1;
+  if (r >= rf[nr - 1]) return nr;
+  int lo = 0, hi = nr - 1;   // largest i with r0[i] <= r
+  while (hi > lo) {
+    const int m = (lo + hi + 1) / 2;
+    if (r0[m] <= r) lo = m; else hi = m - 1;
+  }
+  if (r < rf[lo]) return lo;
+  return -2;
+}
+
+__device__ inline double rvel_cell(const ImapArgs &a, int ix, int iy, int iz, double x0, double y0, double z0,
+                                   double rr) {
+#pragma clang fp contract(off)
+  const int N = a.N;
+  const size_t ngx = (size_t)a.ngx;
+  const float *phi = a.phi;
+  const int ixh = ix == N - 1 ? 0 : ix + 1, ixl = ix == 0 ? N - 1 : ix - 1;
+  const int iyh = iy == N - 1 ? 0 : iy + 1, iyl = iy == 0 ? N - 1 : iy - 1;
+  const int izh = iz == N - 1 ? 0 : iz + 1, izl = iz == 0 ? N - 1 : iz - 1;
+  const size_t z0o = (size_t)iz * N * ngx, y0o = (size_t)iy * ngx;
+  const double u0 = x0 / rr, u1 = y0 / rr, u2 = z0 / rr;
+  const double v0 = (double)phi[ixh + y0o + z0o] - (double)phi[ixl + y0o + z0o];
+  const double v1 = (double)phi[ix + (size_t)iyh * ngx + z0o] - (double)phi[ix + (size_t)iyl * ngx + z0o];
+  const double v2 = (double)phi[ix + y0o + (size_t)izh * N * ngx] - (double)phi[ix + y0o + (size_t)izl * N * ngx];
+  return 0.5 * a.idx * (v0 * u0 + v1 * u1 + v2 * u2);
+}
+
+__global__ __launch_bounds__(256) void k_imap(ImapArgs a, const double *__restrict__ r0s,
+                                              const double *__restrict__ rfs, const int *__restrict__ nsub,
+                                              float *__restrict__ data, int *__restrict__ nadd) {
+#pragma clang fp contract(off)
+  const size_t row = blockIdx.x;
+  const int iz = (int)(row / a.N), iy = (int)(row % a.N);
+  for (int ix = threadIdx.x; ix < a.N; ix += blockDim.x) {
+    const double x0 = ix * a.dx - a.ox, y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
+    const double r0 = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+    const double tmean = f_of_r(a.r_arr, a.tz, a.idr, a.r_last, r0, 0.0, 0.0);
+    if (!(r0 <= a.rmax && r0 >= a.rmin)) continue;
+    if (!(tmean > 0)) continue;
+    const double bias = f_of_r(a.r_arr, a.bz, a.idr, a.r_last, r0, a.bz0, 1.0);
+    const double dnorm = f_of_r(a.r_arr, a.norm, a.idr, a.r_last, r0, a.norm0, a.normf);
+    double rvel = 0;
+    if (r0 > 0) rvel = a.factor_vel * rvel_cell(a, ix, iy, iz, x0, y0, z0, r0);
+    const double dr_rsd = rvel * f_of_r(a.r_arr, a.v1, a.idr, a.r_last, r0, a.v1_0, a.v1_last) *
+                          f_of_r(a.r_arr, a.ih, a.idr, a.r_last, r0, a.ih_0, a.ih_last);
+    const double dl = a.dens[row * (size_t)a.ngx + ix];
+    const float temp = (float)(tmean * bias_model_f64(dl, bias, a.model) * dnorm);
+    int irad = shell_of(r0s, rfs, a.nr, r0);
+    const int ns = irad < 0 ? nsub[0] : (irad >= a.nr ? nsub[a.nr - 1] : nsub[irad]);
+    const double dxs = a.dx / ns;
+    for (int izz = 0; izz < ns; izz++) {
+      const double z = z0 + (izz + 0.5) * dxs;
+      for (int iyy = 0; iyy < ns; iyy++) {
+        const double y = y0 + (iyy + 0.5) * dxs;
+        for (int ixx = 0; ixx < ns; ixx++) {
+          const double x = x0 + (ixx + 0.5) * dxs;
+          const double r = sqrt(x * x + y * y + z * z);
+          double cth = 1, phi = 0;
+          if (r != 0) {
+            const double xn = x / r, yn = y / r;
+            cth = z / r;
+            phi = atan2(yn, xn);
+            if (phi < 0) phi += 2 * M_PI;
+          }
+          irad = shell_of(r0s, rfs, a.nr, r + dr_rsd);
+          if (irad >= 0 && irad < a.nr) {
+            const int64_t pix = ang2pix_ring(a.nside, cth, phi);
+            const size_t o = (size_t)irad * a.npix + pix;
+            atomicAdd(&data[o], temp);
+            atomicAdd(&nadd[o], 1);
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf, float *data,
+                    int32_t *nadd) {
+  if (ipop < 0 || ipop >= c->n_tr[CLR_TRACER_IMAP] || !c->tr[CLR_TRACER_IMAP][ipop].set)
+    return clr_fail(c, CLR_ERR_ARG, "no intensity species %d", ipop);
+  TracerDev &T = c->tr[CLR_TRACER_IMAP][ipop];
+  if (!T.norm_set) return clr_fail(c, CLR_ERR_STATE, "species %d has no norm table", ipop);
+  if (!c->density_done) return clr_fail(c, CLR_ERR_STATE, "density field not computed");
+  if (nside < 1 || nside > 8192 || (nside & (nside - 1))) return clr_fail(c, CLR_ERR_ARG, "bad nside %d", nside);
+  if (nr < 1 || nr > kMaxShells || !r0 || !rf || !data || !nadd)
+    return clr_fail(c, CLR_ERR_ARG, "need 1..%d shells and output maps", kMaxShells);
+  const int N = c->N, na = c->na;
+  // imap_preproc (imap.c:105-133): sort shells by r0 (flouble compare)
+  std::vector<int> order(nr);
+  for (int i = 0; i < nr; i++) order[i] = i;
+  std::vector<float> r0f(nr);
+  for (int i = 0; i < nr; i++) r0f[i] = (float)r0[i];
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return r0f[a] < r0f[b]; });
+  std::vector<double> sr0(nr), srf(nr);
+  for (int i = 0; i < nr; i++) { sr0[i] = r0[order[i]]; srf[i] = rf[order[i]]; }
+  const int64_t npix = 12LL * nside * nside;
+  const double pixel_size = sqrt(4 * M_PI / (double)npix);
+  const double dx = c->L / N;
+  std::vector<int> nsub(nr);
+  for (int i = 0; i < nr; i++) {
+    const double a0 = sr0[i], af = srf[i];
+    const double vol = (af * af * af - a0 * a0 * a0) * pixel_size * pixel_size / 3;
+    const double sct = a0 * pixel_size, scr = af - a0, scv = pow(vol, 0.333333);
+    const double sc0 = fmin(scv, fmin(sct, scr));
+    nsub[i] = (int)(dx / sc0 + 0.5) + 1;
+    if (nsub[i] < 1 || nsub[i] > 4096)
+      return clr_fail(c, CLR_ERR_ARG, "shell %d gives %d sub-samples per cell side", i, nsub[i]);
+  }
+  ImapArgs a{};
+  a.dens = c->dens;
+  a.phi = c->npot;
+  a.r_arr = c->bg;
+  a.tz = T.amp;
+  a.bz = T.bz;
+  a.norm = T.norm;
+  a.v1 = c->bg + 3 * (size_t)na;
+  a.ih = c->bg + 5 * (size_t)na;
+  double e[4];
+  CLR_HIP(c, hipMemcpy(&e[0], a.v1, sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(&e[1], a.v1 + na - 1, sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(&e[2], a.ih, sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(&e[3], a.ih + na - 1, sizeof(double), hipMemcpyDeviceToHost));
+  a.v1_0 = e[0]; a.v1_last = e[1]; a.ih_0 = e[2]; a.ih_last = e[3];
+  a.idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.bz0 = T.bz0;
+  a.norm0 = T.norm0;
+  a.normf = T.normf;
+  a.dx = dx;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.rmin = sr0[0] - 20.;
+  a.rmax = srf[nr - 1] + 20.;
+  a.factor_vel = -c->fgrowth_0 / (1.5 * c->hubble_0 * c->omega_m);
+  a.idx = N / c->L;
+  a.N = N;
+  a.ngx = c->ngx;
+  a.model = T.model;
+  a.nr = nr;
+  a.nside = nside;
+  a.npix = npix;
+
+  const size_t mapn = (size_t)npix * nr;
+  const size_t need = 2 * (size_t)nr * sizeof(double) + (size_t)nr * sizeof(int) + 64 + mapn * (sizeof(float) + sizeof(int));
+  int rc = clr_ensure(c, &c->misc, &c->misc_n, need);
+  if (rc) return rc;
+  double *dr0 = (double *)c->misc, *drf = dr0 + nr;
+  int *dns = (int *)(drf + nr);
+  float *dmap = (float *)(((uintptr_t)(dns + nr) + 63) & ~(uintptr_t)63);
+  int *dnadd = (int *)(dmap + mapn);
+  CLR_HIP(c, hipMemcpyAsync(dr0, sr0.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(drf, srf.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(dns, nsub.data(), nr * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  CLR_HIP(c, hipMemsetAsync(dmap, 0, mapn * (sizeof(float) + sizeof(int)), c->stream));
+  CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_imap, dim3((unsigned)((size_t)N * N)), dim3(N >= 256 ? 256 : N), 0, c->stream, a, dr0,
+                     drf, dns, dmap, dnadd);
+  CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(data, dmap, mapn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(nadd, dnadd, mapn * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->stage_ms[ST_IMAP] = ms;
+  return CLR_OK;
+}

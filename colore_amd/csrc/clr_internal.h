@@ -1,0 +1,108 @@
+// clr_internal.h -- the context behind the C ABI and the launchers each
+// stage file provides.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/colore_amd.h"
+#include "clr_device.h"
+
+struct TracerDev {
+  int set = 0;
+  int model = 0;
+  double *amp = nullptr;    // n(r) / T(r) / K(r), NA
+  double *bz = nullptr;     // b(r), NA
+  double *norm = nullptr;   // norm(r), NA
+  double bz0 = 0;           // f0 of the bias table (its first entry)
+  double norm0 = 1, normf = 1;
+  int norm_set = 0;
+};
+
+struct CatalogDev {
+  float *pos = nullptr;     // 4 floats per source
+  int32_t *ipix = nullptr;
+  long long cap = 0;
+  long long n = 0;
+};
+
+enum StageId { ST_GEN = 0, ST_FFT, ST_XPASS, ST_DENS, ST_NORM, ST_SRCS, ST_KAPPA, ST_ISW, ST_IMAP, ST_COUNT };
+
+struct clr_ctx {
+  int device = 0;
+  int N = 0;
+  int NH = 0;               // N/2+1
+  int ngx = 0;              // 2*(N/2+1)
+  double L = 0, dx = 0;
+  double obs[3] = {0, 0, 0};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double stage_ms[ST_COUNT] = {0};
+
+  float *dens = nullptr, *npot = nullptr;   // padded real grids
+  size_t grid_floats = 0;
+
+  // FP64 tables
+  int tables_set = 0;
+  double *bg = nullptr;     // r, z, d1, v1, pd, ih (NA each)
+  int na = 0;
+  double glob_idr = 0;
+  double r_last = 0;
+  double *pk = nullptr;     // logkarr[numk], pkarr[numk]
+  int numk = 0;
+  double logkmin = 0, logkmax = 0, idlogk = 0, n_scal = 0;
+  double fgrowth_0 = 0, hubble_0 = 0, omega_m = 0, prefac_lensing = 0, r_max = 0;
+
+  TracerDev tr[3][CLR_NPOP_MAX];
+  int n_tr[3] = {0, 0, 0};
+  CatalogDev cat[CLR_NPOP_MAX];
+
+  // scratch
+  float2 *tw = nullptr;     // exp(+2 pi i n/N), n < N
+  float2 *amp = nullptr;    // per integer |k|^2 index: (delta amp, phi amp)
+  size_t amp_n = 0;
+  double *red = nullptr;    // reduction partials
+  size_t red_n = 0;
+  double *red_out = nullptr;
+  size_t red_out_n = 0;
+  int32_t *rowcnt = nullptr;   // per (z,y) row counts / offsets (+1)
+  long long *rowoff = nullptr;
+  size_t rows_n = 0;
+  uint32_t *inj = nullptr;  // injected uniforms (parity mode)
+  size_t inj_n = 0;
+  double *hp_vec = nullptr; // pixel centres scratch
+  size_t hp_n = 0;
+  void *misc = nullptr;     // generic scratch
+  size_t misc_n = 0;
+
+  int have_fields = 0;
+  int density_done = 0;
+  int norms_done = 0;
+  double sigma2 = 0;
+  std::string err;
+};
+
+// status helpers
+int clr_fail(clr_ctx *c, int code, const char *fmt, ...);
+#define CLR_HIP(ctx, call)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return clr_fail((ctx), CLR_ERR_HIP, "%s failed: %s (%s:%d)", #call,         \
+                      hipGetErrorString(e_), __FILE__, __LINE__);                 \
+  } while (0)
+
+int clr_ensure(clr_ctx *c, void **p, size_t *have, size_t bytes);
+clr::BgTables clr_bg(const clr_ctx *c);
+
+// stage launchers (one per file)
+int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_potential,
+                      int rng_mode, const uint32_t *injected, double *sigma2_out);
+int clr_launch_density(clr_ctx *c, int dens_type, double sigma2);
+int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, int *nz_out,
+                     unsigned long long *counts, double *zsum, double *sums);
+int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc);
+int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
+int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out);
+int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf,
+                    float *data, int32_t *nadd);

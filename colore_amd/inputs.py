@@ -1,0 +1,175 @@
+"""Synthetic inputs and param files for the benchmark configurations.
+
+Every config is flat LCDM (Om=0.3, Ob=0.05, h=0.7, sigma8=0.8, ns=0.96) with a
+linear P(k) from the Eisenstein & Hu (1998, ApJ 496, 605) no-wiggle transfer
+function (their eqs. 26, 28-31), 1000 log-spaced k in [1e-4, 1e2] h/Mpc,
+written in the two-column CAMB format the reference reads (cosmo.c:459-463);
+its amplitude is arbitrary since the run renormalises to sigma8.
+
+n(z) files hold dN/dz dOmega in deg^-2 as the reference expects
+(cosmo.c:575-578): dN/dzdOmega = nbar(z) chi(z)^2 / H(z) / RTOD^2, where nbar
+is the requested comoving density ("normalised to X (Mpc/h)^-3" is read as the
+peak density).  Every z-table extends past the redshift of the box corner,
+because the reference's spline returns NaN outside the tabulated range.
+"""
+import math
+import os
+
+import numpy as np
+
+RTOD = 57.2957795
+COSMO = dict(omega_M=0.3, omega_L=0.7, omega_B=0.05, h=0.7, w=-1.0, ns=0.96, sigma_8=0.8)
+
+
+def eh_nowiggle_pk(k_hmpc, om=0.3, ob=0.05, h=0.7, ns=0.96, tcmb=2.7255):
+    """Unnormalised P(k) = k^ns T0(k)^2 with the EH98 no-wiggle T0."""
+    k = np.asarray(k_hmpc, dtype=np.float64)
+    theta = tcmb / 2.7
+    omh2, obh2 = om * h * h, ob * h * h
+    fb = ob / om
+    s = 44.5 * np.log(9.83 / omh2) / np.sqrt(1 + 10 * obh2 ** 0.75)                 # eq. 26, Mpc
+    alpha = 1 - 0.328 * np.log(431 * omh2) * fb + 0.38 * np.log(22.3 * omh2) * fb * fb  # eq. 31
+    kmpc = k * h
+    gamma = om * h * (alpha + (1 - alpha) / (1 + (0.43 * kmpc * s) ** 4))               # eq. 30
+    q = k * theta * theta / gamma                                                        # eq. 28
+    l0 = np.log(2 * math.e + 1.8 * q)
+    c0 = 14.2 + 731.0 / (1 + 62.5 * q)                                                   # eq. 29
+    t0 = l0 / (l0 + c0 * q * q)
+    return k ** ns * t0 * t0
+
+
+def _hub(a, om=0.3):
+    return math.sqrt(om / a ** 3 + (1 - om)) / 2997.92458
+
+
+def _chi(z, om=0.3, n=4000):
+    zz = np.linspace(0, z, n + 1)
+    f = 1.0 / np.sqrt(om * (1 + zz) ** 3 + (1 - om))
+    return 2997.92458 * np.trapezoid(f, zz) if z > 0 else 0.0
+
+
+def _growth(z, om=0.3):
+    """Linear growth normalised to 1 at z=0 (integral solution for LCDM)."""
+    def d(a):
+        aa = np.linspace(1e-6, a, 4001)
+        e = np.sqrt(om / aa ** 3 + (1 - om))
+        return 2.5 * om * np.sqrt(om / a ** 3 + 1 - om) * np.trapezoid(1 / (aa * e) ** 3, aa)
+    return d(1 / (1 + z)) / d(1.0)
+
+
+def _z_of_chi(chi, om=0.3):
+    lo, hi = 0.0, 20.0
+    for _ in range(80):
+        m = 0.5 * (lo + hi)
+        if _chi(m, om) < chi:
+            lo = m
+        else:
+            hi = m
+    return hi
+
+
+def z_table_top(z_max, n_grid):
+    r_max = _chi(z_max)
+    lbox = 2 * r_max * (1 + 2.0 / n_grid)
+    return _z_of_chi(math.sqrt(3) * 0.5 * lbox * 1.02) + 0.1
+
+
+def write_two_col(path, x, y):
+    np.savetxt(path, np.column_stack([x, y]), fmt="%.10e")
+
+
+def _nz_file(path, zs, nbar):
+    chi = np.array([_chi(z) for z in zs])
+    hz = np.array([_hub(1 / (1 + z)) for z in zs])
+    write_two_col(path, zs, nbar * chi ** 2 / hz / RTOD ** 2)
+
+
+def gaussian_nbar(zs, mean, sigma, peak):
+    return peak * np.exp(-0.5 * ((zs - mean) / sigma) ** 2)
+
+
+def flat_nbar(zs, z0, z1, level, edge=0.005):
+    return level / ((1 + np.exp(-(zs - z0) / edge)) * (1 + np.exp((zs - z1) / edge)))
+
+
+# populations: (nbar(z) spec, bias spec, bias model name)
+CONFIGS = {
+    "C1": dict(n_grid=128, z_min=0.05, z_max=0.5, r_smooth=2.0, seed=1001,
+               srcs=[(("gauss", 0.3, 0.08, 1e-3), ("const", 1.0), "exponential")]),
+    "C2": dict(n_grid=512, z_min=0.05, z_max=1.2, r_smooth=2.0, seed=2002,
+               srcs=[(("gauss", 0.5, 0.15, 5e-4), ("1+z",), "exponential"),
+                     (("flat", 0.8, 1.1, 1e-4), ("const", 2.0), "exponential")],
+               kappa=dict(z_out=[1.0], nside=256)),
+    "C3": dict(n_grid=1024, z_min=0.05, z_max=1.6, r_smooth=1.0, seed=3003,
+               srcs=[(("gauss", 0.4, 0.1, 1e-3), ("0.95/D",), "exponential"),
+                     (("gauss", 0.9, 0.15, 3e-4), ("const", 1.5), "exp-truncated"),
+                     (("gauss", 1.3, 0.2, 1e-4), ("const", 2.2), "truncated")],
+               kappa=dict(z_out=[0.5, 1.0, 1.5], nside=512), isw=dict(z_out=[0.5, 1.0, 1.5], nside=512),
+               imap=dict(nside=512, nbands=64, t0=0.2)),
+    "C5": dict(n_grid=2048, z_min=0.05, z_max=2.0, r_smooth=1.0, seed=5005,
+               srcs=[(("gauss", 0.5, 0.15, 5e-4), ("1+z",), "exponential"),
+                     (("flat", 0.8, 1.1, 1e-4), ("const", 2.0), "exponential")]),
+}
+
+
+def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_format="ASCII",
+                 output_density=False, with_maps=True, seed=None, imap_bands=None, imap_nside=None,
+                 kappa_nside=None):
+    """Writes P(k), n(z), b(z) (+T(z), nu table) and the param file of config
+    `name` into outdir; returns the param-file path.  n_grid/z_max override
+    the config's values (small variants for tests)."""
+    cfg = dict(CONFIGS[name])
+    os.makedirs(outdir, exist_ok=True)
+    n = int(n_grid or cfg["n_grid"])
+    zmax = float(z_max or cfg["z_max"])
+    zmin = cfg["z_min"]
+    k = np.logspace(-4, 2, 1000)
+    write_two_col(os.path.join(outdir, "pk.txt"), k, eh_nowiggle_pk(k))
+    ztop = z_table_top(zmax, n)
+    zs = np.linspace(0.0, ztop, 1500)
+    lines = ["global: {", f'  prefix_out = "{outdir}/out";', f'  output_format = "{output_format}";',
+             f"  output_density = {'true' if output_density else 'false'};",
+             f'  pk_filename = "{outdir}/pk.txt";', f"  z_min = {zmin};", f"  z_max = {zmax};",
+             f"  seed = {int(seed if seed is not None else cfg['seed'])};", "  write_pred = false;",
+             "  just_write_pred = false;", "};", "field_par: {", f"  r_smooth = {cfg['r_smooth']};",
+             "  smooth_potential = true;", f"  n_grid = {n};", f"  dens_type = {dens_type};",
+             "  lpt_buffer_fraction = 0.6;", "  lpt_interp_type = 1;", "  output_lpt = 0;", "};",
+             "cosmo_par: {"] + [f"  {kk} = {vv};" for kk, vv in COSMO.items()] + ["};"]
+    for i, (nzs, bzs, model) in enumerate(cfg["srcs"]):
+        if nzs[0] == "gauss":
+            nbar = gaussian_nbar(zs, nzs[1], nzs[2], nzs[3])
+        else:
+            nbar = flat_nbar(zs, nzs[1], nzs[2], nzs[3])
+        _nz_file(os.path.join(outdir, f"nz{i + 1}.txt"), zs, nbar)
+        if bzs[0] == "const":
+            b = np.full_like(zs, bzs[1])
+        elif bzs[0] == "1+z":
+            b = 1 + zs
+        else:
+            b = 0.95 / np.array([_growth(z) for z in zs])
+        write_two_col(os.path.join(outdir, f"bz{i + 1}.txt"), zs, b)
+        lines += [f"srcs{i + 1}: {{", f'  nz_filename = "{outdir}/nz{i + 1}.txt";',
+                  f'  bias_filename = "{outdir}/bz{i + 1}.txt";', "  include_lensing = false;",
+                  "  store_skewers = false;", f'  bias_model = "{model}";', "};"]
+    if with_maps and "imap" in cfg:
+        im = cfg["imap"]
+        nb = imap_bands or im["nbands"]
+        nu_rest = 1420.405
+        nu_lo, nu_hi = nu_rest / (1 + zmax), nu_rest / (1 + zmin)
+        edges = np.linspace(nu_lo, nu_hi, nb + 1)
+        write_two_col(os.path.join(outdir, "nu.txt"), edges[:-1], edges[1:])
+        write_two_col(os.path.join(outdir, "tz.txt"), zs, im["t0"] * np.sqrt(1 + zs))
+        write_two_col(os.path.join(outdir, "bz_imap.txt"), zs, np.ones_like(zs))
+        lines += ["imap1: {", f'  tbak_filename = "{outdir}/tz.txt";', f'  bias_filename = "{outdir}/bz_imap.txt";',
+                  f'  freq_list = "{outdir}/nu.txt";', f"  freq_rest = {nu_rest};",
+                  f"  nside = {imap_nside or im['nside']};", '  bias_model = "exponential";', "};"]
+    for sec in ("kappa", "isw"):
+        if with_maps and sec in cfg:
+            zo = [z for z in cfg[sec]["z_out"] if z <= zmax]
+            if zo:
+                ns = kappa_nside or cfg[sec]["nside"]
+                lines += [f"{sec}: {{", f"  z_out = [{', '.join(str(z) for z in zo)}];", f"  nside = {ns};", "};"]
+    path = os.path.join(outdir, "params.cfg")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path

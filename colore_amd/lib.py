@@ -1,0 +1,307 @@
+"""ctypes bindings of the C ABI (include/colore_amd.h, include/colore_host.h).
+
+The HIP library is required: importing this module on a machine where
+libcolore_amd.so was not built raises immediately, so no caller can fall back
+to a CPU path by accident.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+AMD_SO = os.path.join(_HERE, "libcolore_amd.so")
+HOST_SO = os.path.join(_HERE, "libcolore_host.so")
+
+CLR_NA = 5001
+CLR_GRID_DENS, CLR_GRID_NPOT = 0, 1
+CLR_TRACER_SRCS, CLR_TRACER_IMAP, CLR_TRACER_CSTM = 0, 1, 2
+CLR_BIAS_EXPONENTIAL, CLR_BIAS_EXP_TRUNCATED, CLR_BIAS_TRUNCATED = 0, 1, 2
+CLR_DENS_LOGNORMAL, CLR_DENS_CLIP = 0, 3
+CLR_RNG_PHILOX, CLR_RNG_INJECTED = 0, 1
+STAGES = ["gen", "fft_yz", "fft_x", "density", "norm", "srcs", "kappa", "isw", "imap"]
+
+_d = ctypes.POINTER(ctypes.c_double)
+_f = ctypes.POINTER(ctypes.c_float)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_u32 = ctypes.POINTER(ctypes.c_uint32)
+_u64 = ctypes.POINTER(ctypes.c_ulonglong)
+_vp = ctypes.c_void_p
+
+
+class Tables(ctypes.Structure):
+    """clr_tables (include/colore_amd.h)."""
+    _fields_ = [
+        ("na", ctypes.c_int), ("glob_idr", ctypes.c_double),
+        ("r_arr_r2z", _d), ("z_arr_r2z", _d), ("growth_d_arr", _d), ("growth_v_arr", _d),
+        ("growth_pd_arr", _d), ("ihub_arr", _d),
+        ("numk", ctypes.c_int), ("logkmin", ctypes.c_double), ("logkmax", ctypes.c_double),
+        ("idlogk", ctypes.c_double), ("logkarr", _d), ("pkarr", _d), ("n_scal", ctypes.c_double),
+        ("fgrowth_0", ctypes.c_double), ("hubble_0", ctypes.c_double), ("omega_m", ctypes.c_double),
+        ("prefac_lensing", ctypes.c_double), ("r_max", ctypes.c_double),
+    ]
+
+
+class Csm(ctypes.Structure):
+    """clr_csm (include/colore_host.h)."""
+    _fields_ = [(n, ctypes.c_double) for n in ("OM", "OB", "OL", "OK", "h", "w0", "wa")] + \
+               [(n, ctypes.c_int) for n in ("ksign", "normalDE", "constantw")] + \
+               [(n, ctypes.c_double) for n in ("phorizon", "a_equality", "growth0")]
+
+
+def _sig(lib, name, res, args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+    return fn
+
+
+def load_amd():
+    if not os.path.exists(AMD_SO):
+        raise RuntimeError(f"HIP library {AMD_SO} is not built: run __graft_entry__.build()")
+    lib = ctypes.CDLL(AMD_SO)
+    ci = ctypes.c_int
+    pp = ctypes.POINTER(_vp)
+    _sig(lib, "clr_create", ci, [pp, ci, ctypes.c_double, _d, ci])
+    _sig(lib, "clr_destroy", None, [_vp])
+    _sig(lib, "clr_last_error", ctypes.c_char_p, [_vp])
+    _sig(lib, "clr_version", ci, [])
+    _sig(lib, "clr_device_bytes", ctypes.c_longlong, [_vp])
+    _sig(lib, "clr_upload_tables", ci, [_vp, ctypes.POINTER(Tables)])
+    _sig(lib, "clr_set_tracer", ci, [_vp, ci, ci, ci, _d, _d])
+    _sig(lib, "clr_gaussian_fields", ci, [_vp, ctypes.c_uint32, ctypes.c_double, ci, ci, _u32, _d])
+    _sig(lib, "clr_density", ci, [_vp, ci, ctypes.c_double])
+    _sig(lib, "clr_shell_norms", ci, [_vp, ci, ctypes.c_double, ci, ctypes.POINTER(ci), _u64, _d, _d])
+    _sig(lib, "clr_get_norm_table", ci, [_vp, ci, ci, _d, _d, _d])
+    _sig(lib, "clr_set_norm_table", ci, [_vp, ci, ci, _d, ctypes.c_double, ctypes.c_double])
+    _sig(lib, "clr_sample_sources", ci, [_vp, ci, ctypes.c_uint32, ci, ctypes.POINTER(ctypes.c_longlong)])
+    _sig(lib, "clr_fetch_sources", ci, [_vp, ci, _f, _i32, ctypes.c_longlong])
+    _sig(lib, "clr_source_properties", ci, [_vp, ci, _f, ctypes.c_longlong])
+    _sig(lib, "clr_kappa", ci, [_vp, ci, ci, _d, _f])
+    _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
+    _sig(lib, "clr_imap", ci, [_vp, ci, ci, ci, _d, _d, _f, _i32])
+    _sig(lib, "clr_fetch_grid", ci, [_vp, ci, _f])
+    _sig(lib, "clr_upload_grid", ci, [_vp, ci, _f])
+    _sig(lib, "clr_declare_grids", ci, [_vp, ctypes.c_double, ci])
+    _sig(lib, "clr_stage_times", ci, [_vp, _d, ci])
+    return lib
+
+
+def load_host():
+    if not os.path.exists(HOST_SO):
+        raise RuntimeError(f"host library {HOST_SO} is not built: run __graft_entry__.build()")
+    lib = ctypes.CDLL(HOST_SO)
+    ci, cd = ctypes.c_int, ctypes.c_double
+    _sig(lib, "clr_csm_set", ci, [ctypes.POINTER(Csm), cd, cd, cd, cd, cd, cd, ctypes.c_char_p, ci])
+    for n in ("clr_csm_hubble", "clr_csm_omega_m", "clr_csm_particle_horizon",
+              "clr_csm_radial_comoving_distance", "clr_csm_growth_factor", "clr_csm_f_growth"):
+        _sig(lib, n, cd, [ctypes.POINTER(Csm), cd])
+    _sig(lib, "clr_cspline_eval", None, [ci, _d, _d, ci, _d, _d])
+    _sig(lib, "clr_sigma2_window", cd, [ci, _d, _d, cd, cd, cd, cd, cd, ci])
+    _sig(lib, "clr_run_read", _vp, [ctypes.c_char_p, ctypes.c_char_p, ci])
+    _sig(lib, "clr_run_free", None, [_vp])
+    _sig(lib, "clr_run_scalar", cd, [_vp, ctypes.c_char_p])
+    _sig(lib, "clr_run_tables", ci, [_vp, ctypes.POINTER(Tables)])
+    _sig(lib, "clr_run_tracer", ci, [_vp, ci, ci, ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(ci)])
+    _sig(lib, "clr_run_get_bg", cd, [_vp, cd, ci, ci])
+    _sig(lib, "clr_run_r_of_z", cd, [_vp, cd])
+    _sig(lib, "clr_hp_ang2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, cd, cd])
+    _sig(lib, "clr_hp_vec2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, _d])
+    _sig(lib, "clr_hp_ring2nest", ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong])
+    _sig(lib, "clr_hp_nest2ring", ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong])
+    _sig(lib, "clr_hp_pix2vec_nest", None, [ctypes.c_longlong, ctypes.c_longlong, _d])
+    _sig(lib, "clr_write_density_grid", ci, [ctypes.c_char_p, ci, cd, _f])
+    _sig(lib, "clr_write_healpix_map", ci, [ctypes.c_char_p, ctypes.c_longlong, _f])
+    _sig(lib, "clr_write_catalog_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f])
+    _sig(lib, "clr_write_catalog_ascii", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f])
+    return lib
+
+
+def ptr(a, ct=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+class Run:
+    """A parsed param file with its FP64 tables (the reference's ParamCoLoRe)."""
+
+    def __init__(self, param_file, host=None):
+        self.host = host or load_host()
+        err = ctypes.create_string_buffer(1024)
+        self.h = self.host.clr_run_read(param_file.encode(), err, 1024)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.host.clr_run_free(self.h)
+            self.h = None
+
+    def scalar(self, name):
+        return self.host.clr_run_scalar(self.h, name.encode())
+
+    def tables(self):
+        t = Tables()
+        self.host.clr_run_tables(self.h, ctypes.byref(t))
+        return t
+
+    def table_arrays(self):
+        t = self.tables()
+        na = t.na
+        g = lambda p, n=na: np.ctypeslib.as_array(p, shape=(n,)).copy()
+        return dict(r=g(t.r_arr_r2z), z=g(t.z_arr_r2z), d1=g(t.growth_d_arr), v1=g(t.growth_v_arr),
+                    pd=g(t.growth_pd_arr), ih=g(t.ihub_arr), logk=g(t.logkarr, t.numk),
+                    pk=g(t.pkarr, t.numk), glob_idr=t.glob_idr, numk=t.numk, logkmin=t.logkmin,
+                    logkmax=t.logkmax, idlogk=t.idlogk, ns=t.n_scal, fgrowth_0=t.fgrowth_0,
+                    hubble_0=t.hubble_0, omega_m=t.omega_m, prefac_lensing=t.prefac_lensing,
+                    r_max=t.r_max)
+
+    def tracer(self, kind, ipop):
+        a, b, m = _d(), _d(), ctypes.c_int()
+        if self.host.clr_run_tracer(self.h, kind, ipop, ctypes.byref(a), ctypes.byref(b), ctypes.byref(m)):
+            raise IndexError((kind, ipop))
+        return (np.ctypeslib.as_array(a, shape=(CLR_NA,)).copy(),
+                np.ctypeslib.as_array(b, shape=(CLR_NA,)).copy(), m.value)
+
+
+class Context:
+    """Device context (clr_ctx) with one method per reference stage."""
+
+    def __init__(self, n_grid, l_box, pos_obs=None, device=0, amd=None):
+        self.lib = amd or load_amd()
+        self.N = int(n_grid)
+        self.L = float(l_box)
+        obs = np.ascontiguousarray(pos_obs if pos_obs is not None else [0.5 * l_box] * 3, dtype=np.float64)
+        h = _vp()
+        rc = self.lib.clr_create(ctypes.byref(h), self.N, self.L, ptr(obs), device)
+        self.h = h
+        if rc:
+            msg = self.lib.clr_last_error(h).decode() if h else "clr_create failed"
+            self.close()
+            raise RuntimeError(msg)
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise RuntimeError(f"clr error {rc}: {self.lib.clr_last_error(self.h).decode()}")
+
+    @property
+    def padded_shape(self):
+        return (self.N, self.N, 2 * (self.N // 2 + 1))
+
+    def upload_tables(self, t):
+        self._keep.append(t)
+        self._chk(self.lib.clr_upload_tables(self.h, ctypes.byref(t)))
+
+    def upload_table_arrays(self, ta):
+        arrs = {k: np.ascontiguousarray(ta[k], dtype=np.float64) for k in ("r", "z", "d1", "v1", "pd", "ih", "logk", "pk")}
+        self._keep.append(arrs)
+        t = Tables(na=len(arrs["r"]), glob_idr=ta["glob_idr"], r_arr_r2z=ptr(arrs["r"]), z_arr_r2z=ptr(arrs["z"]),
+                   growth_d_arr=ptr(arrs["d1"]), growth_v_arr=ptr(arrs["v1"]), growth_pd_arr=ptr(arrs["pd"]),
+                   ihub_arr=ptr(arrs["ih"]), numk=len(arrs["pk"]), logkmin=ta["logkmin"], logkmax=ta["logkmax"],
+                   idlogk=ta["idlogk"], logkarr=ptr(arrs["logk"]), pkarr=ptr(arrs["pk"]), n_scal=ta["ns"],
+                   fgrowth_0=ta["fgrowth_0"], hubble_0=ta["hubble_0"], omega_m=ta["omega_m"],
+                   prefac_lensing=ta["prefac_lensing"], r_max=ta["r_max"])
+        self.upload_tables(t)
+
+    def set_tracer(self, kind, ipop, model, amp, bz):
+        amp = np.ascontiguousarray(amp, dtype=np.float64)
+        bz = np.ascontiguousarray(bz, dtype=np.float64)
+        self._chk(self.lib.clr_set_tracer(self.h, kind, ipop, model, ptr(amp), ptr(bz)))
+
+    def gaussian_fields(self, seed, r_smooth, smooth_potential=True, injected=None):
+        s2 = ctypes.c_double()
+        if injected is not None:
+            inj = np.ascontiguousarray(injected, dtype=np.uint32)
+            self._chk(self.lib.clr_gaussian_fields(self.h, seed, r_smooth, int(smooth_potential), CLR_RNG_INJECTED,
+                                                   ptr(inj, ctypes.c_uint32), ctypes.byref(s2)))
+        else:
+            self._chk(self.lib.clr_gaussian_fields(self.h, seed, r_smooth, int(smooth_potential), CLR_RNG_PHILOX,
+                                                   None, ctypes.byref(s2)))
+        return s2.value
+
+    def density(self, dens_type, sigma2):
+        self._chk(self.lib.clr_density(self.h, dens_type, sigma2))
+
+    def shell_norms(self, lognormal_fused=False, sigma2=0.0, ntracers=0, cap=256):
+        nz = ctypes.c_int()
+        cnt = np.zeros(cap, dtype=np.uint64)
+        zs = np.zeros(cap)
+        sums = np.zeros(max(ntracers, 1) * cap)
+        self._chk(self.lib.clr_shell_norms(self.h, int(lognormal_fused), sigma2, cap, ctypes.byref(nz),
+                                           ptr(cnt, ctypes.c_ulonglong), ptr(zs), ptr(sums)))
+        n = nz.value
+        return n, cnt[:n].copy(), zs[:n].copy(), sums[:ntracers * n].reshape(ntracers, n).copy()
+
+    def norm_table(self, kind, ipop):
+        arr = np.zeros(CLR_NA)
+        n0, nf = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.clr_get_norm_table(self.h, kind, ipop, ptr(arr), ctypes.byref(n0), ctypes.byref(nf)))
+        return arr, n0.value, nf.value
+
+    def set_norm_table(self, kind, ipop, arr, n0, nf):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(self.lib.clr_set_norm_table(self.h, kind, ipop, ptr(arr), n0, nf))
+
+    def sample_sources(self, ipop, seed, nside_base=2):
+        n = ctypes.c_longlong()
+        self._chk(self.lib.clr_sample_sources(self.h, ipop, seed, nside_base, ctypes.byref(n)))
+        return n.value
+
+    def fetch_sources(self, ipop, n):
+        pos = np.zeros((n, 4), dtype=np.float32)
+        ipix = np.zeros(n, dtype=np.int32)
+        self._chk(self.lib.clr_fetch_sources(self.h, ipop, ptr(pos, ctypes.c_float), ptr(ipix, ctypes.c_int32), n))
+        return pos, ipix
+
+    def source_properties(self, ipop, n):
+        out = np.zeros((n, 4), dtype=np.float32)
+        self._chk(self.lib.clr_source_properties(self.h, ipop, ptr(out, ctypes.c_float), n))
+        return out
+
+    def kappa(self, nside, r_src):
+        r = np.ascontiguousarray(r_src, dtype=np.float64)
+        out = np.zeros((len(r), 12 * nside * nside), dtype=np.float32)
+        self._chk(self.lib.clr_kappa(self.h, nside, len(r), ptr(r), ptr(out, ctypes.c_float)))
+        return out
+
+    def isw(self, nside, r_src):
+        r = np.ascontiguousarray(r_src, dtype=np.float64)
+        out = np.zeros((len(r), 12 * nside * nside), dtype=np.float32)
+        self._chk(self.lib.clr_isw(self.h, nside, len(r), ptr(r), ptr(out, ctypes.c_float)))
+        return out
+
+    def imap(self, ipop, nside, r0, rf):
+        r0 = np.ascontiguousarray(r0, dtype=np.float64)
+        rf = np.ascontiguousarray(rf, dtype=np.float64)
+        npix = 12 * nside * nside
+        data = np.zeros((len(r0), npix), dtype=np.float32)
+        nadd = np.zeros((len(r0), npix), dtype=np.int32)
+        self._chk(self.lib.clr_imap(self.h, ipop, nside, len(r0), ptr(r0), ptr(rf), ptr(data, ctypes.c_float),
+                                    ptr(nadd, ctypes.c_int32)))
+        return data, nadd
+
+    def fetch_grid(self, which):
+        g = np.empty(self.padded_shape, dtype=np.float32)
+        self._chk(self.lib.clr_fetch_grid(self.h, which, ptr(g, ctypes.c_float)))
+        return g
+
+    def upload_grid(self, which, g):
+        g = np.ascontiguousarray(g, dtype=np.float32)
+        assert g.shape == self.padded_shape, (g.shape, self.padded_shape)
+        self._chk(self.lib.clr_upload_grid(self.h, which, ptr(g, ctypes.c_float)))
+
+    def declare_grids(self, sigma2, density_transformed):
+        self._chk(self.lib.clr_declare_grids(self.h, sigma2, int(density_transformed)))
+
+    def stage_times(self):
+        ms = np.zeros(16)
+        n = self.lib.clr_stage_times(self.h, ptr(ms), 16)
+        return dict(zip(STAGES, ms[:n]))

@@ -1,0 +1,189 @@
+/*
+ * colore_amd.h -- C ABI of the MI355X (gfx950) implementation of CoLoRe's
+ * field -> lognormal -> sources hot path and its lightcone painters.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an int status: 0 on success, <0 for a HIP/runtime failure, >0 for
+ * an input/specification error.  clr_last_error() returns the message of the
+ * last failure on a context.  Calls are synchronous at return (stream-ordered
+ * inside); one host thread per context.  Host buffers are owned by the caller.
+ *
+ * In the reference every stage is `void f(ParamCoLoRe *par)` declared in
+ * src/common.h:436-543 and sequenced by src/main.c:61-141; each function
+ * below names the reference stage it replaces.
+ */
+#ifndef COLORE_AMD_H
+#define COLORE_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct clr_ctx clr_ctx;
+
+/* ---- status codes ---------------------------------------------------- */
+#define CLR_OK            0
+#define CLR_ERR_HIP      (-1)   /* HIP runtime call failed                 */
+#define CLR_ERR_OOM      (-2)   /* device allocation failed                */
+#define CLR_ERR_ARG       1     /* bad argument / unsupported size         */
+#define CLR_ERR_STATE     2     /* stage called out of order               */
+
+/* ---- enumerations ---------------------------------------------------- */
+#define CLR_GRID_DENS     0     /* par->grid_dens (delta)                  */
+#define CLR_GRID_NPOT     1     /* par->grid_npot (Newtonian potential)    */
+
+#define CLR_TRACER_SRCS   0     /* srcsN sections                          */
+#define CLR_TRACER_IMAP   1     /* imapN sections                          */
+#define CLR_TRACER_CSTM   2     /* customN sections                        */
+
+/* bias_model(), src/common.h:414-431; the reference picks one at compile
+ * time (Makefile:11-14), here it is chosen per tracer. */
+#define CLR_BIAS_EXPONENTIAL   0   /* (1+d)^b                 (default branch) */
+#define CLR_BIAS_EXP_TRUNCATED 1   /* _BIAS_MODEL_2                            */
+#define CLR_BIAS_TRUNCATED     2   /* _BIAS_MODEL_3                            */
+
+/* density types, src/common.h:115-118 (1LPT/2LPT are not implemented here) */
+#define CLR_DENS_LOGNORMAL 0
+#define CLR_DENS_CLIP      3
+
+/* random streams for the Fourier modes */
+#define CLR_RNG_PHILOX    0     /* counter-based Philox4x32-10, key = seed  */
+#define CLR_RNG_INJECTED  1     /* two caller-supplied uint32 per mode      */
+
+#define CLR_NA            5001  /* length of the r-tables, src/common.h:131 */
+#define CLR_NPOP_MAX      10    /* src/common.h:132                          */
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Allocates the two padded real grids float[N][N][2(N/2+1)] on `device`
+ * (fourier.c:211-238 allocate_fftw; the z halo slabs of the reference are
+ * the periodic neighbours on one GPU, fourier.c:412-413).  N must be one of
+ * 2^n (32..2048) or 3*2^n (48..1536). */
+int  clr_create(clr_ctx **ctx, int n_grid, double l_box, const double pos_obs[3],
+                int device);
+void clr_destroy(clr_ctx *ctx);
+const char *clr_last_error(const clr_ctx *ctx);
+int  clr_version(void);
+/* Bytes of device memory the context holds now. */
+long long clr_device_bytes(const clr_ctx *ctx);
+
+/* FP64 background tables exactly as held in ParamCoLoRe (src/common.h:220-381,
+ * filled by cosmo_set, src/cosmo.c:669-692) and the P(k) table
+ * (pk_linear_set, src/cosmo.c:444-495). */
+typedef struct {
+  int           na;            /* CLR_NA                                  */
+  double        glob_idr;      /* (NA-1)/r_arr_r2z[NA-1]                  */
+  const double *r_arr_r2z;
+  const double *z_arr_r2z;
+  const double *growth_d_arr;
+  const double *growth_v_arr;
+  const double *growth_pd_arr;
+  const double *ihub_arr;
+  int           numk;          /* P(k) table                              */
+  double        logkmin, logkmax, idlogk;
+  const double *logkarr;
+  const double *pkarr;
+  double        n_scal;        /* ns: low-k extrapolation slope           */
+  double        fgrowth_0;     /* f(a=1)                                  */
+  double        hubble_0;      /* H(a=1) in h/Mpc                         */
+  double        omega_m;
+  double        prefac_lensing;/* 1.5 H0^2 Om                             */
+  double        r_max;         /* chi(z_max): sets the LOS step r_max/(N/2)*/
+} clr_tables;
+int clr_upload_tables(clr_ctx *ctx, const clr_tables *tab);
+
+/* Per-tracer r-tables (srcs_nz_arr/srcs_bz_arr, imap_tz_arr/imap_bz_arr,
+ * cstm_kz_arr/cstm_bz_arr; src/cosmo.c:693-716).  `amp_arr` is n(r) for
+ * sources, T(r) for intensity maps, K(r) for custom maps. */
+int clr_set_tracer(clr_ctx *ctx, int kind, int ipop, int bias_model,
+                   const double *amp_arr, const double *bz_arr);
+
+/* ---- stages ----------------------------------------------------------- */
+
+/* create_cartesian_fields (fourier.c:361-423) = create_grids_fourier
+ * (fourier.c:285-359) + 2x c2r (fourier.c:81-102) + normalisation
+ * (fourier.c:383-398) + compute_sigma_dens (fourier.c:24-79).
+ * r_smooth is the smoothing LENGTH (<=0: none); the reference squares it in
+ * io.c:453-458.  With rng_mode CLR_RNG_INJECTED, `injected` (host memory)
+ * holds 2 uint32 per half-spectrum mode in layout order
+ * kx + (N/2+1)(ky + N kz): (phase draw, modulus draw), as consumed by
+ * rng_delta_gauss (common.c:191-201).  Writes the measured sigma^2. */
+int clr_gaussian_fields(clr_ctx *ctx, uint32_t seed, double r_smooth,
+                        int smooth_potential, int rng_mode,
+                        const uint32_t *injected, double *sigma2_out);
+
+/* compute_physical_density_field (density.c:1105-1126): lognormalize
+ * (density.c:1070-1103) or densclip (density.c:1034-1067), in place. */
+int clr_density(clr_ctx *ctx, int dens_type, double sigma2);
+
+/* compute_density_normalization (density.c:1227-1393).  Computes the FP64
+ * per-shell sums on the device and builds each tracer's NA norm table on the
+ * device.  Optional outputs (may be NULL): nz_out, counts[nz], zsum[nz],
+ * sums[n_tracers][nz] (tracers in order srcs..., imap..., cstm...).
+ * `cap` is the capacity of the arrays (nz <= cap is required when non-NULL).
+ * When `lognormal_fused` is non-zero and dens_type is lognormal, the
+ * lognormal transform of clr_density is applied in the same pass over the
+ * grid (then do not call clr_density first). */
+int clr_shell_norms(clr_ctx *ctx, int lognormal_fused, double sigma2, int cap,
+                    int *nz_out, unsigned long long *counts, double *zsum,
+                    double *sums);
+
+/* Copies a tracer's NA norm table (and its edge values norm0/normf). */
+int clr_get_norm_table(clr_ctx *ctx, int kind, int ipop, double *norm_arr,
+                       double *norm0, double *normf);
+/* Replaces a tracer's NA norm table (host-built tables, surrogates). */
+int clr_set_norm_table(clr_ctx *ctx, int kind, int ipop, const double *norm_arr,
+                       double norm0, double normf);
+
+/* srcs_set_cartesian_single (srcs.c:120-283) for population ipop: Poisson
+ * counts with lambda = n(r) dx^3 bias_model(delta,b(r)) norm(r) for cells
+ * with r < L/2+20 and n(r) > 0, then positions uniform in [x-dx/2,x+dx/2),
+ * RSD shift dz = v_r V1(r) from central differences of phi (srcs.c:87-118),
+ * and the NEST base pixel at nside_base (srcs.c:264-270).  Sources stay on
+ * the device in global z-major cell order; returns their number. */
+int clr_sample_sources(clr_ctx *ctx, int ipop, uint32_t seed, int nside_base,
+                       long long *nsrc_out);
+/* Copies population ipop's CatalogCartesian (common.h:162-167): pos[4n]
+ * = (x, y, z, dz_rsd) relative to the observer, ipix[n]. */
+int clr_fetch_sources(clr_ctx *ctx, int ipop, float *pos4, int32_t *ipix,
+                      long long n);
+/* srcs_get_local_properties_single (srcs.c:386-416): per source RA, Dec
+ * (degrees), z0 = z(r), dz_rsd; written as float[4n]. */
+int clr_source_properties(clr_ctx *ctx, int ipop, float *radecz, long long n);
+
+/* kappa_get_beam_properties (kappa.c:78-175) for pixels 0..npix-1 of a
+ * HEALPix map at nside in NEST order (the reference's pixel set on one rank,
+ * common.c:505-552); out[nplanes][npix] (NEST).  r_src are the source-plane
+ * comoving distances (any order; sorted as kappa_beams_preproc does). */
+int clr_kappa(clr_ctx *ctx, int nside, int nplanes, const double *r_src,
+              float *out);
+/* isw_get_beam_properties (isw.c:78-147); same conventions as clr_kappa. */
+int clr_isw(clr_ctx *ctx, int nside, int nplanes, const double *r_src,
+            float *out);
+/* imap_set_cartesian_single (imap.c:135-245) for intensity species ipop.
+ * r0/rf: the nr shell edges (any order, sorted as imap_preproc does).
+ * Outputs full-sky RING maps data[nr][12 nside^2] (sum of T) and
+ * nadd[nr][12 nside^2] (number of sub-samples), as the reference stores them
+ * before io.c:738-741 divides. */
+int clr_imap(clr_ctx *ctx, int ipop, int nside, int nr, const double *r0,
+             const double *rf, float *data, int32_t *nadd);
+
+/* Grid copies (density dumps io.c:565-595, parity tests).  Padded layout
+ * float[N][N][2(N/2+1)]. */
+int clr_fetch_grid(clr_ctx *ctx, int which, float *host);
+int clr_upload_grid(clr_ctx *ctx, int which, const float *host);
+/* Declares uploaded grids as the output of the field stage (sigma2 is the
+ * measured variance to use) and, if density_transformed, of the density
+ * stage too, so that later stages can run on caller-supplied grids. */
+int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
+
+/* Device time in milliseconds of the last call of each stage (0 if never). */
+int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COLORE_AMD_H */

@@ -1,0 +1,97 @@
+/*
+ * colore_host.h -- host side of the drop-in `CoLoRe <param_file>` driver:
+ * param-file parsing (the reference's libconfig keys, io.c:246-563), FP64
+ * cosmology and r-tables (cosmo.c, cosmo_mad.c), HEALPix helpers and the
+ * output writers (io.c:565-1245).  Plain C; calls the device through
+ * colore_amd.h.
+ */
+#ifndef COLORE_HOST_H
+#define COLORE_HOST_H
+#include <stdint.h>
+#include "colore_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- background cosmology (cosmo_mad.c:160-520) ------------------------- */
+typedef struct {
+  double OM, OB, OL, OK, h, w0, wa;
+  int ksign, normalDE, constantw;
+  double phorizon, a_equality, growth0;
+} clr_csm;
+/* csm_background_set; returns 0, or >0 with a message for invalid input */
+int    clr_csm_set(clr_csm *c, double OmegaM, double OmegaL, double OmegaB, double w, double wa,
+                   double h, char *err, int errlen);
+double clr_csm_hubble(const clr_csm *c, double a);              /* H(a) in h/Mpc      */
+double clr_csm_omega_m(const clr_csm *c, double a);
+double clr_csm_particle_horizon(const clr_csm *c, double a);    /* Mpc/h              */
+double clr_csm_radial_comoving_distance(const clr_csm *c, double a);
+double clr_csm_growth_factor(const clr_csm *c, double a);       /* unnormalised D(a)  */
+double clr_csm_f_growth(const clr_csm *c, double a);
+
+/* natural cubic spline (gsl_interp_cspline): NaN outside [x[0], x[n-1]] */
+void   clr_cspline_eval(int n, const double *x, const double *y, int m, const double *xq, double *yq);
+/* adaptive Gauss-Kronrod (7/15) integral of f on [a,b] */
+double clr_integrate(double (*f)(double, void *), void *p, double a, double b, double relerr);
+/* sigma^2 of the linear field smoothed with window w (0 TopHat, 1 Gauss) of
+ * radius R from a P(k) table (xi2p_L at r=0, cosmo.c:370-433) */
+double clr_sigma2_window(int numk, const double *logk, const double *pk, double logkmin, double logkmax,
+                         double idlogk, double ns, double R, int w);
+
+/* ---- run state: the reference's ParamCoLoRe subset used by this path ---- */
+typedef struct clr_run clr_run;
+
+/* read_run_params (io.c:246-563): parse, set up cosmology and tables.
+ * Returns NULL on error with the message in err. */
+clr_run *clr_run_read(const char *param_file, char *err, int errlen);
+void     clr_run_free(clr_run *r);
+/* scalar by name: "l_box","r_max","r_min","z_min","z_max","n_grid","seed",
+ * "r_smooth","smooth_potential","dens_type","sigma2_gauss","n_srcs","n_imap",
+ * "n_cstm","n_kappa","n_isw","nside_base","nside_kappa","nside_isw",
+ * "fgrowth_0","hubble_0","prefac_lensing","sig8","output_format",
+ * "output_density","sigma8_original"; NaN if unknown. */
+double   clr_run_scalar(const clr_run *r, const char *name);
+/* Points the clr_tables fields at the run's arrays (no copy). */
+int      clr_run_tables(const clr_run *r, clr_tables *out);
+/* Tracer r-tables of tracer kind/ipop (pointers into the run). */
+int      clr_run_tracer(const clr_run *r, int kind, int ipop, const double **amp, const double **bz,
+                        int *bias_model);
+/* Output prefix. */
+const char *clr_run_prefix(const clr_run *r);
+/* get_bg / r_of_z on the run's tables (cosmo.c:40-112). */
+double   clr_run_get_bg(const clr_run *r, double rr, int tag, int ipop);
+double   clr_run_r_of_z(const clr_run *r, double z);
+#define CLR_BG_Z  0
+#define CLR_BG_D1 1
+#define CLR_BG_V1 2
+#define CLR_BG_PD 3
+#define CLR_BG_IH 4
+
+/* Creates the device context and uploads every table of the run. */
+int      clr_run_device(clr_run *r, int device, clr_ctx **ctx, char *err, int errlen);
+/* The stage sequence of main.c:61-141 on the device, with the reference's
+ * stdout report and output files.  Returns 0 or an error code. */
+int      clr_run_execute(clr_run *r, int device);
+
+/* ---- HEALPix (Gorski et al. 2005) ----------------------------------------- */
+long long clr_hp_ang2pix_ring(long long nside, double cth, double phi);
+long long clr_hp_vec2pix_ring(long long nside, const double *v);
+long long clr_hp_ring2nest(long long nside, long long pix);
+long long clr_hp_nest2ring(long long nside, long long pix);
+void      clr_hp_pix2vec_nest(long long nside, long long pix, double *v);
+
+/* ---- writers ------------------------------------------------------------ */
+/* write_density_grid native format (io.c:565-595) from a padded grid */
+int clr_write_density_grid(const char *fname, int N, double L, const float *padded);
+/* he_write_healpix_map (healpix_extra.c:4-57): one 1E column, RING */
+int clr_write_healpix_map(const char *fname, long long nside, const float *map_ring);
+/* write_catalog FITS / ASCII (io.c:1019-1236), 5 columns TYPE RA DEC
+ * Z_COSMO DZ_RSD; radecz = float[4n] (RA, Dec, z0, dz_rsd) */
+int clr_write_catalog_fits(const char *fname, int ipop, long long n, const float *radecz);
+int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const float *radecz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

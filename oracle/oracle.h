@@ -1,0 +1,120 @@
+/*
+ * oracle.h -- CPU FP64 restatement of CoLoRe's field -> lognormal -> sources
+ * path and its lightcone painters.  TEST INFRASTRUCTURE ONLY: used by tests/,
+ * __graft_entry__.smoke() and bench.py's CPU baseline as the checker, never by
+ * the product path.
+ *
+ * Every function follows the reference source line by line (file:line in the
+ * comments of oracle.c).  Random numbers come from an injectable stream:
+ * either Philox4x32-10 counters (the production streams of the GPU path) or,
+ * for the Fourier modes, a caller-supplied buffer (e.g. the reference's
+ * per-thread MT19937 streams, orc_ref_mode_stream).
+ */
+#ifndef COLORE_ORACLE_H
+#define COLORE_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random numbers ---------------------------------------------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+typedef struct { uint32_t mt[624]; int mti; } orc_mt19937;
+void orc_mt_seed(orc_mt19937 *s, uint32_t seed);     /* gsl_rng_set(mt19937) */
+uint32_t orc_mt_next(orc_mt19937 *s);                /* gsl_rng_get          */
+/* The reference's mode stream (fourier.c:295-357): thread ithr of nthreads
+ * owns a contiguous block of kz planes (static schedule as libgomp splits it),
+ * seeds MT19937 with seed+ithr and draws (phase, modulus) per mode with
+ * k^2>0 in loop order.  out[2*m], out[2*m+1] for mode m = kx+(N/2+1)(ky+N kz);
+ * the k=0 mode gets zeros. */
+void orc_ref_mode_stream(int N, uint32_t seed, int nthreads, uint32_t *out);
+
+/* ---- tables ------------------------------------------------------------ */
+typedef struct {
+  int na; double idr;
+  const double *r, *z, *d1, *v1, *pd, *ih;
+} orc_bg;
+typedef struct {
+  int numk; double logkmin, logkmax, idlogk, ns;
+  const double *logk, *pk;
+} orc_pk;
+double orc_f_of_r(const orc_bg *bg, const double *f, double f0, double ff, double r);
+double orc_pk_linear0(const orc_pk *pk, double lgk);
+double orc_bias_model(double d, double b, int model);
+
+/* ---- fields (fourier.c) ------------------------------------------------ */
+/* create_grids_fourier: half spectra as interleaved complex [N][N][N/2+1]. */
+void orc_gen_modes(int N, double L, const orc_pk *pk, uint32_t seed, const uint32_t *inj,
+                   double r_smooth, int smooth_potential, double prefac_lensing,
+                   double *dens_k, double *npot_k);
+/* 3-D c2r, FFTW convention (complex IDFT over z and y, then c2r along x using
+ * Re of kx=0 and kx=N/2), unnormalised.  out: padded [N][N][2(N/2+1)]. */
+void orc_c2r_3d(int N, const double *in, double *out);
+/* Same result by direct summation, O(N^5): for N <= 16 only. */
+void orc_c2r_3d_brute(int N, const double *in, double *out);
+/* 1-D complex inverse DFT (sign +1), in place, n = 2^a 3^b 5^c. */
+void orc_cfft_inv(int n, double *data);
+/* Normalisation (fourier.c:383-398) of both grids and compute_sigma_dens
+ * (fourier.c:24-79); returns sigma^2 (mean and <d^2> in out2 if non-NULL). */
+double orc_normalize_sigma2(int N, double L, double *dens, double *npot, double *out2);
+
+/* ---- density (density.c) ----------------------------------------------- */
+void orc_density(int N, double L, const double obs[3], const orc_bg *bg, double sigma2,
+                 int dens_type, double *grid);
+/* collect_density_normalization_from_grid: returns nz; arrays of capacity cap.
+ * sums[t*nz + iz]. */
+int orc_shell_sums(int N, double L, const double obs[3], const orc_bg *bg, const double *grid,
+                   int ntr, const double *const *bz, const double *bz0, const int *model, int cap,
+                   double *zmax_out, unsigned long long *cnt, double *zsum, double *sums);
+/* compute_density_normalization host part (density.c:1272-1355) for one
+ * tracer: norm table on the NA r-grid. */
+void orc_norm_table(const orc_bg *bg, int nz, double zmax, const unsigned long long *cnt,
+                    const double *zsum, const double *sums_t, double *norm_arr, double *norm0,
+                    double *normf);
+
+/* ---- sources (srcs.c) --------------------------------------------------- */
+typedef struct {
+  int N; double L; double obs[3];
+  const orc_bg *bg;
+  const double *amp, *bz, *norm;     /* n(r), b(r), norm(r) tables (NA) */
+  double bz0, norm0, normf;
+  int model, ipop;
+  uint32_t seed;
+  double fgrowth_0, hubble_0, omega_m;
+} orc_pop;
+/* Poisson count of one cell (srcs.c:158-176) with the Philox stream. */
+int orc_cell_count(const orc_pop *p, const double *dens, int ix, int iy, int iz);
+/* All counts: counts[N^3] in cell order ix + N(iy + N iz); returns total. */
+long long orc_counts(const orc_pop *p, const double *dens, int *counts);
+/* Catalogue in z-major cell order: pos4[4n] (x,y,z,dz_rsd), ipix[n]
+ * (NEST at nside_base).  Returns n (<= cap), or -1 if cap is too small. */
+long long orc_sources(const orc_pop *p, const double *dens, const double *npot, int nside_base,
+                      long long cap, float *pos4, int *ipix);
+/* srcs_get_local_properties_single: RA, Dec, z0, dz_rsd per source. */
+void orc_source_props(const orc_bg *bg, long long n, const float *pos4, float *out4);
+/* Poisson inversion used by both paths. */
+int orc_poisson(double lam, uint64_t cell, uint32_t stream, uint32_t seed);
+
+/* ---- lightcone painters ------------------------------------------------- */
+void orc_kappa_isw(int which, int N, double L, const double obs[3], double r_max,
+                   const orc_bg *bg, const double *npot, int nside, int nplanes,
+                   const double *r_src, double *out);
+void orc_imap(int N, double L, const double obs[3], const orc_bg *bg, const double *dens,
+              const double *npot, const double *tz, const double *bz, double bz0,
+              const double *norm, double norm0, double normf, int model,
+              double fgrowth_0, double hubble_0, double omega_m,
+              int nside, int nr, const double *r0, const double *rf, double *data, int *nadd);
+
+/* ---- HEALPix -------------------------------------------------------------- */
+long long orc_ang2pix_ring(long long nside, double cth, double phi);
+long long orc_vec2pix_ring(long long nside, const double *v);
+long long orc_ring2nest(long long nside, long long pix);
+long long orc_nest2ring(long long nside, long long pix);
+void orc_pix2vec_nest(long long nside, long long pix, double *v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

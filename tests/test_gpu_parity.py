@@ -1,0 +1,163 @@
+"""Stage-by-stage parity of the HIP path (through the C ABI) against the FP64
+oracle, on the same inputs.
+
+Tolerances (FP32 device grids against the FP64 oracle):
+  * Gaussian fields: RMS error <= 2e-6 of the field RMS, max error <= 5e-5 of
+    it; sigma^2 within 1e-5 relative.
+  * lognormal / clipped density from the same Gaussian grid: |err| <= 1e-6 (1+|delta|).
+  * shell sums: counts exact, z sums 1e-12 relative, bias sums 2e-6
+    relative; norm tables 2e-6 relative.
+  * Poisson counts and the catalogue (positions, base pixels): bit-exact given
+    the same density grid and norm tables; dz_rsd within 1e-12 relative.
+  * kappa / ISW maps from the same potential: 1e-5 of the map RMS.
+"""
+import numpy as np
+import pytest
+
+from conftest import make_run
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx_for(run, n):
+    from colore_amd import lib
+    ctx = lib.Context(n, run.scalar("l_box"))
+    ctx.upload_tables(run.tables())
+    ntr = [int(run.scalar(k)) for k in ("n_srcs", "n_imap", "n_cstm")]
+    for kind in range(3):
+        for i in range(ntr[kind]):
+            amp, bz, model = run.tracer(kind, i)
+            ctx.set_tracer(kind, i, model, amp, bz)
+    return ctx
+
+
+def _oracle(run):
+    from oracle_lib import Oracle
+    return Oracle(run.table_arrays())
+
+
+def _field_err(g, o, N):
+    a = g[:, :, :N].astype(np.float64)
+    b = o[:, :, :N]
+    rms = np.sqrt(np.mean(b * b))
+    e = a - b
+    return np.sqrt(np.mean(e * e)) / rms, np.abs(e).max() / rms
+
+
+@pytest.mark.parametrize("N", [32, 64, 128, 192])
+def test_gaussian_fields_philox(workdir, N):
+    path, run = make_run("C1", N, workdir)
+    ctx = _ctx_for(run, N)
+    seed, rs = int(run.scalar("seed")), run.scalar("r_smooth")
+    s2 = ctx.gaussian_fields(seed, rs, True)
+    d, p = ctx.fetch_grid(0), ctx.fetch_grid(1)
+    od, op, os2 = _oracle(run).fields(N, run.scalar("l_box"), seed, rs, True)
+    for g, o in ((d, od), (p, op)):
+        rms_e, max_e = _field_err(g, o, N)
+        assert rms_e < 2e-6 and max_e < 5e-5, (N, rms_e, max_e)
+    assert abs(s2 / os2 - 1) < 1e-5, (s2, os2)
+
+
+def test_gaussian_fields_reference_stream(workdir):
+    """Fed the reference's per-thread MT19937 mode stream (8 threads), the
+    device spectra transform to the oracle's grids."""
+    from oracle_lib import load, p
+    import ctypes
+    N = 64
+    path, run = make_run("C1", N, workdir)
+    lib = load()
+    nmodes = N * N * (N // 2 + 1)
+    inj = np.zeros(2 * nmodes, dtype=np.uint32)
+    lib.orc_ref_mode_stream(N, 1001, 8, p(inj, ctypes.c_uint32))
+    ctx = _ctx_for(run, N)
+    s2 = ctx.gaussian_fields(1001, run.scalar("r_smooth"), True, injected=inj)
+    od, op, os2 = _oracle(run).fields(N, run.scalar("l_box"), 1001, run.scalar("r_smooth"), True, injected=inj)
+    for g, o in ((ctx.fetch_grid(0), od), (ctx.fetch_grid(1), op)):
+        rms_e, max_e = _field_err(g, o, N)
+        assert rms_e < 2e-6 and max_e < 5e-5, (rms_e, max_e)
+    assert abs(s2 / os2 - 1) < 1e-5
+
+
+@pytest.mark.parametrize("dens_type", [0, 3])
+def test_density_transform(workdir, dens_type):
+    N = 64
+    path, run = make_run("C1", N, workdir)
+    ctx = _ctx_for(run, N)
+    s2 = ctx.gaussian_fields(7, run.scalar("r_smooth"), True)
+    g0 = ctx.fetch_grid(0).astype(np.float64)
+    ctx.density(dens_type, s2)
+    g1 = ctx.fetch_grid(0)
+    o = _oracle(run).density(N, run.scalar("l_box"), g0, s2, dens_type)
+    e = np.abs(g1[:, :, :N] - o[:, :, :N]) / (1 + np.abs(o[:, :, :N]))
+    assert e.max() < 1e-6, e.max()
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_shell_norms(workdir, N):
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx = _ctx_for(run, N)
+    s2 = ctx.gaussian_fields(3003, run.scalar("r_smooth"), True)
+    g0 = ctx.fetch_grid(0).astype(np.float64)
+    ntr = int(run.scalar("n_srcs"))
+    nz, cnt, zs, sums = ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=ntr)
+    gl = ctx.fetch_grid(0)
+    orc = _oracle(run)
+    L = run.scalar("l_box")
+    ol = orc.density(N, L, g0, s2, 0)
+    assert np.abs(gl[:, :, :N] - ol[:, :, :N]).max() / (1 + np.abs(ol).max()) < 1e-6
+    tracers = [(run.tracer(0, i)[1], run.tracer(0, i)[2]) for i in range(ntr)]
+    onz, zmax, ocnt, ozs, osums = orc.shell_sums(N, L, gl.astype(np.float64), tracers)
+    assert nz == onz
+    assert np.array_equal(cnt, ocnt)
+    np.testing.assert_allclose(zs, ozs, rtol=1e-12)
+    np.testing.assert_allclose(sums, osums, rtol=2e-6)
+    for i in range(ntr):
+        tab, n0, nf = ctx.norm_table(0, i)
+        otab, on0, onf = orc.norm_table(nz, zmax, cnt, zs, sums[i])
+        np.testing.assert_allclose(tab, otab, rtol=1e-12)
+        assert n0 == on0 and nf == onf
+
+
+@pytest.mark.parametrize("N,ipop", [(64, 0), (128, 0), (128, 1), (128, 2)])
+def test_sources_bit_exact(workdir, N, ipop):
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx = _ctx_for(run, N)
+    seed = 3003
+    s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
+    ntr = int(run.scalar("n_srcs"))
+    ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=ntr)
+    dens = ctx.fetch_grid(0).astype(np.float64)
+    npot = ctx.fetch_grid(1).astype(np.float64)
+    n = ctx.sample_sources(ipop, seed, 2)
+    pos, ipix = ctx.fetch_sources(ipop, n)
+    amp, bz, model = run.tracer(0, ipop)
+    tab, n0, nf = ctx.norm_table(0, ipop)
+    orc = _oracle(run)
+    pop = orc.pop(N, run.scalar("l_box"), amp, bz, tab, n0, nf, model, ipop, seed)
+    tot, counts = orc.counts(pop, dens)
+    assert n == tot and n > 0
+    opos, oipix = orc.sources(pop, dens, npot, 2, cap=n + 1)
+    assert np.array_equal(pos[:, :3], opos[:, :3])
+    assert np.array_equal(ipix, oipix)
+    np.testing.assert_allclose(pos[:, 3], opos[:, 3], rtol=1e-6, atol=1e-12)
+    props = ctx.source_properties(ipop, n)
+    oprops = orc.source_props(pos)
+    np.testing.assert_allclose(props, oprops, rtol=1e-6, atol=1e-5)
+
+
+@pytest.mark.parametrize("which", ["kappa", "isw"])
+def test_los_maps(workdir, which):
+    N = 64
+    path, run = make_run("C2", N, workdir, z_max=1.2)
+    ctx = _ctx_for(run, N)
+    ctx.gaussian_fields(2002, run.scalar("r_smooth"), True)
+    npot = ctx.fetch_grid(1).astype(np.float64)
+    from colore_amd import lib
+    host = lib.load_host()
+    import ctypes
+    r_src = np.array([run.scalar("r_max") * 0.5, run.scalar("r_max") * 0.95])
+    nside = 16
+    out = ctx.kappa(nside, r_src) if which == "kappa" else ctx.isw(nside, r_src)
+    o = _oracle(run).kappa_isw(0 if which == "kappa" else 1, N, run.scalar("l_box"), npot, nside, r_src)
+    rms = np.sqrt(np.mean(o ** 2))
+    assert np.abs(out - o).max() < 1e-5 * rms + 1e-30, (np.abs(out - o).max(), rms)
