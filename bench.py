@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the field -> lognormal -> sources path on one MI355X.
+
+One step = Gaussian modes + 3-D c2r of delta and phi (with sigma^2), the
+lognormal transform fused with the FP64 per-shell normalisation of every
+tracer, and Poisson sources + positions for every population, ending with a
+complete device-resident catalogue.  Workload: 1024^3 FP32 grid, z in
+[0.05, 1.6], three populations (b=0.95/D(z) exponential, b=1.5
+exp-truncated, b=2.2 truncated), seed 3003; inputs are synthetic (EH98
+no-wiggle P(k), Gaussian n(z)).
+
+Prints one JSON line.  `value` is grid cells per second of the whole step.
+`roofline` is for the kernel with the largest device time: algorithmic HBM
+bytes per launch over its mean duration (HIP events on the library's stream),
+against 8 TB/s.  `cpu_baseline` times the FP64 CPU oracle on the same
+configuration at 256^3 on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid cells/sec (field→lognormal→sources) at 1024³ FP32; % HBM roofline"
+HBM_PEAK = 8.0e12
+
+
+def kernel_bytes(N):
+    """Algorithmic HBM bytes per launch of the three FFT kernels (both fields)."""
+    nh = N // 2 + 1
+    half = N * N * nh * 8          # one complex half spectrum
+    real = N * N * N * 4           # one real grid without padding
+    return {
+        "k_gen_z": 2 * half,                   # write both spectra after the z pass
+        "k_col_y": 4 * half,                   # read + write both
+        "k_c2r_x": 2 * half + 2 * real,        # read spectra, write real grids
+    }
+
+
+def cpu_baseline(cfg, n_cpu_grid):
+    """FP64 oracle, same config at n_cpu_grid^3, full step; returns cells/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from colore_amd import inputs, lib
+    from oracle_lib import Oracle
+    d = tempfile.mkdtemp(prefix="colore_cpu_")
+    run = lib.Run(inputs.write_config(cfg, d, n_grid=n_cpu_grid, with_maps=False))
+    orc = Oracle(run.table_arrays())
+    N, L = n_cpu_grid, run.scalar("l_box")
+    seed, rs = int(run.scalar("seed")), run.scalar("r_smooth")
+    npop = int(run.scalar("n_srcs"))
+    t0 = time.perf_counter()
+    dens, npot, s2 = orc.fields(N, L, seed, rs, True)
+    dens = orc.density(N, L, dens, s2, 0)
+    tr = [(run.tracer(0, i)[1], run.tracer(0, i)[2]) for i in range(npop)]
+    nz, zmax, cnt, zs, sums = orc.shell_sums(N, L, dens, tr)
+    nsrc = 0
+    for i in range(npop):
+        tab, n0, nf = orc.norm_table(nz, zmax, cnt, zs, sums[i])
+        amp, bz, model = run.tracer(0, i)
+        pop = orc.pop(N, L, amp, bz, tab, n0, nf, model, i, seed)
+        pos, _ = orc.sources(pop, dens, npot, 2)
+        nsrc += len(pos)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": N ** 3 / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 OpenMP oracle, same cosmology/populations at {N}^3 (full step, {nsrc} sources, "
+                      f"{dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C3")
+    ap.add_argument("--n-grid", type=int, default=None)
+    ap.add_argument("--cpu-grid", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from colore_amd import inputs, lib
+    d = tempfile.mkdtemp(prefix="colore_bench_")
+    run = lib.Run(inputs.write_config(args.config, d, n_grid=args.n_grid, with_maps=False))
+    N, L = int(run.scalar("n_grid")), run.scalar("l_box")
+    seed, rs = int(run.scalar("seed")), run.scalar("r_smooth")
+    npop = int(run.scalar("n_srcs"))
+    ctx = lib.Context(N, L)
+    ctx.upload_tables(run.tables())
+    for i in range(npop):
+        amp, bz, model = run.tracer(0, i)
+        ctx.set_tracer(0, i, model, amp, bz)
+
+    acc = {}
+
+    def step(record):
+        s2 = ctx.gaussian_fields(seed, rs, True)
+        st = ctx.stage_times()
+        ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=npop)
+        st["norm"] = ctx.stage_times()["norm"]
+        ns = [ctx.sample_sources(-1, seed, 2)]
+        st["srcs"] = ctx.stage_times()["srcs"]
+        if record:
+            for k in ("gen", "fft_yz", "fft_x", "norm", "srcs"):
+                acc[k] = acc.get(k, 0.0) + st[k]
+        return ns, s2
+
+    for _ in range(args.warmup):
+        step(False)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ns, s2 = step(True)
+    dt = (time.perf_counter() - t0) / args.steps
+    cells = float(N) ** 3
+    value = cells / dt
+
+    kb = kernel_bytes(N)
+    kt = {"k_gen_z": acc["gen"] / args.steps, "k_col_y": acc["fft_yz"] / args.steps,
+          "k_c2r_x": acc["fft_x"] / args.steps}
+    dom = max(kt, key=kt.get)
+    achieved = kb[dom] / (kt[dom] * 1e-3)
+    p = (N + 2.0) / N
+    nsrc_tot = sum(ns)
+    b_alg = 36 * p + 20.0 * nsrc_tot / cells
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (EH98 no-wiggle P(k), Gaussian n(z)); Philox streams",
+        "config": {"workload": f"{args.config}: {N}^3 FP32 field->lognormal->sources, {npop} populations, "
+                               f"seed {seed}, FP64 shell sums",
+                   "n_grid": N, "l_box": L, "populations": npop, "sources": nsrc_tot, "sigma2": s2},
+        "path_roofline": {"b_alg_per_cell": b_alg, "achieved": value * b_alg, "peak": HBM_PEAK,
+                          "frac": value * b_alg / HBM_PEAK},
+        "stage_ms": {k: v / args.steps for k, v in acc.items()},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes_per_launch": kb[dom], "ms_per_launch": kt[dom]},
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_grid)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -34,12 +34,38 @@ int clr_ensure(clr_ctx *c, void **p, size_t *have, size_t bytes) {
   return CLR_OK;
 }
 
+__global__ void k_d2f(const double *__restrict__ in, float *__restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+int clr_norm_f32(clr_ctx *c, TracerDev &T) {
+  if (!T.normf32) CLR_HIP(c, hipMalloc((void **)&T.normf32, (size_t)c->na * sizeof(float)));
+  hipLaunchKernelGGL(k_d2f, dim3((c->na + 255) / 256), dim3(256), 0, c->stream, T.norm, T.normf32, c->na);
+  CLR_HIP(c, hipGetLastError());
+  return CLR_OK;
+}
+
+int clr_ring2nest_table(clr_ctx *c, int nside) {
+  if (c->r2n && c->r2n_nside == nside) return CLR_OK;
+  const int64_t npix = 12LL * nside * nside;
+  std::vector<int32_t> t(npix);
+  for (int64_t p = 0; p < npix; p++) t[p] = (int32_t)clr::ring2nest(nside, p);
+  if (c->r2n) { CLR_HIP(c, hipStreamSynchronize(c->stream)); hipFree(c->r2n); c->r2n = nullptr; }
+  CLR_HIP(c, hipMalloc((void **)&c->r2n, npix * sizeof(int32_t)));
+  CLR_HIP(c, hipMemcpy(c->r2n, t.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice));
+  c->r2n_nside = nside;
+  return CLR_OK;
+}
+
 static void free_all(clr_ctx *c) {
   auto f = [](void *p) { if (p) hipFree(p); };
   f(c->dens); f(c->npot); f(c->bg); f(c->pk); f(c->tw); f(c->amp); f(c->red); f(c->red_out);
-  f(c->rowcnt); f(c->inj); f(c->hp_vec); f(c->misc);
+  f(c->rowcnt); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d1f); f(c->r2n);
   for (int k = 0; k < 3; k++)
-    for (int i = 0; i < CLR_NPOP_MAX; i++) { f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); }
+    for (int i = 0; i < CLR_NPOP_MAX; i++) {
+      f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].normf32);
+    }
   for (int i = 0; i < CLR_NPOP_MAX; i++) { f(c->cat[i].pos); f(c->cat[i].ipix); }
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -123,6 +149,19 @@ int clr_upload_tables(clr_ctx *c, const clr_tables *t) {
   c->na = na;
   c->glob_idr = t->glob_idr;
   c->r_last = t->r_arr_r2z[na - 1];
+  for (int i = 0; i < 6; i++) {
+    c->h_bg_first[i] = src[i][0];
+    c->h_bg_last[i] = src[i][na - 1];
+  }
+  // get_bg(L/2, BG_Z): the top of the normalisation shells (density.c:1233)
+  c->zmax_half_box = clr::f_of_r(t->r_arr_r2z, t->z_arr_r2z, t->glob_idr, c->r_last, 0.5 * c->L, 0.0,
+                                 t->z_arr_r2z[na - 1]);
+  std::vector<float> d1f(na);
+  for (int i = 0; i < na; i++) d1f[i] = (float)t->growth_d_arr[i];
+  if (c->d1f) hipFree(c->d1f);
+  c->d1f = nullptr;
+  CLR_HIP(c, hipMalloc((void **)&c->d1f, na * sizeof(float)));
+  CLR_HIP(c, hipMemcpy(c->d1f, d1f.data(), na * sizeof(float), hipMemcpyHostToDevice));
   if (c->pk && c->numk != t->numk) { hipFree(c->pk); c->pk = nullptr; }
   if (!c->pk) CLR_HIP(c, hipMalloc((void **)&c->pk, 2 * (size_t)t->numk * sizeof(double)));
   CLR_HIP(c, hipMemcpy(c->pk, t->logkarr, t->numk * sizeof(double), hipMemcpyHostToDevice));
@@ -155,8 +194,12 @@ int clr_set_tracer(clr_ctx *c, int kind, int ipop, int bias_model, const double 
   if (!T.amp) CLR_HIP(c, hipMalloc((void **)&T.amp, b));
   if (!T.bz) CLR_HIP(c, hipMalloc((void **)&T.bz, b));
   if (!T.norm) CLR_HIP(c, hipMalloc((void **)&T.norm, b));
+  if (!T.bzf) CLR_HIP(c, hipMalloc((void **)&T.bzf, (size_t)c->na * sizeof(float)));
   CLR_HIP(c, hipMemcpy(T.amp, amp_arr, b, hipMemcpyHostToDevice));
   CLR_HIP(c, hipMemcpy(T.bz, bz_arr, b, hipMemcpyHostToDevice));
+  std::vector<float> bzf(c->na);
+  for (int i = 0; i < c->na; i++) bzf[i] = (float)bz_arr[i];
+  CLR_HIP(c, hipMemcpy(T.bzf, bzf.data(), (size_t)c->na * sizeof(float), hipMemcpyHostToDevice));
   T.bz0 = bz_arr[0];
   T.model = bias_model;
   T.set = 1;

@@ -1,20 +1,24 @@
 // clr_density.hip -- physical density (density.c:1034-1126) and the per-shell
 // bias normalisation (density.c:1128-1393) on gfx950.
 //
-// k_transform : lognormalize / densclip in place (FP64 geometry and table
-//               lookups, FP32 exp).
-// k_shells    : one pass over the grid; per cell the FP64 radius, z(r) and
-//               shell index s = (int)(z*idz)+1, then count, z and
-//               bias_model(delta, b_t(r)) for every tracer t, accumulated in
-//               FP64.  Lanes of a wavefront that share a shell are reduced
-//               with shuffles and added by one lane into that wavefront's own
-//               LDS slots, so the order of every addition is fixed: the sums
-//               are bitwise repeatable.  With `fused`, the lognormal
-//               transform is applied in the same pass (one read + one write
-//               of the grid instead of two of each).
-// k_shell_final / k_norm_table : per-block partials summed in a fixed order,
-//               norm = count/sum, edge fill, and the linear interpolation in z
-//               onto the NA r-grid (density.c:1272-1355), all on the device.
+// k_transform : lognormalize / densclip in place.
+// k_shells    : one pass over the grid.  Each wavefront walks tiles of
+//               64 (x, one cell per lane) x 4 (y) x 16 (z) cells; per cell the
+//               FP64 radius, z(r) and shell index s = (int)(z*idz)+1 (exactly
+//               the reference's expression, so the counts match bit for bit),
+//               then count, z and bias_model(delta, b_t(r)) for every tracer.
+//               A lane's radius changes slowly along its 64 cells, so it adds
+//               into private FP64 accumulators of its current shell and keeps
+//               at most kSlots finished shells; the lanes are combined once
+//               per tile with DPP wave sums, and lane 0 adds the result into
+//               the wavefront's own LDS slots.  Every addition happens in a
+//               fixed order: the sums are bitwise repeatable.  With `fused`
+//               the lognormal transform is applied in the same pass (one
+//               read and one write of the grid for both stages).
+// k_shell_final / k_norm_table : fixed-order reduction of the per-block
+//               partials, norm = count/sum, edge fill and linear
+//               interpolation in z onto the NA r-grid (density.c:1272-1355),
+//               all on the device.
 #include "clr_internal.h"
 #include <math.h>
 
@@ -22,21 +26,54 @@ using namespace clr;
 
 namespace {
 
-constexpr int kShellBlocks = 1024;
+constexpr int kShellBlocks = 2048;
 constexpr int kShellThreads = 256;
 constexpr int kWaves = kShellThreads / 64;
+constexpr int kTY = 4, kTZ = 16;   // tile rows: kTY x kTZ rows of 64 cells
+constexpr int kSlots = 4;
+constexpr int kMaxTr = 8;
 
-struct TracerArgs {
-  const double *bz[3 * CLR_NPOP_MAX];
-  double bz0[3 * CLR_NPOP_MAX];
-  int model[3 * CLR_NPOP_MAX];
-  int n;
+struct ShellArgs {
+  float *g;
+  int N, ngx, nz, ntr, fused, na;
+  double dx, ox, oy, oz, idr, r_last, z_last, sigma2, idz;
+  const double *r_arr, *zt;
+  const float *d1f;
+  float d1_last_f, idr_f;
+  const float *bzf[kMaxTr];
+  float bz0[kMaxTr];
+  int model[kMaxTr];
 };
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+// f_of_r on an FP32 copy of a table (values that feed sums, not decisions)
+__device__ inline float f_of_r32(const float *__restrict__ f, float idr, int nm1, float r, float f0, float ff) {
+  if (r <= 0.f) return f0;
+  const float x = r * idr;
+  if (x >= (float)nm1) return ff;
+  const int ir = (int)x;
+  const float t = x - (float)ir;
+  return f[ir] + (f[ir + 1] - f[ir]) * t;
+}
+
+template <int CTRL, int RM>
+__device__ inline double dpp_get(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, RM, 0xF, true);
+  const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, RM, 0xF, true);
+  return __hiloint2double(h2, l2);
+}
+
+// Sum over the 64 lanes of a wavefront (all lanes active), broadcast.
+__device__ inline double wave_sum(double x) {
+  x += dpp_get<0xB1, 0xF>(x);    // quad_perm [1,0,3,2]
+  x += dpp_get<0x4E, 0xF>(x);    // quad_perm [2,3,0,1]
+  x += dpp_get<0x141, 0xF>(x);   // row_half_mirror
+  x += dpp_get<0x140, 0xF>(x);   // row_mirror
+  x += dpp_get<0x142, 0xA>(x);   // row_bcast:15 into rows 1,3
+  x += dpp_get<0x143, 0xC>(x);   // row_bcast:31 into rows 2,3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+  return __hiloint2double(hi, lo);
 }
 
 __global__ __launch_bounds__(256) void k_transform(float *__restrict__ g, int N, int ngx, double dx,
@@ -64,86 +101,166 @@ __global__ __launch_bounds__(256) void k_transform(float *__restrict__ g, int N,
   }
 }
 
-// nq = 2 + ntr values per shell: [0] count, [1] z sum, [2+t] tracer t.
-__global__ __launch_bounds__(kShellThreads) void k_shells(
-    float *__restrict__ g, int N, int ngx, double dx, double ox, double oy, double oz,
-    const double *__restrict__ r_arr, const double *__restrict__ zt, const double *__restrict__ d1,
-    double idr, double r_last, double z_last, double d1_last, double sigma2, int fused, int nz,
-    double idz, TracerArgs ta, double *__restrict__ partial) {
+// Combine the slot-k accumulators of all lanes: lanes are grouped by shell
+// id (in order of the lowest lane) and each group is summed with DPP.
+template <int NQ>
+__device__ inline void flush_slot(unsigned long long mask, int key, const double *vals, double *myacc,
+                                  int lane) {
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    const int sl = __builtin_amdgcn_readlane(key, leader);
+    const bool mine = ((mask >> lane) & 1ull) && key == sl;
+    const unsigned long long mm = __ballot(mine);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const double v = wave_sum(mine ? vals[q] : 0.0);
+      if (lane == 0) myacc[sl * NQ + q] += v;
+    }
+    mask &= ~mm;
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *__restrict__ partial) {
 #pragma clang fp contract(off)
-  extern __shared__ double acc[];   // [kWaves][nz][nq]
-  const int nq = 2 + ta.n;
+  constexpr int NQ = 2 + NT;
+  extern __shared__ double acc[];   // [kWaves][nz][NQ]
+  const int nz = a.nz, N = a.N;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  double *myacc = acc + (size_t)wave * nz * nq;
-  for (int i = threadIdx.x; i < kWaves * nz * nq; i += blockDim.x) acc[i] = 0;
+  double *myacc = acc + (size_t)wave * nz * NQ;
+  for (int i = threadIdx.x; i < kWaves * nz * NQ; i += blockDim.x) acc[i] = 0;
   __syncthreads();
-  const double obs[3] = {ox, oy, oz};
-  const size_t nrows = (size_t)N * N;
-  for (size_t row = blockIdx.x; row < nrows; row += gridDim.x) {
-    const int iz = (int)(row / N), iy = (int)(row % N);
-    float *p = g + row * (size_t)ngx;
-    for (int x0i = 0; x0i < N; x0i += blockDim.x) {
-      const int ix = x0i + threadIdx.x;
+  const double obs[3] = {a.ox, a.oy, a.oz};
+  const int ntx = (N + 63) / 64, nty = N / kTY, ntz = N / kTZ;
+  const long ntiles = (long)ntx * nty * ntz;
+  const long nw = (long)gridDim.x * kWaves;
+  for (long tile = (long)blockIdx.x * kWaves + wave; tile < ntiles; tile += nw) {
+    const int tx = (int)(tile % ntx);
+    const long rest = tile / ntx;
+    const int ty = (int)(rest % nty), tz = (int)(rest / nty);
+    const int ix = tx * 64 + lane;
+    const bool valid = ix < N;
+    int nslot = 0, sid[kSlots];
+    double sacc[kSlots][NQ];
+    int cur = -1;
+    double cacc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) cacc[q] = 0;
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) {
+      sid[k] = -1;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) sacc[k][q] = 0;
+    }
+    for (int rr = 0; rr < kTY * kTZ; rr++) {
+      const int iy = ty * kTY + (rr % kTY), iz = tz * kTZ + (rr / kTY);
       int s = -1;
-      double zz = 0, r = 0;
-      float dl = 0.f;
-      if (ix < N) {
+      double v[NQ];
+      if (valid) {
         double x0, y0, z0;
-        r = cell_radius(ix, iy, iz, dx, obs, &x0, &y0, &z0);
-        zz = f_of_r(r_arr, zt, idr, r_last, r, 0.0, z_last);
-        dl = p[ix];
-        if (fused) {
-          const double dg = f_of_r(r_arr, d1, idr, r_last, r, 1.0, d1_last);
-          const float a = (float)(dg * ((double)dl - 0.5 * dg * sigma2));
-          dl = expf(a) - 1.f;
-          p[ix] = dl;
+        const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
+        const double zz = f_of_r(a.r_arr, a.zt, a.idr, a.r_last, r, 0.0, a.z_last);
+        float *pc = a.g + ((size_t)iz * N + iy) * a.ngx + ix;
+        float dl = *pc;
+        const float rf = (float)r;
+        if (a.fused) {
+          const float dg = f_of_r32(a.d1f, a.idr_f, a.na - 1, rf, 1.f, a.d1_last_f);
+          const float arg = dg * (dl - 0.5f * dg * (float)a.sigma2);
+          dl = expf(arg) - 1.f;
+          *pc = dl;
         }
-        const int ind = (int)(zz * idz) + 1;
-        if (ind >= 0 && ind < nz) s = ind;
-      }
-      unsigned long long active = __ballot(s >= 0);
-      while (active) {
-        const int leader = __ffsll((long long)active) - 1;
-        const int sl = __shfl(s, leader);
-        const bool mine = (s == sl);
-        const unsigned long long mmask = __ballot(mine);
-        double v = wave_sum(mine ? 1.0 : 0.0);
-        if (lane == 0) myacc[sl * nq + 0] += v;
-        v = wave_sum(mine ? zz : 0.0);
-        if (lane == 0) myacc[sl * nq + 1] += v;
-        for (int t = 0; t < ta.n; t++) {
-          double term = 0;
-          if (mine) {
-            const double b = f_of_r(r_arr, ta.bz[t], idr, r_last, r, ta.bz0[t], 1.0);
-            term = (double)bias_model_f32(dl, (float)b, ta.model[t]);
+        const int ind = (int)(zz * a.idz) + 1;
+        if (ind >= 0 && ind < nz) {
+          s = ind;
+          v[0] = 1.0;
+          v[1] = zz;
+#pragma unroll
+          for (int t = 0; t < NT; t++) {
+            const float b = f_of_r32(a.bzf[t], a.idr_f, a.na - 1, rf, a.bz0[t], 1.f);
+            v[2 + t] = (double)bias_model_f32(dl, b, a.model[t]);
           }
-          v = wave_sum(term);
-          if (lane == 0) myacc[sl * nq + 2 + t] += v;
         }
-        active &= ~mmask;
+      }
+      const bool change = s >= 0 && s != cur;
+      // a lane whose slots are full flushes them first (wave-collective, rare)
+      if (__ballot(change && cur >= 0 && nslot == kSlots)) {
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) flush_slot<NQ>(__ballot(k < nslot), sid[k], sacc[k], myacc, lane);
+        nslot = 0;
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) {
+          sid[k] = -1;
+#pragma unroll
+          for (int q = 0; q < NQ; q++) sacc[k][q] = 0;
+        }
+      }
+      if (change) {
+        if (cur >= 0) {
+#pragma unroll
+          for (int k = 0; k < kSlots; k++)
+            if (k == nslot) {
+              sid[k] = cur;
+#pragma unroll
+              for (int q = 0; q < NQ; q++) sacc[k][q] = cacc[q];
+            }
+          nslot++;
+        }
+        cur = s;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) cacc[q] = 0;
+      }
+      if (s >= 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) cacc[q] += v[q];
       }
     }
+    // end of tile: the current shell goes into a slot (flushing first if full)
+    if (__ballot(cur >= 0 && nslot == kSlots)) {
+#pragma unroll
+      for (int k = 0; k < kSlots; k++) flush_slot<NQ>(__ballot(k < nslot), sid[k], sacc[k], myacc, lane);
+      nslot = 0;
+    }
+    if (cur >= 0) {
+#pragma unroll
+      for (int k = 0; k < kSlots; k++)
+        if (k == nslot) {
+          sid[k] = cur;
+#pragma unroll
+          for (int q = 0; q < NQ; q++) sacc[k][q] = cacc[q];
+        }
+      nslot++;
+    }
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) flush_slot<NQ>(__ballot(k < nslot), sid[k], sacc[k], myacc, lane);
   }
   __syncthreads();
-  double *out = partial + (size_t)blockIdx.x * nz * nq;
-  for (int i = threadIdx.x; i < nz * nq; i += blockDim.x) {
+  double *out = partial + (size_t)blockIdx.x * nz * NQ;
+  for (int i = threadIdx.x; i < nz * NQ; i += blockDim.x) {
     double v = 0;
-    for (int w = 0; w < kWaves; w++) v += acc[(size_t)w * nz * nq + i];
+    for (int w = 0; w < kWaves; w++) v += acc[(size_t)w * nz * NQ + i];
     out[i] = v;
   }
 }
 
-__global__ void k_shell_final(const double *__restrict__ partial, int nblocks, int nvals,
-                              double *__restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvals) return;
+// out[i] = sum over blocks of partial[b][i], one 256-thread block per value,
+// fixed-order tree: deterministic.
+__global__ __launch_bounds__(256) void k_shell_final(const double *__restrict__ partial, int nblocks, int nvals,
+                                                     double *__restrict__ out) {
+  __shared__ double s[256];
+  const int i = blockIdx.x;
   double v = 0;
-  for (int b = 0; b < nblocks; b++) v += partial[(size_t)b * nvals + i];
-  out[i] = v;
+  for (int b = threadIdx.x; b < nblocks; b += 256) v += partial[(size_t)b * nvals + i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = s[0];
 }
 
 // Per tracer t: norm table on the NA r-grid.  sums = [nz][nq] from
-// k_shell_final; work = scratch of 2*nz doubles per tracer.
+// k_shell_final; work = scratch of 2*nz doubles.
 __global__ void k_norm_table(const double *__restrict__ sums, int nz, int nq, int t, double zmax,
                              const double *__restrict__ r_arr, const double *__restrict__ zt,
                              double idr, double r_last, double z_last, int na, double *__restrict__ work,
@@ -189,6 +306,20 @@ __global__ void k_norm_table(const double *__restrict__ sums, int nz, int nq, in
   }
 }
 
+template <int NT>
+void launch_shells(dim3 g, dim3 b, size_t lds, hipStream_t s, const ShellArgs &a, double *partial) {
+  hipLaunchKernelGGL((k_shells<NT>), g, b, lds, s, a, partial);
+}
+using ShellFn = void (*)(dim3, dim3, size_t, hipStream_t, const ShellArgs &, double *);
+const ShellFn kShellFns[kMaxTr + 1] = {launch_shells<0>, launch_shells<1>, launch_shells<2>, launch_shells<3>,
+                                        launch_shells<4>, launch_shells<5>, launch_shells<6>, launch_shells<7>,
+                                        launch_shells<8>};
+const void *kShellKernels[kMaxTr + 1] = {(const void *)k_shells<0>, (const void *)k_shells<1>,
+                                         (const void *)k_shells<2>, (const void *)k_shells<3>,
+                                         (const void *)k_shells<4>, (const void *)k_shells<5>,
+                                         (const void *)k_shells<6>, (const void *)k_shells<7>,
+                                         (const void *)k_shells<8>};
+
 }  // namespace
 
 int clr_launch_density(clr_ctx *c, int dens_type, double sigma2) {
@@ -197,12 +328,10 @@ int clr_launch_density(clr_ctx *c, int dens_type, double sigma2) {
     return clr_fail(c, CLR_ERR_ARG, "Density type %d not supported", dens_type);
   const int N = c->N;
   const double *r = c->bg, *d1 = c->bg + 2 * c->na;
-  double d1_last;
-  CLR_HIP(c, hipMemcpy(&d1_last, d1 + c->na - 1, sizeof(double), hipMemcpyDeviceToHost));
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
   hipLaunchKernelGGL(k_transform, dim3((unsigned)((size_t)N * N)), dim3(N >= 256 ? 256 : N), 0, c->stream,
                      c->dens, N, c->ngx, c->dx, c->obs[0], c->obs[1], c->obs[2], r, d1, c->glob_idr,
-                     c->r_last, d1_last, sigma2, dens_type);
+                     c->r_last, c->h_bg_last[2], sigma2, dens_type);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
@@ -222,82 +351,87 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   if (!lognormal_fused && !c->density_done)
     return clr_fail(c, CLR_ERR_STATE, "clr_density must run first (or use lognormal_fused)");
   const int N = c->N, na = c->na;
-  const double *r = c->bg, *zt = c->bg + na, *d1 = c->bg + 2 * na;
-  double lastv[2];
-  CLR_HIP(c, hipMemcpy(&lastv[0], zt + na - 1, sizeof(double), hipMemcpyDeviceToHost));
-  CLR_HIP(c, hipMemcpy(&lastv[1], d1 + na - 1, sizeof(double), hipMemcpyDeviceToHost));
-  const double z_last = lastv[0], d1_last = lastv[1];
+  const double *r = c->bg, *zt = c->bg + na;
+  const double z_last = c->h_bg_last[1];
 
   // shells: zmax = z(L/2), nz = (int)(zmax/DZ_SIGMA)+2, idz = (nz-2)/zmax
-  double zmax;
-  {
-    double hv[2];
-    const double rr = 0.5 * c->L;
-    const int ir = (int)(rr * c->glob_idr);
-    if (rr <= 0) zmax = 0;
-    else if (rr >= c->r_last) zmax = z_last;
-    else {
-#pragma clang fp contract(off)
-      CLR_HIP(c, hipMemcpy(hv, zt + ir, 2 * sizeof(double), hipMemcpyDeviceToHost));
-      double rv;
-      CLR_HIP(c, hipMemcpy(&rv, r + ir, sizeof(double), hipMemcpyDeviceToHost));
-      zmax = hv[0] + (hv[1] - hv[0]) * (rr - rv) * c->glob_idr;
-    }
-  }
+  const double zmax = c->zmax_half_box;
   const int nz = (int)(zmax / 0.05) + 2;
   const double idz = (nz - 2) / zmax;
 
-  TracerArgs ta{};
+  ShellArgs a{};
   int tk[3 * CLR_NPOP_MAX], tp[3 * CLR_NPOP_MAX];
-  ta.n = 0;
+  int ntr = 0;
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < c->n_tr[k]; i++) {
       if (!c->tr[k][i].set) return clr_fail(c, CLR_ERR_STATE, "tracer %d/%d has no tables", k, i);
-      ta.bz[ta.n] = c->tr[k][i].bz;
-      ta.bz0[ta.n] = c->tr[k][i].bz0;
-      ta.model[ta.n] = c->tr[k][i].model;
-      tk[ta.n] = k;
-      tp[ta.n] = i;
-      ta.n++;
+      if (ntr >= kMaxTr) return clr_fail(c, CLR_ERR_ARG, "at most %d tracers per normalisation pass", kMaxTr);
+      a.bzf[ntr] = c->tr[k][i].bzf;
+      a.bz0[ntr] = (float)c->tr[k][i].bz0;
+      a.model[ntr] = c->tr[k][i].model;
+      tk[ntr] = k;
+      tp[ntr] = i;
+      ntr++;
     }
-  const int nq = 2 + ta.n;
+  for (int t = 0; t < ntr; t++)
+    if (!a.bzf[t] || !c->tr[tk[t]][tp[t]].norm) return clr_fail(c, CLR_ERR_STATE, "tracer tables missing");
+  if (!c->d1f || !c->bg) return clr_fail(c, CLR_ERR_STATE, "background tables missing");
+  a.g = c->dens;
+  a.N = N;
+  a.ngx = c->ngx;
+  a.nz = nz;
+  a.ntr = ntr;
+  a.fused = lognormal_fused ? 1 : 0;
+  a.na = na;
+  a.dx = c->dx;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.z_last = z_last;
+  a.sigma2 = sigma2;
+  a.idz = idz;
+  a.r_arr = r;
+  a.zt = zt;
+  a.d1f = c->d1f;
+  a.d1_last_f = (float)c->h_bg_last[2];
+  a.idr_f = (float)c->glob_idr;
+  const int nq = 2 + ntr;
   const size_t lds = (size_t)kWaves * nz * nq * sizeof(double);
-  if (lds > 160 * 1024) return clr_fail(c, CLR_ERR_ARG, "too many shells x tracers (%d x %d)", nz, ta.n);
-  if (lds > 64 * 1024) {
-    CLR_HIP(c, hipFuncSetAttribute((const void *)k_shells, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-  }
+  if (lds > 160 * 1024) return clr_fail(c, CLR_ERR_ARG, "too many shells x tracers (%d x %d)", nz, ntr);
+  if (lds > 64 * 1024)
+    CLR_HIP(c, hipFuncSetAttribute(kShellKernels[ntr], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const size_t nvals = (size_t)nz * nq;
-  size_t need = (size_t)kShellBlocks * nvals + nvals + 2 * nz + 2;
+  size_t need = (size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr;
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
   if (rc) return rc;
   double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals, *work = fin + nvals;
 
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  const int fused = lognormal_fused ? 1 : 0;
-  hipLaunchKernelGGL(k_shells, dim3(kShellBlocks), dim3(kShellThreads), lds, c->stream, c->dens, N, c->ngx,
-                     c->dx, c->obs[0], c->obs[1], c->obs[2], r, zt, d1, c->glob_idr, c->r_last, z_last,
-                     d1_last, sigma2, fused, nz, idz, ta, partial);
+  kShellFns[ntr](dim3(kShellBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
   CLR_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_shell_final, dim3((unsigned)((nvals + 255) / 256)), dim3(256), 0, c->stream, partial,
-                     kShellBlocks, (int)nvals, fin);
+  hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,
+                     (int)nvals, fin);
   CLR_HIP(c, hipGetLastError());
-  for (int t = 0; t < ta.n; t++) {
+  for (int t = 0; t < ntr; t++) {
     TracerDev &T = c->tr[tk[t]][tp[t]];
     hipLaunchKernelGGL(k_norm_table, dim3(1), dim3(256), 0, c->stream, fin, nz, nq, t, zmax, r, zt,
-                       c->glob_idr, c->r_last, z_last, na, work, T.norm, work + 2 * nz);
+                       c->glob_idr, c->r_last, z_last, na, work, T.norm, work + 2 * nz + 2 * t);
     CLR_HIP(c, hipGetLastError());
-    double edges[2];
-    CLR_HIP(c, hipMemcpyAsync(edges, work + 2 * nz, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CLR_HIP(c, hipStreamSynchronize(c->stream));
-    T.norm0 = edges[0];
-    T.normf = edges[1];
-    T.norm_set = 1;
   }
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
-  std::vector<double> h(nvals);
+  std::vector<double> h(nvals + 2 * ntr);
   CLR_HIP(c, hipMemcpyAsync(h.data(), fin, nvals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(h.data() + nvals, work + 2 * nz, 2 * ntr * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < ntr; t++) {
+    TracerDev &T = c->tr[tk[t]][tp[t]];
+    T.norm0 = h[nvals + 2 * t];
+    T.normf = h[nvals + 2 * t + 1];
+    T.norm_set = 1;
+  }
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->stage_ms[ST_NORM] = ms;
@@ -308,9 +442,9 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
     if (counts) counts[iz] = (unsigned long long)h[(size_t)iz * nq + 0];
     if (zsum) zsum[iz] = h[(size_t)iz * nq + 1];
     if (sums)
-      for (int t = 0; t < ta.n; t++) sums[(size_t)t * nz + iz] = h[(size_t)iz * nq + 2 + t];
+      for (int t = 0; t < ntr; t++) sums[(size_t)t * nz + iz] = h[(size_t)iz * nq + 2 + t];
   }
-  if (fused) c->density_done = 1;
+  if (lognormal_fused) c->density_done = 1;
   c->norms_done = 1;
   return CLR_OK;
 }

@@ -36,11 +36,20 @@ __host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
   return c;
 }
 
-// Stream identifiers (counter word z).  Counter word x/y = 64-bit site index
-// (mode or cell), word w = draw index within the site.
+// Stream identifiers (counter word z).  Counter word x/y = 64-bit site index,
+// word w = draw index within the site.
+//   modes    : site = kx + (N/2+1)(ky + N (kz mod N/2)); words (0,1) are the
+//              (phase, modulus) draws of kz < N/2, words (2,3) those of
+//              kz + N/2.
+//   Poisson  : site = cell/4 (cell = ix + N(iy + N iz)); word cell%4 is the
+//              cell's uniform; extra uniforms of the lambda > 500 split come
+//              from site = cell in stream_poisson_extra.
+//   positions: site = cell; uniform j of the cell's sources is word j%4 of
+//              draw j/4 (three per source, x y z).
 constexpr uint32_t kKey1        = 0x436F4C52u;   // second key word
 constexpr uint32_t kStreamModes = 0x00000000u;
 __host__ __device__ inline uint32_t stream_poisson(int ipop) { return 0x100u + (uint32_t)ipop; }
+__host__ __device__ inline uint32_t stream_poisson_extra(int ipop) { return 0x80000100u + (uint32_t)ipop; }
 __host__ __device__ inline uint32_t stream_position(int ipop) { return 0x200u + (uint32_t)ipop; }
 
 __host__ __device__ inline u32x4 draw(uint64_t site, uint32_t stream, uint32_t idx, uint32_t seed) {

@@ -87,25 +87,33 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
   const int kx = blockIdx.x * T + c, ky = blockIdx.y;
   const bool valid = kx < NH;
   const int fy = (2 * ky <= N) ? ky : N - ky;
+  static_assert(E % 2 == 0, "modes are drawn in (kz, kz+N/2) pairs");
   float2 vd[E], vp[E];
+  // thread t holds kz = t + P j, j < E; element j sits at v[(j % (E/R0)) R0 + j / (E/R0)]
 #pragma unroll
-  for (int b = 0; b < E / R0; b++) {
+  for (int j = 0; j < E / 2; j++) {
+    const int kz0 = t + P * j;   // < N/2; its partner kz0 + N/2 is element j + E/2
+    u32x4 r{0u, 0u, 0u, 0u};
+    if (!inj) {
+      const uint64_t site = (uint64_t)kx + (uint64_t)NH * ((uint64_t)ky + (uint64_t)N * (uint64_t)kz0);
+      r = draw(site, kStreamModes, 0u, seed);
+    }
 #pragma unroll
-    for (int m = 0; m < R0; m++) {
-      const int kz = t + b * P + m * (N / R0);
+    for (int h = 0; h < 2; h++) {
+      const int jj = j + h * (E / 2);
+      const int kz = kz0 + h * (N / 2);
       const int fz = (2 * kz <= N) ? kz : N - kz;
       const int q = kx * kx + fy * fy + fz * fz;
       float2 d = make_float2(0.f, 0.f), p = d;
       if (valid && q > 0) {
-        const uint64_t mi = (uint64_t)kx + (uint64_t)NH * ((uint64_t)ky + (uint64_t)N * (uint64_t)kz);
         uint32_t x0, x1;
         if (inj) {
+          const uint64_t mi = (uint64_t)kx + (uint64_t)NH * ((uint64_t)ky + (uint64_t)N * (uint64_t)kz);
           x0 = inj[2 * mi];
           x1 = inj[2 * mi + 1];
         } else {
-          const u32x4 r = draw(mi, kStreamModes, 0u, seed);
-          x0 = r.x;
-          x1 = r.y;
+          x0 = h ? r.z : r.x;
+          x1 = h ? r.w : r.y;
         }
         const float w = modulus_draw(x1);
         float sn, cs;
@@ -114,8 +122,9 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
         d = make_float2(a.x * w * cs, a.x * w * sn);
         p = make_float2(a.y * w * cs, a.y * w * sn);
       }
-      vd[b * R0 + m] = d;
-      vp[b * R0 + m] = p;
+      const int vi = (jj % (E / R0)) * R0 + jj / (E / R0);
+      vd[vi] = d;
+      vp[vi] = p;
     }
   }
   const LdsCol<T> io{lds, c};

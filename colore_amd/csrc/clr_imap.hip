@@ -139,7 +139,7 @@ int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, c
     const double sct = a0 * pixel_size, scr = af - a0, scv = pow(vol, 0.333333);
     const double sc0 = fmin(scv, fmin(sct, scr));
     nsub[i] = (int)(dx / sc0 + 0.5) + 1;
-    if (nsub[i] < 1 || nsub[i] > 4096)
+    if (nsub[i] < 1 || nsub[i] > 512)
       return clr_fail(c, CLR_ERR_ARG, "shell %d gives %d sub-samples per cell side", i, nsub[i]);
   }
   ImapArgs a{};

@@ -14,6 +14,8 @@ struct TracerDev {
   double *amp = nullptr;    // n(r) / T(r) / K(r), NA
   double *bz = nullptr;     // b(r), NA
   double *norm = nullptr;   // norm(r), NA
+  float *bzf = nullptr;     // FP32 copies (values feeding FP32 estimates only)
+  float *normf32 = nullptr;
   double bz0 = 0;           // f0 of the bias table (its first entry)
   double norm0 = 1, normf = 1;
   int norm_set = 0;
@@ -45,9 +47,14 @@ struct clr_ctx {
   // FP64 tables
   int tables_set = 0;
   double *bg = nullptr;     // r, z, d1, v1, pd, ih (NA each)
+  float *d1f = nullptr;     // FP32 copy of the growth table
   int na = 0;
   double glob_idr = 0;
   double r_last = 0;
+  double h_bg_first[6] = {0}, h_bg_last[6] = {0};   // first/last entries of each bg table
+  double zmax_half_box = 0;                           // z(L/2), get_bg(L/2, BG_Z)
+  int32_t *r2n = nullptr;   // RING -> NEST table at r2n_nside
+  int r2n_nside = 0;
   double *pk = nullptr;     // logkarr[numk], pkarr[numk]
   int numk = 0;
   double logkmin = 0, logkmax = 0, idlogk = 0, n_scal = 0;
@@ -93,7 +100,8 @@ int clr_fail(clr_ctx *c, int code, const char *fmt, ...);
   } while (0)
 
 int clr_ensure(clr_ctx *c, void **p, size_t *have, size_t bytes);
-clr::BgTables clr_bg(const clr_ctx *c);
+int clr_ring2nest_table(clr_ctx *c, int nside);
+int clr_norm_f32(clr_ctx *c, TracerDev &T);
 
 // stage launchers (one per file)
 int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_potential,

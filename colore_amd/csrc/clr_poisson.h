@@ -2,10 +2,11 @@
 // (srcs.c:136-187) with counter-based uniforms.
 //
 // Sampling algorithm (both the device and the CPU oracle): inversion by
-// sequential search, k = min{k : u <= F(k; lambda)} with u = x/2^32 the first
-// word of Philox(cell, stream_poisson(ipop), 0); for lambda > 500 the count is
-// the sum of m = ceil(lambda/500) inversions at lambda/m (Poisson is closed
-// under sums), using words 1, 2, ... of the same counter sequence.
+// sequential search, k = min{k : u <= F(k; lambda)} with u = x/2^32 the
+// cell's word of Philox(cell/4, stream_poisson(ipop), 0); for lambda > 500
+// the count is the sum of m = ceil(lambda/500) inversions at lambda/m
+// (Poisson is closed under sums), the i-th extra one (i >= 1) using word
+// (i-1)%4 of Philox(cell, stream_poisson_extra(ipop), (i-1)/4).
 //
 // The device evaluates lambda and the search in FP32 first and accepts the
 // result only when u lies further than a stated error bound from both CDF
@@ -33,23 +34,19 @@ __host__ __device__ inline int poisson_inv_f64(double lam, double u) {
   return k;
 }
 
-__host__ __device__ inline int poisson_exact(double lam, uint64_t cell, uint32_t stream, uint32_t seed,
+__host__ __device__ inline int poisson_exact(double lam, uint64_t cell, uint32_t stream_extra, uint32_t seed,
                                              uint32_t first_word) {
 #pragma clang fp contract(off)
   if (!(lam > 0)) return 0;
   if (lam <= kPoisSplit) return poisson_inv_f64(lam, u01(first_word));
   const int m = (int)ceil(lam / kPoisSplit);
   const double lp = lam / m;
-  int k = 0;
+  int k = poisson_inv_f64(lp, u01(first_word));
   u32x4 r{};
-  for (int i = 0; i < m; i++) {
-    uint32_t x;
-    if (i == 0) x = first_word;
-    else {
-      if ((i & 3) == 0 || i == 1) r = draw(cell, stream, (uint32_t)(i >> 2), seed);
-      x = word(r, i & 3);
-    }
-    k += poisson_inv_f64(lp, u01(x));
+  for (int i = 1; i < m; i++) {
+    const int j = i - 1;
+    if ((j & 3) == 0) r = draw(cell, stream_extra, (uint32_t)(j >> 2), seed);
+    k += poisson_inv_f64(lp, u01(word(r, j & 3)));
   }
   return k;
 }

@@ -143,7 +143,10 @@ int clr_set_norm_table(clr_ctx *ctx, int kind, int ipop, const double *norm_arr,
  * with r < L/2+20 and n(r) > 0, then positions uniform in [x-dx/2,x+dx/2),
  * RSD shift dz = v_r V1(r) from central differences of phi (srcs.c:87-118),
  * and the NEST base pixel at nside_base (srcs.c:264-270).  Sources stay on
- * the device in global z-major cell order; returns their number. */
+ * the device in global z-major cell order; returns their number.  ipop = -1
+ * samples every population in one pass over the grid (srcs_set_cartesian,
+ * srcs.c:285-294) with the same catalogues as one call per population, and
+ * returns the total. */
 int clr_sample_sources(clr_ctx *ctx, int ipop, uint32_t seed, int nside_base,
                        long long *nsrc_out);
 /* Copies population ipop's CatalogCartesian (common.h:162-167): pos[4n]

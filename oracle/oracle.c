@@ -163,9 +163,12 @@ void orc_gen_modes(int N, double L, const orc_pk *pk, uint32_t seed, const uint3
         uint32_t x0, x1;
         if (inj) { x0 = inj[2 * index]; x1 = inj[2 * index + 1]; }
         else {
+          /* one Philox draw per (kz, kz+N/2) pair of modes */
           uint32_t r[4];
-          orc_draw(index, 0u, 0u, seed, r);
-          x0 = r[0]; x1 = r[1];
+          const size_t site = (size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * (ii % (N / 2)));
+          orc_draw(site, 0u, 0u, seed, r);
+          const int hi = ii >= N / 2;
+          x0 = r[2 * hi]; x1 = r[2 * hi + 1];
         }
         /* rng_delta_gauss, common.c:191-201 */
         const double phase = 2 * M_PI * orc_u01(x0);
@@ -494,19 +497,23 @@ static int poisson_inv(double lam, double u) {
   return k;
 }
 
-/* inversion by sequential search; lambda > 500 split into ceil(lambda/500)
- * equal parts, each with its own uniform (word i of the counter sequence) */
+/* inversion by sequential search with the cell's word of Philox(cell/4,
+ * stream, 0); lambda > 500 split into ceil(lambda/500) equal parts, the i-th
+ * extra part using word (i-1)%4 of Philox(cell, stream | 0x80000000,
+ * (i-1)/4) */
 int orc_poisson(double lam, uint64_t cell, uint32_t stream, uint32_t seed) {
   if (!(lam > 0)) return 0;
   uint32_t r[4];
-  orc_draw(cell, stream, 0u, seed, r);
-  if (lam <= 500.0) return poisson_inv(lam, orc_u01(r[0]));
+  orc_draw(cell >> 2, stream, 0u, seed, r);
+  const double u0 = orc_u01(r[cell & 3]);
+  if (lam <= 500.0) return poisson_inv(lam, u0);
   const int m = (int)ceil(lam / 500.0);
   const double lp = lam / m;
-  int k = 0;
-  for (int i = 0; i < m; i++) {
-    if (i > 0 && (i & 3) == 0) orc_draw(cell, stream, (uint32_t)(i >> 2), seed, r);
-    k += poisson_inv(lp, orc_u01(r[i & 3]));
+  int k = poisson_inv(lp, u0);
+  for (int i = 1; i < m; i++) {
+    const int j = i - 1;
+    if ((j & 3) == 0) orc_draw(cell, stream | 0x80000000u, (uint32_t)(j >> 2), seed, r);
+    k += poisson_inv(lp, orc_u01(r[j & 3]));
   }
   return k;
 }
@@ -575,41 +582,54 @@ long long orc_sources(const orc_pop *p, const double *dens, const double *npot, 
   const double dx = p->L / N;
   const double factor_vel = -p->fgrowth_0 / (1.5 * p->hubble_0 * p->omega_m);
   const double v1f0 = p->bg->v1[0], v1ff = p->bg->v1[p->bg->na - 1];
-  long long pid = 0;
-  for (int iz = 0; iz < N; iz++) {
+  /* pass 1: counts per (z,y) row; pass 2: every row writes from its offset,
+   * which reproduces the sequential z-major order */
+  long long *off = malloc(((size_t)N * N + 1) * sizeof(long long));
+#pragma omp parallel for schedule(static)
+  for (long row = 0; row < (long)N * N; row++) {
+    long long c = 0;
+    for (int ix = 0; ix < N; ix++) c += orc_cell_count(p, dens, ix, (int)(row % N), (int)(row / N));
+    off[row + 1] = c;
+  }
+  off[0] = 0;
+  for (long row = 0; row < (long)N * N; row++) off[row + 1] += off[row];
+  const long long total = off[(size_t)N * N];
+  if (total > cap) { free(off); return -1; }
+#pragma omp parallel for schedule(static)
+  for (long row = 0; row < (long)N * N; row++) {
+    const int iz = (int)(row / N), iy = (int)(row % N);
     const double z0 = (iz + 0.0) * dx - p->obs[2];
-    for (int iy = 0; iy < N; iy++) {
-      const double y0 = (iy + 0.0) * dx - p->obs[1];
-      for (int ix = 0; ix < N; ix++) {
-        const double x0 = (ix + 0.0) * dx - p->obs[0];
-        const int npp = orc_cell_count(p, dens, ix, iy, iz);
-        if (npp <= 0) continue;
-        const double rr = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
-        const double rv = factor_vel * rvel(N, p->L, npot, ix, iy, iz, x0, y0, z0, rr);
-        const double dz_rsd = rv * orc_f_of_r(p->bg, p->bg->v1, v1f0, v1ff, rr);
-        const uint64_t cell = (uint64_t)ix + (uint64_t)N * ((uint64_t)iy + (uint64_t)N * iz);
-        uint32_t rnd[4] = {0, 0, 0, 0};
-        for (int ip = 0; ip < npp; ip++) {
-          if (pid >= cap) return -1;
-          const double c0[3] = {x0, y0, z0};
-          float q[3];
-          for (int ax = 0; ax < 3; ax++) {
-            const uint32_t j = 3u * (uint32_t)ip + (uint32_t)ax;
-            if ((j & 3u) == 0) orc_draw(cell, 0x200u + (uint32_t)p->ipop, j >> 2, p->seed, rnd);
-            q[ax] = (float)(c0[ax] + dx * (orc_u01(rnd[j & 3u]) - 0.5));
-          }
-          pos4[4 * pid + 0] = q[0];
-          pos4[4 * pid + 1] = q[1];
-          pos4[4 * pid + 2] = q[2];
-          pos4[4 * pid + 3] = (float)dz_rsd;
-          const double v[3] = {q[0], q[1], q[2]};
-          ipix[pid] = (int)orc_ring2nest(nside_base, orc_vec2pix_ring(nside_base, v));
-          pid++;
+    const double y0 = (iy + 0.0) * dx - p->obs[1];
+    long long pid = off[row];
+    for (int ix = 0; ix < N; ix++) {
+      const double x0 = (ix + 0.0) * dx - p->obs[0];
+      const int npp = orc_cell_count(p, dens, ix, iy, iz);
+      if (npp <= 0) continue;
+      const double rr = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+      const double rv = factor_vel * rvel(N, p->L, npot, ix, iy, iz, x0, y0, z0, rr);
+      const double dz_rsd = rv * orc_f_of_r(p->bg, p->bg->v1, v1f0, v1ff, rr);
+      const uint64_t cell = (uint64_t)ix + (uint64_t)N * ((uint64_t)iy + (uint64_t)N * iz);
+      uint32_t rnd[4] = {0, 0, 0, 0};
+      for (int ip = 0; ip < npp; ip++) {
+        const double c0[3] = {x0, y0, z0};
+        float q[3];
+        for (int ax = 0; ax < 3; ax++) {
+          const uint32_t j = 3u * (uint32_t)ip + (uint32_t)ax;
+          if ((j & 3u) == 0) orc_draw(cell, 0x200u + (uint32_t)p->ipop, j >> 2, p->seed, rnd);
+          q[ax] = (float)(c0[ax] + dx * (orc_u01(rnd[j & 3u]) - 0.5));
         }
+        pos4[4 * pid + 0] = q[0];
+        pos4[4 * pid + 1] = q[1];
+        pos4[4 * pid + 2] = q[2];
+        pos4[4 * pid + 3] = (float)dz_rsd;
+        const double v[3] = {q[0], q[1], q[2]};
+        ipix[pid] = (int)orc_ring2nest(nside_base, orc_vec2pix_ring(nside_base, v));
+        pid++;
       }
     }
   }
-  return pid;
+  free(off);
+  return total;
 }
 
 /* srcs_get_local_properties_single, srcs.c:386-416, cart2sph srcs.c:68-85 */

@@ -145,6 +145,25 @@ def test_sources_bit_exact(workdir, N, ipop):
     np.testing.assert_allclose(props, oprops, rtol=1e-6, atol=1e-5)
 
 
+def test_sources_all_populations_one_pass(workdir):
+    """clr_sample_sources(ipop=-1) gives the catalogues of the per-population
+    calls (same streams, same order)."""
+    N = 128
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx = _ctx_for(run, N)
+    s2 = ctx.gaussian_fields(3003, run.scalar("r_smooth"), True)
+    ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=3)
+    per = []
+    for i in range(3):
+        n = ctx.sample_sources(i, 3003, 2)
+        per.append(ctx.fetch_sources(i, n))
+    tot = ctx.sample_sources(-1, 3003, 2)
+    assert tot == sum(len(p[0]) for p in per)
+    for i in range(3):
+        pos, ipix = ctx.fetch_sources(i, len(per[i][0]))
+        assert np.array_equal(pos, per[i][0]) and np.array_equal(ipix, per[i][1])
+
+
 @pytest.mark.parametrize("which", ["kappa", "isw"])
 def test_los_maps(workdir, which):
     N = 64

@@ -1,0 +1,286 @@
+"""Host library (param parsing, cosmology tables, writers, HEALPix) and the
+C ABI surface, on the CPU."""
+import ctypes
+import math
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+from scipy import integrate, interpolate
+
+from colore_amd import inputs, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def host():
+    return lib.load_host()
+
+
+def csm(host, om=0.3, ol=0.7, ob=0.05, h=0.7, w=-1.0):
+    c = lib.Csm()
+    err = ctypes.create_string_buffer(256)
+    assert host.clr_csm_set(ctypes.byref(c), om, ol, ob, w, 0.0, h, err, 256) == 0, err.value
+    return c
+
+
+def test_cosmology_known_answers_from_reference_log(host):
+    """example_CoLoRe.ipynb:298-326: Om=0.3, OL=0.7, Ob=0.3, h=0.7, 256^3, z in [1e-4, 0.15]."""
+    c = csm(host, 0.3, 0.7, 0.3, 0.7)
+    assert f"{c.a_equality:.5f}" == "0.75416"
+    assert f"{c.phorizon:.3E}" == "9.908E+03"
+    assert f"{c.growth0:.3f}" == "0.779"
+    rmin = host.clr_csm_radial_comoving_distance(ctypes.byref(c), 1 / 1.0001)
+    rmax = host.clr_csm_radial_comoving_distance(ctypes.byref(c), 1 / 1.15)
+    lbox = 2 * rmax * (1 + 2.0 / 256)
+    assert f"{rmin:.3f}" == "0.300" and f"{rmax:.3f}" == "434.105"
+    assert f"{lbox:.3f}" == "874.993"
+    assert f"{2 * math.pi / lbox:.3E}" == "7.181E-03" and f"{lbox / 256:.3E}" == "3.418E+00"
+
+
+def test_background_against_quadrature(host):
+    c = csm(host)
+    om, ol = 0.3, 0.7
+    E = lambda a: math.sqrt(om / a ** 3 + ol)
+    for z in (0.1, 0.5, 1.0, 2.0):
+        a = 1 / (1 + z)
+        chi = 2997.92458 * integrate.quad(lambda x: 1 / (x * x * E(x)), a, 1, epsabs=0, epsrel=1e-12)[0]
+        assert abs(host.clr_csm_radial_comoving_distance(ctypes.byref(c), a) / chi - 1) < 1e-6
+        assert abs(host.clr_csm_hubble(ctypes.byref(c), a) - E(a) / 2997.92458) < 1e-15
+        # growth: D(a) ~ (5/2) Om E(a) int_0^a da' / (a' E(a'))^3, normalised at a=1
+        g = lambda aa: 2.5 * om * E(aa) * integrate.quad(lambda x: 1 / (x * E(x)) ** 3, 0, aa, epsrel=1e-12)[0]
+        D = host.clr_csm_growth_factor(ctypes.byref(c), a) / host.clr_csm_growth_factor(ctypes.byref(c), 1.0)
+        assert abs(D / (g(a) / g(1.0)) - 1) < 1e-6
+        # f = dlnD/dlna by finite differences of the same D(a)
+        eps = 1e-4
+        f_fd = (math.log(g(a * (1 + eps))) - math.log(g(a * (1 - eps)))) / (math.log(1 + eps) - math.log(1 - eps))
+        assert abs(host.clr_csm_f_growth(ctypes.byref(c), a) / f_fd - 1) < 1e-5
+
+
+def test_natural_cubic_spline(host):
+    rng = np.random.default_rng(0)
+    x = np.sort(rng.uniform(0, 10, 40))
+    y = np.sin(x) + 0.1 * x
+    xq = np.concatenate([np.linspace(x[0], x[-1], 500), [x[0] - 1e-9, x[-1] + 0.1]])
+    out = np.zeros_like(xq)
+    host.clr_cspline_eval(len(x), lib.ptr(x), lib.ptr(y), len(xq), lib.ptr(xq), lib.ptr(out))
+    ref = interpolate.CubicSpline(x, y, bc_type="natural")(xq[:-2])
+    np.testing.assert_allclose(out[:-2], ref, atol=1e-12)
+    assert np.isnan(out[-2]) and np.isnan(out[-1])
+
+
+def test_sigma2_tophat_and_gauss_against_quad(host):
+    lk = np.linspace(-4, 2, 300)
+    k = 10 ** lk
+    pk = inputs.eh_nowiggle_pk(k) * 1e4
+    idlogk = (len(lk) - 1) / 6.0
+    ns = 0.96
+
+    def pk0(l):
+        ik = int((l + 4) * idlogk)
+        if ik < 0:
+            return pk[0] * 10 ** (ns * (l + 4))
+        if ik < len(lk) - 1:
+            return pk[ik] + (l - lk[ik]) * (pk[ik + 1] - pk[ik]) * idlogk
+        return pk[-1] * 10 ** (-3 * (l - 2))
+
+    def w_th(x):   # TopHat window with the series below x = 0.1 (cosmo.c:251-289)
+        if x >= 0.1:
+            return 3 * (math.sin(x) - x * math.cos(x)) / x ** 3
+        return 1. - 0.1 * x * x + 0.003571429 * x ** 4 - 6.61376E-5 * x ** 6 + 7.51563E-7 * x ** 8
+
+    # P is linear between table nodes: integrate node to node
+    edges = np.concatenate([np.linspace(-16, -4, 25)[:-1], lk])
+    for R, wf, w in ((8.0, 0, w_th), (3.0, 1, lambda x: math.exp(-0.5 * x * x))):
+        f = lambda l: math.log(10) / (2 * math.pi ** 2) * pk0(l) * (10 ** l) ** 3 * w(10 ** l * R) ** 2
+        ref = sum(integrate.quad(f, a, b, epsrel=1e-13, epsabs=0, limit=200)[0]
+                  for a, b in zip(edges[:-1], edges[1:]))
+        got = host.clr_sigma2_window(len(lk), lib.ptr(lk), lib.ptr(pk), -4.0, 2.0, idlogk, ns, R, wf)
+        assert abs(got / ref - 1) < 1e-8
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def test_param_file_parse_and_tables(tmp_path):
+    p = inputs.write_config("C1", str(tmp_path), n_grid=64)
+    run = lib.Run(p)
+    assert run.scalar("n_grid") == 64 and run.scalar("seed") == 1001
+    assert run.scalar("n_srcs") == 1 and run.scalar("nside_base") == 2
+    rmax = run.scalar("r_max")
+    assert abs(run.scalar("l_box") - 2 * rmax * (1 + 2 / 64)) < 1e-9
+    ta = run.table_arrays()
+    host = run.host
+    # tables: z(r(z)) = z, D(0) = 1, V1(0) = 1 (velocity growth normalised at z=0)
+    for z in (0.05, 0.3, 0.5):
+        r = host.clr_run_r_of_z(run.h, z)
+        assert abs(host.clr_run_get_bg(run.h, r, 0, 0) - z) < 2e-5
+    assert ta["d1"][0] == 1.0 and abs(ta["v1"][0] - 1) < 1e-12
+    # sigma8 normalisation: sigma8(TopHat 8) of the normalised table is sigma_8
+    s8 = host.clr_sigma2_window(ta["numk"], lib.ptr(ta["logk"]), lib.ptr(ta["pk"]), ta["logkmin"], ta["logkmax"],
+                                ta["idlogk"], ta["ns"], 8.0, 0)
+    assert abs(math.sqrt(s8) - 0.8) < 1e-9
+    # n(z) in deg^-2 round trip: the table holds the comoving density written
+    amp, bz, model = run.tracer(0, 0)
+    assert model == 0 and np.all(bz[np.isfinite(bz)] == 1.0)
+    r = host.clr_run_r_of_z(run.h, 0.3)
+    i = int(r * ta["glob_idr"])
+    nbar = amp[i] + (amp[i + 1] - amp[i]) * (r - ta["r"][i]) * ta["glob_idr"]
+    assert abs(nbar / 1e-3 - 1) < 2e-3
+
+
+def test_param_errors(tmp_path):
+    base = inputs.write_config("C1", str(tmp_path), n_grid=32)
+    text = open(base).read()
+
+    def err_of(t):
+        q = os.path.join(str(tmp_path), "bad.cfg")
+        _write(q, t)
+        with pytest.raises(RuntimeError) as e:
+            lib.Run(q)
+        return str(e.value)
+
+    assert "Couldn't read variable global.seed" in err_of(re.sub(r"\s*seed = \d+;", "", text))
+    assert "HDF5 format not supported" in err_of(text.replace('"ASCII"', '"HDF5"'))
+    assert "Unrecognized format" in err_of(text.replace('"ASCII"', '"XML"'))
+    assert "Unknown bias model" in err_of(text.replace('"exponential"', '"quadratic"'))
+    assert "Density type 1 not supported" in err_of(text.replace("dens_type = 0", "dens_type = 1"))
+    assert "Source plane 1 outside redshift range" in err_of(text + "kappa: { z_out = [0.9]; nside = 8; };\n")
+    assert "Couldn't open file" in err_of(text.replace("pk.txt", "missing.txt"))
+    # n(z) table that does not reach z_max
+    nz = os.path.join(str(tmp_path), "nz1.txt")
+    d = np.loadtxt(nz)
+    np.savetxt(os.path.join(str(tmp_path), "nz_short.txt"), d[d[:, 0] < 0.3])
+    assert "N(z) z-range is too small" in err_of(text.replace("nz1.txt", "nz_short.txt"))
+
+
+def test_libconfig_syntax(tmp_path):
+    """Comments, '=' and ':' separators, optional ';', int/float autoconvert,
+    arrays, lists, string concatenation, hex integers, case-insensitive bools."""
+    base = inputs.write_config("C1", str(tmp_path), n_grid=32)
+    d = str(tmp_path)
+    t = f"""
+    # comment
+    global : {{ prefix_out = "{d}/" "out" ; output_format="ASCII"
+      output_density = FALSE, pk_filename: "{d}/pk.txt"  // trailing comment
+      z_min = 5e-2; z_max = 0.5; seed = 0x3E9; write_pred = false; just_write_pred = false; }};
+    /* block
+       comment */
+    field_par = {{ r_smooth = 2; smooth_potential = True; n_grid = 32.0; dens_type = 0;
+      lpt_buffer_fraction = 0.6; lpt_interp_type = 1; output_lpt = 0 }}
+    cosmo_par: {{ omega_M = 0.3; omega_L = 0.7; omega_B = 0.05; h = 0.7; w = -1; ns = 0.96; sigma_8 = 0.8; }}
+    srcs1: {{ nz_filename = "{d}/nz1.txt"; bias_filename = "{d}/bz1.txt"; include_lensing = false;
+      store_skewers = false; bias_model = 0; extra_list = ( 1, "two", [3.0] ) }}
+    kappa: {{ z_out = [0.3, 0.45]; nside = 16; }}
+    """
+    q = os.path.join(d, "syn.cfg")
+    _write(q, t)
+    run = lib.Run(q)
+    assert run.scalar("seed") == 1001 and run.scalar("n_grid") == 32 and run.scalar("r_smooth") == 2.0
+    assert run.scalar("n_kappa") == 2 and run.scalar("nside_kappa") == 16 and run.scalar("output_density") == 0
+
+
+def _read_fits_table(path):
+    data = open(path, "rb").read()
+    assert len(data) % 2880 == 0
+    hdus, pos = [], 0
+    while pos < len(data):
+        hdr = {}
+        while True:
+            blk = data[pos:pos + 2880]
+            pos += 2880
+            done = False
+            for i in range(36):
+                c = blk[80 * i:80 * (i + 1)].decode()
+                if c.startswith("END"):
+                    done = True
+                    break
+                if c[8:10] == "= ":
+                    v = c[10:].split(" / ")[0].strip()
+                    hdr[c[:8].strip()] = v.strip("'").strip() if v.startswith("'") else v
+            if done:
+                break
+        size = int(hdr.get("NAXIS1", 0)) * int(hdr.get("NAXIS2", 0)) if hdr.get("NAXIS") == "2" else 0
+        hdus.append((hdr, data[pos:pos + size]))
+        pos += (size + 2879) // 2880 * 2880
+    return hdus
+
+
+def test_fits_writers(tmp_path, host):
+    f = str(tmp_path / "m.fits")
+    nside = 4
+    m = np.arange(12 * nside * nside, dtype=np.float32) * 0.5
+    assert host.clr_write_healpix_map(f.encode(), nside, lib.ptr(m, ctypes.c_float)) == 0
+    (h0, _), (h1, tab) = _read_fits_table(f)
+    assert h0["SIMPLE"] == "T" and h1["XTENSION"] == "BINTABLE" and h1["TFORM1"] == "1E"
+    assert h1["PIXTYPE"] == "HEALPIX" and h1["ORDERING"] == "RING" and h1["NSIDE"] == "4"
+    assert h1["COORDSYS"] == "G" and h1["EXTNAME"] == "BINTABLE" and h1["TTYPE1"] == "map 1"
+    np.testing.assert_array_equal(np.frombuffer(tab, dtype=">f4"), m)
+    c = str(tmp_path / "c.fits")
+    rdz = np.random.default_rng(0).random((100, 4)).astype(np.float32)
+    assert host.clr_write_catalog_fits(c.encode(), 2, 100, lib.ptr(rdz, ctypes.c_float)) == 0
+    (_, _), (h1, tab) = _read_fits_table(c)
+    assert [h1[f"TTYPE{i}"] for i in range(1, 6)] == ["TYPE", "RA", "DEC", "Z_COSMO", "DZ_RSD"]
+    assert [h1[f"TFORM{i}"] for i in range(1, 6)] == ["1J", "1E", "1E", "1E", "1E"]
+    assert h1["TUNIT2"] == "DEG" and h1["CONTENTS"] == "Source catalog"
+    rec = np.frombuffer(tab, dtype=[("t", ">i4"), ("ra", ">f4"), ("dec", ">f4"), ("z", ">f4"), ("dz", ">f4")])
+    assert np.all(rec["t"] == 2)
+    np.testing.assert_array_equal(np.stack([rec["ra"], rec["dec"], rec["z"], rec["dz"]], 1), rdz)
+
+
+def test_density_grid_writer(tmp_path, host):
+    N, L = 8, 123.5
+    g = np.random.default_rng(1).random((N, N, N + 2)).astype(np.float32)
+    f = str(tmp_path / "d.dat")
+    assert host.clr_write_density_grid(f.encode(), N, L, lib.ptr(g, ctypes.c_float)) == 0
+    raw = open(f, "rb").read()
+    nn, sz = struct.unpack("ii", raw[:8])
+    (lb,) = struct.unpack("d", raw[8:16])
+    n, nz, iz0 = struct.unpack("iii", raw[16:28])
+    assert (nn, sz, n, nz, iz0) == (1, 4, N, N, 0) and lb == L
+    np.testing.assert_array_equal(np.frombuffer(raw[28:], dtype=np.float32).reshape(N, N, N), g[:, :, :N])
+
+
+def test_host_healpix_matches_oracle(host):
+    from oracle_lib import load
+    o = load()
+    v, w = np.zeros(3), np.zeros(3)
+    for nside in (1, 2, 8, 32):
+        for p in range(0, 12 * nside * nside, max(1, nside)):
+            assert host.clr_hp_nest2ring(nside, p) == o.orc_nest2ring(nside, p)
+            assert host.clr_hp_ring2nest(nside, p) == o.orc_ring2nest(nside, p)
+            host.clr_hp_pix2vec_nest(nside, p, lib.ptr(v))
+            o.orc_pix2vec_nest(nside, p, lib.ptr(w))
+            assert np.array_equal(v, w)
+            assert host.clr_hp_vec2pix_ring(nside, lib.ptr(v)) == host.clr_hp_nest2ring(nside, p)
+
+
+def test_c_abi_library_exports_every_declared_symbol():
+    import ctypes as ct
+    for hdr, so in (("colore_amd.h", lib.AMD_SO), ("colore_host.h", lib.HOST_SO)):
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names = set(re.findall(r"\b(clr_[a-z0-9_]+)\s*\(", text))
+        names -= {"clr_ctx"}
+        assert len(names) > 10
+        so_lib = ct.CDLL(so)
+        missing = [n for n in sorted(names) if not hasattr(so_lib, n)]
+        assert not missing, (hdr, missing)
+
+
+def test_cli_usage_and_test_memory(tmp_path):
+    exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert "Usage: ./CoLoRe file_name" in r.stderr
+    p = inputs.write_config("C1", str(tmp_path), n_grid=64)
+    r = subprocess.run([exe, "--test-memory", p], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "L_box = 2726.703 Mpc/h, N_grid = 64" in r.stdout   # 2 r_max (1 + 2/64), r_max = chi(0.5)
+    r = subprocess.run([exe, str(tmp_path / "missing.cfg")], capture_output=True, text=True)
+    assert r.returncode == 1 and "Fatal: Couldn't open file" in r.stderr

@@ -1,0 +1,412 @@
+"""Pins of the CPU oracle: published known-answer vectors, library routines
+available here (numpy FFT / MT19937, scipy), closed forms and brute force.
+Each test is chosen so that a plausible slip (dropped term, wrong sign or
+index, transposed operand) fails it."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle_lib import Oracle, load, p
+
+
+@pytest.fixture(scope="module")
+def L():
+    return load()
+
+
+# ---------------------------------------------------------------- RNG ----
+def philox(L, ctr, key):
+    c = np.array(ctr, dtype=np.uint32)
+    k = np.array(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    L.orc_philox4x32_10(p(c, ctypes.c_uint32), p(k, ctypes.c_uint32), p(o, ctypes.c_uint32))
+    return [int(x) for x in o]
+
+
+def test_philox_known_answers(L):
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    assert philox(L, [0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert philox(L, [0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert philox(L, [0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def mt_stream(seed, n):
+    g = np.random.MT19937(0)
+    g._legacy_seeding(seed)
+    return g.random_raw(n).astype(np.uint32)
+
+
+def test_reference_mode_stream_matches_mt19937(L):
+    """Per-thread MT19937 streams (init_genrand seeding, GSL's mt19937_set)
+    consumed (phase, modulus) per mode over each thread's static block of kz
+    planes; the k=0 mode draws nothing."""
+    N, seed, nth = 8, 1001, 3
+    nh = N // 2 + 1
+    out = np.zeros(2 * N * N * nh, dtype=np.uint32)
+    L.orc_ref_mode_stream(N, seed, nth, p(out, ctypes.c_uint32))
+    out = out.reshape(N, N, nh, 2)
+    q, rem = divmod(N, nth)
+    start = 0
+    for t in range(nth):
+        cnt = q + (1 if t < rem else 0)
+        blk = out[start:start + cnt].reshape(-1, 2)
+        if start == 0:
+            assert blk[0, 0] == 0 and blk[0, 1] == 0
+            blk = blk[1:]
+        ref = mt_stream(seed + t, 2 * len(blk)).reshape(-1, 2)
+        assert np.array_equal(blk, ref)
+        start += cnt
+
+
+def test_gsl_seed_zero_maps_to_4357(L):
+    out = np.zeros(2 * 2 * 2 * 2, dtype=np.uint32)
+    L.orc_ref_mode_stream(2, 0, 1, p(out, ctypes.c_uint32))
+    ref = mt_stream(4357, 14)
+    assert np.array_equal(out.reshape(-1, 2)[1:].ravel(), ref)
+
+
+# ---------------------------------------------------------------- FFT ----
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 48, 64, 96, 128, 384, 7, 11, 1024])
+def test_cfft_matches_numpy(L, n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    d = np.empty(2 * n)
+    d[0::2], d[1::2] = x.real, x.imag
+    L.orc_cfft_inv(n, p(d))
+    ref = np.fft.ifft(x) * n
+    np.testing.assert_allclose(d[0::2] + 1j * d[1::2], ref, atol=1e-12 * n)
+
+
+def _half(N, seed):
+    rng = np.random.default_rng(seed)
+    nh = N // 2 + 1
+    h = rng.normal(size=(N, N, nh)) + 1j * rng.normal(size=(N, N, nh))
+    a = np.empty((N, N, nh, 2))
+    a[..., 0], a[..., 1] = h.real, h.imag
+    return h, a.ravel().copy()
+
+
+@pytest.mark.parametrize("N", [4, 6, 8, 12])
+def test_c2r_matches_brute_force(L, N):
+    """Non-Hermitian kx=0 and kx=N/2 planes included (as the reference draws)."""
+    h, a = _half(N, N)
+    out = np.zeros(N * N * 2 * (N // 2 + 1))
+    ref = np.zeros_like(out)
+    L.orc_c2r_3d(N, p(a), p(out))
+    L.orc_c2r_3d_brute(N, p(a), p(ref))
+    np.testing.assert_allclose(out, ref, atol=1e-10 * N ** 3)
+
+
+@pytest.mark.parametrize("N", [16, 24, 32, 48])
+def test_c2r_matches_numpy_irfftn(L, N):
+    h, a = _half(N, 100 + N)
+    out = np.zeros(N * N * 2 * (N // 2 + 1))
+    L.orc_c2r_3d(N, p(a), p(out))
+    ref = np.fft.irfftn(h, s=(N, N, N), axes=(0, 1, 2)) * N ** 3
+    got = out.reshape(N, N, -1)[:, :, :N]
+    np.testing.assert_allclose(got, ref, atol=1e-9 * np.abs(ref).max())
+    # the imaginary parts of the kx=0 and kx=N/2 terms after the (z,y) transforms are ignored
+    h2 = h.copy()
+    h2[:, :, 0] += 1j * 3.0 * (np.arange(N)[:, None] == 0) * (np.arange(N)[None, :] == 0)
+    a2 = np.stack([h2.real, h2.imag], -1).ravel().copy()
+    out2 = np.zeros_like(out)
+    L.orc_c2r_3d(N, p(a2), p(out2))
+    np.testing.assert_allclose(out2, out, atol=1e-9 * np.abs(ref).max())
+
+
+# -------------------------------------------------------- modes / fields ----
+def _tables(nk=200):
+    """Small synthetic tables: r grid, z = r/3000, D = 1/(1+z), P(k) power law."""
+    na = 5001
+    r = np.arange(na) * (10000.0 / (na - 1))
+    z = r / 3000.0
+    ta = dict(r=r, z=z, d1=1 / (1 + z), v1=np.ones(na), pd=np.full(na, 0.5), ih=np.full(na, 3000.0),
+              glob_idr=(na - 1) / 10000.0, ns=0.96, fgrowth_0=0.5, hubble_0=1 / 2997.92458, omega_m=0.3,
+              prefac_lensing=1.5 * 0.3 / 2997.92458 ** 2, r_max=800.0)
+    lk = np.linspace(-4, 2, nk)
+    ta.update(logk=lk, pk=1e4 * 10 ** (-1.5 * lk), logkmin=-4.0, logkmax=2.0, idlogk=(nk - 1) / 6.0, numk=nk)
+    return ta
+
+
+def test_modes_closed_form_and_injection():
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb, rs = 8, 500.0, 5.0
+    nh = N // 2 + 1
+    rng = np.random.default_rng(3)
+    inj = rng.integers(0, 2 ** 32, size=2 * N * N * nh, dtype=np.uint64).astype(np.uint32)
+    d, ph = orc.modes(N, Lb, 1, rs, True, injected=inj)
+    d = d.reshape(N, N, nh, 2)
+    ph = ph.reshape(N, N, nh, 2)
+    dk = 2 * math.pi / Lb
+    assert np.all(d[0, 0, 0] == 0) and np.all(ph[0, 0, 0] == 0)
+    for (i, j, k) in [(1, 0, 0), (0, 3, 2), (5, 7, 4), (4, 4, 4), (7, 1, 3)]:
+        kz = (i if 2 * i <= N else i - N) * dk
+        ky = (j if 2 * j <= N else j - N) * dk
+        kx = k * dk
+        k2 = kx * kx + ky * ky + kz * kz
+        lgk = 0.5 * math.log10(k2)
+        ik = int((lgk + 4) * ta["idlogk"])
+        pk = ta["pk"][ik] + (lgk - ta["logk"][ik]) * (ta["pk"][ik + 1] - ta["pk"][ik]) * ta["idlogk"]
+        s2 = pk / dk ** 3
+        m = k + nh * (j + N * i)
+        u1, u2 = inj[2 * m] / 2 ** 32, inj[2 * m + 1] / 2 ** 32
+        amp = math.sqrt(-s2 * math.log(1 - u2))
+        sm = math.exp(-0.5 * rs * rs * k2)
+        want = amp * np.exp(2j * math.pi * u1) * sm
+        got = d[i, j, k, 0] + 1j * d[i, j, k, 1]
+        assert abs(got - want) <= 1e-12 * abs(want)
+        wphi = -ta["prefac_lensing"] * amp * np.exp(2j * math.pi * u1) / k2 * sm
+        gphi = ph[i, j, k, 0] + 1j * ph[i, j, k, 1]
+        assert abs(gphi - wphi) <= 1e-12 * abs(wphi)
+
+
+def test_mode_power_statistics():
+    """<|delta_k|^2> = P(k)/dk^3 and uniform phases with the Philox stream."""
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb = 32, 800.0
+    d, _ = orc.modes(N, Lb, 42, 0.0, False)
+    d = d.reshape(N, N, N // 2 + 1, 2)
+    c = d[..., 0] + 1j * d[..., 1]
+    dk = 2 * math.pi / Lb
+    f = np.fft.fftfreq(N, 1.0 / N)
+    kx = np.arange(N // 2 + 1)
+    kk = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + kx[None, None, :] ** 2) * dk
+    kk[0, 0, 0] = 1
+    pk = np.interp(np.log10(kk), ta["logk"], ta["pk"])
+    ratio = (np.abs(c) ** 2 / (pk / dk ** 3))[kk > 2 * dk]
+    assert abs(ratio.mean() - 1) < 5 / math.sqrt(ratio.size)
+    ph = np.angle(c[kk > 2 * dk])
+    assert stats.kstest((ph / (2 * math.pi)) % 1.0, "uniform").pvalue > 1e-3
+
+
+def test_normalize_and_sigma2(L):
+    N, Lb = 8, 100.0
+    rng = np.random.default_rng(0)
+    g = rng.normal(size=N * N * 2 * (N // 2 + 1))
+    g2 = g.copy()
+    out = np.zeros(2)
+    s2 = L.orc_normalize_sigma2(N, Lb, p(g2), None, p(out))
+    norm = (math.sqrt(2 * math.pi) / Lb) ** 3
+    cells = (g * norm).reshape(N, N, -1)[:, :, :N]
+    assert abs(s2 - cells.var()) < 1e-12 * cells.var()
+    np.testing.assert_allclose(g2, g * norm, rtol=1e-15)
+
+
+def test_lognormal_and_clip_closed_form():
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb = 8, 300.0
+    rng = np.random.default_rng(1)
+    g = rng.normal(size=(N, N, 2 * (N // 2 + 1))) * 0.5
+    s2 = 0.3
+    ln = orc.density(N, Lb, g, s2, 0)
+    cl = orc.density(N, Lb, g, s2, 3)
+    dx = Lb / N
+    for (iz, iy, ix) in [(0, 0, 0), (4, 4, 4), (7, 2, 5), (3, 6, 1)]:
+        r = math.sqrt(((ix * dx - Lb / 2) ** 2 + (iy * dx - Lb / 2) ** 2 + (iz * dx - Lb / 2) ** 2))
+        D = np.interp(r, ta["r"], ta["d1"]) if r > 0 else 1.0
+        dg = g[iz, iy, ix]
+        assert abs(ln[iz, iy, ix] - (math.exp(D * (dg - 0.5 * D * s2)) - 1)) < 1e-12
+        assert abs(cl[iz, iy, ix] - (max(1 + D * dg, 0) - 1)) < 1e-12
+
+
+def test_bias_models_closed_form(L):
+    bm = L.orc_bias_model
+    for d in (-1.0, -1.5):
+        for m in range(3):
+            assert bm(d, 1.3, m) == 0.0
+    assert abs(bm(0.5, 2.0, 0) - 2.25) < 1e-15
+    assert abs(bm(-0.5, 2.0, 1) - math.exp(2.0 * -0.5 / 0.5)) < 1e-15
+    assert abs(bm(0.5, 2.0, 1) - 2.0) < 1e-15
+    assert bm(-0.6, 2.0, 2) == 0.0 and abs(bm(-0.4, 2.0, 2) - 0.2) < 1e-15
+    # lognormal moment: <(1+delta)^b> = exp(b(b-1) s^2/2) for delta = exp(G - s^2/2) - 1
+    rng = np.random.default_rng(5)
+    s = 0.4
+    G = rng.normal(0, s, 400000)
+    dd = np.exp(G - s * s / 2) - 1
+    for b in (0.5, 1.0, 2.0):
+        m = np.mean([bm(x, b, 0) for x in dd[:20000]])
+        assert abs(m / math.exp(b * (b - 1) * s * s / 2) - 1) < 0.02
+
+
+def test_shell_sums_invariants():
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb = 16, 1200.0
+    zero = np.zeros((N, N, 2 * (N // 2 + 1)))
+    b = np.full(5001, 1.7)
+    nz, zmax, cnt, zs, sums = orc.shell_sums(N, Lb, zero, [(b, 0), (b, 1), (b, 2)])
+    assert nz == int(zmax / 0.05) + 2
+    # delta = 0: every bias model gives 1 -> the sums equal the counts, norm = 1
+    for t in range(3):
+        np.testing.assert_array_equal(sums[t], cnt.astype(float))
+    # count: all cells whose shell index is in range
+    dx = Lb / N
+    ii = np.arange(N) * dx - Lb / 2
+    r = np.sqrt(ii[:, None, None] ** 2 + ii[None, :, None] ** 2 + ii[None, None, :] ** 2)
+    z = np.interp(r, ta["r"], ta["z"])
+    ind = (z * ((nz - 2) / zmax)).astype(int) + 1
+    want = np.bincount(ind[ind < nz], minlength=nz)
+    np.testing.assert_array_equal(cnt, want)
+    idz = (nz - 2) / zmax
+    for s in range(1, nz):
+        if cnt[s]:
+            zm = zs[s] / cnt[s]
+            assert (s - 1) / idz - 1e-12 <= zm < s / idz + 1e-12
+    tab, n0, nf = orc.norm_table(nz, zmax, cnt, zs, sums[0])
+    np.testing.assert_allclose(tab, 1.0, rtol=1e-15)
+    # b = 0 makes the power-law bias 1 for any delta
+    rng = np.random.default_rng(2)
+    g = np.clip(rng.normal(size=zero.shape) * 0.3, -0.95, None)
+    b0 = np.zeros(5001)
+    _, _, cnt2, _, sums2 = orc.shell_sums(N, Lb, g, [(b0, 0)])
+    np.testing.assert_array_equal(sums2[0], cnt2.astype(float))
+    # cells with delta <= -1 contribute nothing (bias_model's first branch)
+    g[:, :, :N] = -1.0
+    _, _, cnt3, _, sums3 = orc.shell_sums(N, Lb, g, [(b0, 0), (b, 1), (b, 2)])
+    assert cnt3.sum() > 0 and np.all(sums3 == 0)
+
+
+def test_norm_table_interpolates_shell_means():
+    ta = _tables()
+    orc = Oracle(ta)
+    nz, zmax = 6, 0.2
+    cnt = np.array([0, 10, 20, 30, 40, 50], dtype=np.uint64)
+    zs = np.array([0, 0.2, 1.5, 3.75, 6.8, 9.5])
+    sums = np.array([0, 8.0, 25.0, 27.0, 44.0, 40.0])
+    tab, n0, nf = orc.norm_table(nz, zmax, cnt, zs, sums)
+    zarr = zs / np.maximum(cnt, 1)
+    nrm = np.where(cnt > 0, cnt / np.where(sums > 0, sums, 1), 0)
+    zarr[0], zarr[-1] = 0, zmax
+    nrm[0], nrm[-1] = nrm[1], nrm[-2]
+    z = np.interp(ta["r"], ta["r"], ta["z"])
+    want = np.interp(z, zarr, nrm)
+    want[z >= zmax] = nrm[-1]
+    assert n0 == nrm[0] and nf == nrm[-1]
+    np.testing.assert_allclose(tab, want, rtol=1e-12)
+
+
+# ------------------------------------------------------------- Poisson ----
+@pytest.mark.parametrize("lam", [0.05, 1.0, 9.2, 37.0, 420.0, 1300.0])
+def test_poisson_distribution(L, lam):
+    n = 60000
+    k = np.array([L.orc_poisson(lam, c, 0x100, 77) for c in range(n)])
+    assert abs(k.mean() - lam) < 5 * math.sqrt(lam / n)
+    assert abs(k.var() / lam - 1) < 6 * math.sqrt(2.0 / n) + 0.02
+    if lam < 50:
+        vals = np.arange(0, int(lam + 8 * math.sqrt(lam) + 5))
+        obs = np.array([(k == v).sum() for v in vals])
+        exp = stats.poisson.pmf(vals, lam) * n
+        m = exp > 5
+        chi2 = ((obs[m] - exp[m]) ** 2 / exp[m]).sum()
+        assert stats.chi2.sf(chi2, m.sum() - 1) > 1e-4
+    assert L.orc_poisson(0.0, 5, 0x100, 1) == 0 and L.orc_poisson(float("nan"), 5, 0x100, 1) == 0
+
+
+def test_source_positions_inside_cells():
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb = 16, 400.0
+    g = np.zeros((N, N, 2 * (N // 2 + 1)))
+    amp = np.full(5001, 2e-3)
+    pop = orc.pop(N, Lb, amp, np.ones(5001), np.ones(5001), 1.0, 1.0, 0, 0, 9)
+    pos, ipix = orc.sources(pop, g, g, 2)
+    dx = Lb / N
+    assert len(pos) > 1000
+    cell = np.floor((pos[:, :3] + Lb / 2) / dx + 0.5)
+    off = pos[:, :3] + Lb / 2 - cell * dx
+    assert np.all(off >= -dx / 2 - 1e-4) and np.all(off < dx / 2 + 1e-4)
+    L_ = load()
+    for i in range(0, len(pos), 97):
+        v = pos[i, :3].astype(np.float64)
+        assert ipix[i] == L_.orc_ring2nest(2, L_.orc_vec2pix_ring(2, p(v)))
+    # z-major order
+    lin = cell[:, 0] + N * (cell[:, 1] + N * cell[:, 2])
+    assert np.all(np.diff(lin) >= 0)
+
+
+# ------------------------------------------------------------- HEALPix ----
+def test_healpix_nside1_centres(L):
+    v = np.zeros(3)
+    for f in range(12):
+        L.orc_pix2vec_nest(1, f, p(v))
+        z = 2 / 3 if f < 4 else (0.0 if f < 8 else -2 / 3)
+        phi = (math.pi / 4 + (f % 4) * math.pi / 2) if f < 4 or f >= 8 else (f - 4) * math.pi / 2
+        assert abs(v[2] - z) < 1e-14
+        assert abs(math.atan2(v[1], v[0]) % (2 * math.pi) - phi % (2 * math.pi)) < 1e-12
+
+
+@pytest.mark.parametrize("nside", [1, 2, 4, 16, 64])
+def test_healpix_bijections_and_roundtrip(L, nside):
+    npix = 12 * nside * nside
+    ring = np.array([L.orc_nest2ring(nside, i) for i in range(npix)])
+    assert sorted(ring) == list(range(npix))
+    assert all(L.orc_ring2nest(nside, int(r)) == i for i, r in enumerate(ring))
+    v = np.zeros(3)
+    for i in range(0, npix, max(1, npix // 500)):
+        L.orc_pix2vec_nest(nside, i, p(v))
+        assert L.orc_ring2nest(nside, L.orc_vec2pix_ring(nside, p(v))) == i
+
+
+def test_healpix_equal_area(L):
+    rng = np.random.default_rng(4)
+    nside, n = 4, 192000
+    z = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * math.pi, n)
+    pix = np.array([L.orc_ang2pix_ring(nside, a, b) for a, b in zip(z, ph)])
+    c = np.bincount(pix, minlength=12 * nside * nside)
+    e = n / (12 * nside * nside)
+    assert stats.chisquare(c).pvalue > 1e-4 and abs(c.mean() - e) < 1e-9
+
+
+# --------------------------------------------------- lightcone painters ----
+def test_kappa_closed_form():
+    """phi = A/2 |x|^2 and D(1+z) = 1: the transverse trace of the tidal
+    tensor is 2A, so kappa = 2A sum (rm - rm^2/r_s) dr over the march."""
+    ta = _tables()
+    ta["z"] = np.zeros(5001)
+    ta["d1"] = np.ones(5001)
+    orc = Oracle(ta)
+    N, Lb = 32, 1000.0
+    ta["r_max"] = Lb / (2 * (1 + 2.0 / N))
+    orc = Oracle(ta)
+    dx = Lb / N
+    i = (np.arange(N) - N // 2) * dx
+    A = 3e-7
+    phi = np.zeros((N, N, 2 * (N // 2 + 1)))
+    phi[:, :, :N] = 0.5 * A * (i[:, None, None] ** 2 + i[None, :, None] ** 2 + i[None, None, :] ** 2)
+    r_s = 0.7 * ta["r_max"]
+    out = orc.kappa_isw(0, N, Lb, phi, 2, [r_s])
+    nr = N // 2
+    dr = ta["r_max"] / nr
+    ih = int(r_s / dr + 0.5)
+    rm = (np.arange(min(ih, nr - 1) + 1) + 0.5) * dr
+    want = 2 * A * np.sum((rm - rm * rm / (ih * dr)) * dr)
+    np.testing.assert_allclose(out[0], want, rtol=1e-9)
+    # ISW of a constant potential: 2 phi sum PD dr
+    phic = np.full_like(phi, 0.25)
+    iw = orc.kappa_isw(1, N, Lb, phic, 2, [r_s])
+    want = 0.25 * 2 * 0.5 * dr * (min(ih, nr - 1) + 1)
+    np.testing.assert_allclose(iw[0], want, rtol=1e-12)
+
+
+def test_imap_constant_temperature():
+    """T = const, b = 0, norm = 1, no RSD: every painted pixel averages to T."""
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb = 16, 600.0
+    g = np.random.default_rng(1).normal(size=(N, N, 2 * (N // 2 + 1))) * 0.2
+    zero = np.zeros_like(g)
+    T = 0.37
+    r0 = np.array([100.0, 160.0, 230.0])
+    rf = np.array([160.0, 230.0, 280.0])
+    data, nadd = orc.imap(N, Lb, g, zero, np.full(5001, T), np.zeros(5001), np.ones(5001), 1.0, 1.0, 0, 4, r0, rf)
+    hit = nadd > 0
+    assert hit.sum() > 100
+    np.testing.assert_allclose(data[hit] / nadd[hit], T, rtol=1e-12)
