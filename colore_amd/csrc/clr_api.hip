@@ -34,18 +34,6 @@ int clr_ensure(clr_ctx *c, void **p, size_t *have, size_t bytes) {
   return CLR_OK;
 }
 
-__global__ void k_d2f(const double *__restrict__ in, float *__restrict__ out, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (float)in[i];
-}
-
-int clr_norm_f32(clr_ctx *c, TracerDev &T) {
-  if (!T.normf32) CLR_HIP(c, hipMalloc((void **)&T.normf32, (size_t)c->na * sizeof(float)));
-  hipLaunchKernelGGL(k_d2f, dim3((c->na + 255) / 256), dim3(256), 0, c->stream, T.norm, T.normf32, c->na);
-  CLR_HIP(c, hipGetLastError());
-  return CLR_OK;
-}
-
 int clr_ring2nest_table(clr_ctx *c, int nside) {
   if (c->r2n && c->r2n_nside == nside) return CLR_OK;
   const int64_t npix = 12LL * nside * nside;
@@ -64,7 +52,7 @@ static void free_all(clr_ctx *c) {
   f(c->rowcnt); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d1f); f(c->r2n);
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < CLR_NPOP_MAX; i++) {
-      f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].normf32);
+      f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].ptab);
     }
   for (int i = 0; i < CLR_NPOP_MAX; i++) { f(c->cat[i].pos); f(c->cat[i].ipix); }
   if (c->ev0) hipEventDestroy(c->ev0);

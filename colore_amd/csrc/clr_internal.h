@@ -14,8 +14,8 @@ struct TracerDev {
   double *amp = nullptr;    // n(r) / T(r) / K(r), NA
   double *bz = nullptr;     // b(r), NA
   double *norm = nullptr;   // norm(r), NA
-  float *bzf = nullptr;     // FP32 copies (values feeding FP32 estimates only)
-  float *normf32 = nullptr;
+  float *bzf = nullptr;     // FP32 copy of bz (feeds FP32 estimates only)
+  void *ptab = nullptr;     // float4 per bin: (n dx^3, b, norm, flags) for sources
   double bz0 = 0;           // f0 of the bias table (its first entry)
   double norm0 = 1, normf = 1;
   int norm_set = 0;
@@ -101,7 +101,6 @@ int clr_fail(clr_ctx *c, int code, const char *fmt, ...);
 
 int clr_ensure(clr_ctx *c, void **p, size_t *have, size_t bytes);
 int clr_ring2nest_table(clr_ctx *c, int nside);
-int clr_norm_f32(clr_ctx *c, TracerDev &T);
 
 // stage launchers (one per file)
 int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_potential,

@@ -1,21 +1,29 @@
 // clr_sources.hip -- srcs_set_cartesian_single (srcs.c:120-283) for one or
 // all populations, and srcs_get_local_properties_single (srcs.c:386-416).
 //
-// A segment is up to 1024 consecutive cells of one (z,y) row; each lane owns
-// 4 consecutive cells (two 8-byte loads, one Philox draw per population for
-// the 4 Poisson uniforms).  Population geometry (radius, table index) is
-// computed once per cell for every population of the pass.
-//   k_count : per cell and population the Poisson count (clr_poisson.h);
-//             per segment and population the total -> segcnt.
-//   k_scan_*: exclusive prefix of the segment totals (64-bit), per population.
-//   k_fill  : recomputes the counts (counter-based draws make this exact),
-//             scans the segment in x order and writes the sources of every
-//             occupied cell: uniform position in [x-dx/2, x+dx/2)
-//             (srcs.c:261-263), RSD shift from central differences of phi
-//             (get_rvel, srcs.c:87-118), NEST base pixel (vec2pix_ring then a
-//             RING->NEST table, srcs.c:264-270).
-// Each catalogue is in global z-major cell order, sources of a cell in draw
-// order -- the order the reference produces on one thread.
+// k_sources: one pass over the density grid.  A block takes 1024
+// consecutive cells (z-major flat order; a lane owns 8 cells of one row) in
+// the order of a ticket it draws when it starts, and for every population
+//   A. decides most cells from 16 uniform bits and an FP32 lambda whose error
+//      is bounded (clr_poisson.h): k = 0 at once; the few others are queued
+//      in LDS (cell, lambda, bound);
+//   B. lets all lanes work the queue: the full 32-bit uniform, the FP32
+//      search, and where that is too close to call the reference's FP64
+//      expressions (count_exact) -- every count equals the FP64 one;
+//   C. scans the counts of its cells;
+//   D. gets the number of sources of all earlier blocks by decoupled
+//      look-back over per-block status words (a block waits only on blocks
+//      with a smaller ticket, which are already running and publish their
+//      own count before they wait, so every wait ends); and
+//   E. writes one descriptor (cell, index within the cell) per source at
+//      its final place in the catalogue: global z-major cell order, draw
+//      order within a cell -- the order of the reference on one thread.
+// k_write: one thread per source turns its descriptor into the position
+//   in [x-dx/2, x+dx/2) (srcs.c:261-263), the RSD shift from central
+//   differences of phi (get_rvel, srcs.c:87-118) and the NEST base pixel
+//   (vec2pix_ring then a RING->NEST table, srcs.c:264-270).
+// A pass writes nothing past the end of a catalogue; when one is too small
+// for its count the host grows it and runs the pass again.
 #include "clr_internal.h"
 #include "clr_poisson.h"
 #include <math.h>
@@ -24,202 +32,331 @@ using namespace clr;
 
 namespace {
 
+constexpr int kFlagPos = 1, kFlagNonPos = 2, kFlagNan = 4;
+constexpr int kLanes = 128, kCellsPerLane = 8, kBlockCells = kLanes * kCellsPerLane;
+constexpr int kSpinLimit = 1 << 24;   // look-back polls before giving up (~1 s)
+
 struct PopTab {
   const double *amp, *bz, *norm;
-  const float *bzf, *normf32;
+  const float4 *tab;   // per bin: (n dx^3, b, norm, flags)
   double bz0, norm0, normf;
   int model, ipop;
-  float4 *pos;
-  int32_t *ipix;
+  uint4 *desc;         // the catalogue's position buffer, holding descriptors
+  long long cap;
+  unsigned long long *status;   // one word per block: count << 2 | state
 };
 
 struct SrcArgs {
-  const float *g, *phi;
-  const double *r_arr, *v1;
-  double idr, r_last, v1_0, v1_last;
-  double dx, cell_vol, half_box_20, factor_vel, idx, ox, oy, oz;
-  float idr_f;
-  int N, ngx, na, nseg, seglen, npop, nside_base;
+  const float *g;
+  const double *r_arr;
+  double idr, r_last;
+  double dx, cell_vol, half_box_20, ox, oy, oz;
+  int N, ngx, na, npop;
+  long long nblocks;
   uint32_t seed;
-  const int32_t *r2n;
+  unsigned int *ticket;
+  unsigned long long *totals;
+  int *bad;
   PopTab pop[CLR_NPOP_MAX];
 };
 
-// FP32 table lookup for values that only feed the FP32 lambda estimate
-__device__ inline float f_of_r32(const float *__restrict__ f, float idr, int nm1, float r, float f0, float ff) {
-  if (r <= 0.f) return f0;
-  const float x = r * idr;
-  if (x >= (float)nm1) return ff;
-  const int ir = (int)x;
-  const float t = x - (float)ir;
-  return f[ir] + (f[ir + 1] - f[ir]) * t;
+// per bin i: n(r_i) dx^3, b(r_i), norm(r_i) in FP32 and flags for the
+// interval [r_i, r_i+1] of the FP64 tables:
+//   kFlagNan    n is NaN at an end: the interpolated n is NaN, no sources;
+//   kFlagNonPos n <= 0 at both ends: the FP64 interpolation is <= 0 (it is
+//               monotone in each end and both ends bound it), no sources;
+//   kFlagPos    n, norm > 0 and finite at both ends with ratios <= 2 and b
+//               finite: the FP32 interpolation is within 1e-6 relative.
+__global__ void k_pop_table(const double *__restrict__ amp, const double *__restrict__ bz,
+                            const double *__restrict__ norm, double cv, int na, float4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  const int j = i + 1 < na ? i + 1 : i;
+  const double f0 = amp[i], f1 = amp[j], n0 = norm[i], n1 = norm[j], b0 = bz[i], b1 = bz[j];
+  // both ends inside FP32's normal range, within a factor 2 of each other
+  auto close_pos = [](double u, double v) {
+    return u > 1e-30 && u < 1e30 && v > 1e-30 && v < 1e30 && u <= 2 * v && v <= 2 * u;
+  };
+  int fl = 0;
+  if (isnan(f0) || isnan(f1)) fl = kFlagNan;
+  else if (f0 <= 0 && f1 <= 0) fl = kFlagNonPos;
+  else if (close_pos(f0 * cv, f1 * cv) && close_pos(n0, n1) && fabs(b0) < 1e6 && fabs(b1) < 1e6) fl = kFlagPos;
+  out[i] = make_float4((float)(f0 * cv), (float)b0, (float)n0, __int_as_float(fl));
 }
 
-// Count of cell `cell` for population P given its exact radius r.
-__device__ inline int cell_count(const SrcArgs &a, const PopTab &P, double r, float dl, uint64_t cell,
-                                 uint32_t word0) {
+// The reference's FP64 count of cell (ix,iy,iz) (srcs.c:136-187) from the
+// full 32-bit uniform x.
+__device__ __noinline__ int count_exact(int ix, int iy, int iz, float dl, uint64_t cell, uint32_t x,
+                                        const double *r_arr, const double *amp, const double *bz,
+                                        const double *norm, double idr, double r_last, double bz0, double norm0,
+                                        double normf, double dx, double ox, double oy, double oz, double cell_vol,
+                                        double half_box_20, int model, int ipop, uint32_t seed) {
 #pragma clang fp contract(off)
-  if (!(r < a.half_box_20)) return 0;
-  const double nd = f_of_r(a.r_arr, P.amp, a.idr, a.r_last, r, 0.0, 0.0);
+  const double obs[3] = {ox, oy, oz};
+  double x0, y0, z0;
+  const double r = cell_radius(ix, iy, iz, dx, obs, &x0, &y0, &z0);
+  if (!(r < half_box_20)) return 0;
+  const double nd = f_of_r(r_arr, amp, idr, r_last, r, 0.0, 0.0);
   if (!(nd > 0)) return 0;
-  const float rf = (float)r;
-  const float b32 = f_of_r32(P.bzf, a.idr_f, a.na - 1, rf, (float)P.bz0, 1.f);
-  const float n32 = f_of_r32(P.normf32, a.idr_f, a.na - 1, rf, (float)P.norm0, (float)P.normf);
-  const float lam32 = (float)(nd * a.cell_vol) * n32 * bias_model_f32(dl, b32, P.model);
-  const int k = poisson_fast_f32(lam32, word0);
-  if (k >= 0) return k;
-  const double bias = f_of_r(a.r_arr, P.bz, a.idr, a.r_last, r, P.bz0, 1.0);
-  const double dnorm = f_of_r(a.r_arr, P.norm, a.idr, a.r_last, r, P.norm0, P.normf);
-  const double lam = nd * a.cell_vol * bias_model_f64((double)dl, bias, P.model) * dnorm;
-  return poisson_exact(lam, cell, stream_poisson_extra(P.ipop), a.seed, word0);
+  const double bias = f_of_r(r_arr, bz, idr, r_last, r, bz0, 1.0);
+  const double dnorm = f_of_r(r_arr, norm, idr, r_last, r, norm0, normf);
+  const double lam = nd * cell_vol * bias_model_f64((double)dl, bias, model) * dnorm;
+  return poisson_exact(lam, cell, stream_poisson_extra(ipop), seed, x);
 }
 
-// The lane's 4 cells: delta values, radii and offsets.
-struct Quad {
-  float d[4];
-  double r[4], x0[4];
-  double y0, z0;
-  int ix0, iy, iz;
-  bool ok[4];
-  uint64_t cell0;
+// FP32 bias_model with an absolute bound E on its error against the FP64
+// value at the same (delta, b): b carries the interpolation error of the
+// table (bmax = larger |b| at the bin ends), exp2f/__expf/__log2f theirs.
+__device__ inline float bias_fast(int model, float b, float bmax, float dl, float l2, float qd, float &E) {
+  if (dl <= -1.f) {
+    E = 0.f;
+    return 0.f;
+  }
+  if (model == 0 || (model == 1 && dl < 0.f)) {
+    float v, s;
+    if (model == 0) {
+      v = exp2f(b * l2);
+      s = bmax * fabsf(l2) * 0.6931472f;
+    } else {
+      v = __expf(b * qd);
+      s = bmax * fabsf(qd);
+    }
+    E = v * (2e-6f + 1e-6f * bmax + 2e-6f * s);
+    return v;
+  }
+  const float v = 1.f + b * dl;
+  E = 3e-7f + 1e-6f * bmax * fabsf(dl);
+  return model == 2 ? fmaxf(v, 0.f) : v;
+}
+
+// queue entry: cell within the block | hi << 16; lambda (< 0: FP64 path); bound
+struct QEntry {
+  uint32_t ch;
+  float lam, err;
 };
 
-__device__ inline void load_quad(const SrcArgs &a, size_t seg, int lane, Quad &q) {
-#pragma clang fp contract(off)
-  const size_t row = seg / a.nseg;
-  const int s = (int)(seg % a.nseg);
-  q.iz = (int)(row / a.N);
-  q.iy = (int)(row % a.N);
-  q.ix0 = s * a.seglen + 4 * lane;
-  const bool in = q.ix0 < a.N && 4 * lane < a.seglen;
-  const float *p = a.g + row * (size_t)a.ngx + (in ? q.ix0 : 0);
-  float2 v0 = make_float2(0.f, 0.f), v1 = v0;
-  if (in) {
-    v0 = *(const float2 *)p;
-    v1 = *(const float2 *)(p + 2);
-  }
-  q.d[0] = v0.x; q.d[1] = v0.y; q.d[2] = v1.x; q.d[3] = v1.y;
-  q.z0 = (q.iz + 0.0) * a.dx - a.oz;
-  q.y0 = (q.iy + 0.0) * a.dx - a.oy;
-  q.cell0 = (uint64_t)q.ix0 + (uint64_t)a.N * ((uint64_t)q.iy + (uint64_t)a.N * (uint64_t)q.iz);
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    q.ok[c] = in;
-    const double xx = (q.ix0 + c + 0.0) * a.dx - a.ox;
-    q.x0[c] = xx;
-    q.r[c] = sqrt(xx * xx + q.y0 * q.y0 + q.z0 * q.z0);
-  }
+__device__ inline void publish(unsigned long long *w, unsigned long long v, unsigned int state) {
+  __hip_atomic_store(w, (v << 2) | state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline void quad_counts(const SrcArgs &a, const PopTab &P, const Quad &q, int k[4]) {
-  u32x4 rr{0u, 0u, 0u, 0u};
-  bool any = false;
-#pragma unroll
-  for (int c = 0; c < 4; c++) any |= q.ok[c] && q.r[c] < a.half_box_20;
-  if (any) rr = draw(q.cell0 >> 2, stream_poisson(P.ipop), 0u, a.seed);
-#pragma unroll
-  for (int c = 0; c < 4; c++) k[c] = q.ok[c] ? cell_count(a, P, q.r[c], q.d[c], q.cell0 + c, word(rr, c)) : 0;
-}
-
-template <class T>
-__device__ inline T block_excl_scan(T v, T *sh, T *total) {
-  // inclusive scan within the wavefront, then across wavefronts through LDS
-  const int lane = threadIdx.x % 64, w = threadIdx.x / 64, nw = blockDim.x / 64;
-  T x = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const T y = __shfl_up(x, off);
-    if (lane >= off) x += y;
-  }
-  __syncthreads();
-  if (lane == 63) sh[w] = x;
-  __syncthreads();
-  T base = 0, tot = 0;
-  for (int i = 0; i < nw; i++) {
-    if (i < w) base += sh[i];
-    tot += sh[i];
-  }
-  *total = tot;
-  return base + x - v;
-}
-
-__global__ __launch_bounds__(256) void k_count(SrcArgs a, int32_t *__restrict__ segcnt, size_t nsegs) {
-  __shared__ int32_t sh[CLR_NPOP_MAX][4];
-  const size_t seg = blockIdx.x;
-  const int lane = threadIdx.x;
-  Quad q;
-  load_quad(a, seg, lane, q);
-  for (int ip = 0; ip < a.npop; ip++) {
-    int k[4];
-    quad_counts(a, a.pop[ip], q, k);
-    int t = k[0] + k[1] + k[2] + k[3];
-    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
-    if ((threadIdx.x & 63) == 0) sh[ip][threadIdx.x >> 6] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x < (unsigned)a.npop) {
-    int t = 0;
-    for (int w = 0; w < (int)(blockDim.x / 64); w++) t += sh[threadIdx.x][w];
-    segcnt[(size_t)threadIdx.x * nsegs + seg] = t;
-  }
-}
-
-// exclusive scan of n int32 -> int64 offsets, bsum[nb] = total.  Three
-// kernels: per-block sums, scan of the block sums (one block), block-local scan.
-constexpr int kScanB = 1024;
-__global__ __launch_bounds__(kScanB) void k_scan_blocks(const int32_t *__restrict__ in, size_t n,
-                                                        long long *__restrict__ bsum) {
-  __shared__ long long s[kScanB];
-  const size_t i = (size_t)blockIdx.x * kScanB + threadIdx.x;
-  s[threadIdx.x] = i < n ? in[i] : 0;
-  __syncthreads();
-  for (int w = kScanB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
-}
-__global__ __launch_bounds__(kScanB) void k_scan_top(long long *__restrict__ bsum, size_t nb) {
-  __shared__ long long s[kScanB];
-  __shared__ long long carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (size_t base = 0; base < nb; base += kScanB) {
-    const size_t i = base + threadIdx.x;
-    const long long v = i < nb ? bsum[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < kScanB; off <<= 1) {
-      long long t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
+// Number of sources in the blocks before `t` (state 1: the block's own
+// count, state 2: inclusive of all before it).  Called by one whole wave;
+// every lane returns the same value.
+__device__ long long look_back(unsigned long long *st, long long t, int *bad) {
+  const int lane = threadIdx.x & 63;
+  long long excl = 0;
+  long long j0 = t - 1;
+  while (j0 >= 0) {
+    const long long j = j0 - lane;
+    unsigned long long w = 0;
+    if (j >= 0) {
+      int spins = 0;
+      while (((w = __hip_atomic_load(st + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 3u) == 0u) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > kSpinLimit) {
+          atomicOr(bad, 2);
+          w = 2u;   // an inclusive zero: the wait ends, the host reports it
+          break;
+        }
+      }
     }
-    if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == kScanB - 1) carry += s[kScanB - 1];
-    __syncthreads();
+    // lanes up to the nearest inclusive word (or all 64) contribute
+    const unsigned long long incl = __ballot((j >= 0) && (w & 3u) == 2u);
+    const int stop = incl ? __ffsll((long long)incl) - 1 : 63;
+    long long v = (j >= 0 && lane <= stop) ? (long long)(w >> 2) : 0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    excl += v;
+    if (incl) break;
+    j0 -= 64;
   }
-  if (threadIdx.x == 0) bsum[nb] = carry;
-}
-__global__ __launch_bounds__(kScanB) void k_scan_apply(const int32_t *__restrict__ in, size_t n,
-                                                       const long long *__restrict__ bsum,
-                                                       long long *__restrict__ out) {
-  __shared__ long long s[kScanB];
-  const size_t i = (size_t)blockIdx.x * kScanB + threadIdx.x;
-  const long long v = i < n ? in[i] : 0;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < kScanB; off <<= 1) {
-    long long t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
-  }
-  if (i < n) out[i] = bsum[blockIdx.x] + s[threadIdx.x] - v;
+  return excl;
 }
 
-__device__ inline double get_rvel(const SrcArgs &a, int ix, int iy, int iz, double x0, double y0, double z0,
+__global__ __launch_bounds__(kLanes) void k_sources(SrcArgs a) {
+  __shared__ QEntry queue[kBlockCells];
+  __shared__ int cnt[kBlockCells];
+  __shared__ int qn;
+  __shared__ unsigned int s_ticket;
+  __shared__ long long s_base;
+  __shared__ int s_sum[kLanes / 64];
+  const int lane = threadIdx.x;
+  if (lane == 0) {
+    s_ticket = atomicAdd(a.ticket, 1u);
+    qn = 0;
+  }
+  __syncthreads();
+  const long long t = s_ticket;
+  const size_t cell0 = (size_t)t * kBlockCells + (size_t)kCellsPerLane * lane;
+  const int N = a.N;
+  const size_t row = cell0 / (size_t)N;
+  const int ix0 = (int)(cell0 - row * N);
+  const int iy = (int)(row % N), iz = (int)(row / N);
+
+  // --- the lane's 8 cells: density and geometry --------------------------
+  float d[kCellsPerLane], tt[kCellsPerLane], l2[kCellsPerLane], qd[kCellsPerLane];
+  int ir[kCellsPerLane];
+  unsigned fastm = 0, inm = 0;
+  {
+    const float2 *p = (const float2 *)(a.g + row * (size_t)a.ngx + ix0);
+#pragma unroll
+    for (int h = 0; h < kCellsPerLane / 2; h++) {
+      const float2 v = p[h];
+      d[2 * h] = v.x;
+      d[2 * h + 1] = v.y;
+    }
+  }
+  {
+#pragma clang fp contract(off)
+    const double y0 = (iy + 0.0) * a.dx - a.oy, z0 = (iz + 0.0) * a.dx - a.oz;
+#pragma unroll
+    for (int c = 0; c < kCellsPerLane; c++) {
+      const double xx = (ix0 + c + 0.0) * a.dx - a.ox;
+      const double r2 = xx * xx + y0 * y0 + z0 * z0;   // the reference's order
+      // r = r2 / sqrt(r2): FP32 rsqrt refined once in FP64 (rel. error ~2e-14)
+      const float y = rsqrtf((float)r2);
+      const double yd = (double)y * (1.5 - 0.5 * r2 * (double)y * (double)y);
+      const double r = r2 * yd;
+      const double xr = r * a.idr;
+      const int i = (int)xr;
+      const double fr = xr - i;
+      ir[c] = i;
+      tt[c] = (float)fr;
+      const bool fast = r2 > 0 && fr > 1e-9 && fr < 1 - 1e-9 && i < a.na - 2 &&
+                        fabs(r - a.half_box_20) > 1e-9 * a.half_box_20;
+      fastm |= (fast ? 1u : 0u) << c;
+      inm |= (r < a.half_box_20 ? 1u : 0u) << c;
+      const float dl = d[c];
+      l2[c] = dl > -1.f ? __log2f(1.f + dl) : 0.f;
+      qd[c] = dl > -1.f ? __fdividef(dl, 1.f + dl) : 0.f;
+    }
+  }
+
+  for (int ip = 0; ip < a.npop; ip++) {
+    const PopTab &P = a.pop[ip];
+    // --- A: decide from the high 16 bits ---------------------------------
+    const u32x4 rr = draw(cell0 >> 3, stream_poisson(P.ipop), 0u, a.seed);
+#pragma unroll
+    for (int c = 0; c < kCellsPerLane; c++) {
+      const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
+      float lam = -1.f, err = 0.f;
+      bool queue_it = true;
+      if ((fastm >> c) & 1u) {
+        if (!((inm >> c) & 1u)) {
+          queue_it = false;
+        } else {
+          const float4 T0 = P.tab[ir[c]], T1 = P.tab[ir[c] + 1];
+          const int fl = __float_as_int(T0.w);
+          if (fl & (kFlagNonPos | kFlagNan)) {
+            queue_it = false;
+          } else if (fl & kFlagPos) {
+            const float tc = tt[c];
+            const float nd = T0.x + (T1.x - T0.x) * tc;
+            const float b = T0.y + (T1.y - T0.y) * tc;
+            const float nm = T0.z + (T1.z - T0.z) * tc;
+            float E;
+            const float bias = bias_fast(P.model, b, fmaxf(fabsf(T0.y), fabsf(T1.y)), d[c], l2[c], qd[c], E);
+            const float nn = nd * nm;
+            lam = nn * bias;
+            err = 2e-6f * lam + nn * E;
+            if (poisson_zero_from_hi(lam, __expf(-lam), err, hi)) queue_it = false;
+          }
+        }
+      }
+      if (queue_it) {
+        const int slot = atomicAdd(&qn, 1);
+        queue[slot] = QEntry{(uint32_t)(kCellsPerLane * lane + c) | (hi << 16), lam, err};
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kCellsPerLane; c++) cnt[kCellsPerLane * lane + c] = 0;
+    __syncthreads();
+    // --- B: the queue, all lanes busy --------------------------------------
+    const int nq = qn;
+    for (int i = lane; i < nq; i += kLanes) {
+      const QEntry e = queue[i];
+      const int cb = (int)(e.ch & 0xFFFFu);
+      const uint32_t hi = e.ch >> 16;
+      const size_t cell = (size_t)t * kBlockCells + cb;
+      const uint32_t x = (hi << 16) | poisson_lo16(cell, P.ipop, a.seed);
+      int kk = -1;
+      if (e.lam >= 0.f) kk = poisson_fast_f32(e.lam, __expf(-e.lam), e.err, x);
+      if (kk < 0) {
+        const size_t rw = cell / (size_t)N;
+        const int cx = (int)(cell - rw * N);
+        kk = count_exact(cx, (int)(rw % N), (int)(rw / N), a.g[rw * (size_t)a.ngx + cx], cell, x, a.r_arr, P.amp,
+                         P.bz, P.norm, a.idr, a.r_last, P.bz0, P.norm0, P.normf, a.dx, a.ox, a.oy, a.oz,
+                         a.cell_vol, a.half_box_20, P.model, P.ipop, a.seed);
+      }
+      cnt[cb] = kk;
+    }
+    __syncthreads();
+    // --- C: scan of the block's counts -------------------------------------
+    int k[kCellsPerLane];
+    int mine = 0;
+#pragma unroll
+    for (int c = 0; c < kCellsPerLane; c++) {
+      k[c] = cnt[kCellsPerLane * lane + c];
+      mine += k[c];
+    }
+    int incl = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off);
+      if ((lane & 63) >= off) incl += y;
+    }
+    if ((lane & 63) == 63) s_sum[lane >> 6] = incl;
+    if (lane == 0) qn = 0;
+    __syncthreads();
+    int wbase = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kLanes / 64; w++) {
+      if (w < (lane >> 6)) wbase += s_sum[w];
+      total += s_sum[w];
+    }
+    // --- D: global offset (wave 0) -----------------------------------------
+    if (lane < 64) {
+      unsigned long long *st = P.status;
+      long long base = 0;
+      if (t == 0) {
+        if (lane == 0) publish(st, (unsigned long long)total, 2u);
+      } else {
+        if (lane == 0) publish(st + t, (unsigned long long)total, 1u);
+        base = look_back(st, t, a.bad);
+        if (lane == 0) publish(st + t, (unsigned long long)(base + total), 2u);
+      }
+      if (lane == 0) {
+        s_base = base;
+        if (t == a.nblocks - 1) a.totals[ip] = (unsigned long long)(base + total);
+      }
+    }
+    __syncthreads();
+    // --- E: descriptors ------------------------------------------------------
+    if (total > 0) {
+      long long pid = s_base + wbase + incl - mine;
+#pragma unroll
+      for (int c = 0; c < kCellsPerLane; c++) {
+        const uint64_t cell = (uint64_t)cell0 + c;
+        for (int is = 0; is < k[c]; is++, pid++)
+          if (pid < P.cap) P.desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
+      }
+    }
+  }
+}
+
+struct WriteArgs {
+  const float *phi;
+  const double *r_arr, *v1;
+  double idr, r_last, v1_0, v1_last;
+  double dx, factor_vel, idx, ox, oy, oz;
+  int N, ngx, nside_base, ipop;
+  uint32_t seed;
+  const int32_t *r2n;
+  float4 *pos;
+  int32_t *ipix;
+  long long n;
+};
+
+__device__ inline double get_rvel(const WriteArgs &a, int ix, int iy, int iz, double x0, double y0, double z0,
                                   double rr) {
 #pragma clang fp contract(off)
   const float *__restrict__ phi = a.phi;
@@ -269,44 +406,36 @@ __device__ inline int vec2pix_ring32(int nside, double x, double y, double z) {
   return 12 * nside * nside - 2 * ir * (ir + 1) + ip;
 }
 
-__global__ __launch_bounds__(256) void k_fill(SrcArgs a, const long long *__restrict__ segoff, size_t nsegs) {
+__global__ __launch_bounds__(256) void k_write(WriteArgs a) {
 #pragma clang fp contract(off)
-  __shared__ long long sh[4];
-  const size_t seg = blockIdx.x;
-  const int lane = threadIdx.x;
-  Quad q;
-  load_quad(a, seg, lane, q);
-  for (int ip = 0; ip < a.npop; ip++) {
-    const PopTab &P = a.pop[ip];
-    int k[4];
-    quad_counts(a, P, q, k);
-    const long long mine = k[0] + k[1] + k[2] + k[3];
-    long long tot;
-    long long base = block_excl_scan<long long>(mine, sh, &tot) + segoff[(size_t)ip * (nsegs + 1) + seg];
-    if (mine == 0) continue;
-    const uint32_t stream = stream_position(P.ipop);
-    for (int c = 0; c < 4; c++) {
-      if (k[c] == 0) continue;
-      const int ix = q.ix0 + c;
-      const uint64_t cell = q.cell0 + c;
-      const double rvel = a.factor_vel * get_rvel(a, ix, q.iy, q.iz, q.x0[c], q.y0, q.z0, q.r[c]);
-      const double dz_rsd = rvel * f_of_r(a.r_arr, a.v1, a.idr, a.r_last, q.r[c], a.v1_0, a.v1_last);
-      const double ctr[3] = {q.x0[c], q.y0, q.z0};
-      u32x4 rnd{};
-      for (int is = 0; is < k[c]; is++) {
-        float pp[3];
+  const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= a.n) return;
+  const uint4 ds = ((const uint4 *)a.pos)[pid];
+  const uint64_t cell = (uint64_t)ds.x | ((uint64_t)ds.y << 32);
+  const int is = (int)ds.z;
+  const int N = a.N;
+  const uint64_t rw = cell / (uint64_t)N;
+  const int ix = (int)(cell - rw * N), iy = (int)(rw % N), iz = (int)(rw / N);
+  const double x0 = (ix + 0.0) * a.dx - a.ox, y0 = (iy + 0.0) * a.dx - a.oy, z0 = (iz + 0.0) * a.dx - a.oz;
+  const double rr = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+  const double rvel = a.factor_vel * get_rvel(a, ix, iy, iz, x0, y0, z0, rr);
+  const double dz_rsd = rvel * f_of_r(a.r_arr, a.v1, a.idr, a.r_last, rr, a.v1_0, a.v1_last);
+  const uint32_t stream = stream_position(a.ipop);
+  const double ctr[3] = {x0, y0, z0};
+  float pp[3];
+  u32x4 rnd{};
+  uint32_t last = 0xFFFFFFFFu;
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-          const uint32_t j = 3u * (uint32_t)is + (uint32_t)ax;
-          if ((j & 3u) == 0u) rnd = draw(cell, stream, j >> 2, a.seed);
-          pp[ax] = (float)(ctr[ax] + a.dx * (u01(word(rnd, (int)(j & 3u))) - 0.5));
-        }
-        P.pos[base] = make_float4(pp[0], pp[1], pp[2], (float)dz_rsd);
-        P.ipix[base] = a.r2n[vec2pix_ring32(a.nside_base, (double)pp[0], (double)pp[1], (double)pp[2])];
-        base++;
-      }
+  for (int ax = 0; ax < 3; ax++) {
+    const uint32_t j = 3u * (uint32_t)is + (uint32_t)ax;
+    if ((j >> 2) != last) {
+      rnd = draw(cell, stream, j >> 2, a.seed);
+      last = j >> 2;
     }
+    pp[ax] = (float)(ctr[ax] + a.dx * (u01(word(rnd, (int)(j & 3u))) - 0.5));
   }
+  a.pos[pid] = make_float4(pp[0], pp[1], pp[2], (float)dz_rsd);
+  a.ipix[pid] = a.r2n[vec2pix_ring32(a.nside_base, (double)pp[0], (double)pp[1], (double)pp[2])];
 }
 
 // Src fields of srcs_get_local_properties_single: RA, Dec, z0, dz_rsd.
@@ -341,121 +470,151 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     return clr_fail(c, CLR_ERR_ARG, "nside_base=%d must be a power of two <= 64", nside_base);
   const int p0 = ipop < 0 ? 0 : ipop, p1 = ipop < 0 ? nall : ipop + 1;
   const int N = c->N, na = c->na;
+  const size_t ncells = (size_t)N * N * N;
+  if (N % kCellsPerLane || ncells % kBlockCells)
+    return clr_fail(c, CLR_ERR_ARG, "grid of %d^3 cells is not a whole number of %d-cell blocks", N, kBlockCells);
   for (int i = p0; i < p1; i++) {
     const TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
     if (!T.set || !T.norm_set) return clr_fail(c, CLR_ERR_STATE, "population %d has no norm table", i);
   }
   int rc = clr_ring2nest_table(c, nside_base);
   if (rc) return rc;
-  // FP32 copies of the norm tables (allocated on first use): before the
-  // kernel arguments take their addresses
+  const double dx = c->L / N, cell_vol = dx * dx * dx;
+  // packed FP32 tables, rebuilt from the current norm tables
   for (int i = p0; i < p1; i++) {
-    rc = clr_norm_f32(c, c->tr[CLR_TRACER_SRCS][i]);
-    if (rc) return rc;
+    TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
+    if (!T.ptab) CLR_HIP(c, hipMalloc((void **)&T.ptab, (size_t)na * sizeof(float4)));
+    hipLaunchKernelGGL(k_pop_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, T.amp, T.bz, T.norm, cell_vol,
+                       na, (float4 *)T.ptab);
+    CLR_HIP(c, hipGetLastError());
   }
+  const int np = p1 - p0;
+  const long long nblocks = (long long)(ncells / kBlockCells);
+  // scratch: status words (np x nblocks), totals, ticket, error flag
+  const size_t status_bytes = (size_t)np * nblocks * sizeof(unsigned long long);
+  const size_t scratch = status_bytes + CLR_NPOP_MAX * sizeof(unsigned long long) + 16;
+  rc = clr_ensure(c, (void **)&c->rowcnt, &c->rows_n, scratch);
+  if (rc) return rc;
+  unsigned long long *status = (unsigned long long *)c->rowcnt;
+  unsigned long long *totals = status + (size_t)np * nblocks;
+  unsigned int *ticket = (unsigned int *)(totals + CLR_NPOP_MAX);
+  int *bad = (int *)(ticket + 1);
+
   SrcArgs a{};
   a.g = c->dens;
-  a.phi = c->npot;
   a.r_arr = c->bg;
-  a.v1 = c->bg + 3 * (size_t)na;
   a.idr = c->glob_idr;
   a.r_last = c->r_last;
-  a.v1_0 = c->h_bg_first[3];
-  a.v1_last = c->h_bg_last[3];
-  a.dx = c->L / N;
-  a.cell_vol = a.dx * a.dx * a.dx;
+  a.dx = dx;
+  a.cell_vol = cell_vol;
   a.half_box_20 = c->L / 2 + 20.;
-  a.factor_vel = -c->fgrowth_0 / (1.5 * c->hubble_0 * c->omega_m);
-  a.idx = N / c->L;
   a.ox = c->obs[0];
   a.oy = c->obs[1];
   a.oz = c->obs[2];
-  a.idr_f = (float)c->glob_idr;
   a.N = N;
   a.ngx = c->ngx;
   a.na = na;
-  a.seglen = N < 1024 ? N : 1024;
-  a.nseg = N / a.seglen;
-  a.npop = p1 - p0;
-  a.nside_base = nside_base;
+  a.npop = np;
+  a.nblocks = nblocks;
   a.seed = seed;
-  a.r2n = c->r2n;
+  a.ticket = ticket;
+  a.totals = totals;
+  a.bad = bad;
   for (int i = p0; i < p1; i++) {
     const TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
     PopTab &P = a.pop[i - p0];
     P.amp = T.amp;
     P.bz = T.bz;
     P.norm = T.norm;
-    P.bzf = T.bzf;
-    P.normf32 = T.normf32;
+    P.tab = (const float4 *)T.ptab;
     P.bz0 = T.bz0;
     P.norm0 = T.norm0;
     P.normf = T.normf;
     P.model = T.model;
     P.ipop = i;
+    P.status = status + (size_t)(i - p0) * nblocks;
+    if (!P.amp || !P.bz || !P.norm || !P.tab) return clr_fail(c, CLR_ERR_STATE, "population %d tables missing", i);
   }
-  const size_t nsegs = (size_t)N * N * a.nseg;
-  const size_t nb = (nsegs + kScanB - 1) / kScanB;
-  const int np = a.npop;
-  rc = clr_ensure(c, (void **)&c->rowcnt, &c->rows_n,
-                  np * (nsegs * sizeof(int32_t) + 8 + (nsegs + 1) * sizeof(long long) + (nb + 1) * sizeof(long long)));
-  if (rc) return rc;
-  int32_t *segcnt = c->rowcnt;
-  long long *segoff = (long long *)(((uintptr_t)(segcnt + np * nsegs) + 7) & ~(uintptr_t)7);
-  long long *bsum = segoff + np * (nsegs + 1);
+  if (!c->r2n) return clr_fail(c, CLR_ERR_STATE, "no RING->NEST table");
+  if (!a.g || !a.r_arr) return clr_fail(c, CLR_ERR_STATE, "density grid or background tables missing");
 
-  const int bt = ((a.seglen / 4) + 63) / 64 * 64;
-  for (int i = p0; i < p1; i++)
-    if (!a.pop[i - p0].normf32 || !a.pop[i - p0].bzf || !a.pop[i - p0].norm || !a.pop[i - p0].amp)
-      return clr_fail(c, CLR_ERR_STATE, "population %d tables are not on the device", i);
-  if (!a.r2n) return clr_fail(c, CLR_ERR_STATE, "no RING->NEST table");
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_count, dim3((unsigned)nsegs), dim3(bt), 0, c->stream, a, segcnt, nsegs);
-  CLR_HIP(c, hipGetLastError());
-  for (int i = 0; i < np; i++) {
-    const int32_t *in = segcnt + (size_t)i * nsegs;
-    long long *bs = bsum + (size_t)i * (nb + 1);
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(kScanB), 0, c->stream, in, nsegs, bs);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanB), 0, c->stream, bs, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanB), 0, c->stream, in, nsegs, bs,
-                       segoff + (size_t)i * (nsegs + 1));
+  std::vector<unsigned long long> tot(CLR_NPOP_MAX, 0ull);
+  for (int pass = 0; pass < 2; pass++) {
+    for (int i = 0; i < np; i++) {
+      CatalogDev &C = c->cat[p0 + i];
+      a.pop[i].desc = (uint4 *)C.pos;
+      a.pop[i].cap = C.pos ? C.cap : 0;
+    }
+    CLR_HIP(c, hipMemsetAsync(status, 0, scratch, c->stream));
+    hipLaunchKernelGGL(k_sources, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
-  }
-  std::vector<long long> tot(np);
-  for (int i = 0; i < np; i++)
-    CLR_HIP(c, hipMemcpyAsync(&tot[i], bsum + (size_t)i * (nb + 1) + nb, sizeof(long long), hipMemcpyDeviceToHost,
+    int hbad = 0;
+    CLR_HIP(c, hipMemcpyAsync(tot.data(), totals, CLR_NPOP_MAX * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               c->stream));
-  CLR_HIP(c, hipStreamSynchronize(c->stream));
+    CLR_HIP(c, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CLR_HIP(c, hipStreamSynchronize(c->stream));
+    if (hbad) return clr_fail(c, CLR_ERR_STATE, "source pass: look-back did not complete (code %d)", hbad);
+    bool grown = false;
+    for (int i = 0; i < np; i++) {
+      CatalogDev &C = c->cat[p0 + i];
+      const long long n = (long long)tot[i];
+      if (n > C.cap) {
+        if (pass == 1) return clr_fail(c, CLR_ERR_STATE, "catalogue %d still too small", p0 + i);
+        if (C.pos) (void)hipFree(C.pos);
+        if (C.ipix) (void)hipFree(C.ipix);
+        C.pos = nullptr;
+        C.ipix = nullptr;
+        C.cap = 0;
+        const long long cap = n + n / 16 + 1024;
+        if (hipMalloc((void **)&C.pos, (size_t)cap * 4 * sizeof(float)) != hipSuccess ||
+            hipMalloc((void **)&C.ipix, (size_t)cap * sizeof(int32_t)) != hipSuccess) {
+          (void)hipGetLastError();
+          return clr_fail(c, CLR_ERR_OOM, "cannot allocate catalogue of %lld sources", n);
+        }
+        C.cap = cap;
+        grown = true;
+      }
+    }
+    if (!grown) break;
+  }
   long long total = 0;
   for (int i = 0; i < np; i++) {
     CatalogDev &C = c->cat[p0 + i];
-    if (tot[i] > C.cap) {
-      if (C.pos) hipFree(C.pos);
-      if (C.ipix) hipFree(C.ipix);
-      C.pos = nullptr;
-      C.ipix = nullptr;
-      C.cap = 0;
-      const long long cap = tot[i] + tot[i] / 16 + 1024;
-      if (hipMalloc((void **)&C.pos, (size_t)cap * 4 * sizeof(float)) != hipSuccess ||
-          hipMalloc((void **)&C.ipix, (size_t)cap * sizeof(int32_t)) != hipSuccess) {
-        (void)hipGetLastError();
-        return clr_fail(c, CLR_ERR_OOM, "cannot allocate catalogue of %lld sources", tot[i]);
-      }
-      C.cap = cap;
-    }
-    C.n = tot[i];
-    a.pop[i].pos = (float4 *)C.pos;
-    a.pop[i].ipix = C.ipix;
-    total += tot[i];
-  }
-  if (total > 0) {
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)nsegs), dim3(bt), 0, c->stream, a, segoff, nsegs);
+    C.n = (long long)tot[i];
+    total += C.n;
+    if (C.n == 0) continue;
+    WriteArgs w{};
+    w.phi = c->npot;
+    w.r_arr = c->bg;
+    w.v1 = c->bg + 3 * (size_t)na;
+    w.idr = c->glob_idr;
+    w.r_last = c->r_last;
+    w.v1_0 = c->h_bg_first[3];
+    w.v1_last = c->h_bg_last[3];
+    w.dx = dx;
+    w.factor_vel = -c->fgrowth_0 / (1.5 * c->hubble_0 * c->omega_m);
+    w.idx = N / c->L;
+    w.ox = c->obs[0];
+    w.oy = c->obs[1];
+    w.oz = c->obs[2];
+    w.N = N;
+    w.ngx = c->ngx;
+    w.nside_base = nside_base;
+    w.ipop = p0 + i;
+    w.seed = seed;
+    w.r2n = c->r2n;
+    w.pos = (float4 *)C.pos;
+    w.ipix = C.ipix;
+    w.n = C.n;
+    if (!w.phi || !w.pos || !w.ipix) return clr_fail(c, CLR_ERR_STATE, "no potential or catalogue buffer");
+    hipLaunchKernelGGL(k_write, dim3((unsigned)((C.n + 255) / 256)), dim3(256), 0, c->stream, w);
     CLR_HIP(c, hipGetLastError());
   }
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
   float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->stage_ms[ST_SRCS] = ms;
   if (nsrc) *nsrc = total;
   return CLR_OK;
@@ -474,7 +633,7 @@ int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n) {
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "source properties: %s", hipGetErrorString(e));
   return CLR_OK;
 }

@@ -497,15 +497,19 @@ static int poisson_inv(double lam, double u) {
   return k;
 }
 
-/* inversion by sequential search with the cell's word of Philox(cell/4,
- * stream, 0); lambda > 500 split into ceil(lambda/500) equal parts, the i-th
- * extra part using word (i-1)%4 of Philox(cell, stream | 0x80000000,
- * (i-1)/4) */
+/* inversion by sequential search with u = (hi 2^16 + lo)/2^32: hi the 16-bit
+ * half (cell % 8) of Philox(cell/8, stream, 0), lo the low 16 bits of word 0
+ * of Philox(cell, stream | 0x40000000, 0); lambda > 500 split into
+ * ceil(lambda/500) equal parts, the i-th extra part using word (i-1)%4 of
+ * Philox(cell, stream | 0x80000000, (i-1)/4) */
 int orc_poisson(double lam, uint64_t cell, uint32_t stream, uint32_t seed) {
   if (!(lam > 0)) return 0;
-  uint32_t r[4];
-  orc_draw(cell >> 2, stream, 0u, seed, r);
-  const double u0 = orc_u01(r[cell & 3]);
+  uint32_t r[4], q[4];
+  orc_draw(cell >> 3, stream, 0u, seed, r);
+  orc_draw(cell, stream | 0x40000000u, 0u, seed, q);
+  const uint32_t hi = (r[(cell & 7) >> 1] >> (((cell & 7) & 1) * 16)) & 0xFFFFu;
+  const uint32_t x = (hi << 16) | (q[0] & 0xFFFFu);
+  const double u0 = orc_u01(x);
   if (lam <= 500.0) return poisson_inv(lam, u0);
   const int m = (int)ceil(lam / 500.0);
   const double lp = lam / m;
