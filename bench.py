@@ -29,16 +29,60 @@ METRIC = "grid cells/sec (field→lognormal→sources) at 1024³ FP32; % HBM roo
 HBM_PEAK = 8.0e12
 
 
-def kernel_bytes(N):
-    """Algorithmic HBM bytes per launch of the three FFT kernels (both fields)."""
+def kernel_bytes(N, nsrc):
+    """Algorithmic HBM bytes per launch of each timed kernel.
+
+    k_gen_z  writes both half spectra (delta and phi) after the z pass;
+    k_col_y  reads and writes both half spectra (y pass);
+    k_c2r_x  reads both half spectra and writes both padded real grids;
+    k_shells reads delta_G and writes the lognormal field in place (cells only);
+    k_sources reads the density of every cell once (descriptors are its own
+              intermediate, not counted);
+    k_write  gathers the 6-point phi stencil per source and writes pos + ipix.
+    """
     nh = N // 2 + 1
     half = N * N * nh * 8          # one complex half spectrum
-    real = N * N * N * 4           # one real grid without padding
+    real = N * N * (N + 2) * 4     # one padded real grid
+    cells = N ** 3
     return {
-        "k_gen_z": 2 * half,                   # write both spectra after the z pass
-        "k_col_y": 4 * half,                   # read + write both
-        "k_c2r_x": 2 * half + 2 * real,        # read spectra, write real grids
+        "k_gen_z": 2 * half,
+        "k_col_y": 4 * half,
+        "k_c2r_x": 2 * half + 2 * real,
+        "k_shells": 2 * cells * 4,
+        "k_sources": cells * 4,
+        "k_write": nsrc * (16 + 4 + 6 * 4),
     }
+
+
+# stage_times() key holding each kernel's device time
+KERNEL_STAGE = {"k_gen_z": "gen", "k_col_y": "fft_yz", "k_c2r_x": "fft_x", "k_shells": "norm",
+                "k_sources": "src_pass", "k_write": "src_write"}
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (scaled by the same counters on a plain copy of known size),
+    read from profiles/<kernel>_traffic.txt; None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size, for scaling the PMC byte
+    counters (run under rocprofv3 --pmc next to the bench)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
 
 
 def cpu_baseline(cfg, n_cpu_grid):
@@ -80,7 +124,12 @@ def main():
     ap.add_argument("--n-grid", type=int, default=None)
     ap.add_argument("--cpu-grid", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (PMC counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
@@ -106,9 +155,11 @@ def main():
         ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=npop)
         st["norm"] = ctx.stage_times()["norm"]
         ns = [ctx.sample_sources(-1, seed, 2)]
-        st["srcs"] = ctx.stage_times()["srcs"]
+        st_s = ctx.stage_times()
+        for k in ("srcs", "src_pass", "src_write"):
+            st[k] = st_s[k]
         if record:
-            for k in ("gen", "fft_yz", "fft_x", "norm", "srcs"):
+            for k in ("gen", "fft_yz", "fft_x", "norm", "srcs", "src_pass", "src_write"):
                 acc[k] = acc.get(k, 0.0) + st[k]
         return ns, s2
 
@@ -121,13 +172,12 @@ def main():
     cells = float(N) ** 3
     value = cells / dt
 
-    kb = kernel_bytes(N)
-    kt = {"k_gen_z": acc["gen"] / args.steps, "k_col_y": acc["fft_yz"] / args.steps,
-          "k_c2r_x": acc["fft_x"] / args.steps}
+    nsrc_tot = sum(ns)
+    kb = kernel_bytes(N, nsrc_tot)
+    kt = {k: acc[st] / args.steps for k, st in KERNEL_STAGE.items()}
     dom = max(kt, key=kt.get)
     achieved = kb[dom] / (kt[dom] * 1e-3)
     p = (N + 2.0) / N
-    nsrc_tot = sum(ns)
     b_alg = 36 * p + 20.0 * nsrc_tot / cells
 
     out = {
@@ -150,8 +200,10 @@ def main():
                           "frac": value * b_alg / HBM_PEAK},
         "stage_ms": {k: v / args.steps for k, v in acc.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic_per_launch(dom),
                      "bytes_per_launch": kb[dom], "ms_per_launch": kt[dom]},
+        "kernels": {k: {"ms": kt[k], "GB/s": kb[k] / (kt[k] * 1e-3) / 1e9 if kt[k] > 0 else None}
+                    for k in kt},
     }
     ctx.close()
     if not args.no_cpu_baseline:

@@ -57,6 +57,8 @@ static void free_all(clr_ctx *c) {
   for (int i = 0; i < CLR_NPOP_MAX; i++) { f(c->cat[i].pos); f(c->cat[i].ipix); }
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->ev2) hipEventDestroy(c->ev2);
+  if (c->ev3) hipEventDestroy(c->ev3);
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -87,6 +89,8 @@ int clr_create(clr_ctx **out, int n_grid, double l_box, const double pos_obs[3],
   CLR_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   CLR_HIP(c, hipEventCreate(&c->ev0));
   CLR_HIP(c, hipEventCreate(&c->ev1));
+  CLR_HIP(c, hipEventCreate(&c->ev2));
+  CLR_HIP(c, hipEventCreate(&c->ev3));
   c->grid_floats = (size_t)n_grid * n_grid * c->ngx;
   if (hipMalloc((void **)&c->dens, c->grid_floats * sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&c->npot, c->grid_floats * sizeof(float)) != hipSuccess) {

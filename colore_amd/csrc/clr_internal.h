@@ -28,7 +28,12 @@ struct CatalogDev {
   long long n = 0;
 };
 
-enum StageId { ST_GEN = 0, ST_FFT, ST_XPASS, ST_DENS, ST_NORM, ST_SRCS, ST_KAPPA, ST_ISW, ST_IMAP, ST_COUNT };
+enum StageId {
+  ST_GEN = 0, ST_FFT, ST_XPASS, ST_DENS, ST_NORM, ST_SRCS, ST_KAPPA, ST_ISW, ST_IMAP,
+  ST_SRC_PASS,    // the last k_sources launch of a sample_sources call
+  ST_SRC_WRITE,   // its k_write launches
+  ST_COUNT
+};
 
 struct clr_ctx {
   int device = 0;
@@ -38,7 +43,7 @@ struct clr_ctx {
   double L = 0, dx = 0;
   double obs[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   double stage_ms[ST_COUNT] = {0};
 
   float *dens = nullptr, *npot = nullptr;   // padded real grids

@@ -547,8 +547,10 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
       a.pop[i].cap = C.pos ? C.cap : 0;
     }
     CLR_HIP(c, hipMemsetAsync(status, 0, scratch, c->stream));
+    CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
     hipLaunchKernelGGL(k_sources, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
+    CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
     int hbad = 0;
     CLR_HIP(c, hipMemcpyAsync(tot.data(), totals, CLR_NPOP_MAX * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               c->stream));
@@ -578,6 +580,10 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     }
     if (!grown) break;
   }
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
+  c->stage_ms[ST_SRC_PASS] = ms;
+  CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
   long long total = 0;
   for (int i = 0; i < np; i++) {
     CatalogDev &C = c->cat[p0 + i];
@@ -611,9 +617,11 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     hipLaunchKernelGGL(k_write, dim3((unsigned)((C.n + 255) / 256)), dim3(256), 0, c->stream, w);
     CLR_HIP(c, hipGetLastError());
   }
+  CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
-  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
+  c->stage_ms[ST_SRC_WRITE] = ms;
   (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->stage_ms[ST_SRCS] = ms;
   if (nsrc) *nsrc = total;

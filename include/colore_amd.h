@@ -61,7 +61,8 @@ typedef struct clr_ctx clr_ctx;
 /* Allocates the two padded real grids float[N][N][2(N/2+1)] on `device`
  * (fourier.c:211-238 allocate_fftw; the z halo slabs of the reference are
  * the periodic neighbours on one GPU, fourier.c:412-413).  N must be one of
- * 2^n (32..2048) or 3*2^n (48..1536). */
+ * 32, 64, 128, 192, 256, 384, 512, 1024, 2048 (the sizes with a tuned FFT
+ * tile set); other sizes return CLR_ERR_ARG. */
 int  clr_create(clr_ctx **ctx, int n_grid, double l_box, const double pos_obs[3],
                 int device);
 void clr_destroy(clr_ctx *ctx);
@@ -183,7 +184,11 @@ int clr_upload_grid(clr_ctx *ctx, int which, const float *host);
  * stage too, so that later stages can run on caller-supplied grids. */
 int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
 
-/* Device time in milliseconds of the last call of each stage (0 if never). */
+/* Device time in milliseconds of the last call of each stage (0 if never),
+ * in this order: mode generation + z pass, y pass, x pass, density
+ * transform, shell sums, sources (whole call), kappa, ISW, imap, the last
+ * counting pass of the sources call, its per-source writer.  Fills at most
+ * n values; returns the number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 
 #ifdef __cplusplus
