@@ -29,16 +29,18 @@ METRIC = "grid cells/sec (field→lognormal→sources) at 1024³ FP32; % HBM roo
 HBM_PEAK = 8.0e12
 
 
-def kernel_bytes(N, nsrc):
+def kernel_bytes(N, nsrc, npop):
     """Algorithmic HBM bytes per launch of each timed kernel.
 
     k_gen_z  writes both half spectra (delta and phi) after the z pass;
     k_col_y  reads and writes both half spectra (y pass);
     k_c2r_x  reads both half spectra and writes both padded real grids;
     k_shells reads delta_G and writes the lognormal field in place (cells only);
-    k_sources reads the density of every cell once (descriptors are its own
-              intermediate, not counted);
-    k_write  gathers the 6-point phi stencil per source and writes pos + ipix.
+    k_count  reads the density of every cell once and writes one count byte
+             per cell and population;
+    k_fill+k_write  read the count bytes, write and read one 16-byte
+             descriptor per source, gather the 6-point phi stencil and write
+             pos + ipix.
     """
     nh = N // 2 + 1
     half = N * N * nh * 8          # one complex half spectrum
@@ -49,14 +51,14 @@ def kernel_bytes(N, nsrc):
         "k_col_y": 4 * half,
         "k_c2r_x": 2 * half + 2 * real,
         "k_shells": 2 * cells * 4,
-        "k_sources": cells * 4,
-        "k_write": nsrc * (16 + 4 + 6 * 4),
+        "k_count": cells * (4 + npop),
+        "k_fill+k_write": cells * npop + nsrc * (16 + 16 + 16 + 4 + 6 * 4),
     }
 
 
 # stage_times() key holding each kernel's device time
 KERNEL_STAGE = {"k_gen_z": "gen", "k_col_y": "fft_yz", "k_c2r_x": "fft_x", "k_shells": "norm",
-                "k_sources": "src_pass", "k_write": "src_write"}
+                "k_count": "src_pass", "k_fill+k_write": "src_write"}
 
 
 def traffic_per_launch(kernel):
@@ -173,7 +175,7 @@ def main():
     value = cells / dt
 
     nsrc_tot = sum(ns)
-    kb = kernel_bytes(N, nsrc_tot)
+    kb = kernel_bytes(N, nsrc_tot, npop)
     kt = {k: acc[st] / args.steps for k, st in KERNEL_STAGE.items()}
     dom = max(kt, key=kt.get)
     achieved = kb[dom] / (kt[dom] * 1e-3)

@@ -1,29 +1,25 @@
 // clr_sources.hip -- srcs_set_cartesian_single (srcs.c:120-283) for one or
 // all populations, and srcs_get_local_properties_single (srcs.c:386-416).
 //
-// k_sources: one pass over the density grid.  A block takes 1024
-// consecutive cells (z-major flat order; a lane owns 8 cells of one row) in
-// the order of a ticket it draws when it starts, and for every population
-//   A. decides most cells from 16 uniform bits and an FP32 lambda whose error
-//      is bounded (clr_poisson.h): k = 0 at once; the few others are queued
-//      in LDS (cell, lambda, bound);
-//   B. lets all lanes work the queue: the full 32-bit uniform, the FP32
-//      search, and where that is too close to call the reference's FP64
-//      expressions (count_exact) -- every count equals the FP64 one;
-//   C. scans the counts of its cells;
-//   D. gets the number of sources of all earlier blocks by decoupled
-//      look-back over per-block status words (a block waits only on blocks
-//      with a smaller ticket, which are already running and publish their
-//      own count before they wait, so every wait ends); and
-//   E. writes one descriptor (cell, index within the cell) per source at
-//      its final place in the catalogue: global z-major cell order, draw
-//      order within a cell -- the order of the reference on one thread.
-// k_write: one thread per source turns its descriptor into the position
-//   in [x-dx/2, x+dx/2) (srcs.c:261-263), the RSD shift from central
-//   differences of phi (get_rvel, srcs.c:87-118) and the NEST base pixel
-//   (vec2pix_ring then a RING->NEST table, srcs.c:264-270).
-// A pass writes nothing past the end of a catalogue; when one is too small
-// for its count the host grows it and runs the pass again.
+// Blocks are one wavefront over 512 consecutive cells (z-major flat order; a
+// lane owns 8 cells of one row).
+// k_count : per population, decides most cells from 16 uniform bits and an
+//           FP32 lambda whose error is bounded (clr_poisson.h): k = 0 at once;
+//           the few others are compacted (ballot) into an LDS queue that all
+//           lanes then work: the full 32-bit uniform, the FP32 search, and
+//           where that is too close to call the reference's FP64 expressions
+//           (count_exact) -- every count equals the FP64 one.  Writes one byte
+//           per cell and population (255 = "255 or more": recounted by
+//           k_fill) and the block's total.
+// k_scan_*: exclusive prefix of the block totals (64-bit), per population.
+// k_fill  : reads the bytes, scans its cells and writes one descriptor
+//           (cell, index within the cell) per source at its final place in
+//           the catalogue: global z-major cell order, draw order within a
+//           cell -- the order of the reference on one thread.
+// k_write : one thread per source turns its descriptor into the position
+//           in [x-dx/2, x+dx/2) (srcs.c:261-263), the RSD shift from central
+//           differences of phi (get_rvel, srcs.c:87-118) and the NEST base
+//           pixel (vec2pix_ring then a RING->NEST table, srcs.c:264-270).
 #include "clr_internal.h"
 #include "clr_poisson.h"
 #include <math.h>
@@ -32,18 +28,18 @@ using namespace clr;
 
 namespace {
 
-constexpr int kFlagPos = 1, kFlagNonPos = 2, kFlagNan = 4;
-constexpr int kLanes = 128, kCellsPerLane = 8, kBlockCells = kLanes * kCellsPerLane;
-constexpr int kSpinLimit = 1 << 24;   // look-back polls before giving up (~1 s)
+constexpr int kLanes = 64, kCellsPerLane = 8, kBlockCells = kLanes * kCellsPerLane;
+constexpr int kByteEscape = 255;   // a stored count of 255 means "recount this cell"
 
 struct PopTab {
   const double *amp, *bz, *norm;
-  const float4 *tab;   // per bin: (n dx^3, b, norm, flags)
+  const float4 *tab;         // k_pop_table's bins (na + 1 entries)
   double bz0, norm0, normf;
   int model, ipop;
-  uint4 *desc;         // the catalogue's position buffer, holding descriptors
-  long long cap;
-  unsigned long long *status;   // one word per block: count << 2 | state
+  uint8_t *cnt8;             // one byte per cell
+  uint32_t *btot;            // one total per block
+  const long long *boff;     // exclusive prefix of btot
+  uint4 *desc;               // the catalogue's position buffer, holding descriptors
 };
 
 struct SrcArgs {
@@ -52,36 +48,55 @@ struct SrcArgs {
   double idr, r_last;
   double dx, cell_vol, half_box_20, ox, oy, oz;
   int N, ngx, na, npop;
-  long long nblocks;
   uint32_t seed;
-  unsigned int *ticket;
-  unsigned long long *totals;
-  int *bad;
   PopTab pop[CLR_NPOP_MAX];
 };
 
-// per bin i: n(r_i) dx^3, b(r_i), norm(r_i) in FP32 and flags for the
-// interval [r_i, r_i+1] of the FP64 tables:
-//   kFlagNan    n is NaN at an end: the interpolated n is NaN, no sources;
-//   kFlagNonPos n <= 0 at both ends: the FP64 interpolation is <= 0 (it is
-//               monotone in each end and both ends bound it), no sources;
-//   kFlagPos    n, norm > 0 and finite at both ends with ratios <= 2 and b
-//               finite: the FP32 interpolation is within 1e-6 relative.
+// Per bin i (the interval [r_i, r_i+1] of the FP64 tables) one float4 that
+// the sampler gathers once per cell:
+//   Pos   (x >= 0): n, norm > 0 and finite at both ends, P = n dx^3 norm in
+//         [1e-30, 1e30] at both ends within a factor 2, |b| < 1e6:
+//         x = P(r_i), y = P(r_i+1) - P(r_i), z = b(r_i), w = b(r_i+1) - b(r_i).
+//         The sampler interpolates P linearly; the reference interpolates n
+//         and norm separately, whose product differs by at most
+//         |dn dnorm| dx^3 / 4, a relative bound kept (maximum over bins) in
+//         the x of the extra element na.
+//   Zero  (x = -1): n is NaN at an end, or n <= 0 at both ends (the FP64
+//         interpolation is then <= 0 or NaN): no sources.
+//   Tiny  (x = -2): n >= 0 and norm > 0 at both ends with P <= 1e-20: the
+//         reference's lambda is at most 1e-20 bias, far below 2^-33, where
+//         exp(-lambda) exceeds every uniform: no sources unless the bias is
+//         huge (z, w as for Pos).
+//   Other (x = -3): the FP64 path decides.
+constexpr float kBinZero = -1.f, kBinTiny = -2.f, kBinOther = -3.f;
+
 __global__ void k_pop_table(const double *__restrict__ amp, const double *__restrict__ bz,
                             const double *__restrict__ norm, double cv, int na, float4 *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= na) return;
   const int j = i + 1 < na ? i + 1 : i;
   const double f0 = amp[i], f1 = amp[j], n0 = norm[i], n1 = norm[j], b0 = bz[i], b1 = bz[j];
-  // both ends inside FP32's normal range, within a factor 2 of each other
-  auto close_pos = [](double u, double v) {
-    return u > 1e-30 && u < 1e30 && v > 1e-30 && v < 1e30 && u <= 2 * v && v <= 2 * u;
-  };
-  int fl = 0;
-  if (isnan(f0) || isnan(f1)) fl = kFlagNan;
-  else if (f0 <= 0 && f1 <= 0) fl = kFlagNonPos;
-  else if (close_pos(f0 * cv, f1 * cv) && close_pos(n0, n1) && fabs(b0) < 1e6 && fabs(b1) < 1e6) fl = kFlagPos;
-  out[i] = make_float4((float)(f0 * cv), (float)b0, (float)n0, __int_as_float(fl));
+  const bool bfin = fabs(b0) < 1e6 && fabs(b1) < 1e6;
+  const bool npos = n0 > 0 && n1 > 0 && n0 < 1e300 && n1 < 1e300;
+  const float bz0 = (float)b0, dbz = (float)(b1 - b0);
+  if (isnan(f0) || isnan(f1) || (f0 <= 0 && f1 <= 0)) {
+    out[i] = make_float4(kBinZero, 0.f, 0.f, 0.f);
+    return;
+  }
+  if (f0 >= 0 && f1 >= 0 && npos && bfin) {
+    const double P0 = f0 * cv * n0, P1 = f1 * cv * n1;
+    if (P0 <= 1e-20 && P1 <= 1e-20) {
+      out[i] = make_float4(kBinTiny, 0.f, bz0, dbz);
+      return;
+    }
+    if (P0 >= 1e-30 && P1 >= 1e-30 && P0 <= 1e30 && P1 <= 1e30 && P0 <= 2 * P1 && P1 <= 2 * P0) {
+      const double rel = fabs((f1 - f0) * (n1 - n0)) * cv / 4 / fmin(P0, P1);
+      atomicMax((unsigned int *)&out[na].x, __float_as_uint((float)rel * 1.0001f));
+      out[i] = make_float4((float)P0, (float)(P1 - P0), bz0, dbz);
+      return;
+    }
+  }
+  out[i] = make_float4(kBinOther, 0.f, 0.f, 0.f);
 }
 
 // The reference's FP64 count of cell (ix,iy,iz) (srcs.c:136-187) from the
@@ -107,7 +122,7 @@ __device__ __noinline__ int count_exact(int ix, int iy, int iz, float dl, uint64
 // FP32 bias_model with an absolute bound E on its error against the FP64
 // value at the same (delta, b): b carries the interpolation error of the
 // table (bmax = larger |b| at the bin ends), exp2f/__expf/__log2f theirs.
-__device__ inline float bias_fast(int model, float b, float bmax, float dl, float l2, float qd, float &E) {
+__device__ inline float bias_fast(int model, float b, float bmax, float dl, float &E) {
   if (dl <= -1.f) {
     E = 0.f;
     return 0.f;
@@ -115,9 +130,11 @@ __device__ inline float bias_fast(int model, float b, float bmax, float dl, floa
   if (model == 0 || (model == 1 && dl < 0.f)) {
     float v, s;
     if (model == 0) {
+      const float l2 = __log2f(1.f + dl);
       v = exp2f(b * l2);
       s = bmax * fabsf(l2) * 0.6931472f;
     } else {
+      const float qd = __fdividef(dl, 1.f + dl);
       v = __expf(b * qd);
       s = bmax * fabsf(qd);
     }
@@ -129,71 +146,39 @@ __device__ inline float bias_fast(int model, float b, float bmax, float dl, floa
   return model == 2 ? fmaxf(v, 0.f) : v;
 }
 
+// The count of one cell from nothing but its index, by the FP64 path (k_fill's
+// recount of cells whose byte saturated).
+__device__ __forceinline__ int count_cell(const SrcArgs &a, const PopTab &P, uint64_t cell) {
+  const int N = a.N;
+  const uint64_t rw = cell / (uint64_t)N;
+  const int cx = (int)(cell - rw * N);
+  const u32x4 r = draw(cell >> 3, stream_poisson(P.ipop), 0u, a.seed);
+  const uint32_t x = (poisson_hi16(r, cell) << 16) | poisson_lo16(cell, P.ipop, a.seed);
+  return count_exact(cx, (int)(rw % N), (int)(rw / N), a.g[rw * (size_t)a.ngx + cx], cell, x, a.r_arr, P.amp,
+                     P.bz, P.norm, a.idr, a.r_last, P.bz0, P.norm0, P.normf, a.dx, a.ox, a.oy, a.oz, a.cell_vol,
+                     a.half_box_20, P.model, P.ipop, a.seed);
+}
+
 // queue entry: cell within the block | hi << 16; lambda (< 0: FP64 path); bound
 struct QEntry {
   uint32_t ch;
   float lam, err;
 };
 
-__device__ inline void publish(unsigned long long *w, unsigned long long v, unsigned int state) {
-  __hip_atomic_store(w, (v << 2) | state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Number of sources in the blocks before `t` (state 1: the block's own
-// count, state 2: inclusive of all before it).  Called by one whole wave;
-// every lane returns the same value.
-__device__ long long look_back(unsigned long long *st, long long t, int *bad) {
-  const int lane = threadIdx.x & 63;
-  long long excl = 0;
-  long long j0 = t - 1;
-  while (j0 >= 0) {
-    const long long j = j0 - lane;
-    unsigned long long w = 0;
-    if (j >= 0) {
-      int spins = 0;
-      while (((w = __hip_atomic_load(st + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 3u) == 0u) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > kSpinLimit) {
-          atomicOr(bad, 2);
-          w = 2u;   // an inclusive zero: the wait ends, the host reports it
-          break;
-        }
-      }
-    }
-    // lanes up to the nearest inclusive word (or all 64) contribute
-    const unsigned long long incl = __ballot((j >= 0) && (w & 3u) == 2u);
-    const int stop = incl ? __ffsll((long long)incl) - 1 : 63;
-    long long v = (j >= 0 && lane <= stop) ? (long long)(w >> 2) : 0;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    excl += v;
-    if (incl) break;
-    j0 -= 64;
-  }
-  return excl;
-}
-
-__global__ __launch_bounds__(kLanes) void k_sources(SrcArgs a) {
+__global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
   __shared__ QEntry queue[kBlockCells];
   __shared__ int cnt[kBlockCells];
-  __shared__ int qn;
-  __shared__ unsigned int s_ticket;
-  __shared__ long long s_base;
-  __shared__ int s_sum[kLanes / 64];
   const int lane = threadIdx.x;
-  if (lane == 0) {
-    s_ticket = atomicAdd(a.ticket, 1u);
-    qn = 0;
-  }
-  __syncthreads();
-  const long long t = s_ticket;
-  const size_t cell0 = (size_t)t * kBlockCells + (size_t)kCellsPerLane * lane;
+  const size_t blk = blockIdx.x;
+  const size_t cell0 = blk * kBlockCells + (size_t)kCellsPerLane * lane;
   const int N = a.N;
   const size_t row = cell0 / (size_t)N;
   const int ix0 = (int)(cell0 - row * N);
   const int iy = (int)(row % N), iz = (int)(row / N);
+  const unsigned long long below = (1ull << lane) - 1ull;
 
   // --- the lane's 8 cells: density and geometry --------------------------
-  float d[kCellsPerLane], tt[kCellsPerLane], l2[kCellsPerLane], qd[kCellsPerLane];
+  float d[kCellsPerLane], tt[kCellsPerLane];
   int ir[kCellsPerLane];
   unsigned fastm = 0, inm = 0;
   {
@@ -219,22 +204,26 @@ __global__ __launch_bounds__(kLanes) void k_sources(SrcArgs a) {
       const double xr = r * a.idr;
       const int i = (int)xr;
       const double fr = xr - i;
-      ir[c] = i;
-      tt[c] = (float)fr;
       const bool fast = r2 > 0 && fr > 1e-9 && fr < 1 - 1e-9 && i < a.na - 2 &&
                         fabs(r - a.half_box_20) > 1e-9 * a.half_box_20;
+      ir[c] = fast ? i : 0;   // a valid bin for every cell: the gathers below need no branch
+      tt[c] = (float)fr;
       fastm |= (fast ? 1u : 0u) << c;
       inm |= (r < a.half_box_20 ? 1u : 0u) << c;
-      const float dl = d[c];
-      l2[c] = dl > -1.f ? __log2f(1.f + dl) : 0.f;
-      qd[c] = dl > -1.f ? __fdividef(dl, 1.f + dl) : 0.f;
     }
   }
 
   for (int ip = 0; ip < a.npop; ip++) {
     const PopTab &P = a.pop[ip];
-    // --- A: decide from the high 16 bits ---------------------------------
+#pragma unroll
+    for (int c = 0; c < kCellsPerLane; c++) cnt[kCellsPerLane * lane + c] = 0;
+    // all 8 bins in flight at once
+    float4 T[kCellsPerLane];
+#pragma unroll
+    for (int c = 0; c < kCellsPerLane; c++) T[c] = P.tab[ir[c]];
     const u32x4 rr = draw(cell0 >> 3, stream_poisson(P.ipop), 0u, a.seed);
+    const float relerr = P.tab[a.na].x;   // product-interpolation bound of the table
+    int qn = 0;   // uniform
 #pragma unroll
     for (int c = 0; c < kCellsPerLane; c++) {
       const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
@@ -243,40 +232,35 @@ __global__ __launch_bounds__(kLanes) void k_sources(SrcArgs a) {
       if ((fastm >> c) & 1u) {
         if (!((inm >> c) & 1u)) {
           queue_it = false;
-        } else {
-          const float4 T0 = P.tab[ir[c]], T1 = P.tab[ir[c] + 1];
-          const int fl = __float_as_int(T0.w);
-          if (fl & (kFlagNonPos | kFlagNan)) {
-            queue_it = false;
-          } else if (fl & kFlagPos) {
-            const float tc = tt[c];
-            const float nd = T0.x + (T1.x - T0.x) * tc;
-            const float b = T0.y + (T1.y - T0.y) * tc;
-            const float nm = T0.z + (T1.z - T0.z) * tc;
-            float E;
-            const float bias = bias_fast(P.model, b, fmaxf(fabsf(T0.y), fabsf(T1.y)), d[c], l2[c], qd[c], E);
-            const float nn = nd * nm;
-            lam = nn * bias;
-            err = 2e-6f * lam + nn * E;
-            if (poisson_zero_from_hi(lam, __expf(-lam), err, hi)) queue_it = false;
-          }
+        } else if (T[c].x >= 0.f) {
+          const float tc = tt[c];
+          const float nn = T[c].x + T[c].y * tc;
+          const float b = T[c].z + T[c].w * tc;
+          float E;
+          const float bias = bias_fast(P.model, b, fmaxf(fabsf(T[c].z), fabsf(T[c].z + T[c].w)), d[c], E);
+          lam = nn * bias;
+          err = lam * (2e-6f + relerr) + nn * E;
+          if (poisson_zero_from_hi(lam, __expf(-lam), err, hi)) queue_it = false;
+        } else if (T[c].x == kBinZero) {
+          queue_it = false;
+        } else if (T[c].x == kBinTiny) {
+          float E;
+          const float b = T[c].z + T[c].w * tt[c];
+          const float bias = bias_fast(P.model, b, fmaxf(fabsf(T[c].z), fabsf(T[c].z + T[c].w)), d[c], E);
+          if (bias + E < 1e9f) queue_it = false;
         }
       }
-      if (queue_it) {
-        const int slot = atomicAdd(&qn, 1);
-        queue[slot] = QEntry{(uint32_t)(kCellsPerLane * lane + c) | (hi << 16), lam, err};
-      }
+      const unsigned long long m = __ballot(queue_it);
+      if (queue_it)
+        queue[qn + __popcll(m & below)] = QEntry{(uint32_t)(kCellsPerLane * lane + c) | (hi << 16), lam, err};
+      qn += __popcll(m);
     }
-#pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) cnt[kCellsPerLane * lane + c] = 0;
-    __syncthreads();
-    // --- B: the queue, all lanes busy --------------------------------------
-    const int nq = qn;
-    for (int i = lane; i < nq; i += kLanes) {
+    __syncthreads();   // one wavefront: orders the LDS writes above before the reads below
+    for (int i = lane; i < qn; i += kLanes) {
       const QEntry e = queue[i];
       const int cb = (int)(e.ch & 0xFFFFu);
       const uint32_t hi = e.ch >> 16;
-      const size_t cell = (size_t)t * kBlockCells + cb;
+      const size_t cell = blk * kBlockCells + cb;
       const uint32_t x = (hi << 16) | poisson_lo16(cell, P.ipop, a.seed);
       int kk = -1;
       if (e.lam >= 0.f) kk = poisson_fast_f32(e.lam, __expf(-e.lam), e.err, x);
@@ -290,55 +274,108 @@ __global__ __launch_bounds__(kLanes) void k_sources(SrcArgs a) {
       cnt[cb] = kk;
     }
     __syncthreads();
-    // --- C: scan of the block's counts -------------------------------------
-    int k[kCellsPerLane];
-    int mine = 0;
+    // bytes of the lane's 8 cells, the block total
+    uint32_t lo = 0, hi8 = 0;
+    long long mine = 0;
 #pragma unroll
     for (int c = 0; c < kCellsPerLane; c++) {
-      k[c] = cnt[kCellsPerLane * lane + c];
+      const int k = cnt[kCellsPerLane * lane + c];
+      mine += k;
+      const uint32_t b = (uint32_t)(k < kByteEscape ? k : kByteEscape);
+      if (c < 4) lo |= b << (8 * c);
+      else hi8 |= b << (8 * (c - 4));
+    }
+    *(uint2 *)(P.cnt8 + cell0) = make_uint2(lo, hi8);
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+    if (lane == 0) P.btot[blk] = (uint32_t)mine;
+  }
+}
+
+// exclusive scan of n uint32 -> int64 offsets (out[n] = total).  Three
+// kernels: per-block sums, scan of the block sums (one block), block-local scan.
+constexpr int kScanB = 1024;
+__global__ __launch_bounds__(kScanB) void k_scan_blocks(const uint32_t *__restrict__ in, size_t n,
+                                                        long long *__restrict__ bsum) {
+  __shared__ long long s[kScanB];
+  const size_t i = (size_t)blockIdx.x * kScanB + threadIdx.x;
+  s[threadIdx.x] = i < n ? in[i] : 0;
+  __syncthreads();
+  for (int w = kScanB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+__global__ __launch_bounds__(kScanB) void k_scan_top(long long *__restrict__ bsum, size_t nb) {
+  __shared__ long long s[kScanB];
+  __shared__ long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (size_t base = 0; base < nb; base += kScanB) {
+    const size_t i = base + threadIdx.x;
+    const long long v = i < nb ? bsum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanB; off <<= 1) {
+      const long long u = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += u;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == kScanB - 1) carry += s[kScanB - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[nb] = carry;
+}
+__global__ __launch_bounds__(kScanB) void k_scan_apply(const uint32_t *__restrict__ in, size_t n,
+                                                       const long long *__restrict__ bsum,
+                                                       long long *__restrict__ out) {
+  __shared__ long long s[kScanB];
+  const size_t i = (size_t)blockIdx.x * kScanB + threadIdx.x;
+  const long long v = i < n ? in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanB; off <<= 1) {
+    const long long u = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += u;
+    __syncthreads();
+  }
+  if (i < n) out[i] = bsum[blockIdx.x] + s[threadIdx.x] - v;
+  if (i == n - 1) out[n] = bsum[gridDim.x];
+}
+
+__global__ __launch_bounds__(kLanes) void k_fill(SrcArgs a) {
+  const int lane = threadIdx.x;
+  const size_t blk = blockIdx.x;
+  const uint64_t cell0 = (uint64_t)blk * kBlockCells + (uint64_t)kCellsPerLane * lane;
+  for (int ip = 0; ip < a.npop; ip++) {
+    const PopTab &P = a.pop[ip];
+    const long long b0 = P.boff[blk];
+    if (P.boff[blk + 1] == b0) continue;   // no sources in this block (uniform)
+    const uint2 w = *(const uint2 *)(P.cnt8 + cell0);
+    int k[kCellsPerLane];
+    long long mine = 0;
+#pragma unroll
+    for (int c = 0; c < kCellsPerLane; c++) {
+      k[c] = (int)(((c < 4 ? w.x : w.y) >> (8 * (c & 3))) & 0xFFu);
+      if (k[c] == kByteEscape) k[c] = count_cell(a, P, cell0 + c);
       mine += k[c];
     }
-    int incl = mine;
+    long long incl = mine;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(incl, off);
-      if ((lane & 63) >= off) incl += y;
+      const long long y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
     }
-    if ((lane & 63) == 63) s_sum[lane >> 6] = incl;
-    if (lane == 0) qn = 0;
-    __syncthreads();
-    int wbase = 0, total = 0;
+    long long pid = b0 + incl - mine;
 #pragma unroll
-    for (int w = 0; w < kLanes / 64; w++) {
-      if (w < (lane >> 6)) wbase += s_sum[w];
-      total += s_sum[w];
-    }
-    // --- D: global offset (wave 0) -----------------------------------------
-    if (lane < 64) {
-      unsigned long long *st = P.status;
-      long long base = 0;
-      if (t == 0) {
-        if (lane == 0) publish(st, (unsigned long long)total, 2u);
-      } else {
-        if (lane == 0) publish(st + t, (unsigned long long)total, 1u);
-        base = look_back(st, t, a.bad);
-        if (lane == 0) publish(st + t, (unsigned long long)(base + total), 2u);
-      }
-      if (lane == 0) {
-        s_base = base;
-        if (t == a.nblocks - 1) a.totals[ip] = (unsigned long long)(base + total);
-      }
-    }
-    __syncthreads();
-    // --- E: descriptors ------------------------------------------------------
-    if (total > 0) {
-      long long pid = s_base + wbase + incl - mine;
-#pragma unroll
-      for (int c = 0; c < kCellsPerLane; c++) {
-        const uint64_t cell = (uint64_t)cell0 + c;
-        for (int is = 0; is < k[c]; is++, pid++)
-          if (pid < P.cap) P.desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
-      }
+    for (int c = 0; c < kCellsPerLane; c++) {
+      const uint64_t cell = cell0 + c;
+      for (int is = 0; is < k[c]; is++, pid++)
+        P.desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
     }
   }
 }
@@ -483,22 +520,19 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   // packed FP32 tables, rebuilt from the current norm tables
   for (int i = p0; i < p1; i++) {
     TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
-    if (!T.ptab) CLR_HIP(c, hipMalloc((void **)&T.ptab, (size_t)na * sizeof(float4)));
+    if (!T.ptab) CLR_HIP(c, hipMalloc((void **)&T.ptab, (size_t)(na + 1) * sizeof(float4)));
+    CLR_HIP(c, hipMemsetAsync((float4 *)T.ptab + na, 0, sizeof(float4), c->stream));
     hipLaunchKernelGGL(k_pop_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, T.amp, T.bz, T.norm, cell_vol,
                        na, (float4 *)T.ptab);
     CLR_HIP(c, hipGetLastError());
   }
   const int np = p1 - p0;
-  const long long nblocks = (long long)(ncells / kBlockCells);
-  // scratch: status words (np x nblocks), totals, ticket, error flag
-  const size_t status_bytes = (size_t)np * nblocks * sizeof(unsigned long long);
-  const size_t scratch = status_bytes + CLR_NPOP_MAX * sizeof(unsigned long long) + 16;
-  rc = clr_ensure(c, (void **)&c->rowcnt, &c->rows_n, scratch);
+  const size_t nblocks = ncells / kBlockCells;
+  const size_t nsb = (nblocks + kScanB - 1) / kScanB;
+  // scratch per population: bytes (ncells), block totals, offsets, scan partials
+  const size_t per_pop = ncells + nblocks * 4 + (nblocks + 1) * 8 + (nsb + 1) * 8 + 64;
+  rc = clr_ensure(c, (void **)&c->rowcnt, &c->rows_n, (size_t)np * per_pop);
   if (rc) return rc;
-  unsigned long long *status = (unsigned long long *)c->rowcnt;
-  unsigned long long *totals = status + (size_t)np * nblocks;
-  unsigned int *ticket = (unsigned int *)(totals + CLR_NPOP_MAX);
-  int *bad = (int *)(ticket + 1);
 
   SrcArgs a{};
   a.g = c->dens;
@@ -515,14 +549,17 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   a.ngx = c->ngx;
   a.na = na;
   a.npop = np;
-  a.nblocks = nblocks;
   a.seed = seed;
-  a.ticket = ticket;
-  a.totals = totals;
-  a.bad = bad;
+  std::vector<long long *> bsum(np);
   for (int i = p0; i < p1; i++) {
     const TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
     PopTab &P = a.pop[i - p0];
+    uint8_t *base = (uint8_t *)c->rowcnt + (size_t)(i - p0) * per_pop;
+    P.cnt8 = base;
+    P.btot = (uint32_t *)(base + ncells);
+    long long *off = (long long *)(((uintptr_t)(P.btot + nblocks) + 7) & ~(uintptr_t)7);
+    P.boff = off;
+    bsum[i - p0] = off + nblocks + 1;
     P.amp = T.amp;
     P.bz = T.bz;
     P.norm = T.norm;
@@ -532,63 +569,60 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     P.normf = T.normf;
     P.model = T.model;
     P.ipop = i;
-    P.status = status + (size_t)(i - p0) * nblocks;
     if (!P.amp || !P.bz || !P.norm || !P.tab) return clr_fail(c, CLR_ERR_STATE, "population %d tables missing", i);
   }
   if (!c->r2n) return clr_fail(c, CLR_ERR_STATE, "no RING->NEST table");
   if (!a.g || !a.r_arr) return clr_fail(c, CLR_ERR_STATE, "density grid or background tables missing");
 
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  std::vector<unsigned long long> tot(CLR_NPOP_MAX, 0ull);
-  for (int pass = 0; pass < 2; pass++) {
-    for (int i = 0; i < np; i++) {
-      CatalogDev &C = c->cat[p0 + i];
-      a.pop[i].desc = (uint4 *)C.pos;
-      a.pop[i].cap = C.pos ? C.cap : 0;
-    }
-    CLR_HIP(c, hipMemsetAsync(status, 0, scratch, c->stream));
-    CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
-    hipLaunchKernelGGL(k_sources, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
+  CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
+  hipLaunchKernelGGL(k_count, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
+  CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
+  std::vector<long long> tot(np);
+  for (int i = 0; i < np; i++) {
+    const PopTab &P = a.pop[i];
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i]);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanB), 0, c->stream, bsum[i], nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i],
+                       (long long *)P.boff);
     CLR_HIP(c, hipGetLastError());
-    CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
-    int hbad = 0;
-    CLR_HIP(c, hipMemcpyAsync(tot.data(), totals, CLR_NPOP_MAX * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              c->stream));
-    CLR_HIP(c, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    CLR_HIP(c, hipStreamSynchronize(c->stream));
-    if (hbad) return clr_fail(c, CLR_ERR_STATE, "source pass: look-back did not complete (code %d)", hbad);
-    bool grown = false;
-    for (int i = 0; i < np; i++) {
-      CatalogDev &C = c->cat[p0 + i];
-      const long long n = (long long)tot[i];
-      if (n > C.cap) {
-        if (pass == 1) return clr_fail(c, CLR_ERR_STATE, "catalogue %d still too small", p0 + i);
-        if (C.pos) (void)hipFree(C.pos);
-        if (C.ipix) (void)hipFree(C.ipix);
-        C.pos = nullptr;
-        C.ipix = nullptr;
-        C.cap = 0;
-        const long long cap = n + n / 16 + 1024;
-        if (hipMalloc((void **)&C.pos, (size_t)cap * 4 * sizeof(float)) != hipSuccess ||
-            hipMalloc((void **)&C.ipix, (size_t)cap * sizeof(int32_t)) != hipSuccess) {
-          (void)hipGetLastError();
-          return clr_fail(c, CLR_ERR_OOM, "cannot allocate catalogue of %lld sources", n);
-        }
-        C.cap = cap;
-        grown = true;
-      }
-    }
-    if (!grown) break;
+    CLR_HIP(c, hipMemcpyAsync(&tot[i], P.boff + nblocks, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
   }
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
   c->stage_ms[ST_SRC_PASS] = ms;
-  CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
   long long total = 0;
   for (int i = 0; i < np; i++) {
     CatalogDev &C = c->cat[p0 + i];
-    C.n = (long long)tot[i];
-    total += C.n;
+    const long long n = tot[i];
+    if (n > C.cap) {
+      if (C.pos) (void)hipFree(C.pos);
+      if (C.ipix) (void)hipFree(C.ipix);
+      C.pos = nullptr;
+      C.ipix = nullptr;
+      C.cap = 0;
+      const long long cap = n + n / 16 + 1024;
+      if (hipMalloc((void **)&C.pos, (size_t)cap * 4 * sizeof(float)) != hipSuccess ||
+          hipMalloc((void **)&C.ipix, (size_t)cap * sizeof(int32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        return clr_fail(c, CLR_ERR_OOM, "cannot allocate catalogue of %lld sources", n);
+      }
+      C.cap = cap;
+    }
+    C.n = n;
+    a.pop[i].desc = (uint4 *)C.pos;
+    total += n;
+    if (n > 0 && !C.pos) return clr_fail(c, CLR_ERR_STATE, "no catalogue buffer");
+  }
+  CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
+  if (total > 0) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
+    CLR_HIP(c, hipGetLastError());
+  }
+  for (int i = 0; i < np; i++) {
+    CatalogDev &C = c->cat[p0 + i];
     if (C.n == 0) continue;
     WriteArgs w{};
     w.phi = c->npot;

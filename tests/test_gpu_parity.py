@@ -118,11 +118,10 @@ def test_shell_norms(workdir, N):
         assert n0 == on0 and nf == onf
 
 
-@pytest.mark.parametrize("N,ipop", [(64, 0), (128, 0), (128, 1), (128, 2)])
-def test_sources_bit_exact(workdir, N, ipop):
-    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+def _sources_vs_oracle(run, N, ipop, seed):
+    """Samples population ipop on the device and checks the catalogue against
+    the oracle's; returns (context, oracle, positions, oracle per-cell counts)."""
     ctx = _ctx_for(run, N)
-    seed = 3003
     s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
     ntr = int(run.scalar("n_srcs"))
     ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=ntr)
@@ -140,7 +139,30 @@ def test_sources_bit_exact(workdir, N, ipop):
     assert np.array_equal(pos[:, :3], opos[:, :3])
     assert np.array_equal(ipix, oipix)
     np.testing.assert_allclose(pos[:, 3], opos[:, 3], rtol=1e-6, atol=1e-12)
-    props = ctx.source_properties(ipop, n)
+    return ctx, orc, pos, counts
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_sources_bit_exact_large_lambda(workdir, N):
+    """One population at 1e-3 (Mpc/h)^-3 on coarse grids: lambda per cell
+    reaches the hundreds, past the FP32 search (lambda >= 64) and, at N=32,
+    past 500 where the count is a sum of inversions."""
+    path, run = make_run("C1", N, workdir, with_maps=False)
+    counts = _sources_vs_oracle(run, N, 0, 1001)[3]
+    assert counts.max() > (500 if N == 32 else 64), counts.max()
+
+
+def test_sources_bit_exact_192(workdir):
+    """A 3*2^n grid: 1024-cell blocks span several rows."""
+    path, run = make_run("C3", 192, workdir, z_max=1.6, with_maps=False)
+    _sources_vs_oracle(run, 192, 1, 3003)
+
+
+@pytest.mark.parametrize("N,ipop", [(64, 0), (128, 0), (128, 1), (128, 2)])
+def test_sources_bit_exact(workdir, N, ipop):
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx, orc, pos, _ = _sources_vs_oracle(run, N, ipop, 3003)
+    props = ctx.source_properties(ipop, len(pos))
     oprops = orc.source_props(pos)
     np.testing.assert_allclose(props, oprops, rtol=1e-6, atol=1e-5)
 
