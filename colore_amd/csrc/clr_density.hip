@@ -26,7 +26,7 @@ using namespace clr;
 
 namespace {
 
-constexpr int kShellBlocks = 2048;
+constexpr int kShellBlocks = 8192;
 constexpr int kShellThreads = 256;
 constexpr int kWaves = kShellThreads / 64;
 constexpr int kTY = 4, kTZ = 16;   // tile rows: kTY x kTZ rows of 64 cells
@@ -38,21 +38,26 @@ struct ShellArgs {
   int N, ngx, nz, ntr, fused, na;
   double dx, ox, oy, oz, idr, r_last, z_last, sigma2, idz;
   const double *r_arr, *zt;
-  const float *d1f;
+  const float4 *tab;   // per bin: D, b_0 .. b_{NT-1} in (NT+4)/4 float4s (k_pack_shell_table)
   float d1_last_f, idr_f;
-  const float *bzf[kMaxTr];
   float bz0[kMaxTr];
   int model[kMaxTr];
 };
 
-// f_of_r on an FP32 copy of a table (values that feed sums, not decisions)
-__device__ inline float f_of_r32(const float *__restrict__ f, float idr, int nm1, float r, float f0, float ff) {
-  if (r <= 0.f) return f0;
-  const float x = r * idr;
-  if (x >= (float)nm1) return ff;
-  const int ir = (int)x;
-  const float t = x - (float)ir;
-  return f[ir] + (f[ir + 1] - f[ir]) * t;
+// FP32 tables of one shell pass packed per bin: component 0 is D(r), component
+// 1 + t is b_t(r); w float4s per bin.
+struct ShellTabSrc {
+  const float *d1f;
+  const float *bzf[kMaxTr];
+};
+__global__ void k_pack_shell_table(ShellTabSrc src, int ntr, int na, int w, float4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  float v[4 * 3];
+  for (int j = 0; j < 4 * w; j++) v[j] = 0.f;
+  v[0] = src.d1f[i];
+  for (int t = 0; t < ntr; t++) v[1 + t] = src.bzf[t][i];
+  for (int k = 0; k < w; k++) out[(size_t)i * w + k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
 
 template <int CTRL, int RM>
@@ -124,6 +129,7 @@ template <int NT>
 __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *__restrict__ partial) {
 #pragma clang fp contract(off)
   constexpr int NQ = 2 + NT;
+  constexpr int W = (NT + 4) / 4;
   extern __shared__ double acc[];   // [kWaves][nz][NQ]
   const int nz = a.nz, N = a.N;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -152,22 +158,48 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
 #pragma unroll
       for (int q = 0; q < NQ; q++) sacc[k][q] = 0;
     }
+    // the next row's density is loaded while the current one is processed
+    float dnext = valid ? a.g[((size_t)(tz * kTZ) * N + ty * kTY) * a.ngx + ix] : 0.f;
     for (int rr = 0; rr < kTY * kTZ; rr++) {
       const int iy = ty * kTY + (rr % kTY), iz = tz * kTZ + (rr / kTY);
       int s = -1;
       double v[NQ];
+      float dl = dnext;
+      if (valid && rr + 1 < kTY * kTZ) {
+        const int iy2 = ty * kTY + ((rr + 1) % kTY), iz2 = tz * kTZ + ((rr + 1) / kTY);
+        dnext = a.g[((size_t)iz2 * N + iy2) * a.ngx + ix];
+      }
       if (valid) {
         double x0, y0, z0;
         const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
         const double zz = f_of_r(a.r_arr, a.zt, a.idr, a.r_last, r, 0.0, a.z_last);
-        float *pc = a.g + ((size_t)iz * N + iy) * a.ngx + ix;
-        float dl = *pc;
-        const float rf = (float)r;
+        // FP32 tables: one bin lookup for D and every b_t (f_of_r32 semantics)
+        float comp[4 * W];
+        {
+          const float rf = (float)r;
+          const float xq = rf * a.idr_f;
+          const bool e0 = !(rf > 0.f), e1 = xq >= (float)(a.na - 1);
+          const int i0 = (e0 || e1) ? 0 : (int)xq;
+          const float tq = xq - (float)i0;
+#pragma unroll
+          for (int k = 0; k < W; k++) {
+            const float4 A = a.tab[(size_t)i0 * W + k], B = a.tab[(size_t)(i0 + 1) * W + k];
+            comp[4 * k + 0] = A.x + (B.x - A.x) * tq;
+            comp[4 * k + 1] = A.y + (B.y - A.y) * tq;
+            comp[4 * k + 2] = A.z + (B.z - A.z) * tq;
+            comp[4 * k + 3] = A.w + (B.w - A.w) * tq;
+          }
+          if (e0 || e1) {
+            comp[0] = e0 ? 1.f : a.d1_last_f;
+#pragma unroll
+            for (int t = 0; t < NT; t++) comp[1 + t] = e0 ? a.bz0[t] : 1.f;
+          }
+        }
         if (a.fused) {
-          const float dg = f_of_r32(a.d1f, a.idr_f, a.na - 1, rf, 1.f, a.d1_last_f);
+          const float dg = comp[0];
           const float arg = dg * (dl - 0.5f * dg * (float)a.sigma2);
           dl = expf(arg) - 1.f;
-          *pc = dl;
+          a.g[((size_t)iz * N + iy) * a.ngx + ix] = dl;
         }
         const int ind = (int)(zz * a.idz) + 1;
         if (ind >= 0 && ind < nz) {
@@ -175,10 +207,7 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
           v[0] = 1.0;
           v[1] = zz;
 #pragma unroll
-          for (int t = 0; t < NT; t++) {
-            const float b = f_of_r32(a.bzf[t], a.idr_f, a.na - 1, rf, a.bz0[t], 1.f);
-            v[2 + t] = (double)bias_model_f32(dl, b, a.model[t]);
-          }
+          for (int t = 0; t < NT; t++) v[2 + t] = (double)bias_model_f32(dl, comp[1 + t], a.model[t]);
         }
       }
       const bool change = s >= 0 && s != cur;
@@ -360,13 +389,14 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   const double idz = (nz - 2) / zmax;
 
   ShellArgs a{};
+  ShellTabSrc tsrc{};
   int tk[3 * CLR_NPOP_MAX], tp[3 * CLR_NPOP_MAX];
   int ntr = 0;
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < c->n_tr[k]; i++) {
       if (!c->tr[k][i].set) return clr_fail(c, CLR_ERR_STATE, "tracer %d/%d has no tables", k, i);
       if (ntr >= kMaxTr) return clr_fail(c, CLR_ERR_ARG, "at most %d tracers per normalisation pass", kMaxTr);
-      a.bzf[ntr] = c->tr[k][i].bzf;
+      tsrc.bzf[ntr] = c->tr[k][i].bzf;
       a.bz0[ntr] = (float)c->tr[k][i].bz0;
       a.model[ntr] = c->tr[k][i].model;
       tk[ntr] = k;
@@ -374,7 +404,7 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
       ntr++;
     }
   for (int t = 0; t < ntr; t++)
-    if (!a.bzf[t] || !c->tr[tk[t]][tp[t]].norm) return clr_fail(c, CLR_ERR_STATE, "tracer tables missing");
+    if (!tsrc.bzf[t] || !c->tr[tk[t]][tp[t]].norm) return clr_fail(c, CLR_ERR_STATE, "tracer tables missing");
   if (!c->d1f || !c->bg) return clr_fail(c, CLR_ERR_STATE, "background tables missing");
   a.g = c->dens;
   a.N = N;
@@ -394,7 +424,7 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   a.idz = idz;
   a.r_arr = r;
   a.zt = zt;
-  a.d1f = c->d1f;
+  tsrc.d1f = c->d1f;
   a.d1_last_f = (float)c->h_bg_last[2];
   a.idr_f = (float)c->glob_idr;
   const int nq = 2 + ntr;
@@ -407,8 +437,15 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
   if (rc) return rc;
   double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals, *work = fin + nvals;
+  const int w = (ntr + 4) / 4;
+  rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
+  if (rc) return rc;
+  a.tab = (const float4 *)c->misc;
 
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, ntr, na, w,
+                     (float4 *)c->misc);
+  CLR_HIP(c, hipGetLastError());
   kShellFns[ntr](dim3(kShellBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
   CLR_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,

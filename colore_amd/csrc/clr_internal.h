@@ -30,8 +30,8 @@ struct CatalogDev {
 
 enum StageId {
   ST_GEN = 0, ST_FFT, ST_XPASS, ST_DENS, ST_NORM, ST_SRCS, ST_KAPPA, ST_ISW, ST_IMAP,
-  ST_SRC_PASS,    // the last k_sources launch of a sample_sources call
-  ST_SRC_WRITE,   // its k_write launches
+  ST_SRC_PASS,    // k_count of a sample_sources call
+  ST_SRC_WRITE,   // its k_fill and k_write launches
   ST_COUNT
 };
 

@@ -186,9 +186,10 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
 
 /* Device time in milliseconds of the last call of each stage (0 if never),
  * in this order: mode generation + z pass, y pass, x pass, density
- * transform, shell sums, sources (whole call), kappa, ISW, imap, the last
- * counting pass of the sources call, its per-source writer.  Fills at most
- * n values; returns the number of stages. */
+ * transform, shell sums, sources (whole call), kappa, ISW, imap, the
+ * counting pass of the sources call, its catalogue writers (descriptors and
+ * per-source positions).  Fills at most n values; returns the number of
+ * stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 
 #ifdef __cplusplus
