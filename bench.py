@@ -61,16 +61,18 @@ KERNEL_STAGE = {"k_gen_z": "gen", "k_col_y": "fft_yz", "k_c2r_x": "fft_x", "k_sh
                 "k_count": "src_pass", "k_fill+k_write": "src_write"}
 
 
-def traffic_per_launch(kernel):
-    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE (scaled by the same counters on a plain copy of known size),
-    read from profiles/<kernel>_traffic.txt; None when no such summary exists."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.txt")
+def traffic_per_launch(kernel, config, N):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE
+    and WRITE_SIZE (each scaled by the same counter on a plain 1 GiB device
+    copy), from profiles/hbm_traffic_<N>_<config>.txt; None when that
+    workload has no such summary."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{N}_{config}.txt")
     if not os.path.exists(path):
         return None
+    key = f"hbm_bytes_per_launch {kernel}:"
     with open(path) as f:
         for line in f:
-            if line.startswith("hbm_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     return None
 
@@ -202,7 +204,7 @@ def main():
                           "frac": value * b_alg / HBM_PEAK},
         "stage_ms": {k: v / args.steps for k, v in acc.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic_per_launch(dom),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic_per_launch(dom, args.config, N),
                      "bytes_per_launch": kb[dom], "ms_per_launch": kt[dom]},
         "kernels": {k: {"ms": kt[k], "GB/s": kb[k] / (kt[k] * 1e-3) / 1e9 if kt[k] > 0 else None}
                     for k in kt},
