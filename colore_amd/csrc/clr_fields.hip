@@ -71,6 +71,18 @@ __device__ inline float modulus_draw(uint32_t x) {
   return sqrtf(-l);
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has its
+// own L2.  A tile of T columns covers T*8 bytes of a row of the half
+// spectrum, which is not aligned to 128-byte lines, so neighbouring tiles
+// share lines: giving every XCD whole planes (all tiles of a plane, one after
+// the other) makes the shared lines meet in one L2 instead of being fetched
+// and written back by two.  nplanes must be a multiple of 8.
+__device__ inline void xcd_tile(unsigned b, unsigned ntile, unsigned nplanes, int &tile, int &plane) {
+  const unsigned xcd = b % 8u, k = b / 8u;
+  plane = (int)(xcd * (nplanes / 8u) + k / ntile);
+  tile = (int)(k % ntile);
+}
+
 // ---------------------------------------------------------------------------
 // fused mode generation + z pass
 // ---------------------------------------------------------------------------
@@ -81,10 +93,12 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
                                                  const uint32_t *__restrict__ inj) {
   using RL = RadixList<Rs...>;
   static_assert(RL::product == N, "radices must multiply to N");
-  constexpr int NH = N / 2 + 1, E = N / P, R0 = RL::first, RZ = RL::last;
+  constexpr int NH = N / 2 + 1, E = N / P, R0 = RL::first, RZ = RL::last, NTILE = (NH + T - 1) / T;
   __shared__ float2 lds[N * T];
   const int c = threadIdx.x % T, t = threadIdx.x / T;
-  const int kx = blockIdx.x * T + c, ky = blockIdx.y;
+  int tile, ky;
+  xcd_tile(blockIdx.x, NTILE, N, tile, ky);
+  const int kx = tile * T + c;
   const bool valid = kx < NH;
   const int fy = (2 * ky <= N) ? ky : N - ky;
   static_assert(E % 2 == 0, "modes are drawn in (kz, kz+N/2) pairs");
@@ -154,10 +168,12 @@ template <int N, int P, int T, int... Rs>
 __global__ __launch_bounds__(P *T) void k_col_y(float2 *__restrict__ g0, float2 *__restrict__ g1,
                                                  const float2 *__restrict__ tw) {
   using RL = RadixList<Rs...>;
-  constexpr int NH = N / 2 + 1, E = N / P, R0 = RL::first, RZ = RL::last;
+  constexpr int NH = N / 2 + 1, E = N / P, R0 = RL::first, RZ = RL::last, NTILE = (NH + T - 1) / T;
   __shared__ float2 lds[N * T];
   const int c = threadIdx.x % T, t = threadIdx.x / T;
-  const int kx = blockIdx.x * T + c, z = blockIdx.y;
+  int tile, z;
+  xcd_tile(blockIdx.x, NTILE, N, tile, z);
+  const int kx = tile * T + c;
   const bool valid = kx < NH;
   float2 *base = (blockIdx.z == 0 ? g0 : g1) + (size_t)z * N * NH + kx;
   float2 v[E];
@@ -361,21 +377,18 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   float2 *d = (float2 *)c->dens, *p = (float2 *)c->npot;
   const int ntile = (NH + L.colT - 1) / L.colT;
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  L.gen(dim3(ntile, N), dim3(L.colThreads), c->stream, d, p, c->amp, c->tw, seed, inj);
+  L.gen(dim3((unsigned)(ntile * N)), dim3(L.colThreads), c->stream, d, p, c->amp, c->tw, seed, inj);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
-  L.coly(dim3(ntile, N, 2), dim3(L.colThreads), c->stream, d, p, c->tw);
+  L.coly(dim3((unsigned)(ntile * N), 1, 2), dim3(L.colThreads), c->stream, d, p, c->tw);
   CLR_HIP(c, hipGetLastError());
-  hipEvent_t e2;
-  CLR_HIP(c, hipEventCreate(&e2));
+  hipEvent_t e2 = c->ev2, e3 = c->ev3;
   CLR_HIP(c, hipEventRecord(e2, c->stream));
   const float norm = (float)pow(sqrt(2 * M_PI) / c->L, 3);
   L.c2r(dim3((unsigned)nxb, 2), dim3(L.xThreads), c->stream, c->dens, c->npot, norm, c->red, nrows, c->tw);
   CLR_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, c->stream, c->red, nxb, c->red_out);
   CLR_HIP(c, hipGetLastError());
-  hipEvent_t e3;
-  CLR_HIP(c, hipEventCreate(&e3));
   CLR_HIP(c, hipEventRecord(e3, c->stream));
   double s[2];
   CLR_HIP(c, hipMemcpyAsync(s, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -384,8 +397,6 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   hipEventElapsedTime(&ms, c->ev0, c->ev1);  c->stage_ms[ST_GEN] = ms;
   hipEventElapsedTime(&ms, c->ev1, e2);      c->stage_ms[ST_FFT] = ms;
   hipEventElapsedTime(&ms, e2, e3);          c->stage_ms[ST_XPASS] = ms;
-  hipEventDestroy(e2);
-  hipEventDestroy(e3);
 
   const double ntot = (double)N * (double)N * (double)N;
   const double mean = s[0] / ntot;
