@@ -119,31 +119,62 @@ __device__ __noinline__ int count_exact(int ix, int iy, int iz, float dl, uint64
   return poisson_exact(lam, cell, stream_poisson_extra(ipop), seed, x);
 }
 
-// FP32 bias_model with an absolute bound E on its error against the FP64
-// value at the same (delta, b): b carries the interpolation error of the
-// table (bmax = larger |b| at the bin ends), exp2f/__expf/__log2f theirs.
-__device__ inline float bias_fast(int model, float b, float bmax, float dl, float &E) {
-  if (dl <= -1.f) {
-    E = 0.f;
-    return 0.f;
+// FP32 bias_model (common.h:414-431) for a bias model known at compile time,
+// branch-free, with an absolute bound E on its error against the FP64 value
+// at the same (delta, b): b carries the interpolation error of the table
+// (bmax = larger |b| at the bin ends), the raw hardware exp2/log2 (about 1
+// ulp; results below FP32's normal range flush to 0, which only lowers lambda
+// where k = 0 is certain anyway) and __fdividef theirs.
+template <int MODEL>
+__device__ inline float bias_fast_t(float b, float bmax, float dl, float &E) {
+  float v, e;
+  if (MODEL == 0) {
+    const float l2 = __builtin_amdgcn_logf(1.f + dl);
+    v = __builtin_amdgcn_exp2f(b * l2);
+    e = v * (2e-6f + 1e-6f * bmax + 2e-6f * bmax * fabsf(l2) * 0.6931472f);
+  } else if (MODEL == 1) {
+    const float qd = __fdividef(dl, 1.f + dl);
+    const float ve = __expf(b * qd);
+    const bool neg = dl < 0.f;
+    v = neg ? ve : 1.f + b * dl;
+    e = neg ? ve * (2e-6f + 1e-6f * bmax + 2e-6f * bmax * fabsf(qd)) : 3e-7f + 1e-6f * bmax * fabsf(dl);
+  } else {
+    v = fmaxf(1.f + b * dl, 0.f);
+    e = 3e-7f + 1e-6f * bmax * fabsf(dl);
   }
-  if (model == 0 || (model == 1 && dl < 0.f)) {
-    float v, s;
-    if (model == 0) {
-      const float l2 = __log2f(1.f + dl);
-      v = exp2f(b * l2);
-      s = bmax * fabsf(l2) * 0.6931472f;
-    } else {
-      const float qd = __fdividef(dl, 1.f + dl);
-      v = __expf(b * qd);
-      s = bmax * fabsf(qd);
-    }
-    E = v * (2e-6f + 1e-6f * bmax + 2e-6f * s);
-    return v;
+  const bool dead = !(dl > -1.f);
+  E = dead ? 0.f : e;
+  return dead ? 0.f : v;
+}
+
+// Phase A of k_count for the lane's 8 cells of one population: which cells
+// the 16 high uniform bits cannot settle as k = 0 (bit c of the result), and
+// their FP32 lambda (-1: FP64 path) and error bound.
+template <int MODEL>
+__device__ inline unsigned undecided_cells(const float4 (&T)[kCellsPerLane], const float (&d)[kCellsPerLane],
+                                           const float (&tt)[kCellsPerLane], unsigned fastm, unsigned inm,
+                                           const u32x4 &rr, float relerr, float (&lam)[kCellsPerLane],
+                                           float (&err)[kCellsPerLane]) {
+  unsigned qm = 0;
+#pragma unroll
+  for (int c = 0; c < kCellsPerLane; c++) {
+    const float4 Tc = T[c];
+    const float t = tt[c];
+    const float b = Tc.z + Tc.w * t;
+    float E;
+    const float bias = bias_fast_t<MODEL>(b, fmaxf(fabsf(Tc.z), fabsf(Tc.z + Tc.w)), d[c], E);
+    const float nn = Tc.x + Tc.y * t;
+    const float l = nn * bias;
+    const float e = l * (2e-6f + relerr) + nn * E;
+    const bool pos = Tc.x >= 0.f;
+    const bool fast = (fastm >> c) & 1u, in = (inm >> c) & 1u;
+    const bool zero = poisson_zero_from_hi(l, __expf(-l), e, poisson_hi16(rr, (uint64_t)c));
+    const bool done = fast && (!in || Tc.x == kBinZero || (pos && zero) || (Tc.x == kBinTiny && bias + E < 1e9f));
+    qm |= (done ? 0u : 1u) << c;
+    lam[c] = (fast && in && pos) ? l : -1.f;
+    err[c] = e;
   }
-  const float v = 1.f + b * dl;
-  E = 3e-7f + 1e-6f * bmax * fabsf(dl);
-  return model == 2 ? fmaxf(v, 0.f) : v;
+  return qm;
 }
 
 // The count of one cell from nothing but its index, by the FP64 path (k_fill's
@@ -172,17 +203,26 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
   const size_t blk = blockIdx.x;
   const size_t cell0 = blk * kBlockCells + (size_t)kCellsPerLane * lane;
   const int N = a.N;
-  const size_t row = cell0 / (size_t)N;
-  const int ix0 = (int)(cell0 - row * N);
-  const int iy = (int)(row % N), iz = (int)(row / N);
+  // row and column of the lane's first cell: the block part in scalar
+  // 64-bit arithmetic, the lane part in 32 bits (rows < 2^22)
+  const size_t base = blk * kBlockCells;
+  const size_t row0 = base / (size_t)N;
+  const int o = (int)(base - row0 * N) + kCellsPerLane * lane;
+  const int drow = o / N;
+  const int ix0 = o - drow * N;
+  const unsigned row = (unsigned)row0 + (unsigned)drow;
+  const int iy = (int)(row % (unsigned)N), iz = (int)(row / (unsigned)N);
   const unsigned long long below = (1ull << lane) - 1ull;
 
   // --- the lane's 8 cells: density and geometry --------------------------
+  // The fast path needs the radius only to well inside its margins (bin
+  // fraction and box edge both 1e-9 away), so it is evaluated here in the
+  // cheapest stable order; count_exact recomputes it the reference's way.
   float d[kCellsPerLane], tt[kCellsPerLane];
   int ir[kCellsPerLane];
   unsigned fastm = 0, inm = 0;
   {
-    const float2 *p = (const float2 *)(a.g + row * (size_t)a.ngx + ix0);
+    const float2 *p = (const float2 *)(a.g + (size_t)row * a.ngx + ix0);
 #pragma unroll
     for (int h = 0; h < kCellsPerLane / 2; h++) {
       const float2 v = p[h];
@@ -191,68 +231,54 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
     }
   }
   {
-#pragma clang fp contract(off)
-    const double y0 = (iy + 0.0) * a.dx - a.oy, z0 = (iz + 0.0) * a.dx - a.oz;
+    const double y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
+    const double yz2 = y0 * y0 + z0 * z0;
+    const double xx0 = ix0 * a.dx - a.ox;
+    const double hb = a.half_box_20, hbm = 1e-9 * a.half_box_20;
 #pragma unroll
     for (int c = 0; c < kCellsPerLane; c++) {
-      const double xx = (ix0 + c + 0.0) * a.dx - a.ox;
-      const double r2 = xx * xx + y0 * y0 + z0 * z0;   // the reference's order
+      const double xx = xx0 + c * a.dx;
+      const double r2 = xx * xx + yz2;
       // r = r2 / sqrt(r2): FP32 rsqrt refined once in FP64 (rel. error ~2e-14)
-      const float y = rsqrtf((float)r2);
+      const float y = __builtin_amdgcn_rsqf((float)r2);
       const double yd = (double)y * (1.5 - 0.5 * r2 * (double)y * (double)y);
       const double r = r2 * yd;
       const double xr = r * a.idr;
       const int i = (int)xr;
       const double fr = xr - i;
-      const bool fast = r2 > 0 && fr > 1e-9 && fr < 1 - 1e-9 && i < a.na - 2 &&
-                        fabs(r - a.half_box_20) > 1e-9 * a.half_box_20;
+      const bool fast = r2 > 0 && fr > 1e-9 && fr < 1 - 1e-9 && i < a.na - 2 && fabs(r - hb) > hbm;
       ir[c] = fast ? i : 0;   // a valid bin for every cell: the gathers below need no branch
       tt[c] = (float)fr;
       fastm |= (fast ? 1u : 0u) << c;
-      inm |= (r < a.half_box_20 ? 1u : 0u) << c;
+      inm |= (r < hb ? 1u : 0u) << c;
     }
   }
 
   for (int ip = 0; ip < a.npop; ip++) {
     const PopTab &P = a.pop[ip];
+    const float4 *tab = P.tab;
+    const int model = P.model, pipop = P.ipop;
 #pragma unroll
     for (int c = 0; c < kCellsPerLane; c++) cnt[kCellsPerLane * lane + c] = 0;
     // all 8 bins in flight at once
     float4 T[kCellsPerLane];
 #pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) T[c] = P.tab[ir[c]];
-    const u32x4 rr = draw(cell0 >> 3, stream_poisson(P.ipop), 0u, a.seed);
-    const float relerr = P.tab[a.na].x;   // product-interpolation bound of the table
+    for (int c = 0; c < kCellsPerLane; c++) T[c] = tab[ir[c]];
+    const u32x4 rr = draw(cell0 >> 3, stream_poisson(pipop), 0u, a.seed);
+    const float relerr = tab[a.na].x;   // product-interpolation bound of the table
+    float lam[kCellsPerLane], err[kCellsPerLane];
+    unsigned qm;
+    if (model == 0) qm = undecided_cells<0>(T, d, tt, fastm, inm, rr, relerr, lam, err);
+    else if (model == 1) qm = undecided_cells<1>(T, d, tt, fastm, inm, rr, relerr, lam, err);
+    else qm = undecided_cells<2>(T, d, tt, fastm, inm, rr, relerr, lam, err);
     int qn = 0;   // uniform
 #pragma unroll
     for (int c = 0; c < kCellsPerLane; c++) {
-      const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
-      float lam = -1.f, err = 0.f;
-      bool queue_it = true;
-      if ((fastm >> c) & 1u) {
-        if (!((inm >> c) & 1u)) {
-          queue_it = false;
-        } else if (T[c].x >= 0.f) {
-          const float tc = tt[c];
-          const float nn = T[c].x + T[c].y * tc;
-          const float b = T[c].z + T[c].w * tc;
-          float E;
-          const float bias = bias_fast(P.model, b, fmaxf(fabsf(T[c].z), fabsf(T[c].z + T[c].w)), d[c], E);
-          lam = nn * bias;
-          err = lam * (2e-6f + relerr) + nn * E;
-          if (poisson_zero_from_hi(lam, __expf(-lam), err, hi)) queue_it = false;
-        } else if (T[c].x == kBinZero) {
-          queue_it = false;
-        } else if (T[c].x == kBinTiny) {
-          float E;
-          const float b = T[c].z + T[c].w * tt[c];
-          const float bias = bias_fast(P.model, b, fmaxf(fabsf(T[c].z), fabsf(T[c].z + T[c].w)), d[c], E);
-          if (bias + E < 1e9f) queue_it = false;
-        }
-      }
+      const bool queue_it = (qm >> c) & 1u;
       const unsigned long long m = __ballot(queue_it);
       if (queue_it)
-        queue[qn + __popcll(m & below)] = QEntry{(uint32_t)(kCellsPerLane * lane + c) | (hi << 16), lam, err};
+        queue[qn + __popcll(m & below)] =
+            QEntry{(uint32_t)(kCellsPerLane * lane + c) | (poisson_hi16(rr, (uint64_t)c) << 16), lam[c], err[c]};
       qn += __popcll(m);
     }
     __syncthreads();   // one wavefront: orders the LDS writes above before the reads below
