@@ -17,21 +17,13 @@ namespace clr {
 // ---------------------------------------------------------------------------
 struct u32x4 { uint32_t x, y, z, w; };
 
-__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __umulhi(a, b);
-#else
-  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
-#endif
-}
-
 __host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int i = 0; i < 10; i++) {
     if (i) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = mulhi32(0xD2511F53u, c.x);
-    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = mulhi32(0xCD9E8D57u, c.z);
-    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    // one 32x32->64 product per multiplier (a single v_mad_u64_u32 on gfx950)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = u32x4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
   }
   return c;
 }
@@ -41,9 +33,11 @@ __host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
 //   modes    : site = kx + (N/2+1)(ky + N (kz mod N/2)); words (0,1) are the
 //              (phase, modulus) draws of kz < N/2, words (2,3) those of
 //              kz + N/2.
-//   Poisson  : site = cell/4 (cell = ix + N(iy + N iz)); word cell%4 is the
-//              cell's uniform; extra uniforms of the lambda > 500 split come
-//              from site = cell in stream_poisson_extra.
+//   Poisson  : (cell = ix + N(iy + N iz)) the high 16 bits of the cell's
+//              uniform are half cell%8 of the draw at site cell/8, the low 16
+//              bits word 0 of site cell in stream 0x40000100+pop; extra
+//              uniforms of the lambda > 500 split come from site = cell in
+//              stream_poisson_extra (clr_poisson.h).
 //   positions: site = cell; uniform j of the cell's sources is word j%4 of
 //              draw j/4 (three per source, x y z).
 constexpr uint32_t kKey1        = 0x436F4C52u;   // second key word
