@@ -4,17 +4,21 @@
 // k_transform : lognormalize / densclip in place.
 // k_shells    : one pass over the grid.  Each wavefront walks tiles of
 //               64 (x, one cell per lane) x 4 (y) x 16 (z) cells; per cell the
-//               FP64 radius, z(r) and shell index s = (int)(z*idz)+1 (exactly
-//               the reference's expression, so the counts match bit for bit),
-//               then count, z and bias_model(delta, b_t(r)) for every tracer.
-//               A lane's radius changes slowly along its 64 cells, so it adds
-//               into private FP64 accumulators of its current shell and keeps
-//               at most kSlots finished shells; the lanes are combined once
-//               per tile with DPP wave sums, and lane 0 adds the result into
-//               the wavefront's own LDS slots.  Every addition happens in a
-//               fixed order: the sums are bitwise repeatable.  With `fused`
-//               the lognormal transform is applied in the same pass (one
-//               read and one write of the grid for both stages).
+//               radius, then from one packed FP32 table D(r), z(r) and every
+//               b_t(r); the shell index s = (int)(z*idz)+1 is taken from the
+//               FP32 z where that is more than 5e-4 of a shell away from a
+//               shell edge, otherwise from the reference's FP64 expressions
+//               (so the counts match bit for bit); then count, z and
+//               bias_model(delta, b_t(r)) for every tracer.  A lane's radius
+//               changes slowly along its 64 cells, so it adds into private
+//               FP64 accumulators of its current shell and logs finished
+//               shells (at most kSlots) in its own LDS slots; the lanes are
+//               combined once per tile with DPP wave sums, and lane 0 adds the
+//               result into the wavefront's own LDS accumulators.  Every
+//               addition happens in a fixed order: the sums are bitwise
+//               repeatable.  With `fused` the lognormal transform is applied
+//               in the same pass (one read and one write of the grid for both
+//               stages).
 // k_shell_final / k_norm_table : fixed-order reduction of the per-block
 //               partials, norm = count/sum, edge fill and linear
 //               interpolation in z onto the NA r-grid (density.c:1272-1355),
@@ -30,7 +34,7 @@ constexpr int kShellBlocks = 8192;
 constexpr int kShellThreads = 256;
 constexpr int kWaves = kShellThreads / 64;
 constexpr int kTY = 4, kTZ = 16;   // tile rows: kTY x kTZ rows of 64 cells
-constexpr int kSlots = 4;
+constexpr int kSlots = 3;
 constexpr int kMaxTr = 8;
 
 struct ShellArgs {
@@ -38,16 +42,17 @@ struct ShellArgs {
   int N, ngx, nz, ntr, fused, na;
   double dx, ox, oy, oz, idr, r_last, z_last, sigma2, idz;
   const double *r_arr, *zt;
-  const float4 *tab;   // per bin: D, b_0 .. b_{NT-1} in (NT+4)/4 float4s (k_pack_shell_table)
-  float d1_last_f, idr_f;
+  const float4 *tab;   // per bin: D, z, b_0 .. b_{NT-1} in (NT+5)/4 float4s (k_pack_shell_table)
+  float d1_last_f, z_last_f, idr_f, idz_f;
   float bz0[kMaxTr];
   int model[kMaxTr];
 };
 
-// FP32 tables of one shell pass packed per bin: component 0 is D(r), component
-// 1 + t is b_t(r); w float4s per bin.
+// FP32 tables of one shell pass packed per bin: component 0 is D(r),
+// component 1 is z(r), component 2 + t is b_t(r); w float4s per bin.
 struct ShellTabSrc {
   const float *d1f;
+  const double *zt;
   const float *bzf[kMaxTr];
 };
 __global__ void k_pack_shell_table(ShellTabSrc src, int ntr, int na, int w, float4 *__restrict__ out) {
@@ -56,7 +61,8 @@ __global__ void k_pack_shell_table(ShellTabSrc src, int ntr, int na, int w, floa
   float v[4 * 3];
   for (int j = 0; j < 4 * w; j++) v[j] = 0.f;
   v[0] = src.d1f[i];
-  for (int t = 0; t < ntr; t++) v[1 + t] = src.bzf[t][i];
+  v[1] = (float)src.zt[i];
+  for (int t = 0; t < ntr; t++) v[2 + t] = src.bzf[t][i];
   for (int k = 0; k < w; k++) out[(size_t)i * w + k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
 
@@ -106,8 +112,9 @@ __global__ __launch_bounds__(256) void k_transform(float *__restrict__ g, int N,
   }
 }
 
-// Combine the slot-k accumulators of all lanes: lanes are grouped by shell
-// id (in order of the lowest lane) and each group is summed with DPP.
+// Combine the slot-k accumulators of all lanes (slot values in LDS,
+// vals[q * 64 + lane]): lanes are grouped by shell id (in order of the lowest
+// lane) and each group is summed with DPP.
 template <int NQ>
 __device__ inline void flush_slot(unsigned long long mask, int key, const double *vals, double *myacc,
                                   int lane) {
@@ -118,46 +125,56 @@ __device__ inline void flush_slot(unsigned long long mask, int key, const double
     const unsigned long long mm = __ballot(mine);
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-      const double v = wave_sum(mine ? vals[q] : 0.0);
+      const double v = wave_sum(mine ? vals[q * 64 + lane] : 0.0);
       if (lane == 0) myacc[sl * NQ + q] += v;
     }
     mask &= ~mm;
   }
 }
 
+// LDS per block: kWaves x nz x NQ accumulators, then per wavefront kSlots x
+// NQ x 64 logged shell sums and kSlots x 64 shell ids.
+template <int NT>
+constexpr size_t shell_lds_bytes(int nz) {
+  return (size_t)kWaves * nz * (2 + NT) * sizeof(double) +
+         (size_t)kWaves * kSlots * 64 * ((2 + NT) * sizeof(double) + sizeof(int));
+}
+
 template <int NT>
 __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *__restrict__ partial) {
 #pragma clang fp contract(off)
   constexpr int NQ = 2 + NT;
-  constexpr int W = (NT + 4) / 4;
-  extern __shared__ double acc[];   // [kWaves][nz][NQ]
+  constexpr int W = (NT + 5) / 4;
+  extern __shared__ double acc[];
   const int nz = a.nz, N = a.N;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   double *myacc = acc + (size_t)wave * nz * NQ;
+  double *logv = acc + (size_t)kWaves * nz * NQ + (size_t)wave * kSlots * NQ * 64;   // [slot][q][lane]
+  int *logid = (int *)(acc + (size_t)kWaves * nz * NQ + (size_t)kWaves * kSlots * NQ * 64) + wave * kSlots * 64;
   for (int i = threadIdx.x; i < kWaves * nz * NQ; i += blockDim.x) acc[i] = 0;
   __syncthreads();
   const double obs[3] = {a.ox, a.oy, a.oz};
+  const float ox = (float)a.ox, oy = (float)a.oy, oz = (float)a.oz, dxf = (float)a.dx;
   const int ntx = (N + 63) / 64, nty = N / kTY, ntz = N / kTZ;
   const long ntiles = (long)ntx * nty * ntz;
   const long nw = (long)gridDim.x * kWaves;
+  // all lanes' logged shells into the wavefront's accumulators
+  auto flush_all = [&](int nslot) {
+#pragma unroll
+    for (int k = 0; k < kSlots; k++)
+      flush_slot<NQ>(__ballot(k < nslot), logid[k * 64 + lane], logv + (size_t)k * NQ * 64, myacc, lane);
+  };
   for (long tile = (long)blockIdx.x * kWaves + wave; tile < ntiles; tile += nw) {
     const int tx = (int)(tile % ntx);
     const long rest = tile / ntx;
     const int ty = (int)(rest % nty), tz = (int)(rest / nty);
     const int ix = tx * 64 + lane;
     const bool valid = ix < N;
-    int nslot = 0, sid[kSlots];
-    double sacc[kSlots][NQ];
-    int cur = -1;
+    int nslot = 0, cur = -1;
     double cacc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) cacc[q] = 0;
-#pragma unroll
-    for (int k = 0; k < kSlots; k++) {
-      sid[k] = -1;
-#pragma unroll
-      for (int q = 0; q < NQ; q++) sacc[k][q] = 0;
-    }
+    const float x32 = ix * dxf - ox;
     // the next row's density is loaded while the current one is processed
     float dnext = valid ? a.g[((size_t)(tz * kTZ) * N + ty * kTY) * a.ngx + ix] : 0.f;
     for (int rr = 0; rr < kTY * kTZ; rr++) {
@@ -170,15 +187,14 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
         dnext = a.g[((size_t)iz2 * N + iy2) * a.ngx + ix];
       }
       if (valid) {
-        double x0, y0, z0;
-        const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
-        const double zz = f_of_r(a.r_arr, a.zt, a.idr, a.r_last, r, 0.0, a.z_last);
-        // FP32 tables: one bin lookup for D and every b_t (f_of_r32 semantics)
+        // FP32 radius (absolute error ~1e-3 cell) and one bin lookup for D,
+        // z and every b_t (f_of_r32 semantics)
+        const float y32 = iy * dxf - oy, z32 = iz * dxf - oz;
+        const float rf = __builtin_amdgcn_sqrtf(x32 * x32 + y32 * y32 + z32 * z32);
         float comp[4 * W];
+        const float xq = rf * a.idr_f;
+        const bool e0 = !(rf > 0.f), e1 = xq >= (float)(a.na - 1);
         {
-          const float rf = (float)r;
-          const float xq = rf * a.idr_f;
-          const bool e0 = !(rf > 0.f), e1 = xq >= (float)(a.na - 1);
           const int i0 = (e0 || e1) ? 0 : (int)xq;
           const float tq = xq - (float)i0;
 #pragma unroll
@@ -191,8 +207,9 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
           }
           if (e0 || e1) {
             comp[0] = e0 ? 1.f : a.d1_last_f;
+            comp[1] = e0 ? 0.f : a.z_last_f;
 #pragma unroll
-            for (int t = 0; t < NT; t++) comp[1 + t] = e0 ? a.bz0[t] : 1.f;
+            for (int t = 0; t < NT; t++) comp[2 + t] = e0 ? a.bz0[t] : 1.f;
           }
         }
         if (a.fused) {
@@ -201,37 +218,38 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
           dl = expf(arg) - 1.f;
           a.g[((size_t)iz * N + iy) * a.ngx + ix] = dl;
         }
-        const int ind = (int)(zz * a.idz) + 1;
+        // shell index: FP32 z (error < 5e-5 shell) unless within 5e-4 shell of
+        // an edge (or at the table ends), where the reference's FP64 radius
+        // and z decide
+        double zz = (double)comp[1];
+        const float sf = comp[1] * a.idz_f;
+        int ind = (int)sf + 1;
+        const float fr = sf - floorf(sf);
+        if (e0 || e1 || !(fr > 5e-4f && fr < 1.f - 5e-4f)) {
+          double x0, y0, z0;
+          const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
+          zz = f_of_r(a.r_arr, a.zt, a.idr, a.r_last, r, 0.0, a.z_last);
+          ind = (int)(zz * a.idz) + 1;
+        }
         if (ind >= 0 && ind < nz) {
           s = ind;
           v[0] = 1.0;
           v[1] = zz;
 #pragma unroll
-          for (int t = 0; t < NT; t++) v[2 + t] = (double)bias_model_f32(dl, comp[1 + t], a.model[t]);
+          for (int t = 0; t < NT; t++) v[2 + t] = (double)bias_model_f32(dl, comp[2 + t], a.model[t]);
         }
       }
       const bool change = s >= 0 && s != cur;
       // a lane whose slots are full flushes them first (wave-collective, rare)
       if (__ballot(change && cur >= 0 && nslot == kSlots)) {
-#pragma unroll
-        for (int k = 0; k < kSlots; k++) flush_slot<NQ>(__ballot(k < nslot), sid[k], sacc[k], myacc, lane);
+        flush_all(nslot);
         nslot = 0;
-#pragma unroll
-        for (int k = 0; k < kSlots; k++) {
-          sid[k] = -1;
-#pragma unroll
-          for (int q = 0; q < NQ; q++) sacc[k][q] = 0;
-        }
       }
       if (change) {
         if (cur >= 0) {
 #pragma unroll
-          for (int k = 0; k < kSlots; k++)
-            if (k == nslot) {
-              sid[k] = cur;
-#pragma unroll
-              for (int q = 0; q < NQ; q++) sacc[k][q] = cacc[q];
-            }
+          for (int q = 0; q < NQ; q++) logv[((size_t)nslot * NQ + q) * 64 + lane] = cacc[q];
+          logid[nslot * 64 + lane] = cur;
           nslot++;
         }
         cur = s;
@@ -245,22 +263,16 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
     }
     // end of tile: the current shell goes into a slot (flushing first if full)
     if (__ballot(cur >= 0 && nslot == kSlots)) {
-#pragma unroll
-      for (int k = 0; k < kSlots; k++) flush_slot<NQ>(__ballot(k < nslot), sid[k], sacc[k], myacc, lane);
+      flush_all(nslot);
       nslot = 0;
     }
     if (cur >= 0) {
 #pragma unroll
-      for (int k = 0; k < kSlots; k++)
-        if (k == nslot) {
-          sid[k] = cur;
-#pragma unroll
-          for (int q = 0; q < NQ; q++) sacc[k][q] = cacc[q];
-        }
+      for (int q = 0; q < NQ; q++) logv[((size_t)nslot * NQ + q) * 64 + lane] = cacc[q];
+      logid[nslot * 64 + lane] = cur;
       nslot++;
     }
-#pragma unroll
-    for (int k = 0; k < kSlots; k++) flush_slot<NQ>(__ballot(k < nslot), sid[k], sacc[k], myacc, lane);
+    flush_all(nslot);
   }
   __syncthreads();
   double *out = partial + (size_t)blockIdx.x * nz * NQ;
@@ -425,10 +437,16 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   a.r_arr = r;
   a.zt = zt;
   tsrc.d1f = c->d1f;
+  tsrc.zt = zt;
+  a.z_last_f = (float)z_last;
+  a.idz_f = (float)idz;
   a.d1_last_f = (float)c->h_bg_last[2];
   a.idr_f = (float)c->glob_idr;
   const int nq = 2 + ntr;
-  const size_t lds = (size_t)kWaves * nz * nq * sizeof(double);
+  static size_t (*const kLds[kMaxTr + 1])(int) = {shell_lds_bytes<0>, shell_lds_bytes<1>, shell_lds_bytes<2>,
+                                                   shell_lds_bytes<3>, shell_lds_bytes<4>, shell_lds_bytes<5>,
+                                                   shell_lds_bytes<6>, shell_lds_bytes<7>, shell_lds_bytes<8>};
+  const size_t lds = kLds[ntr](nz);
   if (lds > 160 * 1024) return clr_fail(c, CLR_ERR_ARG, "too many shells x tracers (%d x %d)", nz, ntr);
   if (lds > 64 * 1024)
     CLR_HIP(c, hipFuncSetAttribute(kShellKernels[ntr], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -437,7 +455,7 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
   if (rc) return rc;
   double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals, *work = fin + nvals;
-  const int w = (ntr + 4) / 4;
+  const int w = (ntr + 5) / 4;
   rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
   if (rc) return rc;
   a.tab = (const float4 *)c->misc;

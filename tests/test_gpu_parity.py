@@ -5,8 +5,8 @@ Tolerances (FP32 device grids against the FP64 oracle):
   * Gaussian fields: RMS error <= 2e-6 of the field RMS, max error <= 5e-5 of
     it; sigma^2 within 1e-5 relative.
   * lognormal / clipped density from the same Gaussian grid: |err| <= 1e-6 (1+|delta|).
-  * shell sums: counts exact, z sums 1e-12 relative, bias sums 2e-6
-    relative; norm tables 2e-6 relative.
+  * shell sums: counts exact; z and bias sums (FP32 per-cell terms summed
+    in FP64) 2e-6 relative; norm tables from the same sums 1e-12 relative.
   * Poisson counts and the catalogue (positions, base pixels): bit-exact given
     the same density grid and norm tables; dz_rsd within 1e-12 relative.
   * kappa / ISW maps from the same potential: 1e-5 of the map RMS.
@@ -109,7 +109,7 @@ def test_shell_norms(workdir, N):
     onz, zmax, ocnt, ozs, osums = orc.shell_sums(N, L, gl.astype(np.float64), tracers)
     assert nz == onz
     assert np.array_equal(cnt, ocnt)
-    np.testing.assert_allclose(zs, ozs, rtol=1e-12)
+    np.testing.assert_allclose(zs, ozs, rtol=2e-6)
     np.testing.assert_allclose(sums, osums, rtol=2e-6)
     for i in range(ntr):
         tab, n0, nf = ctx.norm_table(0, i)
