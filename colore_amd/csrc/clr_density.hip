@@ -132,6 +132,15 @@ __device__ inline void flush_slot(unsigned long long mask, int key, const double
   }
 }
 
+// bias_model (common.h:414-431) for the shell sums, on the raw hardware
+// exp2/log2 and a fast reciprocal (a few ulp: the sums are of FP32 terms).
+__device__ inline float bias_sum_term(float d, float b, int model) {
+  if (!(d > -1.f)) return 0.f;
+  if (model == 1) return d < 0.f ? __expf(b * __fdividef(d, 1.f + d)) : 1.f + b * d;
+  if (model == 2) return fmaxf(1.f + b * d, 0.f);
+  return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(1.f + d));
+}
+
 // LDS per block: kWaves x nz x NQ accumulators, then per wavefront kSlots x
 // NQ x 64 logged shell sums and kSlots x 64 shell ids.
 template <int NT>
@@ -175,31 +184,51 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
 #pragma unroll
     for (int q = 0; q < NQ; q++) cacc[q] = 0;
     const float x32 = ix * dxf - ox;
-    // the next row's density is loaded while the current one is processed
+    // The next row's density and table bins are loaded while the current row
+    // is processed.  Bin lookup: FP32 radius (absolute error ~1e-3 cell), one
+    // index for D, z and every b_t (f_of_r32 semantics).
+    struct Bin {
+      float4 A[W], B[W];
+      float tq;
+      bool e0, e1;
+    };
+    auto fetch = [&](int rr2, Bin &bn) {
+      const int iy2 = ty * kTY + (rr2 % kTY), iz2 = tz * kTZ + (rr2 / kTY);
+      const float y32 = iy2 * dxf - oy, z32 = iz2 * dxf - oz;
+      const float rf = __builtin_amdgcn_sqrtf(x32 * x32 + y32 * y32 + z32 * z32);
+      const float xq = rf * a.idr_f;
+      bn.e0 = !(rf > 0.f);
+      bn.e1 = xq >= (float)(a.na - 1);
+      const int i0 = (bn.e0 || bn.e1) ? 0 : (int)xq;
+      bn.tq = xq - (float)i0;
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        bn.A[k] = a.tab[(size_t)i0 * W + k];
+        bn.B[k] = a.tab[(size_t)(i0 + 1) * W + k];
+      }
+    };
+    Bin bnext;
+    fetch(0, bnext);
     float dnext = valid ? a.g[((size_t)(tz * kTZ) * N + ty * kTY) * a.ngx + ix] : 0.f;
     for (int rr = 0; rr < kTY * kTZ; rr++) {
       const int iy = ty * kTY + (rr % kTY), iz = tz * kTZ + (rr / kTY);
       int s = -1;
       double v[NQ];
       float dl = dnext;
-      if (valid && rr + 1 < kTY * kTZ) {
+      const Bin bn = bnext;
+      if (rr + 1 < kTY * kTZ) {
         const int iy2 = ty * kTY + ((rr + 1) % kTY), iz2 = tz * kTZ + ((rr + 1) / kTY);
-        dnext = a.g[((size_t)iz2 * N + iy2) * a.ngx + ix];
+        if (valid) dnext = a.g[((size_t)iz2 * N + iy2) * a.ngx + ix];
+        fetch(rr + 1, bnext);
       }
       if (valid) {
-        // FP32 radius (absolute error ~1e-3 cell) and one bin lookup for D,
-        // z and every b_t (f_of_r32 semantics)
-        const float y32 = iy * dxf - oy, z32 = iz * dxf - oz;
-        const float rf = __builtin_amdgcn_sqrtf(x32 * x32 + y32 * y32 + z32 * z32);
         float comp[4 * W];
-        const float xq = rf * a.idr_f;
-        const bool e0 = !(rf > 0.f), e1 = xq >= (float)(a.na - 1);
+        const bool e0 = bn.e0, e1 = bn.e1;
         {
-          const int i0 = (e0 || e1) ? 0 : (int)xq;
-          const float tq = xq - (float)i0;
+          const float tq = bn.tq;
 #pragma unroll
           for (int k = 0; k < W; k++) {
-            const float4 A = a.tab[(size_t)i0 * W + k], B = a.tab[(size_t)(i0 + 1) * W + k];
+            const float4 A = bn.A[k], B = bn.B[k];
             comp[4 * k + 0] = A.x + (B.x - A.x) * tq;
             comp[4 * k + 1] = A.y + (B.y - A.y) * tq;
             comp[4 * k + 2] = A.z + (B.z - A.z) * tq;
@@ -236,7 +265,7 @@ __global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *_
           v[0] = 1.0;
           v[1] = zz;
 #pragma unroll
-          for (int t = 0; t < NT; t++) v[2 + t] = (double)bias_model_f32(dl, comp[2 + t], a.model[t]);
+          for (int t = 0; t < NT; t++) v[2 + t] = (double)bias_sum_term(dl, comp[2 + t], a.model[t]);
         }
       }
       const bool change = s >= 0 && s != cur;
