@@ -130,8 +130,9 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
           x1 = h ? r.w : r.y;
         }
         const float w = modulus_draw(x1);
-        float sn, cs;
-        sincospif((float)x0 * 4.6566128730773926e-10f, &sn, &cs);   // angle 2 pi u
+        // angle 2 pi u: v_sin_f32 / v_cos_f32 take their argument in turns
+        const float turns = (float)x0 * 2.3283064365386963e-10f;
+        const float sn = __builtin_amdgcn_sinf(turns), cs = __builtin_amdgcn_cosf(turns);
         const float2 a = amp[q];
         d = make_float2(a.x * w * cs, a.x * w * sn);
         p = make_float2(a.y * w * cs, a.y * w * sn);
