@@ -49,7 +49,7 @@ int clr_ring2nest_table(clr_ctx *c, int nside) {
 static void free_all(clr_ctx *c) {
   auto f = [](void *p) { if (p) hipFree(p); };
   f(c->dens); f(c->npot); f(c->bg); f(c->pk); f(c->tw); f(c->amp); f(c->red); f(c->red_out);
-  f(c->rowcnt); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d1f); f(c->r2n);
+  f(c->rowcnt); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d1f); f(c->r2n);
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < CLR_NPOP_MAX; i++) {
       f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].ptab);

@@ -84,6 +84,8 @@ struct clr_ctx {
   size_t inj_n = 0;
   double *hp_vec = nullptr; // pixel centres scratch
   size_t hp_n = 0;
+  void *esc = nullptr;      // sources: escape list of saturated count bytes (+ counter)
+  size_t esc_bytes = 0;
   void *misc = nullptr;     // generic scratch
   size_t misc_n = 0;
 

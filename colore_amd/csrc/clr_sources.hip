@@ -23,13 +23,15 @@
 #include "clr_internal.h"
 #include "clr_poisson.h"
 #include <math.h>
+#include <algorithm>
+#include <vector>
 
 using namespace clr;
 
 namespace {
 
 constexpr int kLanes = 64, kCellsPerLane = 8, kBlockCells = kLanes * kCellsPerLane;
-constexpr int kByteEscape = 255;   // a stored count of 255 means "recount this cell"
+constexpr int kByteEscape = 255;   // a stored count of 255: the count is in the escape list
 
 struct PopTab {
   const double *amp, *bz, *norm;
@@ -49,6 +51,11 @@ struct SrcArgs {
   double dx, cell_vol, half_box_20, ox, oy, oz;
   int N, ngx, na, npop;
   uint32_t seed;
+  // exact counts of cells whose byte saturated: key = pop << 56 | cell,
+  // value = count; appended by k_count, sorted by key on the host for k_fill
+  ulonglong2 *esc;
+  unsigned int *esc_n;
+  unsigned int esc_cap, esc_count;
   PopTab pop[CLR_NPOP_MAX];
 };
 
@@ -177,18 +184,6 @@ __device__ inline unsigned undecided_cells(const float4 (&T)[kCellsPerLane], con
   return qm;
 }
 
-// The count of one cell from nothing but its index, by the FP64 path (k_fill's
-// recount of cells whose byte saturated).
-__device__ __forceinline__ int count_cell(const SrcArgs &a, const PopTab &P, uint64_t cell) {
-  const int N = a.N;
-  const uint64_t rw = cell / (uint64_t)N;
-  const int cx = (int)(cell - rw * N);
-  const u32x4 r = draw(cell >> 3, stream_poisson(P.ipop), 0u, a.seed);
-  const uint32_t x = (poisson_hi16(r, cell) << 16) | poisson_lo16(cell, P.ipop, a.seed);
-  return count_exact(cx, (int)(rw % N), (int)(rw / N), a.g[rw * (size_t)a.ngx + cx], cell, x, a.r_arr, P.amp,
-                     P.bz, P.norm, a.idr, a.r_last, P.bz0, P.norm0, P.normf, a.dx, a.ox, a.oy, a.oz, a.cell_vol,
-                     a.half_box_20, P.model, P.ipop, a.seed);
-}
 
 // queue entry: cell within the block | hi << 16; lambda (< 0: FP64 path); bound
 struct QEntry {
@@ -312,6 +307,19 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
       else hi8 |= b << (8 * (c - 4));
     }
     *(uint2 *)(P.cnt8 + cell0) = make_uint2(lo, hi8);
+    if (((lo | hi8) & 0x80808080u) && (((lo & (lo >> 1)) | (hi8 & (hi8 >> 1))) & 0x40404040u)) {
+      // some byte may be 255 (rare): append the exact counts to the escape list
+#pragma unroll
+      for (int c = 0; c < kCellsPerLane; c++) {
+        const int k = cnt[kCellsPerLane * lane + c];
+        if (k >= kByteEscape) {
+          const unsigned e = atomicAdd(a.esc_n, 1u);
+          if (e < a.esc_cap)
+            a.esc[e] = make_ulonglong2(((unsigned long long)ip << 56) | (unsigned long long)(cell0 + c),
+                                       (unsigned long long)k);
+        }
+      }
+    }
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
     if (lane == 0) P.btot[blk] = (uint32_t)mine;
   }
@@ -373,35 +381,62 @@ __global__ __launch_bounds__(kScanB) void k_scan_apply(const uint32_t *__restric
   if (i == n - 1) out[n] = bsum[gridDim.x];
 }
 
+// count of an escaped cell from the sorted escape list
+__device__ inline int escaped_count(const ulonglong2 *esc, unsigned n, unsigned long long key) {
+  unsigned lo = 0, hi = n;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi) / 2;
+    if (esc[mid].x < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && esc[lo].x == key) ? (int)esc[lo].y : 0;
+}
+
 __global__ __launch_bounds__(kLanes) void k_fill(SrcArgs a) {
+  constexpr int G = 4;   // populations whose loads are in flight together
   const int lane = threadIdx.x;
   const size_t blk = blockIdx.x;
   const uint64_t cell0 = (uint64_t)blk * kBlockCells + (uint64_t)kCellsPerLane * lane;
-  for (int ip = 0; ip < a.npop; ip++) {
-    const PopTab &P = a.pop[ip];
-    const long long b0 = P.boff[blk];
-    if (P.boff[blk + 1] == b0) continue;   // no sources in this block (uniform)
-    const uint2 w = *(const uint2 *)(P.cnt8 + cell0);
-    int k[kCellsPerLane];
-    long long mine = 0;
+  for (int g0 = 0; g0 < a.npop; g0 += G) {
+    long long b0[G], b1[G];
+    uint2 w[G];
 #pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) {
-      k[c] = (int)(((c < 4 ? w.x : w.y) >> (8 * (c & 3))) & 0xFFu);
-      if (k[c] == kByteEscape) k[c] = count_cell(a, P, cell0 + c);
-      mine += k[c];
+    for (int q = 0; q < G; q++) {
+      b0[q] = b1[q] = 0;
+      w[q] = make_uint2(0u, 0u);
+      if (g0 + q < a.npop) {
+        const PopTab &P = a.pop[g0 + q];
+        b0[q] = P.boff[blk];
+        b1[q] = P.boff[blk + 1];
+        w[q] = *(const uint2 *)(P.cnt8 + cell0);
+      }
     }
-    long long incl = mine;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const long long y = __shfl_up(incl, off);
-      if (lane >= off) incl += y;
-    }
-    long long pid = b0 + incl - mine;
+    for (int q = 0; q < G; q++) {
+      if (g0 + q >= a.npop || b1[q] == b0[q]) continue;   // uniform: no sources in this block
+      int k[kCellsPerLane];
+      long long mine = 0;
 #pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) {
-      const uint64_t cell = cell0 + c;
-      for (int is = 0; is < k[c]; is++, pid++)
-        P.desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
+      for (int c = 0; c < kCellsPerLane; c++) {
+        k[c] = (int)(((c < 4 ? w[q].x : w[q].y) >> (8 * (c & 3))) & 0xFFu);
+        if (k[c] == kByteEscape)
+          k[c] = escaped_count(a.esc, a.esc_count, ((unsigned long long)(g0 + q) << 56) | (cell0 + c));
+        mine += k[c];
+      }
+      long long incl = mine;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const long long y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+      }
+      long long pid = b0[q] + incl - mine;
+      uint4 *desc = a.pop[g0 + q].desc;
+#pragma unroll
+      for (int c = 0; c < kCellsPerLane; c++) {
+        const uint64_t cell = cell0 + c;
+        for (int is = 0; is < k[c]; is++, pid++)
+          if (pid < b1[q]) desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
+      }
     }
   }
 }
@@ -474,9 +509,10 @@ __global__ __launch_bounds__(256) void k_write(WriteArgs a) {
   const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= a.n) return;
   const uint4 ds = ((const uint4 *)a.pos)[pid];
-  const uint64_t cell = (uint64_t)ds.x | ((uint64_t)ds.y << 32);
-  const int is = (int)ds.z;
   const int N = a.N;
+  uint64_t cell = (uint64_t)ds.x | ((uint64_t)ds.y << 32);
+  if (cell >= (uint64_t)N * N * N) cell = 0;   // never: every slot holds a descriptor
+  const int is = (int)ds.z;
   const uint64_t rw = cell / (uint64_t)N;
   const int ix = (int)(cell - rw * N), iy = (int)(rw % N), iz = (int)(rw / N);
   const double x0 = (ix + 0.0) * a.dx - a.ox, y0 = (iy + 0.0) * a.dx - a.oy, z0 = (iz + 0.0) * a.dx - a.oz;
@@ -601,21 +637,45 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   if (!a.g || !a.r_arr) return clr_fail(c, CLR_ERR_STATE, "density grid or background tables missing");
 
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
-  hipLaunchKernelGGL(k_count, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
-  CLR_HIP(c, hipGetLastError());
-  CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
   std::vector<long long> tot(np);
-  for (int i = 0; i < np; i++) {
-    const PopTab &P = a.pop[i];
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i]);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanB), 0, c->stream, bsum[i], nsb);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i],
-                       (long long *)P.boff);
+  unsigned int nesc = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    // escape list: a counter, then entries (grown and recounted if it overflows)
+    const size_t want = c->esc_bytes > 0 ? c->esc_bytes : (size_t)16 + 16 * 65536;
+    rc = clr_ensure(c, &c->esc, &c->esc_bytes, want);
+    if (rc) return rc;
+    a.esc_n = (unsigned int *)c->esc;
+    a.esc = (ulonglong2 *)((char *)c->esc + 16);
+    a.esc_cap = (unsigned int)((c->esc_bytes - 16) / 16);
+    CLR_HIP(c, hipMemsetAsync(a.esc_n, 0, sizeof(unsigned int), c->stream));
+    CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
+    hipLaunchKernelGGL(k_count, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
-    CLR_HIP(c, hipMemcpyAsync(&tot[i], P.boff + nblocks, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
+    for (int i = 0; i < np; i++) {
+      const PopTab &P = a.pop[i];
+      hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i]);
+      hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanB), 0, c->stream, bsum[i], nsb);
+      hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i],
+                         (long long *)P.boff);
+      CLR_HIP(c, hipGetLastError());
+      CLR_HIP(c, hipMemcpyAsync(&tot[i], P.boff + nblocks, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    }
+    CLR_HIP(c, hipMemcpyAsync(&nesc, a.esc_n, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    CLR_HIP(c, hipStreamSynchronize(c->stream));
+    if (nesc <= a.esc_cap) break;
+    if (attempt == 1) return clr_fail(c, CLR_ERR_STATE, "escape list overflow (%u entries)", nesc);
+    rc = clr_ensure(c, &c->esc, &c->esc_bytes, (size_t)16 + 16 * ((size_t)nesc + nesc / 4 + 1024));
+    if (rc) return rc;
   }
-  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  a.esc_count = nesc;
+  if (nesc > 0) {
+    // sorted by key for k_fill's binary search
+    std::vector<ulonglong2> h(nesc);
+    CLR_HIP(c, hipMemcpy(h.data(), a.esc, (size_t)nesc * sizeof(ulonglong2), hipMemcpyDeviceToHost));
+    std::sort(h.begin(), h.end(), [](const ulonglong2 &u, const ulonglong2 &v) { return u.x < v.x; });
+    CLR_HIP(c, hipMemcpy(a.esc, h.data(), (size_t)nesc * sizeof(ulonglong2), hipMemcpyHostToDevice));
+  }
   float ms = 0;
   (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
   c->stage_ms[ST_SRC_PASS] = ms;
