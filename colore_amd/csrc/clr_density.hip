@@ -34,7 +34,7 @@ constexpr int kShellBlocks = 8192;
 constexpr int kShellThreads = 256;
 constexpr int kWaves = kShellThreads / 64;
 constexpr int kTY = 4, kTZ = 16;   // tile rows: kTY x kTZ rows of 64 cells
-constexpr int kSlots = 3;
+constexpr int kSlots = 2;
 constexpr int kMaxTr = 8;
 
 struct ShellArgs {

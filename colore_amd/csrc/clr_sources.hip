@@ -31,7 +31,8 @@ using namespace clr;
 namespace {
 
 constexpr int kLanes = 64, kCellsPerLane = 8, kBlockCells = kLanes * kCellsPerLane;
-constexpr int kByteEscape = 255;   // a stored count of 255: the count is in the escape list
+constexpr int kByteEscape = 255;    // stored for counts >= kBytePending: the count is in the escape list
+constexpr int kBytePending = 254;   // k_count left the cell to k_exact (never seen by k_fill)
 
 struct PopTab {
   const double *amp, *bz, *norm;
@@ -56,6 +57,11 @@ struct SrcArgs {
   ulonglong2 *esc;
   unsigned int *esc_n;
   unsigned int esc_cap, esc_count;
+  // cells whose count needs the FP64 path: (cell lo, cell hi, pass population,
+  // 32-bit uniform), appended by k_count and counted by k_exact
+  uint4 *exq;
+  unsigned int *exq_n;
+  unsigned int exq_cap;
   PopTab pop[CLR_NPOP_MAX];
 };
 
@@ -154,15 +160,21 @@ __device__ inline float bias_fast_t(float b, float bmax, float dl, float &E) {
   return dead ? 0.f : v;
 }
 
-// Phase A of k_count for the lane's 8 cells of one population: which cells
-// the 16 high uniform bits cannot settle as k = 0 (bit c of the result), and
-// their FP32 lambda (-1: FP64 path) and error bound.
+// queue entry: cell within the block | hi << 16; lambda (< 0: FP64 path); bound
+struct QEntry {
+  uint32_t ch;
+  float lam, err;
+};
+
+// Phase A of k_count for the lane's 8 cells of one population: the cells the
+// 16 high uniform bits cannot settle as k = 0 go to the LDS queue (ballot
+// compaction in lane order, qn uniform) with their FP32 lambda (-1: FP64
+// path) and error bound.
 template <int MODEL>
-__device__ inline unsigned undecided_cells(const float4 (&T)[kCellsPerLane], const float (&d)[kCellsPerLane],
-                                           const float (&tt)[kCellsPerLane], unsigned fastm, unsigned inm,
-                                           const u32x4 &rr, float relerr, float (&lam)[kCellsPerLane],
-                                           float (&err)[kCellsPerLane]) {
-  unsigned qm = 0;
+__device__ inline void queue_undecided(const float4 (&T)[kCellsPerLane], const float (&d)[kCellsPerLane],
+                                       const float (&tt)[kCellsPerLane], unsigned fastm, unsigned inm,
+                                       const u32x4 &rr, float relerr, int lane, QEntry *queue, int &qn) {
+  const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
   for (int c = 0; c < kCellsPerLane; c++) {
     const float4 Tc = T[c];
@@ -175,23 +187,18 @@ __device__ inline unsigned undecided_cells(const float4 (&T)[kCellsPerLane], con
     const float e = l * (2e-6f + relerr) + nn * E;
     const bool pos = Tc.x >= 0.f;
     const bool fast = (fastm >> c) & 1u, in = (inm >> c) & 1u;
-    const bool zero = poisson_zero_from_hi(l, __expf(-l), e, poisson_hi16(rr, (uint64_t)c));
+    const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
+    const bool zero = poisson_zero_from_hi(l, __expf(-l), e, hi);
     const bool done = fast && (!in || Tc.x == kBinZero || (pos && zero) || (Tc.x == kBinTiny && bias + E < 1e9f));
-    qm |= (done ? 0u : 1u) << c;
-    lam[c] = (fast && in && pos) ? l : -1.f;
-    err[c] = e;
+    const unsigned long long m = __ballot(!done);
+    if (!done)
+      queue[qn + __popcll(m & below)] =
+          QEntry{(uint32_t)(kCellsPerLane * lane + c) | (hi << 16), (fast && in && pos) ? l : -1.f, e};
+    qn += __popcll(m);
   }
-  return qm;
 }
 
-
-// queue entry: cell within the block | hi << 16; lambda (< 0: FP64 path); bound
-struct QEntry {
-  uint32_t ch;
-  float lam, err;
-};
-
-__global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
+__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4))) void k_count(SrcArgs a) {
   __shared__ QEntry queue[kBlockCells];
   __shared__ int cnt[kBlockCells];
   const int lane = threadIdx.x;
@@ -207,7 +214,6 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
   const int ix0 = o - drow * N;
   const unsigned row = (unsigned)row0 + (unsigned)drow;
   const int iy = (int)(row % (unsigned)N), iz = (int)(row / (unsigned)N);
-  const unsigned long long below = (1ull << lane) - 1ull;
 
   // --- the lane's 8 cells: density and geometry --------------------------
   // The fast path needs the radius only to well inside its margins (bin
@@ -261,21 +267,10 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
     for (int c = 0; c < kCellsPerLane; c++) T[c] = tab[ir[c]];
     const u32x4 rr = draw(cell0 >> 3, stream_poisson(pipop), 0u, a.seed);
     const float relerr = tab[a.na].x;   // product-interpolation bound of the table
-    float lam[kCellsPerLane], err[kCellsPerLane];
-    unsigned qm;
-    if (model == 0) qm = undecided_cells<0>(T, d, tt, fastm, inm, rr, relerr, lam, err);
-    else if (model == 1) qm = undecided_cells<1>(T, d, tt, fastm, inm, rr, relerr, lam, err);
-    else qm = undecided_cells<2>(T, d, tt, fastm, inm, rr, relerr, lam, err);
     int qn = 0;   // uniform
-#pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) {
-      const bool queue_it = (qm >> c) & 1u;
-      const unsigned long long m = __ballot(queue_it);
-      if (queue_it)
-        queue[qn + __popcll(m & below)] =
-            QEntry{(uint32_t)(kCellsPerLane * lane + c) | (poisson_hi16(rr, (uint64_t)c) << 16), lam[c], err[c]};
-      qn += __popcll(m);
-    }
+    if (model == 0) queue_undecided<0>(T, d, tt, fastm, inm, rr, relerr, lane, queue, qn);
+    else if (model == 1) queue_undecided<1>(T, d, tt, fastm, inm, rr, relerr, lane, queue, qn);
+    else queue_undecided<2>(T, d, tt, fastm, inm, rr, relerr, lane, queue, qn);
     __syncthreads();   // one wavefront: orders the LDS writes above before the reads below
     for (int i = lane; i < qn; i += kLanes) {
       const QEntry e = queue[i];
@@ -285,12 +280,9 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
       const uint32_t x = (hi << 16) | poisson_lo16(cell, P.ipop, a.seed);
       int kk = -1;
       if (e.lam >= 0.f) kk = poisson_fast_f32(e.lam, __expf(-e.lam), e.err, x);
-      if (kk < 0) {
-        const size_t rw = cell / (size_t)N;
-        const int cx = (int)(cell - rw * N);
-        kk = count_exact(cx, (int)(rw % N), (int)(rw / N), a.g[rw * (size_t)a.ngx + cx], cell, x, a.r_arr, P.amp,
-                         P.bz, P.norm, a.idr, a.r_last, P.bz0, P.norm0, P.normf, a.dx, a.ox, a.oy, a.oz,
-                         a.cell_vol, a.half_box_20, P.model, P.ipop, a.seed);
+      if (kk < 0) {   // too close to call in FP32: k_exact decides
+        const unsigned j = atomicAdd(a.exq_n, 1u);
+        if (j < a.exq_cap) a.exq[j] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)ip, x);
       }
       cnt[cb] = kk;
     }
@@ -301,8 +293,8 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
 #pragma unroll
     for (int c = 0; c < kCellsPerLane; c++) {
       const int k = cnt[kCellsPerLane * lane + c];
-      mine += k;
-      const uint32_t b = (uint32_t)(k < kByteEscape ? k : kByteEscape);
+      mine += k < 0 ? 0 : k;
+      const uint32_t b = (uint32_t)(k < 0 ? kBytePending : (k < kBytePending ? k : kByteEscape));
       if (c < 4) lo |= b << (8 * c);
       else hi8 |= b << (8 * (c - 4));
     }
@@ -312,7 +304,7 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
 #pragma unroll
       for (int c = 0; c < kCellsPerLane; c++) {
         const int k = cnt[kCellsPerLane * lane + c];
-        if (k >= kByteEscape) {
+        if (k >= kBytePending) {
           const unsigned e = atomicAdd(a.esc_n, 1u);
           if (e < a.esc_cap)
             a.esc[e] = make_ulonglong2(((unsigned long long)ip << 56) | (unsigned long long)(cell0 + c),
@@ -322,6 +314,31 @@ __global__ __launch_bounds__(kLanes) void k_count(SrcArgs a) {
     }
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
     if (lane == 0) P.btot[blk] = (uint32_t)mine;
+  }
+}
+
+// The FP64 counts of the cells k_count left pending: block totals, bytes and
+// escape entries completed in place (one thread per entry).
+__global__ __launch_bounds__(256) void k_exact(SrcArgs a) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned n = *a.exq_n;
+  if (j >= n || j >= a.exq_cap) return;
+  const uint4 e = a.exq[j];
+  const uint64_t cell = (uint64_t)e.x | ((uint64_t)e.y << 32);
+  const int ip = (int)e.z;
+  const PopTab &P = a.pop[ip];
+  const int N = a.N;
+  const uint64_t rw = cell / (uint64_t)N;
+  const int cx = (int)(cell - rw * N);
+  const int k = count_exact(cx, (int)(rw % N), (int)(rw / N), a.g[rw * (size_t)a.ngx + cx], cell, e.w, a.r_arr,
+                            P.amp, P.bz, P.norm, a.idr, a.r_last, P.bz0, P.norm0, P.normf, a.dx, a.ox, a.oy, a.oz,
+                            a.cell_vol, a.half_box_20, P.model, P.ipop, a.seed);
+  if (k > 0) atomicAdd(&P.btot[cell / kBlockCells], (uint32_t)k);
+  P.cnt8[cell] = (uint8_t)(k < kBytePending ? k : kByteEscape);
+  if (k >= kBytePending) {
+    const unsigned q = atomicAdd(a.esc_n, 1u);
+    if (q < a.esc_cap)
+      a.esc[q] = make_ulonglong2(((unsigned long long)ip << 56) | (unsigned long long)cell, (unsigned long long)k);
   }
 }
 
@@ -647,9 +664,19 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     a.esc_n = (unsigned int *)c->esc;
     a.esc = (ulonglong2 *)((char *)c->esc + 16);
     a.esc_cap = (unsigned int)((c->esc_bytes - 16) / 16);
+    // FP64 queue: a counter, then entries
+    const size_t xwant = c->misc_n >= 16 + 16 * 4096 ? c->misc_n : (size_t)16 + 16 * 65536;
+    rc = clr_ensure(c, &c->misc, &c->misc_n, xwant);
+    if (rc) return rc;
+    a.exq_n = (unsigned int *)c->misc;
+    a.exq = (uint4 *)((char *)c->misc + 16);
+    a.exq_cap = (unsigned int)((c->misc_n - 16) / 16);
     CLR_HIP(c, hipMemsetAsync(a.esc_n, 0, sizeof(unsigned int), c->stream));
+    CLR_HIP(c, hipMemsetAsync(a.exq_n, 0, sizeof(unsigned int), c->stream));
     CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
     hipLaunchKernelGGL(k_count, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
+    CLR_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_exact, dim3((a.exq_cap + 255) / 256), dim3(256), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
     CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
     for (int i = 0; i < np; i++) {
@@ -661,12 +688,20 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
       CLR_HIP(c, hipGetLastError());
       CLR_HIP(c, hipMemcpyAsync(&tot[i], P.boff + nblocks, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
     }
+    unsigned int nexq = 0;
     CLR_HIP(c, hipMemcpyAsync(&nesc, a.esc_n, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    CLR_HIP(c, hipMemcpyAsync(&nexq, a.exq_n, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
     CLR_HIP(c, hipStreamSynchronize(c->stream));
-    if (nesc <= a.esc_cap) break;
-    if (attempt == 1) return clr_fail(c, CLR_ERR_STATE, "escape list overflow (%u entries)", nesc);
-    rc = clr_ensure(c, &c->esc, &c->esc_bytes, (size_t)16 + 16 * ((size_t)nesc + nesc / 4 + 1024));
-    if (rc) return rc;
+    if (nesc <= a.esc_cap && nexq <= a.exq_cap) break;
+    if (attempt == 1) return clr_fail(c, CLR_ERR_STATE, "source lists overflow (%u, %u entries)", nesc, nexq);
+    if (nesc > a.esc_cap) {
+      rc = clr_ensure(c, &c->esc, &c->esc_bytes, (size_t)16 + 16 * ((size_t)nesc + nesc / 4 + 1024));
+      if (rc) return rc;
+    }
+    if (nexq > a.exq_cap) {
+      rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)16 + 16 * ((size_t)nexq + nexq / 4 + 1024));
+      if (rc) return rc;
+    }
   }
   a.esc_count = nesc;
   if (nesc > 0) {
