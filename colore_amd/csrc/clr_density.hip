@@ -34,7 +34,7 @@ constexpr int kShellBlocks = 8192;
 constexpr int kShellThreads = 256;
 constexpr int kWaves = kShellThreads / 64;
 constexpr int kTY = 4, kTZ = 16;   // tile rows: kTY x kTZ rows of 64 cells
-constexpr int kSlots = 2;
+constexpr int kSlots = 1;
 constexpr int kMaxTr = 8;
 
 struct ShellArgs {
@@ -150,7 +150,8 @@ constexpr size_t shell_lds_bytes(int nz) {
 }
 
 template <int NT>
-__global__ __launch_bounds__(kShellThreads) void k_shells(ShellArgs a, double *__restrict__ partial) {
+__global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_shells(ShellArgs a,
+                                                                                      double *__restrict__ partial) {
 #pragma clang fp contract(off)
   constexpr int NQ = 2 + NT;
   constexpr int W = (NT + 5) / 4;
