@@ -197,6 +197,15 @@ double clr_run_scalar(const clr_run *r, const char *n) {
   return NAN;
 }
 
+int clr_run_imap_shells(const clr_run *r, int ipop, int *nside, int *nr, const double **r0, const double **rf) {
+  if (ipop < 0 || ipop >= r->n_tr[1]) return 1;
+  *nside = r->nside_imap[ipop];
+  *nr = r->imap_nr[ipop];
+  *r0 = r->imap_r0[ipop];
+  *rf = r->imap_rf[ipop];
+  return 0;
+}
+
 int clr_run_tables(const clr_run *r, clr_tables *t) {
   memset(t, 0, sizeof(*t));
   t->na = CLR_NA;

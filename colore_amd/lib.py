@@ -103,6 +103,8 @@ def load_host():
     _sig(lib, "clr_run_scalar", cd, [_vp, ctypes.c_char_p])
     _sig(lib, "clr_run_tables", ci, [_vp, ctypes.POINTER(Tables)])
     _sig(lib, "clr_run_tracer", ci, [_vp, ci, ci, ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(ci)])
+    _sig(lib, "clr_run_imap_shells", ci, [_vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(_d),
+                                          ctypes.POINTER(_d)])
     _sig(lib, "clr_run_get_bg", cd, [_vp, cd, ci, ci])
     _sig(lib, "clr_run_r_of_z", cd, [_vp, cd])
     _sig(lib, "clr_hp_ang2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, cd, cd])
@@ -154,6 +156,16 @@ class Run:
                     logkmax=t.logkmax, idlogk=t.idlogk, ns=t.n_scal, fgrowth_0=t.fgrowth_0,
                     hubble_0=t.hubble_0, omega_m=t.omega_m, prefac_lensing=t.prefac_lensing,
                     r_max=t.r_max)
+
+    def imap_shells(self, ipop):
+        """(nside, r0, rf) of intensity species ipop."""
+        ns, nr, r0, rf = ctypes.c_int(), ctypes.c_int(), _d(), _d()
+        if self.host.clr_run_imap_shells(self.h, ipop, ctypes.byref(ns), ctypes.byref(nr), ctypes.byref(r0),
+                                         ctypes.byref(rf)):
+            raise IndexError(ipop)
+        n = nr.value
+        return (ns.value, np.ctypeslib.as_array(r0, shape=(n,)).copy(),
+                np.ctypeslib.as_array(rf, shape=(n,)).copy())
 
     def tracer(self, kind, ipop):
         a, b, m = _d(), _d(), ctypes.c_int()

@@ -57,6 +57,10 @@ int      clr_run_tables(const clr_run *r, clr_tables *out);
 /* Tracer r-tables of tracer kind/ipop (pointers into the run). */
 int      clr_run_tracer(const clr_run *r, int kind, int ipop, const double **amp, const double **bz,
                         int *bias_model);
+/* Shells of intensity species ipop (imap_preproc, imap.c:105-122: comoving
+ * r0/rf from the frequency table, pointers into the run): nside and nr. */
+int      clr_run_imap_shells(const clr_run *r, int ipop, int *nside, int *nr, const double **r0,
+                             const double **rf);
 /* Output prefix. */
 const char *clr_run_prefix(const clr_run *r);
 /* get_bg / r_of_z on the run's tables (cosmo.c:40-112). */

@@ -11,6 +11,8 @@ Tolerances (FP32 device grids against the FP64 oracle):
     the same density grid and norm tables; dz_rsd within 1e-12 relative.
   * kappa / ISW maps from the same potential: 1e-5 of the map RMS.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -202,3 +204,58 @@ def test_los_maps(workdir, which):
     o = _oracle(run).kappa_isw(0 if which == "kappa" else 1, N, run.scalar("l_box"), npot, nside, r_src)
     rms = np.sqrt(np.mean(o ** 2))
     assert np.abs(out - o).max() < 1e-5 * rms + 1e-30, (np.abs(out - o).max(), rms)
+
+
+def test_imap_vs_oracle(workdir):
+    """21cm intensity shells painted on the device against the oracle's walk
+    over the same grids: identical hit counts per pixel, temperatures summed
+    in FP32 atomics within 1e-5 of the FP64 sums."""
+    N = 64
+    path, run = make_run("C3", N, workdir, z_max=1.6, imap_bands=8, imap_nside=16)
+    ctx = _ctx_for(run, N)
+    s2 = ctx.gaussian_fields(3003, run.scalar("r_smooth"), True)
+    ntr = int(run.scalar("n_srcs")) + int(run.scalar("n_imap"))
+    ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=ntr)
+    dens = ctx.fetch_grid(0).astype(np.float64)
+    npot = ctx.fetch_grid(1).astype(np.float64)
+    nside, r0, rf = run.imap_shells(0)
+    data, nadd = ctx.imap(0, nside, r0, rf)
+    tz, bz, model = run.tracer(1, 0)
+    tab, n0, nf = ctx.norm_table(1, 0)
+    od, on = _oracle(run).imap(N, run.scalar("l_box"), dens, npot, tz, bz, tab, n0, nf, model, nside, r0, rf)
+    assert nadd.sum() > 0
+    assert np.array_equal(nadd, on)
+    scale = np.abs(od).max()
+    assert np.abs(data - od).max() <= 1e-5 * scale, (np.abs(data - od).max(), scale)
+
+
+def test_cli_end_to_end(tmp_path):
+    """`CoLoRe param.cfg` on the device: every output file of the run, the
+    catalogues holding exactly the sources the library samples for the same
+    seed and grids."""
+    import glob
+    import subprocess
+    from conftest import ROOT
+    from colore_amd import inputs, lib
+    N = 64
+    p = inputs.write_config("C3", str(tmp_path), n_grid=N, z_max=1.6, output_format="FITS", output_density=True,
+                            imap_bands=4, imap_nside=16, kappa_nside=16)
+    exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
+    r = subprocess.run([exe, p], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    files = sorted(os.path.basename(f) for f in glob.glob(str(tmp_path / "out*")))
+    for want in ("out_dens_gaussian_0.dat", "out_dens_lightcone_0.dat", "out_srcs_s1_0.fits",
+                 "out_srcs_s3_0.fits", "out_kappa_z000.fits", "out_isw_z002.fits", "out_imap_s1_nu003.fits"):
+        assert want in files, (want, files)
+    run = lib.Run(p)
+    ctx = _ctx_for(run, N)
+    seed = int(run.scalar("seed"))
+    s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
+    ctx.density(0, s2)
+    ctx.shell_norms(lognormal_fused=False, sigma2=s2, ntracers=int(run.scalar("n_srcs")) + int(run.scalar("n_imap")))
+    for i in range(int(run.scalar("n_srcs"))):
+        n = ctx.sample_sources(i, seed, int(run.scalar("nside_base")))
+        with open(tmp_path / f"out_srcs_s{i + 1}_0.fits", "rb") as f:
+            head = f.read(2880 * 4).decode("ascii", "replace")
+        k = head.index("NAXIS2")
+        assert int(head[k + 10:k + 30].strip()) == n, (i, head[k:k + 30], n)
