@@ -77,6 +77,44 @@ def traffic_per_launch(kernel, config, N):
     return None
 
 
+def time_maps(run, ctx, config, sigma2):
+    """kappa and ISW at the workload's source planes and the 21cm shells of its
+    first intensity species, on the last step's grids: device ms per call
+    (median of 3) and work rates."""
+    import numpy as np
+    from colore_amd import lib
+    out = {}
+    N = int(run.scalar("n_grid"))
+    for which in ("kappa", "isw"):
+        n = int(run.scalar(f"n_{which}"))
+        if n == 0:
+            continue
+        nside = int(run.scalar(f"nside_{which}"))
+        from colore_amd import inputs
+        zs = [z for z in inputs.CONFIGS[config][which]["z_out"] if z <= run.scalar("z_max")]
+        r_src = np.array([lib.load_host().clr_run_r_of_z(run.h, z) for z in zs])
+        ts = []
+        for _ in range(3):
+            (ctx.kappa if which == "kappa" else ctx.isw)(nside, r_src)
+            ts.append(ctx.stage_times()[which])
+        ms = sorted(ts)[1]
+        out[which] = {"ms": ms, "nside": nside, "planes": n,
+                      "pixel_steps_per_s": 12 * nside * nside * (N // 2) / (ms * 1e-3)}
+    if int(run.scalar("n_imap")) > 0:
+        # the intensity species joins the shell sums of the last step's density
+        amp, bz, model = run.tracer(1, 0)
+        ctx.set_tracer(1, 0, model, amp, bz)
+        ctx.shell_norms(lognormal_fused=False, sigma2=sigma2, ntracers=int(run.scalar("n_srcs")) + 1)
+        nside, r0, rf = run.imap_shells(0)
+        ts = []
+        for _ in range(3):
+            ctx.imap(0, nside, r0, rf)
+            ts.append(ctx.stage_times()["imap"])
+        ms = sorted(ts)[1]
+        out["imap"] = {"ms": ms, "nside": nside, "shells": len(r0), "cells_per_s": N ** 3 / (ms * 1e-3)}
+    return out
+
+
 def copy_probe(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size, for scaling the PMC byte
     counters (run under rocprofv3 --pmc next to the bench)."""
@@ -128,6 +166,8 @@ def main():
     ap.add_argument("--n-grid", type=int, default=None)
     ap.add_argument("--cpu-grid", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--maps", action="store_true",
+                    help="also time the kappa/ISW line-of-sight maps and the 21cm painting of the workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (PMC counter calibration)")
     args = ap.parse_args()
@@ -141,7 +181,7 @@ def main():
 
     from colore_amd import inputs, lib
     d = tempfile.mkdtemp(prefix="colore_bench_")
-    run = lib.Run(inputs.write_config(args.config, d, n_grid=args.n_grid, with_maps=False))
+    run = lib.Run(inputs.write_config(args.config, d, n_grid=args.n_grid, with_maps=args.maps))
     N, L = int(run.scalar("n_grid")), run.scalar("l_box")
     seed, rs = int(run.scalar("seed")), run.scalar("r_smooth")
     npop = int(run.scalar("n_srcs"))
@@ -209,6 +249,8 @@ def main():
         "kernels": {k: {"ms": kt[k], "GB/s": kb[k] / (kt[k] * 1e-3) / 1e9 if kt[k] > 0 else None}
                     for k in kt},
     }
+    if args.maps:
+        out["maps"] = time_maps(run, ctx, args.config, s2)
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_grid)
