@@ -34,6 +34,20 @@ int clr_ensure(clr_ctx *c, void **p, size_t *have, size_t bytes) {
   return CLR_OK;
 }
 
+int clr_phi_ready(clr_ctx *c, int host) {
+  if (!c->phi_pending) return CLR_OK;
+  if (!host) {
+    CLR_HIP(c, hipStreamWaitEvent(c->stream, c->ev_phi, 0));
+    return CLR_OK;
+  }
+  CLR_HIP(c, hipStreamSynchronize(c->stream2));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev4, c->ev5);
+  c->stage_ms[ST_FFT_PHI] = ms;
+  c->phi_pending = 0;
+  return CLR_OK;
+}
+
 int clr_ring2nest_table(clr_ctx *c, int nside) {
   if (c->r2n && c->r2n_nside == nside) return CLR_OK;
   const int64_t npix = 12LL * nside * nside;
@@ -58,7 +72,12 @@ static void free_all(clr_ctx *c) {
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev2) hipEventDestroy(c->ev2);
+  if (c->ev_gen) hipEventDestroy(c->ev_gen);
+  if (c->ev_phi) hipEventDestroy(c->ev_phi);
+  if (c->ev4) hipEventDestroy(c->ev4);
+  if (c->ev5) hipEventDestroy(c->ev5);
   if (c->ev3) hipEventDestroy(c->ev3);
+  if (c->stream2) { hipStreamSynchronize(c->stream2); hipStreamDestroy(c->stream2); }
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -87,6 +106,11 @@ int clr_create(clr_ctx **out, int n_grid, double l_box, const double pos_obs[3],
   *out = c;
   CLR_HIP(c, hipSetDevice(device));
   CLR_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  CLR_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  CLR_HIP(c, hipEventCreateWithFlags(&c->ev_gen, hipEventDisableTiming));
+  CLR_HIP(c, hipEventCreateWithFlags(&c->ev_phi, hipEventDisableTiming));
+  CLR_HIP(c, hipEventCreate(&c->ev4));
+  CLR_HIP(c, hipEventCreate(&c->ev5));
   CLR_HIP(c, hipEventCreate(&c->ev0));
   CLR_HIP(c, hipEventCreate(&c->ev1));
   CLR_HIP(c, hipEventCreate(&c->ev2));
@@ -112,6 +136,7 @@ void clr_destroy(clr_ctx *ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
   free_all(ctx);
   delete ctx;
 }
@@ -293,6 +318,8 @@ int clr_fetch_grid(clr_ctx *c, int which, float *host) {
   if (which != CLR_GRID_DENS && which != CLR_GRID_NPOT) return clr_fail(c, CLR_ERR_ARG, "bad grid %d", which);
   CLR_HIP(c, hipSetDevice(c->device));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
+  int rc = clr_phi_ready(c, 1);
+  if (rc) return rc;
   CLR_HIP(c, hipMemcpy(host, which == CLR_GRID_DENS ? c->dens : c->npot, c->grid_floats * sizeof(float),
                        hipMemcpyDeviceToHost));
   return CLR_OK;
@@ -303,6 +330,8 @@ int clr_upload_grid(clr_ctx *c, int which, const float *host) {
   if (which != CLR_GRID_DENS && which != CLR_GRID_NPOT) return clr_fail(c, CLR_ERR_ARG, "bad grid %d", which);
   CLR_HIP(c, hipSetDevice(c->device));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
+  int rc = clr_phi_ready(c, 1);
+  if (rc) return rc;
   CLR_HIP(c, hipMemcpy(which == CLR_GRID_DENS ? c->dens : c->npot, host, c->grid_floats * sizeof(float),
                        hipMemcpyHostToDevice));
   return CLR_OK;
@@ -310,6 +339,8 @@ int clr_upload_grid(clr_ctx *c, int which, const float *host) {
 
 int clr_declare_grids(clr_ctx *c, double sigma2, int density_transformed) {
   if (!c) return CLR_ERR_ARG;
+  int rc = clr_phi_ready(c, 1);
+  if (rc) return rc;
   c->have_fields = 1;
   c->sigma2 = sigma2;
   c->density_done = density_transformed ? 1 : 0;
@@ -320,6 +351,11 @@ int clr_declare_grids(clr_ctx *c, double sigma2, int density_transformed) {
 int clr_stage_times(const clr_ctx *c, double *ms, int n) {
   if (!c || !ms) return CLR_ERR_ARG;
   for (int i = 0; i < n && i < ST_COUNT; i++) ms[i] = c->stage_ms[i];
+  // phi's passes: read without waiting (the last finished value otherwise)
+  if (c->phi_pending && ST_FFT_PHI < n && hipEventQuery(c->ev5) == hipSuccess) {
+    float f = 0;
+    if (hipEventElapsedTime(&f, c->ev4, c->ev5) == hipSuccess) ms[ST_FFT_PHI] = f;
+  }
   return ST_COUNT;
 }
 

@@ -377,20 +377,38 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
 
   float2 *d = (float2 *)c->dens, *p = (float2 *)c->npot;
   const int ntile = (NH + L.colT - 1) / L.colT;
+  const float norm = (float)pow(sqrt(2 * M_PI) / c->L, 3);
+  // the previous call's phi passes (stream2) must be done before the modes
+  // are overwritten
+  rc = clr_phi_ready(c, 0);
+  if (rc) return rc;
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
   L.gen(dim3((unsigned)(ntile * N)), dim3(L.colThreads), c->stream, d, p, c->amp, c->tw, seed, inj);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
-  L.coly(dim3((unsigned)(ntile * N), 1, 2), dim3(L.colThreads), c->stream, d, p, c->tw);
+  // delta: y and x passes with the sigma^2 partials
+  L.coly(dim3((unsigned)(ntile * N), 1, 1), dim3(L.colThreads), c->stream, d, d, c->tw);
   CLR_HIP(c, hipGetLastError());
   hipEvent_t e2 = c->ev2, e3 = c->ev3;
   CLR_HIP(c, hipEventRecord(e2, c->stream));
-  const float norm = (float)pow(sqrt(2 * M_PI) / c->L, 3);
-  L.c2r(dim3((unsigned)nxb, 2), dim3(L.xThreads), c->stream, c->dens, c->npot, norm, c->red, nrows, c->tw);
+  L.c2r(dim3((unsigned)nxb, 1), dim3(L.xThreads), c->stream, c->dens, c->dens, norm, c->red, nrows, c->tw);
   CLR_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, c->stream, c->red, nxb, c->red_out);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(e3, c->stream));
+  // phi: y and x passes on stream2 after the density's (memory-bound like
+  // them), overlapping the instruction-bound stages that follow on stream;
+  // every reader of npot waits on ev_phi
+  CLR_HIP(c, hipEventRecord(c->ev_gen, c->stream));
+  CLR_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_gen, 0));
+  CLR_HIP(c, hipEventRecord(c->ev4, c->stream2));
+  L.coly(dim3((unsigned)(ntile * N), 1, 1), dim3(L.colThreads), c->stream2, p, p, c->tw);
+  CLR_HIP(c, hipGetLastError());
+  L.c2r(dim3((unsigned)nxb, 1), dim3(L.xThreads), c->stream2, c->npot, c->npot, norm, nullptr, nrows, c->tw);
+  CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev5, c->stream2));
+  CLR_HIP(c, hipEventRecord(c->ev_phi, c->stream2));
+  c->phi_pending = 1;
   double s[2];
   CLR_HIP(c, hipMemcpyAsync(s, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CLR_HIP(c, hipStreamSynchronize(c->stream));

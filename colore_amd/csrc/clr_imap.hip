@@ -144,6 +144,10 @@ int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, c
   }
   ImapArgs a{};
   a.dens = c->dens;
+  {
+    const int prc = clr_phi_ready(c, 0);   // phi's passes may still run on the second stream
+    if (prc) return prc;
+  }
   a.phi = c->npot;
   a.r_arr = c->bg;
   a.tz = T.amp;

@@ -32,6 +32,7 @@ enum StageId {
   ST_GEN = 0, ST_FFT, ST_XPASS, ST_DENS, ST_NORM, ST_SRCS, ST_KAPPA, ST_ISW, ST_IMAP,
   ST_SRC_PASS,    // k_count of a sample_sources call
   ST_SRC_WRITE,   // its k_fill and k_write launches
+  ST_FFT_PHI,     // y and x passes of phi (second stream, overlapping later stages)
   ST_COUNT
 };
 
@@ -43,6 +44,11 @@ struct clr_ctx {
   double L = 0, dx = 0;
   double obs[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
+  // phi's y and x passes run on stream2 while stream goes on with the density;
+  // ev_phi marks their end, every reader of npot waits on it (clr_phi_ready)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_gen = nullptr, ev_phi = nullptr, ev4 = nullptr, ev5 = nullptr;
+  int phi_pending = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   double stage_ms[ST_COUNT] = {0};
 
@@ -115,6 +121,9 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
 int clr_launch_density(clr_ctx *c, int dens_type, double sigma2);
 int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, int *nz_out,
                      unsigned long long *counts, double *zsum, double *sums);
+// Makes npot safe to use: host = 0 orders c->stream after phi's passes,
+// host = 1 waits for them on the host.
+int clr_phi_ready(clr_ctx *c, int host);
 int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc);
 int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
 int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out);

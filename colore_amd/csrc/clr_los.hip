@@ -122,6 +122,10 @@ int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *
   for (int i = 0; i < nplanes; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rsf[a] < rsf[b]; });
   LosArgs a{};
+  {
+    const int prc = clr_phi_ready(c, 0);   // phi's passes may still run on the second stream
+    if (prc) return prc;
+  }
   a.phi = c->npot;
   a.N = N;
   a.ngx = c->ngx;

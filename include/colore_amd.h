@@ -6,7 +6,11 @@
  * returns an int status: 0 on success, <0 for a HIP/runtime failure, >0 for
  * an input/specification error.  clr_last_error() returns the message of the
  * last failure on a context.  Calls are synchronous at return (stream-ordered
- * inside); one host thread per context.  Host buffers are owned by the caller.
+ * inside), with one exception: the y and x passes of the potential phi run on
+ * a second stream and may still be running when clr_gaussian_fields returns;
+ * every call that reads phi (sources, kappa, ISW, imap, grid copies) waits
+ * for them.  One host thread per context.  Host buffers are owned by the
+ * caller.
  *
  * In the reference every stage is `void f(ParamCoLoRe *par)` declared in
  * src/common.h:436-543 and sequenced by src/main.c:61-141; each function
@@ -188,8 +192,10 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
  * in this order: mode generation + z pass, y pass, x pass, density
  * transform, shell sums, sources (whole call), kappa, ISW, imap, the
  * counting pass of the sources call, its catalogue writers (descriptors and
- * per-source positions).  Fills at most n values; returns the number of
- * stages. */
+ * per-source positions), the y and x passes of phi (these run on a second
+ * stream, overlapping the density's passes and the stages after them; every
+ * call that reads phi waits for them).  Fills at most n values; returns the
+ * number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 
 #ifdef __cplusplus
