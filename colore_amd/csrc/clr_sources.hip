@@ -36,7 +36,7 @@ constexpr int kBytePending = 254;   // k_count left the cell to k_exact (never s
 
 struct PopTab {
   const double *amp, *bz, *norm;
-  const float4 *tab;         // k_pop_table's bins (na + 1 entries)
+  const float4 *tab;         // k_pop_table's bins, bound and coarse bounds
   double bz0, norm0, normf;
   int model, ipop;
   uint8_t *cnt8;             // one byte per cell
@@ -83,6 +83,19 @@ struct SrcArgs {
 //   Other (x = -3): the FP64 path decides.
 constexpr float kBinZero = -1.f, kBinTiny = -2.f, kBinOther = -3.f;
 
+// After the bins and the bound (element na) come the coarse bounds, written
+// by k_pop_coarse: entry j covers the bins [16j, 16j+16), i.e. the nodes
+// 16j..16j+16, as (Pmax, bmin, bmax, 0):
+//   Pmax >= n(r) dx^3 norm(r) for the reference's separate FP64 interpolants
+//   of n and norm anywhere in those bins (the largest n times the largest
+//   norm at the nodes, rounded up; NaN nodes of n give no sources there),
+//   bmin, bmax: the extreme b at the nodes.
+//   (+inf, -inf, +inf): no bound -- a node with n = +-inf, norm < 0 or not
+//   finite, b < 0, |b| >= 1e6 or NaN, or Pmax > 1e30.
+constexpr int kCoarse = 16;
+inline int coarse_count(int na) { return (na + kCoarse - 1) / kCoarse; }
+inline size_t pop_table_len(int na) { return (size_t)na + 1 + coarse_count(na); }
+
 __global__ void k_pop_table(const double *__restrict__ amp, const double *__restrict__ bz,
                             const double *__restrict__ norm, double cv, int na, float4 *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,6 +123,31 @@ __global__ void k_pop_table(const double *__restrict__ amp, const double *__rest
     }
   }
   out[i] = make_float4(kBinOther, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_pop_coarse(const double *__restrict__ amp, const double *__restrict__ bz,
+                             const double *__restrict__ norm, double cv, int na, float4 *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= (na + kCoarse - 1) / kCoarse) return;
+  const int i0 = j * kCoarse, i1 = min(i0 + kCoarse, na - 1);
+  double nmax = 0, mmax = 0, bmin = 1e300, bmax = 0;
+  bool ok = true;
+  for (int i = i0; i <= i1; i++) {
+    const double n = amp[i], m = norm[i], b = bz[i];
+    if (isinf(n)) ok = false;
+    else if (n > nmax) nmax = n;
+    if (m >= 0 && m < 1e300) mmax = fmax(mmax, m);
+    else ok = false;
+    if (b >= 0 && b < 1e6) {
+      bmin = fmin(bmin, b);
+      bmax = fmax(bmax, b);
+    } else {
+      ok = false;
+    }
+  }
+  const double P = nmax * cv * mmax;
+  out[na + 1 + j] = ok && P <= 1e30 ? make_float4((float)(P * (1 + 1e-6)), (float)bmin, (float)bmax, 0.f)
+                                    : make_float4(INFINITY, -INFINITY, INFINITY, 0.f);
 }
 
 // The reference's FP64 count of cell (ix,iy,iz) (srcs.c:136-187) from the
@@ -160,47 +198,86 @@ __device__ inline float bias_fast_t(float b, float bmax, float dl, float &E) {
   return dead ? 0.f : v;
 }
 
-// queue entry: cell within the block | hi << 16; lambda (< 0: FP64 path); bound
-struct QEntry {
-  uint32_t ch;
-  float lam, err;
-};
-
-// Phase A of k_count for the lane's 8 cells of one population: the cells the
-// 16 high uniform bits cannot settle as k = 0 go to the LDS queue (ballot
-// compaction in lane order, qn uniform) with their FP32 lambda (-1: FP64
-// path) and error bound.
 template <int MODEL>
-__device__ inline void queue_undecided(const float4 (&T)[kCellsPerLane], const float (&d)[kCellsPerLane],
-                                       const float (&tt)[kCellsPerLane], unsigned fastm, unsigned inm,
-                                       const u32x4 &rr, float relerr, int lane, QEntry *queue, int &qn) {
-  const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int c = 0; c < kCellsPerLane; c++) {
-    const float4 Tc = T[c];
-    const float t = tt[c];
-    const float b = Tc.z + Tc.w * t;
-    float E;
-    const float bias = bias_fast_t<MODEL>(b, fmaxf(fabsf(Tc.z), fabsf(Tc.z + Tc.w)), d[c], E);
-    const float nn = Tc.x + Tc.y * t;
-    const float l = nn * bias;
-    const float e = l * (2e-6f + relerr) + nn * E;
-    const bool pos = Tc.x >= 0.f;
-    const bool fast = (fastm >> c) & 1u, in = (inm >> c) & 1u;
-    const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
-    const bool zero = poisson_zero_from_hi(l, __expf(-l), e, hi);
-    const bool done = fast && (!in || Tc.x == kBinZero || (pos && zero) || (Tc.x == kBinTiny && bias + E < 1e9f));
-    const unsigned long long m = __ballot(!done);
-    if (!done)
-      queue[qn + __popcll(m & below)] =
-          QEntry{(uint32_t)(kCellsPerLane * lane + c) | (hi << 16), (fast && in && pos) ? l : -1.f, e};
-    qn += __popcll(m);
-  }
+__device__ inline float bias_upper(float dmax, float blo, float bhi) {
+  float E1, E2;
+  const float v1 = bias_fast_t<MODEL>(blo, bhi, dmax, E1);
+  const float v2 = bias_fast_t<MODEL>(bhi, bhi, dmax, E2);
+  return fmaxf(v1 + E1, v2 + E2) * (1.f + 1e-6f);
 }
 
+__device__ inline float bias_fast_rt(int model, float b, float bmax, float dl, float &E) {
+  if (model == 0) return bias_fast_t<0>(b, bmax, dl, E);
+  if (model == 1) return bias_fast_t<1>(b, bmax, dl, E);
+  return bias_fast_t<2>(b, bmax, dl, E);
+}
+
+// Appends a cell to the FP64 queue of k_exact with its full uniform.
+__device__ inline void to_exact(const SrcArgs &a, size_t cell, int ip, uint32_t x) {
+  const unsigned j = atomicAdd(a.exq_n, 1u);
+  if (j < a.exq_cap) a.exq[j] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)ip, x);
+}
+
+// Count of one cell that the lane bound left open (srcs.c:136-187): the
+// cell's own geometry and bin, then
+//   outside r < L/2+20, or an empty bin: 0;
+//   a Pos bin: the FP32 lambda and its error bound decide k = 0 from the 16
+//   high bits, else the FP32 search with the full uniform decides;
+//   a Tiny bin: 0 while the bias stays below 1e9;
+// and everything else -- the radius within 1e-9 of a bin or box edge, the
+// last bins, Other bins, a search too close to call -- goes to k_exact (-1).
+__device__ inline int settle_cell(const SrcArgs &a, const float4 *tab, int model, int pipop, int ip, size_t cell,
+                                  int ix, int iy, int iz, uint32_t hi) {
+  const float dl = a.g[((size_t)iz * a.N + iy) * a.ngx + ix];
+  const double xx = ix * a.dx - a.ox, y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
+  const double r2 = xx * xx + (y0 * y0 + z0 * z0);
+  // r = r2 / sqrt(r2): FP32 rsqrt refined once in FP64 (rel. error ~2e-14)
+  const float y = __builtin_amdgcn_rsqf((float)r2);
+  const double yd = (double)y * (1.5 - 0.5 * r2 * (double)y * (double)y);
+  const double r = r2 * yd;
+  const double xr = r * a.idr;
+  const int i = (int)xr;
+  const double fr = xr - i;
+  const double hb = a.half_box_20;
+  const bool fast = r2 > 0 && fr > 1e-9 && fr < 1 - 1e-9 && i < a.na - 2 && fabs(r - hb) > 1e-9 * hb;
+  if (fast && !(r < hb)) return 0;
+  if (fast) {
+    const float4 T = tab[i];
+    const float t = (float)fr;
+    if (T.x == kBinZero) return 0;
+    if (T.x == kBinTiny) {
+      float E;
+      const float bias = bias_fast_rt(model, T.z + T.w * t, fmaxf(fabsf(T.z), fabsf(T.z + T.w)), dl, E);
+      if (bias + E < 1e9f) return 0;
+    } else if (T.x >= 0.f) {
+      float E;
+      const float bias = bias_fast_rt(model, T.z + T.w * t, fmaxf(fabsf(T.z), fabsf(T.z + T.w)), dl, E);
+      const float nn = T.x + T.y * t;
+      const float l = nn * bias;
+      const float err = l * (2e-6f + tab[a.na].x) + nn * E;
+      const float ex = __expf(-l);
+      if (poisson_zero_from_hi(l, ex, err, hi)) return 0;
+      const uint32_t x = (hi << 16) | poisson_lo16(cell, pipop, a.seed);
+      const int k = poisson_fast_f32(l, ex, err, x);
+      if (k >= 0) return k;
+      to_exact(a, cell, ip, x);
+      return -1;
+    }
+  }
+  to_exact(a, cell, ip, (hi << 16) | poisson_lo16(cell, pipop, a.seed));
+  return -1;
+}
+
+// queue entry: cell within the block (bits 0-8) | population (bits 9-12) |
+// the uniform's high 16 bits << 16
+constexpr int kQueueCap = 2 * kBlockCells;
+
 __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4))) void k_count(SrcArgs a) {
-  __shared__ QEntry queue[kBlockCells];
-  __shared__ int cnt[kBlockCells];
+  __shared__ uint32_t queue[kQueueCap];
+  __shared__ uint2 sbytes[CLR_NPOP_MAX][kLanes];   // one byte per cell and population
+  __shared__ uint32_t stot[CLR_NPOP_MAX];
+  __shared__ const float4 *stab[CLR_NPOP_MAX];
+  __shared__ int smodel[CLR_NPOP_MAX];
   const int lane = threadIdx.x;
   const size_t blk = blockIdx.x;
   const size_t cell0 = blk * kBlockCells + (size_t)kCellsPerLane * lane;
@@ -209,111 +286,140 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4))) voi
   // 64-bit arithmetic, the lane part in 32 bits (rows < 2^22)
   const size_t base = blk * kBlockCells;
   const size_t row0 = base / (size_t)N;
-  const int o = (int)(base - row0 * N) + kCellsPerLane * lane;
+  const int obase = (int)(base - row0 * N);
+  const int o = obase + kCellsPerLane * lane;
   const int drow = o / N;
   const int ix0 = o - drow * N;
   const unsigned row = (unsigned)row0 + (unsigned)drow;
   const int iy = (int)(row % (unsigned)N), iz = (int)(row / (unsigned)N);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int ip = 0; ip < a.npop; ip++) {
+    sbytes[ip][lane] = make_uint2(0u, 0u);
+    if (lane == 0) {
+      stot[ip] = 0;
+      stab[ip] = a.pop[ip].tab;
+      smodel[ip] = a.pop[ip].model;
+    }
+  }
 
-  // --- the lane's 8 cells: density and geometry --------------------------
-  // The fast path needs the radius only to well inside its margins (bin
-  // fraction and box edge both 1e-9 away), so it is evaluated here in the
-  // cheapest stable order; count_exact recomputes it the reference's way.
-  float d[kCellsPerLane], tt[kCellsPerLane];
-  int ir[kCellsPerLane];
-  unsigned fastm = 0, inm = 0;
+  // --- the lane's 8 cells: largest delta and the range of their radii -------
+  // Every count of the lane is 0 where u < exp(-lambda_max) for an upper
+  // bound lambda_max of the reference's lambda over the 8 cells:
+  //   lambda = n(r) dx^3 norm(r) bias_model(delta, b(r)) <= Pmax * bias_max
+  // with Pmax and the range of b from the coarse bounds of the bins that the
+  // radii can fall in, and bias_max the larger corner (delta_max, b_min or
+  // b_max): with b >= 0 every bias model is non-decreasing in delta and
+  // monotone in b.  Cells outside r < L/2+20 have lambda = 0 and need no
+  // special case.  The radii come from the two end cells (the nearest point
+  // of the row when it lies between them), in FP32 with a 1e-5 margin.
+  float dmax;
   {
     const float2 *p = (const float2 *)(a.g + (size_t)row * a.ngx + ix0);
+    float d[kCellsPerLane];
 #pragma unroll
     for (int h = 0; h < kCellsPerLane / 2; h++) {
       const float2 v = p[h];
       d[2 * h] = v.x;
       d[2 * h + 1] = v.y;
     }
+    dmax = d[0];
+#pragma unroll
+    for (int c = 1; c < kCellsPerLane; c++) dmax = fmaxf(dmax, d[c]);
   }
+  bool out;
+  int ilo, ihi;
   {
     const double y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
     const double yz2 = y0 * y0 + z0 * z0;
-    const double xx0 = ix0 * a.dx - a.ox;
-    const double hb = a.half_box_20, hbm = 1e-9 * a.half_box_20;
-#pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) {
-      const double xx = xx0 + c * a.dx;
-      const double r2 = xx * xx + yz2;
-      // r = r2 / sqrt(r2): FP32 rsqrt refined once in FP64 (rel. error ~2e-14)
-      const float y = __builtin_amdgcn_rsqf((float)r2);
-      const double yd = (double)y * (1.5 - 0.5 * r2 * (double)y * (double)y);
-      const double r = r2 * yd;
-      const double xr = r * a.idr;
-      const int i = (int)xr;
-      const double fr = xr - i;
-      const bool fast = r2 > 0 && fr > 1e-9 && fr < 1 - 1e-9 && i < a.na - 2 && fabs(r - hb) > hbm;
-      ir[c] = fast ? i : 0;   // a valid bin for every cell: the gathers below need no branch
-      tt[c] = (float)fr;
-      fastm |= (fast ? 1u : 0u) << c;
-      inm |= (r < hb ? 1u : 0u) << c;
-    }
+    const double xa = ix0 * a.dx - a.ox, xb = xa + (kCellsPerLane - 1) * a.dx;
+    const double ra2 = xa * xa + yz2, rb2 = xb * xb + yz2;
+    const double lo2 = (xa <= 0 && xb >= 0) ? yz2 : fmin(ra2, rb2);
+    const float rlo = sqrtf((float)lo2) * (1.f - 1e-5f);
+    const float rhi = sqrtf((float)fmax(ra2, rb2)) * (1.f + 1e-5f);
+    const float hb = (float)a.half_box_20 * (1.f + 1e-5f);
+    const float idr = (float)a.idr;
+    out = rlo > hb;
+    ilo = (int)(rlo * idr * (1.f - 1e-5f));
+    ihi = (int)(fminf(rhi, hb) * idr * (1.f + 1e-5f));
   }
 
+  int qn = 0;   // uniform
   for (int ip = 0; ip < a.npop; ip++) {
     const PopTab &P = a.pop[ip];
-    const float4 *tab = P.tab;
-    const int model = P.model, pipop = P.ipop;
-#pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) cnt[kCellsPerLane * lane + c] = 0;
-    // all 8 bins in flight at once
-    float4 T[kCellsPerLane];
-#pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) T[c] = tab[ir[c]];
-    const u32x4 rr = draw(cell0 >> 3, stream_poisson(pipop), 0u, a.seed);
-    const float relerr = tab[a.na].x;   // product-interpolation bound of the table
-    int qn = 0;   // uniform
-    if (model == 0) queue_undecided<0>(T, d, tt, fastm, inm, rr, relerr, lane, queue, qn);
-    else if (model == 1) queue_undecided<1>(T, d, tt, fastm, inm, rr, relerr, lane, queue, qn);
-    else queue_undecided<2>(T, d, tt, fastm, inm, rr, relerr, lane, queue, qn);
-    __syncthreads();   // one wavefront: orders the LDS writes above before the reads below
-    for (int i = lane; i < qn; i += kLanes) {
-      const QEntry e = queue[i];
-      const int cb = (int)(e.ch & 0xFFFFu);
-      const uint32_t hi = e.ch >> 16;
-      const size_t cell = blk * kBlockCells + cb;
-      const uint32_t x = (hi << 16) | poisson_lo16(cell, P.ipop, a.seed);
-      int kk = -1;
-      if (e.lam >= 0.f) kk = poisson_fast_f32(e.lam, __expf(-e.lam), e.err, x);
-      if (kk < 0) {   // too close to call in FP32: k_exact decides
-        const unsigned j = atomicAdd(a.exq_n, 1u);
-        if (j < a.exq_cap) a.exq[j] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)ip, x);
-      }
-      cnt[cb] = kk;
-    }
-    __syncthreads();
-    // bytes of the lane's 8 cells, the block total
-    uint32_t lo = 0, hi8 = 0;
-    long long mine = 0;
-#pragma unroll
-    for (int c = 0; c < kCellsPerLane; c++) {
-      const int k = cnt[kCellsPerLane * lane + c];
-      mine += k < 0 ? 0 : k;
-      const uint32_t b = (uint32_t)(k < 0 ? kBytePending : (k < kBytePending ? k : kByteEscape));
-      if (c < 4) lo |= b << (8 * c);
-      else hi8 |= b << (8 * (c - 4));
-    }
-    *(uint2 *)(P.cnt8 + cell0) = make_uint2(lo, hi8);
-    if (((lo | hi8) & 0x80808080u) && (((lo & (lo >> 1)) | (hi8 & (hi8 >> 1))) & 0x40404040u)) {
-      // some byte may be 255 (rare): append the exact counts to the escape list
-#pragma unroll
-      for (int c = 0; c < kCellsPerLane; c++) {
-        const int k = cnt[kCellsPerLane * lane + c];
-        if (k >= kBytePending) {
-          const unsigned e = atomicAdd(a.esc_n, 1u);
-          if (e < a.esc_cap)
-            a.esc[e] = make_ulonglong2(((unsigned long long)ip << 56) | (unsigned long long)(cell0 + c),
-                                       (unsigned long long)k);
+    const int model = P.model;
+    // --- A: the lane bound, then the cells it leaves open to the queue ------
+    int hcrit = 65536;   // hi < hcrit: u < (hi+1)/2^16 <= exp(-lambda_max), k = 0
+    if (!out) {
+      hcrit = 0;
+      if (ihi < a.na - 2) {
+        const float4 *coarse = P.tab + a.na + 1;
+        float pm = 0.f, blo = 3.4e38f, bhi = 0.f;
+        for (int j = ilo / kCoarse; j <= ihi / kCoarse; j++) {
+          const float4 C = coarse[j];
+          pm = fmaxf(pm, C.x);
+          blo = fminf(blo, C.y);
+          bhi = fmaxf(bhi, C.z);
+        }
+        if (pm == 0.f) {
+          hcrit = 65536;
+        } else if (pm < 1e30f && blo >= 0.f) {
+          float bub;
+          if (model == 0) bub = bias_upper<0>(dmax, blo, bhi);
+          else if (model == 1) bub = bias_upper<1>(dmax, blo, bhi);
+          else bub = bias_upper<2>(dmax, blo, bhi);
+          const float lub = pm * bub * (1.f + 1e-5f);
+          // __expf: relative error below 1e-6 + 2e-7 lub
+          if (lub < 16.f) hcrit = (int)(65536.f * __expf(-lub) * (1.f - 2e-6f - 4e-7f * lub));
         }
       }
     }
-    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
-    if (lane == 0) P.btot[blk] = (uint32_t)mine;
+    if (__ballot(hcrit < 65536) != 0ull) {
+      const u32x4 rr = draw(cell0 >> 3, stream_poisson(P.ipop), 0u, a.seed);
+#pragma unroll
+      for (int c = 0; c < kCellsPerLane; c++) {
+        const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
+        const bool open = (int)hi >= hcrit;
+        const unsigned long long m = __ballot(open);
+        if (m != 0ull) {
+          if (open)
+            queue[qn + __popcll(m & below)] = (uint32_t)(kCellsPerLane * lane + c) | ((uint32_t)ip << 9) | (hi << 16);
+          qn += __popcll(m);
+        }
+      }
+    }
+    // --- B: the queue, all lanes busy; run when another population might
+    // not fit, and after the last -------------------------------------------
+    if (qn > kQueueCap - kBlockCells || ip == a.npop - 1) {
+      __syncthreads();   // one wavefront: orders the LDS writes above before the reads below
+      for (int i = lane; i < qn; i += kLanes) {
+        const uint32_t e = queue[i];
+        const int cb = (int)(e & (kBlockCells - 1));
+        const int q = (int)((e >> 9) & 15u);
+        const uint32_t hi = e >> 16;
+        const size_t cell = base + cb;
+        const int oc = obase + cb;
+        const int dr = oc / N;
+        const unsigned rc = (unsigned)row0 + (unsigned)dr;
+        const int k = settle_cell(a, stab[q], smodel[q], a.pop[0].ipop + q, q, cell, oc - dr * N,
+                                  (int)(rc % (unsigned)N), (int)(rc / (unsigned)N), hi);
+        ((uint8_t *)sbytes[q])[cb] = (uint8_t)(k < 0 ? kBytePending : (k < kBytePending ? k : kByteEscape));
+        if (k > 0) atomicAdd(&stot[q], (uint32_t)k);
+        if (k >= kBytePending) {
+          // the exact count of a saturated byte goes to the escape list
+          const unsigned x = atomicAdd(a.esc_n, 1u);
+          if (x < a.esc_cap)
+            a.esc[x] = make_ulonglong2(((unsigned long long)q << 56) | (unsigned long long)cell, (unsigned long long)k);
+        }
+      }
+      __syncthreads();
+      qn = 0;
+    }
+  }
+  // --- the bytes of the lane's 8 cells and the block totals ------------------
+  for (int ip = 0; ip < a.npop; ip++) {
+    const PopTab &P = a.pop[ip];
+    *(uint2 *)(P.cnt8 + cell0) = sbytes[ip][lane];
+    if (lane == 0) P.btot[blk] = stot[ip];
   }
 }
 
@@ -599,10 +705,12 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   // packed FP32 tables, rebuilt from the current norm tables
   for (int i = p0; i < p1; i++) {
     TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
-    if (!T.ptab) CLR_HIP(c, hipMalloc((void **)&T.ptab, (size_t)(na + 1) * sizeof(float4)));
+    if (!T.ptab) CLR_HIP(c, hipMalloc((void **)&T.ptab, pop_table_len(na) * sizeof(float4)));
     CLR_HIP(c, hipMemsetAsync((float4 *)T.ptab + na, 0, sizeof(float4), c->stream));
     hipLaunchKernelGGL(k_pop_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, T.amp, T.bz, T.norm, cell_vol,
                        na, (float4 *)T.ptab);
+    hipLaunchKernelGGL(k_pop_coarse, dim3((coarse_count(na) + 255) / 256), dim3(256), 0, c->stream, T.amp, T.bz,
+                       T.norm, cell_vol, na, (float4 *)T.ptab);
     CLR_HIP(c, hipGetLastError());
   }
   const int np = p1 - p0;
