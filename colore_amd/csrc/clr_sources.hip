@@ -3,14 +3,17 @@
 //
 // Blocks are one wavefront over 512 consecutive cells (z-major flat order; a
 // lane owns 8 cells of one row).
-// k_count : per population, decides most cells from 16 uniform bits and an
-//           FP32 lambda whose error is bounded (clr_poisson.h): k = 0 at once;
-//           the few others are compacted (ballot) into an LDS queue that all
-//           lanes then work: the full 32-bit uniform, the FP32 search, and
-//           where that is too close to call the reference's FP64 expressions
-//           (count_exact) -- every count equals the FP64 one.  Writes one byte
-//           per cell and population (255 = "255 or more": recounted by
-//           k_fill) and the block's total.
+// k_count : per lane and population, one upper bound on lambda over the
+//           lane's 8 cells (largest delta, coarse bounds of the bins their
+//           radii can fall in) settles k = 0 wherever the 16 high uniform
+//           bits lie below exp(-bound); the few open cells are compacted
+//           (ballot) into an LDS queue that all lanes then work: the cell's
+//           own FP32 lambda with a bounded error (clr_poisson.h), the full
+//           32-bit uniform and the FP32 search, and where that is too close
+//           to call the reference's FP64 expressions (k_exact, count_exact)
+//           -- every count equals the FP64 one.  Writes one byte per cell
+//           and population (255 = "255 or more", the count is in the escape
+//           list) and the block's total.
 // k_scan_*: exclusive prefix of the block totals (64-bit), per population.
 // k_fill  : reads the bytes, scans its cells and writes one descriptor
 //           (cell, index within the cell) per source at its final place in
@@ -227,8 +230,7 @@ __device__ inline void to_exact(const SrcArgs &a, size_t cell, int ip, uint32_t 
 // and everything else -- the radius within 1e-9 of a bin or box edge, the
 // last bins, Other bins, a search too close to call -- goes to k_exact (-1).
 __device__ inline int settle_cell(const SrcArgs &a, const float4 *tab, int model, int pipop, int ip, size_t cell,
-                                  int ix, int iy, int iz, uint32_t hi) {
-  const float dl = a.g[((size_t)iz * a.N + iy) * a.ngx + ix];
+                                  int ix, int iy, int iz, float dl, uint32_t hi) {
   const double xx = ix * a.dx - a.ox, y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
   const double r2 = xx * xx + (y0 * y0 + z0 * z0);
   // r = r2 / sqrt(r2): FP32 rsqrt refined once in FP64 (rel. error ~2e-14)
@@ -269,12 +271,14 @@ __device__ inline int settle_cell(const SrcArgs &a, const float4 *tab, int model
 }
 
 // queue entry: cell within the block (bits 0-8) | population (bits 9-12) |
-// the uniform's high 16 bits << 16
-constexpr int kQueueCap = 2 * kBlockCells;
+// the uniform's high 16 bits << 16.  The queue is worked when another
+// population's 512 entries might not fit, so usually once per block.
+constexpr int kQueueCap = 3 * kBlockCells / 2;
 
-__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4))) void k_count(SrcArgs a) {
+__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(8))) void k_count(SrcArgs a) {
   __shared__ uint32_t queue[kQueueCap];
-  __shared__ uint2 sbytes[CLR_NPOP_MAX][kLanes];   // one byte per cell and population
+  extern __shared__ uint2 sbytes[];   // [npop][kLanes]: one byte per cell and population
+  __shared__ uint2 sgeo[kLanes];      // per lane: first column | row y << 16, row z
   __shared__ uint32_t stot[CLR_NPOP_MAX];
   __shared__ const float4 *stab[CLR_NPOP_MAX];
   __shared__ int smodel[CLR_NPOP_MAX];
@@ -293,8 +297,9 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4))) voi
   const unsigned row = (unsigned)row0 + (unsigned)drow;
   const int iy = (int)(row % (unsigned)N), iz = (int)(row / (unsigned)N);
   const unsigned long long below = (1ull << lane) - 1ull;
+  sgeo[lane] = make_uint2((uint32_t)ix0 | ((uint32_t)iy << 16), (uint32_t)iz);
   for (int ip = 0; ip < a.npop; ip++) {
-    sbytes[ip][lane] = make_uint2(0u, 0u);
+    sbytes[ip * kLanes + lane] = make_uint2(0u, 0u);
     if (lane == 0) {
       stot[ip] = 0;
       stab[ip] = a.pop[ip].tab;
@@ -397,12 +402,12 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int q = (int)((e >> 9) & 15u);
         const uint32_t hi = e >> 16;
         const size_t cell = base + cb;
-        const int oc = obase + cb;
-        const int dr = oc / N;
-        const unsigned rc = (unsigned)row0 + (unsigned)dr;
-        const int k = settle_cell(a, stab[q], smodel[q], a.pop[0].ipop + q, q, cell, oc - dr * N,
-                                  (int)(rc % (unsigned)N), (int)(rc / (unsigned)N), hi);
-        ((uint8_t *)sbytes[q])[cb] = (uint8_t)(k < 0 ? kBytePending : (k < kBytePending ? k : kByteEscape));
+        const uint2 gl = sgeo[cb / kCellsPerLane];
+        const int ix = (int)(gl.x & 0xFFFFu) + cb % kCellsPerLane, iy = (int)(gl.x >> 16), iz = (int)gl.y;
+        const float dl = a.g[((size_t)iz * N + iy) * a.ngx + ix];
+        const int k = settle_cell(a, stab[q], smodel[q], a.pop[0].ipop + q, q, cell, ix, iy, iz, dl, hi);
+        ((uint8_t *)(sbytes + q * kLanes))[cb] =
+            (uint8_t)(k < 0 ? kBytePending : (k < kBytePending ? k : kByteEscape));
         if (k > 0) atomicAdd(&stot[q], (uint32_t)k);
         if (k >= kBytePending) {
           // the exact count of a saturated byte goes to the escape list
@@ -418,7 +423,7 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(4))) voi
   // --- the bytes of the lane's 8 cells and the block totals ------------------
   for (int ip = 0; ip < a.npop; ip++) {
     const PopTab &P = a.pop[ip];
-    *(uint2 *)(P.cnt8 + cell0) = sbytes[ip][lane];
+    *(uint2 *)(P.cnt8 + cell0) = sbytes[ip * kLanes + lane];
     if (lane == 0) P.btot[blk] = stot[ip];
   }
 }
@@ -782,7 +787,8 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     CLR_HIP(c, hipMemsetAsync(a.esc_n, 0, sizeof(unsigned int), c->stream));
     CLR_HIP(c, hipMemsetAsync(a.exq_n, 0, sizeof(unsigned int), c->stream));
     CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
-    hipLaunchKernelGGL(k_count, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
+    hipLaunchKernelGGL(k_count, dim3((unsigned)nblocks), dim3(kLanes), (size_t)np * kLanes * sizeof(uint2), c->stream,
+                       a);
     CLR_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_exact, dim3((a.exq_cap + 255) / 256), dim3(256), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
