@@ -3,7 +3,9 @@
 //
 // k_transform : lognormalize / densclip in place.
 // k_shells    : one pass over the grid.  Each wavefront walks tiles of
-//               64 (x, one cell per lane) x 4 (y) x 16 (z) cells; per cell the
+//               16 (x) x 16 (y) x 16 (z) cells, 64 rows of 16 x 4 cells (one
+//               cell per lane: a compact footprint keeps the table bins the
+//               lanes gather close together); per cell the
 //               radius, then from one packed FP32 table D(r), z(r) and every
 //               b_t(r); the shell index s = (int)(z*idz)+1 is taken from the
 //               FP32 z where that is more than 5e-4 of a shell away from a
@@ -33,8 +35,21 @@ namespace {
 constexpr int kShellBlocks = 8192;
 constexpr int kShellThreads = 256;
 constexpr int kWaves = kShellThreads / 64;
-constexpr int kTY = 4, kTZ = 16;   // tile rows: kTY x kTZ rows of 64 cells
+// A wavefront row is kTX consecutive x cells of kSub = 64/kTX adjacent y
+// rows (lane = x + kTX * sub-row): a short x extent keeps the radii, and so
+// the table bins the lanes gather, close together.  A tile is kTY x kTZ such
+// rows: kTX x (kTY kSub) x kTZ cells.
+#ifndef CLR_SHELL_TX
+#define CLR_SHELL_TX 16
+#endif
+constexpr int kTX = CLR_SHELL_TX, kSub = 64 / kTX;
+constexpr int kTY = 4, kTZ = 16;
 constexpr int kSlots = 1;
+#ifndef CLR_SHELL_PREFETCH
+#define CLR_SHELL_PREFETCH 2
+#endif
+constexpr int kPrefetch = CLR_SHELL_PREFETCH;   // rows of density in flight per wavefront
+static_assert((kTY * kTZ) % kPrefetch == 0, "prefetch distance must divide the tile rows");
 constexpr int kMaxTr = 8;
 
 struct ShellArgs {
@@ -133,12 +148,26 @@ __device__ inline void flush_slot(unsigned long long mask, int key, const double
 }
 
 // bias_model (common.h:414-431) for the shell sums, on the raw hardware
-// exp2/log2 and a fast reciprocal (a few ulp: the sums are of FP32 terms).
-__device__ inline float bias_sum_term(float d, float b, int model) {
-  if (!(d > -1.f)) return 0.f;
-  if (model == 1) return d < 0.f ? __expf(b * __fdividef(d, 1.f + d)) : 1.f + b * d;
-  if (model == 2) return fmaxf(1.f + b * d, 0.f);
-  return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(1.f + d));
+// exp2/log2 and a fast reciprocal (a few ulp: the sums are of FP32 terms),
+// without branches (the model is uniform, the sign of d is not):
+//   0: (1+d)^b = exp2(b l2),            l2 = log2(1+d)
+//   1: d < 0: exp(b d/(1+d)) = exp2(b q), q = log2(e) d/(1+d); else 1 + b d
+//   2: max(1 + b d, 0)
+// l2 and q are shared by the tracers; d <= -1 (or NaN) gives 0.
+__device__ inline float bias_sum_term(float d, float l2, float q, float b, int model) {
+  const float e = __builtin_amdgcn_exp2f(b * (model == 0 ? l2 : q));
+  const float lin = fmaxf(__builtin_fmaf(b, d, 1.f), 0.f);
+  const bool use_exp = model == 0 || (model == 1 && d < 0.f);
+  return d > -1.f ? (use_exp ? e : lin) : 0.f;
+}
+
+// e^x to about 2 ulp: the hardware exp2 of x log2(e) rounded, times the
+// first-order correction for that rounding (log2(e) = 1.44269502 + 1.925963e-8).
+__device__ inline float exp_lognormal(float x) {
+  const float t = x * 1.44269502f;
+  const float e = __builtin_fmaf(x, 1.44269502f, -t) + x * 1.925963e-8f;
+  const float r = __builtin_amdgcn_exp2f(t);
+  return __builtin_fmaf(r, e * 0.693147181f, r);
 }
 
 // LDS per block: kWaves x nz x NQ accumulators, then per wavefront kSlots x
@@ -149,10 +178,10 @@ constexpr size_t shell_lds_bytes(int nz) {
          (size_t)kWaves * kSlots * 64 * ((2 + NT) * sizeof(double) + sizeof(int));
 }
 
-template <int NT>
+// M0: every tracer uses bias model 0 (no other model's code in the loop).
+template <int NT, bool M0>
 __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_shells(ShellArgs a,
                                                                                       double *__restrict__ partial) {
-#pragma clang fp contract(off)
   constexpr int NQ = 2 + NT;
   constexpr int W = (NT + 5) / 4;
   extern __shared__ double acc[];
@@ -165,7 +194,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
   __syncthreads();
   const double obs[3] = {a.ox, a.oy, a.oz};
   const float ox = (float)a.ox, oy = (float)a.oy, oz = (float)a.oz, dxf = (float)a.dx;
-  const int ntx = (N + 63) / 64, nty = N / kTY, ntz = N / kTZ;
+  const int ntx = (N + kTX - 1) / kTX, nty = N / (kTY * kSub), ntz = N / kTZ;
   const long ntiles = (long)ntx * nty * ntz;
   const long nw = (long)gridDim.x * kWaves;
   // all lanes' logged shells into the wavefront's accumulators
@@ -178,7 +207,8 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
     const int tx = (int)(tile % ntx);
     const long rest = tile / ntx;
     const int ty = (int)(rest % nty), tz = (int)(rest / nty);
-    const int ix = tx * 64 + lane;
+    const int ix = tx * kTX + lane % kTX, ly = lane / kTX;
+    const int iy0 = ty * kTY * kSub + ly;   // the lane's y in the tile's first row
     const bool valid = ix < N;
     int nslot = 0, cur = -1;
     double cacc[NQ];
@@ -194,7 +224,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
       bool e0, e1;
     };
     auto fetch = [&](int rr2, Bin &bn) {
-      const int iy2 = ty * kTY + (rr2 % kTY), iz2 = tz * kTZ + (rr2 / kTY);
+      const int iy2 = iy0 + (rr2 % kTY) * kSub, iz2 = tz * kTZ + (rr2 / kTY);
       const float y32 = iy2 * dxf - oy, z32 = iz2 * dxf - oz;
       const float rf = __builtin_amdgcn_sqrtf(x32 * x32 + y32 * y32 + z32 * z32);
       const float xq = rf * a.idr_f;
@@ -208,20 +238,28 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
         bn.B[k] = a.tab[(size_t)(i0 + 1) * W + k];
       }
     };
+    // the lane's cell in the tile's first row; rows at uniform offsets from it
+    float *const tb = a.g + ((size_t)(tz * kTZ) * N + iy0) * a.ngx + (valid ? ix : 0);
+    const auto roff = [&](int r) { return ((size_t)(r / kTY) * N + (r % kTY) * kSub) * a.ngx; };
+    // densities kPrefetch rows ahead (the loads that wait on HBM), table bins
+    // one row ahead (cache hits), issued bins first so that waiting for them
+    // never waits for a later density
+    float dq[kPrefetch];
+#pragma unroll
+    for (int j = 0; j < kPrefetch; j++) dq[j] = valid ? tb[roff(j)] : 0.f;
     Bin bnext;
     fetch(0, bnext);
-    float dnext = valid ? a.g[((size_t)(tz * kTZ) * N + ty * kTY) * a.ngx + ix] : 0.f;
-    for (int rr = 0; rr < kTY * kTZ; rr++) {
-      const int iy = ty * kTY + (rr % kTY), iz = tz * kTZ + (rr / kTY);
+    for (int r0 = 0; r0 < kTY * kTZ; r0 += kPrefetch)
+#pragma unroll
+    for (int j = 0; j < kPrefetch; j++) {
+      const int rr = r0 + j;
+      const int iy = iy0 + (rr % kTY) * kSub, iz = tz * kTZ + (rr / kTY);
       int s = -1;
       double v[NQ];
-      float dl = dnext;
+      float dl = dq[j];
       const Bin bn = bnext;
-      if (rr + 1 < kTY * kTZ) {
-        const int iy2 = ty * kTY + ((rr + 1) % kTY), iz2 = tz * kTZ + ((rr + 1) / kTY);
-        if (valid) dnext = a.g[((size_t)iz2 * N + iy2) * a.ngx + ix];
-        fetch(rr + 1, bnext);
-      }
+      if (rr + 1 < kTY * kTZ) fetch(rr + 1, bnext);
+      if (rr + kPrefetch < kTY * kTZ && valid) dq[j] = tb[roff(rr + kPrefetch)];
       if (valid) {
         float comp[4 * W];
         const bool e0 = bn.e0, e1 = bn.e1;
@@ -245,8 +283,8 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
         if (a.fused) {
           const float dg = comp[0];
           const float arg = dg * (dl - 0.5f * dg * (float)a.sigma2);
-          dl = expf(arg) - 1.f;
-          a.g[((size_t)iz * N + iy) * a.ngx + ix] = dl;
+          dl = exp_lognormal(arg) - 1.f;
+          tb[roff(rr)] = dl;
         }
         // shell index: FP32 z (error < 5e-5 shell) unless within 5e-4 shell of
         // an edge (or at the table ends), where the reference's FP64 radius
@@ -256,6 +294,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
         int ind = (int)sf + 1;
         const float fr = sf - floorf(sf);
         if (e0 || e1 || !(fr > 5e-4f && fr < 1.f - 5e-4f)) {
+#pragma clang fp contract(off)
           double x0, y0, z0;
           const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
           zz = f_of_r(a.r_arr, a.zt, a.idr, a.r_last, r, 0.0, a.z_last);
@@ -265,8 +304,12 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
           s = ind;
           v[0] = 1.0;
           v[1] = zz;
+          const float l2 = __builtin_amdgcn_logf(1.f + dl);
+          const float q = M0 ? 0.f : 1.44269504f * dl * __builtin_amdgcn_rcpf(1.f + dl);
 #pragma unroll
-          for (int t = 0; t < NT; t++) v[2 + t] = (double)bias_sum_term(dl, comp[2 + t], a.model[t]);
+          for (int t = 0; t < NT; t++)
+            v[2 + t] = (double)(M0 ? (dl > -1.f ? __builtin_amdgcn_exp2f(comp[2 + t] * l2) : 0.f)
+                                   : bias_sum_term(dl, l2, q, comp[2 + t], a.model[t]));
         }
       }
       const bool change = s >= 0 && s != cur;
@@ -377,19 +420,18 @@ __global__ void k_norm_table(const double *__restrict__ sums, int nz, int nq, in
   }
 }
 
-template <int NT>
+template <int NT, bool M0>
 void launch_shells(dim3 g, dim3 b, size_t lds, hipStream_t s, const ShellArgs &a, double *partial) {
-  hipLaunchKernelGGL((k_shells<NT>), g, b, lds, s, a, partial);
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void *)k_shells<NT, M0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_shells<NT, M0>), g, b, lds, s, a, partial);
 }
 using ShellFn = void (*)(dim3, dim3, size_t, hipStream_t, const ShellArgs &, double *);
-const ShellFn kShellFns[kMaxTr + 1] = {launch_shells<0>, launch_shells<1>, launch_shells<2>, launch_shells<3>,
-                                        launch_shells<4>, launch_shells<5>, launch_shells<6>, launch_shells<7>,
-                                        launch_shells<8>};
-const void *kShellKernels[kMaxTr + 1] = {(const void *)k_shells<0>, (const void *)k_shells<1>,
-                                         (const void *)k_shells<2>, (const void *)k_shells<3>,
-                                         (const void *)k_shells<4>, (const void *)k_shells<5>,
-                                         (const void *)k_shells<6>, (const void *)k_shells<7>,
-                                         (const void *)k_shells<8>};
+#define CLR_SHELL_FNS(M0)                                                                                  \
+  {launch_shells<0, M0>, launch_shells<1, M0>, launch_shells<2, M0>, launch_shells<3, M0>, launch_shells<4, M0>, \
+   launch_shells<5, M0>, launch_shells<6, M0>, launch_shells<7, M0>, launch_shells<8, M0>}
+const ShellFn kShellFns[2][kMaxTr + 1] = {CLR_SHELL_FNS(false), CLR_SHELL_FNS(true)};
+#undef CLR_SHELL_FNS
 
 }  // namespace
 
@@ -478,8 +520,8 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
                                                    shell_lds_bytes<6>, shell_lds_bytes<7>, shell_lds_bytes<8>};
   const size_t lds = kLds[ntr](nz);
   if (lds > 160 * 1024) return clr_fail(c, CLR_ERR_ARG, "too many shells x tracers (%d x %d)", nz, ntr);
-  if (lds > 64 * 1024)
-    CLR_HIP(c, hipFuncSetAttribute(kShellKernels[ntr], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  bool all0 = true;
+  for (int t = 0; t < ntr; t++) all0 = all0 && a.model[t] == CLR_BIAS_EXPONENTIAL;
   const size_t nvals = (size_t)nz * nq;
   size_t need = (size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr;
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
@@ -494,7 +536,7 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, ntr, na, w,
                      (float4 *)c->misc);
   CLR_HIP(c, hipGetLastError());
-  kShellFns[ntr](dim3(kShellBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
+  kShellFns[all0 ? 1 : 0][ntr](dim3(kShellBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
   CLR_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,
                      (int)nvals, fin);

@@ -59,16 +59,26 @@ __global__ void k_amp(float2 *__restrict__ amp, int qmax, double dk, const doubl
   amp[q] = make_float2((float)ad, (float)ap);
 }
 
-// sqrt(-ln(1-u)) for u = x/2^32, accurate over the whole range.
+// sqrt(-ln(1-u)) for u = x/2^32 to a few 1e-7 relative, without branches:
+//   u < 1/2:  -ln(1-u) = 2 atanh(s) = 2 (s + s^3/3 + s^5/5 + ...) with
+//             s = u/(2-u) <= 1/3, summed to s^13 (the rest is < 2e-8 of it);
+//   u >= 1/2: the hardware log2 of 1-u = (2^32 - x)/2^32, which that integer
+//             gives to FP32 rounding (-ln(1-u) >= ln 2 there).
 __device__ inline float modulus_draw(uint32_t x) {
-  float l;
-  if (x < 0x80000000u) {
-    l = log1pf(-(float)x * 2.3283064365386963e-10f);
-  } else {
-    const float om = (float)(4294967296.0 - (double)x) * 2.3283064365386963e-10f;  // 1-u
-    l = logf(om);
-  }
-  return sqrtf(-l);
+  const float u = (float)x * 2.3283064365386963e-10f;
+  const float s = u * __builtin_amdgcn_rcpf(2.f - u);
+  const float s2 = s * s;
+  float p = 1.f / 13.f;
+  p = p * s2 + 1.f / 11.f;
+  p = p * s2 + 1.f / 9.f;
+  p = p * s2 + 1.f / 7.f;
+  p = p * s2 + 1.f / 5.f;
+  p = p * s2 + 1.f / 3.f;
+  p = p * s2 + 1.f;
+  const float small = 2.f * s * p;
+  const float om = (float)(0u - x) * 2.3283064365386963e-10f;
+  const float large = -0.69314718055994531f * __builtin_amdgcn_logf(om);
+  return __builtin_amdgcn_sqrtf(x < 0x80000000u ? small : large);
 }
 
 // Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has its
