@@ -334,7 +334,7 @@ bool field_launch_for(int N, FieldLaunch *L) {
     case 256:  *L = {COLCFG(256, 16, 16, 16, 16), XCFG(256, 8, 16, 16, 8), 16, 256, 16, 128}; return true;
     case 384:  *L = {COLCFG(384, 16, 16, 8, 8, 6), XCFG(384, 16, 16, 4, 4, 12), 16, 256, 16, 256}; return true;
     case 512:  *L = {COLCFG(512, 64, 16, 8, 8, 8), XCFG(512, 16, 16, 16, 16), 16, 1024, 16, 256}; return true;
-    case 1024: *L = {COLCFG(1024, 64, 16, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 16, 1024, 8, 256}; return true;
+    case 1024: *L = {COLCFG(1024, 32, 8, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256}; return true;
     case 2048: *L = {COLCFG(2048, 128, 8, 16, 16, 8), XCFG(2048, 64, 4, 16, 16, 4), 8, 1024, 4, 256}; return true;
     default: return false;
   }
