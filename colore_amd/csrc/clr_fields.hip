@@ -38,13 +38,20 @@ __device__ double pk_linear0(const double *__restrict__ logk, const double *__re
   return pk[numk - 1] * pow(10.0, -3 * (lgk - logkmax));
 }
 
-__global__ void k_amp(float2 *__restrict__ amp, int qmax, double dk, const double *__restrict__ logk,
+// compact: only the delta amplitude, as float[qmax+1] (phi's is delta's times
+// -prefac/k^2, formed in k_gen_z); otherwise (delta, phi) as float2.
+__global__ void k_amp(void *__restrict__ out, int compact, int qmax, double dk, const double *__restrict__ logk,
                       const double *__restrict__ pk, int numk, double logkmin, double logkmax,
                       double idlogk, double ns, double r2, int do_smooth, int smooth_pot, double prefac) {
 #pragma clang fp contract(off)
+  float2 *amp = (float2 *)out;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q > qmax) return;
-  if (q == 0) { amp[0] = make_float2(0.f, 0.f); return; }
+  if (q == 0) {
+    if (compact) ((float *)out)[0] = 0.f;
+    else amp[0] = make_float2(0.f, 0.f);
+    return;
+  }
   const double k2 = dk * dk * (double)q;
   const double lgk = 0.5 * log10(k2);
   const double idk3 = 1.0 / (dk * dk * dk);
@@ -56,7 +63,8 @@ __global__ void k_amp(float2 *__restrict__ amp, int qmax, double dk, const doubl
     ad *= sm;
     if (smooth_pot) ap *= sm;
   }
-  amp[q] = make_float2((float)ad, (float)ap);
+  if (compact) ((float *)out)[q] = (float)ad;
+  else amp[q] = make_float2((float)ad, (float)ap);
 }
 
 // sqrt(-ln(1-u)) for u = x/2^32 to a few 1e-7 relative, without branches:
@@ -98,7 +106,7 @@ __device__ inline void xcd_tile(unsigned b, unsigned ntile, unsigned nplanes, in
 // ---------------------------------------------------------------------------
 template <int N, int P, int T, int... Rs>
 __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float2 *__restrict__ npot,
-                                                 const float2 *__restrict__ amp,
+                                                 const float2 *__restrict__ amp, int compact, float phi_ratio,
                                                  const float2 *__restrict__ tw, uint32_t seed,
                                                  const uint32_t *__restrict__ inj) {
   using RL = RadixList<Rs...>;
@@ -143,7 +151,15 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
         // angle 2 pi u: v_sin_f32 / v_cos_f32 take their argument in turns
         const float turns = (float)x0 * 2.3283064365386963e-10f;
         const float sn = __builtin_amdgcn_sinf(turns), cs = __builtin_amdgcn_cosf(turns);
-        const float2 a = amp[q];
+        // the compact table (4 bytes per |k|^2) stays in L2; phi's amplitude
+        // is delta's times -prefac/k^2 (a few ulp)
+        float2 a;
+        if (compact) {
+          a.x = ((const float *)amp)[q];
+          a.y = a.x * (phi_ratio * __builtin_amdgcn_rcpf((float)q));
+        } else {
+          a = amp[q];
+        }
         d = make_float2(a.x * w * cs, a.x * w * sn);
         p = make_float2(a.y * w * cs, a.y * w * sn);
       }
@@ -298,7 +314,7 @@ __global__ __launch_bounds__(1024) void k_reduce_pairs(const double *__restrict_
 // per-N configuration
 // ---------------------------------------------------------------------------
 struct FieldLaunch {
-  void (*gen)(dim3, dim3, hipStream_t, float2 *, float2 *, const float2 *, const float2 *, uint32_t,
+  void (*gen)(dim3, dim3, hipStream_t, float2 *, float2 *, const float2 *, int, float, const float2 *, uint32_t,
               const uint32_t *);
   void (*coly)(dim3, dim3, hipStream_t, float2 *, float2 *, const float2 *);
   void (*c2r)(dim3, dim3, hipStream_t, float *, float *, float, double *, size_t, const float2 *);
@@ -306,9 +322,9 @@ struct FieldLaunch {
 };
 
 template <int N, int P, int T, int... Rs> struct ColKernels {
-  static void gen(dim3 g, dim3 b, hipStream_t s, float2 *d, float2 *p, const float2 *amp, const float2 *tw,
-                  uint32_t seed, const uint32_t *inj) {
-    hipLaunchKernelGGL((k_gen_z<N, P, T, Rs...>), g, b, 0, s, d, p, amp, tw, seed, inj);
+  static void gen(dim3 g, dim3 b, hipStream_t s, float2 *d, float2 *p, const float2 *amp, int compact,
+                  float phi_ratio, const float2 *tw, uint32_t seed, const uint32_t *inj) {
+    hipLaunchKernelGGL((k_gen_z<N, P, T, Rs...>), g, b, 0, s, d, p, amp, compact, phi_ratio, tw, seed, inj);
   }
   static void coly(dim3 g, dim3 b, hipStream_t s, float2 *d, float2 *p, const float2 *tw) {
     hipLaunchKernelGGL((k_col_y<N, P, T, Rs...>), g, b, 0, s, d, p, tw);
@@ -363,8 +379,11 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   const double dk = 2 * M_PI / c->L;
   const int do_smooth = r_smooth > 0;
   const double r2 = do_smooth ? r_smooth * r_smooth : 0.0;
-  hipLaunchKernelGGL(k_amp, dim3((qmax + 256) / 256), dim3(256), 0, c->stream, c->amp, qmax, dk, c->pk,
-                     c->pk + c->numk, c->numk, c->logkmin, c->logkmax, c->idlogk, c->n_scal, r2, do_smooth,
+  // phi's amplitude is delta's times -prefac/k^2 unless only delta is smoothed
+  const int compact = !(do_smooth && !smooth_potential);
+  const float phi_ratio = (float)(-c->prefac_lensing / (dk * dk));
+  hipLaunchKernelGGL(k_amp, dim3((qmax + 256) / 256), dim3(256), 0, c->stream, (void *)c->amp, compact, qmax, dk,
+                     c->pk, c->pk + c->numk, c->numk, c->logkmin, c->logkmax, c->idlogk, c->n_scal, r2, do_smooth,
                      smooth_potential, c->prefac_lensing);
   CLR_HIP(c, hipGetLastError());
 
@@ -393,7 +412,8 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   rc = clr_phi_ready(c, 0);
   if (rc) return rc;
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  L.gen(dim3((unsigned)(ntile * N)), dim3(L.colThreads), c->stream, d, p, c->amp, c->tw, seed, inj);
+  L.gen(dim3((unsigned)(ntile * N)), dim3(L.colThreads), c->stream, d, p, c->amp, compact, phi_ratio, c->tw, seed,
+        inj);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
   // delta: y and x passes with the sigma^2 partials

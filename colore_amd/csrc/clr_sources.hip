@@ -852,12 +852,12 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     if (n > 0 && !C.pos) return clr_fail(c, CLR_ERR_STATE, "no catalogue buffer");
   }
   CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
-  rc = clr_phi_ready(c, 0);   // k_write reads phi
-  if (rc) return rc;
   if (total > 0) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
   }
+  rc = clr_phi_ready(c, 0);   // k_write reads phi
+  if (rc) return rc;
   for (int i = 0; i < np; i++) {
     CatalogDev &C = c->cat[p0 + i];
     if (C.n == 0) continue;

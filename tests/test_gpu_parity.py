@@ -60,6 +60,23 @@ def test_gaussian_fields_philox(workdir, N):
     assert abs(s2 / os2 - 1) < 1e-5, (s2, os2)
 
 
+def test_gaussian_fields_unsmoothed_potential(workdir):
+    """smooth_potential = false: the potential's amplitudes come from their
+    own table (the density alone is smoothed)."""
+    N = 64
+    path, run = make_run("C1", N, workdir)
+    ctx = _ctx_for(run, N)
+    seed, rs = int(run.scalar("seed")), run.scalar("r_smooth")
+    assert rs > 0
+    s2 = ctx.gaussian_fields(seed, rs, False)
+    d, p = ctx.fetch_grid(0), ctx.fetch_grid(1)
+    od, op, os2 = _oracle(run).fields(N, run.scalar("l_box"), seed, rs, False)
+    for g, o in ((d, od), (p, op)):
+        rms_e, max_e = _field_err(g, o, N)
+        assert rms_e < 2e-6 and max_e < 5e-5, (rms_e, max_e)
+    assert abs(s2 / os2 - 1) < 1e-5, (s2, os2)
+
+
 def test_gaussian_fields_reference_stream(workdir):
     """Fed the reference's per-thread MT19937 mode stream (8 threads), the
     device spectra transform to the oracle's grids."""
