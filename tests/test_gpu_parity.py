@@ -137,10 +137,16 @@ def test_shell_norms(workdir, N):
         assert n0 == on0 and nf == onf
 
 
-def _sources_vs_oracle(run, N, ipop, seed):
+def _sources_vs_oracle(run, N, ipop, seed, edit=None):
     """Samples population ipop on the device and checks the catalogue against
-    the oracle's; returns (context, oracle, positions, oracle per-cell counts)."""
+    the oracle's; returns (context, oracle, positions, oracle per-cell counts).
+    edit(amp, bz, model) -> (amp, bz, model) replaces the population's tables
+    on both sides."""
     ctx = _ctx_for(run, N)
+    amp, bz, model = run.tracer(0, ipop)
+    if edit is not None:
+        amp, bz, model = edit(amp.copy(), bz.copy(), model)
+        ctx.set_tracer(0, ipop, model, amp, bz)
     s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
     ntr = int(run.scalar("n_srcs"))
     ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=ntr)
@@ -148,7 +154,6 @@ def _sources_vs_oracle(run, N, ipop, seed):
     npot = ctx.fetch_grid(1).astype(np.float64)
     n = ctx.sample_sources(ipop, seed, 2)
     pos, ipix = ctx.fetch_sources(ipop, n)
-    amp, bz, model = run.tracer(0, ipop)
     tab, n0, nf = ctx.norm_table(0, ipop)
     orc = _oracle(run)
     pop = orc.pop(N, run.scalar("l_box"), amp, bz, tab, n0, nf, model, ipop, seed)
@@ -175,6 +180,37 @@ def test_sources_bit_exact_192(workdir):
     """A 3*2^n grid: 1024-cell blocks span several rows."""
     path, run = make_run("C3", 192, workdir, z_max=1.6, with_maps=False)
     _sources_vs_oracle(run, 192, 1, 3003)
+
+
+@pytest.mark.parametrize("model", [0, 2])
+def test_sources_negative_bias(workdir, model):
+    """b(z) < 0: the lane bound does not apply (the bias grows toward voids),
+    so every cell inside the sphere takes the per-cell path.  (Model 1 with
+    b < 0 is left out: exp(b d/(1+d)) is unbounded as d -> -1.)"""
+    path, run = make_run("C3", 64, workdir, z_max=1.6, with_maps=False)
+
+    def edit(amp, bz, m):
+        return amp, np.full_like(bz, -0.6), model
+
+    _sources_vs_oracle(run, 64, 0, 3003, edit)
+
+
+def test_sources_gaps_in_nz(workdir):
+    """n(z) with empty stretches, a single nonzero node and a negative node:
+    zero bins, bins the FP64 path decides and coarse bounds of 0 next to
+    populated ones."""
+    path, run = make_run("C3", 128, workdir, z_max=1.6, with_maps=False)
+
+    def edit(amp, bz, m):
+        na = len(amp)
+        amp[na // 5: na // 4] = 0.0
+        amp[na // 3] = 0.0
+        amp[na // 3 + 40] = -amp[na // 3 + 41]
+        amp[na // 2: na // 2 + 300] = 0.0
+        amp[na // 2 + 150] = 3 * amp[na // 2 + 301]
+        return amp, bz, m
+
+    _sources_vs_oracle(run, 128, 0, 3003, edit)
 
 
 @pytest.mark.parametrize("N,ipop", [(64, 0), (128, 0), (128, 1), (128, 2)])
