@@ -61,7 +61,7 @@ def kernel_bytes(N, nsrc, npop):
 
 
 # stage_times() key holding each kernel's device time
-KERNEL_STAGE = {"k_gen_z": "gen", "k_col_y": "fft_yz", "k_c2r_x": "fft_x", "phi y+x": "fft_phi", "k_shells": "norm",
+KERNEL_STAGE = {"k_gen_z": "gen", "k_col_y": "fft_yz", "k_c2r_x": "fft_x", "phi y+x": "fft_phi", "k_shells": "shells",
                 "k_count": "src_pass", "k_fill+k_write": "src_write"}
 
 
@@ -201,13 +201,14 @@ def main():
         s2 = ctx.gaussian_fields(seed, rs, True)
         st = ctx.stage_times()
         ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=npop)
-        st["norm"] = ctx.stage_times()["norm"]
+        st_n = ctx.stage_times()
+        st["norm"], st["shells"] = st_n["norm"], st_n["shells"]
         ns = [ctx.sample_sources(-1, seed, 2)]
         st_s = ctx.stage_times()
         for k in ("srcs", "src_pass", "src_write", "fft_phi"):
             st[k] = st_s[k]
         if record:
-            for k in ("gen", "fft_yz", "fft_x", "fft_phi", "norm", "srcs", "src_pass", "src_write"):
+            for k in ("gen", "fft_yz", "fft_x", "fft_phi", "norm", "shells", "srcs", "src_pass", "src_write"):
                 acc[k] = acc.get(k, 0.0) + st[k]
         return ns, s2
 

@@ -536,8 +536,10 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, ntr, na, w,
                      (float4 *)c->misc);
   CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
   kShellFns[all0 ? 1 : 0][ntr](dim3(kShellBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
   CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
   hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,
                      (int)nvals, fin);
   CLR_HIP(c, hipGetLastError());
@@ -562,6 +564,8 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->stage_ms[ST_NORM] = ms;
+  hipEventElapsedTime(&ms, c->ev2, c->ev3);
+  c->stage_ms[ST_SHELLS] = ms;
   if (nz_out) *nz_out = nz;
   if ((counts || zsum || sums) && cap < nz)
     return clr_fail(c, CLR_ERR_ARG, "output capacity %d < nz=%d", cap, nz);

@@ -33,6 +33,7 @@ enum StageId {
   ST_SRC_PASS,    // k_count of a sample_sources call
   ST_SRC_WRITE,   // its k_fill and k_write launches
   ST_FFT_PHI,     // y and x passes of phi (second stream, overlapping later stages)
+  ST_SHELLS,      // the k_shells launch of a normalisation call
   ST_COUNT
 };
 

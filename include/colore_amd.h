@@ -190,12 +190,12 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
 
 /* Device time in milliseconds of the last call of each stage (0 if never),
  * in this order: mode generation + z pass, y pass, x pass, density
- * transform, shell sums, sources (whole call), kappa, ISW, imap, the
- * counting pass of the sources call, its catalogue writers (descriptors and
- * per-source positions), the y and x passes of phi (these run on a second
+ * transform, shell sums (whole call), sources (whole call), kappa, ISW, imap,
+ * the counting pass of the sources call, its catalogue writers (descriptors
+ * and per-source positions), the y and x passes of phi (these run on a second
  * stream, overlapping the density's passes and the stages after them; every
- * call that reads phi waits for them).  Fills at most n values; returns the
- * number of stages. */
+ * call that reads phi waits for them), the shell-sum kernel alone.  Fills at
+ * most n values; returns the number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 
 #ifdef __cplusplus
