@@ -99,7 +99,7 @@ int clr_create(clr_ctx **out, int n_grid, double l_box, const double pos_obs[3],
   c->device = device;
   c->N = n_grid;
   c->NH = n_grid / 2 + 1;
-  c->ngx = 2 * c->NH;
+  c->ngx = clr::grid_pitch(n_grid);
   c->L = l_box;
   c->dx = l_box / n_grid;
   for (int i = 0; i < 3; i++) c->obs[i] = pos_obs ? pos_obs[i] : 0.5 * l_box;
@@ -320,8 +320,10 @@ int clr_fetch_grid(clr_ctx *c, int which, float *host) {
   CLR_HIP(c, hipStreamSynchronize(c->stream));
   int rc = clr_phi_ready(c, 1);
   if (rc) return rc;
-  CLR_HIP(c, hipMemcpy(host, which == CLR_GRID_DENS ? c->dens : c->npot, c->grid_floats * sizeof(float),
-                       hipMemcpyDeviceToHost));
+  // device rows are grid_pitch(N) floats, host rows 2(N/2+1)
+  const size_t hrow = (size_t)2 * c->NH * sizeof(float), drow = (size_t)c->ngx * sizeof(float);
+  CLR_HIP(c, hipMemcpy2D(host, hrow, which == CLR_GRID_DENS ? c->dens : c->npot, drow, hrow, (size_t)c->N * c->N,
+                         hipMemcpyDeviceToHost));
   return CLR_OK;
 }
 
@@ -332,8 +334,9 @@ int clr_upload_grid(clr_ctx *c, int which, const float *host) {
   CLR_HIP(c, hipStreamSynchronize(c->stream));
   int rc = clr_phi_ready(c, 1);
   if (rc) return rc;
-  CLR_HIP(c, hipMemcpy(which == CLR_GRID_DENS ? c->dens : c->npot, host, c->grid_floats * sizeof(float),
-                       hipMemcpyHostToDevice));
+  const size_t hrow = (size_t)2 * c->NH * sizeof(float), drow = (size_t)c->ngx * sizeof(float);
+  CLR_HIP(c, hipMemcpy2D(which == CLR_GRID_DENS ? c->dens : c->npot, drow, host, hrow, hrow, (size_t)c->N * c->N,
+                         hipMemcpyHostToDevice));
   return CLR_OK;
 }
 

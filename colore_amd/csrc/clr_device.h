@@ -11,6 +11,12 @@
 
 namespace clr {
 
+// Floats per x row of a real grid in device memory: the 2(N/2+1) of the
+// in-place c2r layout rounded up to 32, so that every row starts on a
+// 128-byte cache line (the host-side layout keeps 2(N/2+1); the grid copies
+// convert).  Half of it is the row pitch of the complex half spectrum.
+__host__ __device__ constexpr int grid_pitch(int N) { return (2 * (N / 2 + 1) + 31) / 32 * 32; }
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11).  Counter-based: one call
 // maps (counter[4], key[2]) to four independent 32-bit words.

@@ -112,6 +112,7 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
   using RL = RadixList<Rs...>;
   static_assert(RL::product == N, "radices must multiply to N");
   constexpr int NH = N / 2 + 1, E = N / P, R0 = RL::first, RZ = RL::last, NTILE = (NH + T - 1) / T;
+  constexpr int NHP = grid_pitch(N) / 2;   // row pitch of the half spectrum
   __shared__ float2 lds[N * T];
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   int tile, ky;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
 #pragma unroll
     for (int q = 0; q < RZ; q++) {
       const int z = t + b * P + q * (N / RZ);
-      if (valid) dens[((size_t)z * N + ky) * NH + kx] = vd[b * RZ + q];
+      if (valid) dens[((size_t)z * N + ky) * NHP + kx] = vd[b * RZ + q];
     }
   __syncthreads();
   Stages<N, P, N, 1, Rs...>::run(vp, t, io, tw);
@@ -184,32 +185,33 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
 #pragma unroll
     for (int q = 0; q < RZ; q++) {
       const int z = t + b * P + q * (N / RZ);
-      if (valid) npot[((size_t)z * N + ky) * NH + kx] = vp[b * RZ + q];
+      if (valid) npot[((size_t)z * N + ky) * NHP + kx] = vp[b * RZ + q];
     }
 }
 
 // ---------------------------------------------------------------------------
-// y pass: column c of plane z holds element y at base[(z*N + y)*NH + kx]
+// y pass: column c of plane z holds element y at base[(z*N + y)*NHP + kx]
 // ---------------------------------------------------------------------------
 template <int N, int P, int T, int... Rs>
 __global__ __launch_bounds__(P *T) void k_col_y(float2 *__restrict__ g0, float2 *__restrict__ g1,
                                                  const float2 *__restrict__ tw) {
   using RL = RadixList<Rs...>;
   constexpr int NH = N / 2 + 1, E = N / P, R0 = RL::first, RZ = RL::last, NTILE = (NH + T - 1) / T;
+  constexpr int NHP = grid_pitch(N) / 2;   // row pitch of the half spectrum
   __shared__ float2 lds[N * T];
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   int tile, z;
   xcd_tile(blockIdx.x, NTILE, N, tile, z);
   const int kx = tile * T + c;
   const bool valid = kx < NH;
-  float2 *base = (blockIdx.z == 0 ? g0 : g1) + (size_t)z * N * NH + kx;
+  float2 *base = (blockIdx.z == 0 ? g0 : g1) + (size_t)z * N * NHP + kx;
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / R0; b++)
 #pragma unroll
     for (int m = 0; m < R0; m++) {
       const int y = t + b * P + m * (N / R0);
-      v[b * R0 + m] = valid ? base[(size_t)y * NH] : make_float2(0.f, 0.f);
+      v[b * R0 + m] = valid ? base[(size_t)y * NHP] : make_float2(0.f, 0.f);
     }
   const LdsCol<T> io{lds, c};
   Stages<N, P, N, 1, Rs...>::run(v, t, io, tw);
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(P *T) void k_col_y(float2 *__restrict__ g0, float2 
 #pragma unroll
     for (int q = 0; q < RZ; q++) {
       const int y = t + b * P + q * (N / RZ);
-      if (valid) base[(size_t)y * NH] = v[b * RZ + q];
+      if (valid) base[(size_t)y * NHP] = v[b * RZ + q];
     }
 }
 
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(P *RPB) void k_c2r_x(float *__restrict__ g0, float 
   const bool valid = row < nrows;
   const int field = blockIdx.y;
   float *gbase = field == 0 ? g0 : g1;
-  float2 *X = (float2 *)(gbase + (valid ? row : 0) * (size_t)(2 * NH));
+  float2 *X = (float2 *)(gbase + (valid ? row : 0) * (size_t)grid_pitch(N));
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / R0; b++)

@@ -41,7 +41,7 @@ struct clr_ctx {
   int device = 0;
   int N = 0;
   int NH = 0;               // N/2+1
-  int ngx = 0;              // 2*(N/2+1)
+  int ngx = 0;              // floats per device row: grid_pitch(N) >= 2*(N/2+1)
   double L = 0, dx = 0;
   double obs[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
