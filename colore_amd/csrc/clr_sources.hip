@@ -2,7 +2,7 @@
 // all populations, and srcs_get_local_properties_single (srcs.c:386-416).
 //
 // Blocks are one wavefront over 512 consecutive cells (z-major flat order; a
-// lane owns 8 cells of one row).
+// lane owns 8 cells of one row); k_count's wavefronts take two such blocks.
 // k_count : per lane and population, one upper bound on lambda over the
 //           lane's 8 cells (largest delta, coarse bounds of the bins their
 //           radii can fall in) settles k = 0 wherever the 16 high uniform
@@ -270,28 +270,35 @@ __device__ inline int settle_cell(const SrcArgs &a, const float4 *tab, int model
   return -1;
 }
 
-// queue entry: cell within the block (bits 0-8) | population (bits 9-12) |
-// the uniform's high 16 bits << 16.  The queue is worked when another
-// population's 512 entries might not fit, so usually once per block.
-constexpr int kQueueCap = 3 * kBlockCells / 2;
+// k_count works on 1024 cells per wavefront (16 per lane, two of the 512-cell
+// blocks whose totals k_scan and k_fill use), so that its fixed costs per
+// wavefront -- the density loads' latency, the coarse-bound gathers, the queue
+// pass -- are shared by twice the cells.
+constexpr int kCountLaneCells = 16;
+constexpr int kCountCells = kLanes * kCountLaneCells;
+constexpr int kCountSubs = kCountCells / kBlockCells;
+// queue entry: cell within the wavefront's 1024 (bits 0-9) | population
+// (bits 10-13) | the uniform's high 16 bits << 16.  The queue is worked when
+// another population's 1024 entries might not fit, so usually once.
+constexpr int kQueueCap = kCountCells + kCountCells / 4;
 
-__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(8))) void k_count(SrcArgs a) {
+__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(6))) void k_count(SrcArgs a) {
   __shared__ uint32_t queue[kQueueCap];
-  extern __shared__ uint2 sbytes[];   // [npop][kLanes]: one byte per cell and population
+  extern __shared__ uint4 sbytes[];   // [npop][kLanes]: one byte per cell and population
   __shared__ uint2 sgeo[kLanes];      // per lane: first column | row y << 16, row z
-  __shared__ uint32_t stot[CLR_NPOP_MAX];
+  __shared__ uint32_t stot[CLR_NPOP_MAX][kCountSubs];
   __shared__ const float4 *stab[CLR_NPOP_MAX];
   __shared__ int smodel[CLR_NPOP_MAX];
   const int lane = threadIdx.x;
   const size_t blk = blockIdx.x;
-  const size_t cell0 = blk * kBlockCells + (size_t)kCellsPerLane * lane;
+  const size_t base = blk * kCountCells;
+  const size_t cell0 = base + (size_t)kCountLaneCells * lane;
   const int N = a.N;
-  // row and column of the lane's first cell: the block part in scalar
+  // row and column of the lane's first cell: the wavefront part in scalar
   // 64-bit arithmetic, the lane part in 32 bits (rows < 2^22)
-  const size_t base = blk * kBlockCells;
   const size_t row0 = base / (size_t)N;
   const int obase = (int)(base - row0 * N);
-  const int o = obase + kCellsPerLane * lane;
+  const int o = obase + kCountLaneCells * lane;
   const int drow = o / N;
   const int ix0 = o - drow * N;
   const unsigned row = (unsigned)row0 + (unsigned)drow;
@@ -299,17 +306,17 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(8))) voi
   const unsigned long long below = (1ull << lane) - 1ull;
   sgeo[lane] = make_uint2((uint32_t)ix0 | ((uint32_t)iy << 16), (uint32_t)iz);
   for (int ip = 0; ip < a.npop; ip++) {
-    sbytes[ip * kLanes + lane] = make_uint2(0u, 0u);
+    sbytes[ip * kLanes + lane] = make_uint4(0u, 0u, 0u, 0u);
+    if (lane < kCountSubs) stot[ip][lane] = 0;
     if (lane == 0) {
-      stot[ip] = 0;
       stab[ip] = a.pop[ip].tab;
       smodel[ip] = a.pop[ip].model;
     }
   }
 
-  // --- the lane's 8 cells: largest delta and the range of their radii -------
+  // --- the lane's 16 cells: largest delta and the range of their radii ------
   // Every count of the lane is 0 where u < exp(-lambda_max) for an upper
-  // bound lambda_max of the reference's lambda over the 8 cells:
+  // bound lambda_max of the reference's lambda over the 16 cells:
   //   lambda = n(r) dx^3 norm(r) bias_model(delta, b(r)) <= Pmax * bias_max
   // with Pmax and the range of b from the coarse bounds of the bins that the
   // radii can fall in, and bias_max the larger corner (delta_max, b_min or
@@ -319,24 +326,21 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(8))) voi
   // of the row when it lies between them), in FP32 with a 1e-5 margin.
   float dmax;
   {
-    const float2 *p = (const float2 *)(a.g + (size_t)row * a.ngx + ix0);
-    float d[kCellsPerLane];
+    const float4 *p = (const float4 *)(a.g + (size_t)row * a.ngx + ix0);   // rows start on 128 bytes
+    float4 v[kCountLaneCells / 4];
 #pragma unroll
-    for (int h = 0; h < kCellsPerLane / 2; h++) {
-      const float2 v = p[h];
-      d[2 * h] = v.x;
-      d[2 * h + 1] = v.y;
-    }
-    dmax = d[0];
+    for (int h = 0; h < kCountLaneCells / 4; h++) v[h] = p[h];
+    dmax = fmaxf(fmaxf(v[0].x, v[0].y), fmaxf(v[0].z, v[0].w));
 #pragma unroll
-    for (int c = 1; c < kCellsPerLane; c++) dmax = fmaxf(dmax, d[c]);
+    for (int h = 1; h < kCountLaneCells / 4; h++)
+      dmax = fmaxf(dmax, fmaxf(fmaxf(v[h].x, v[h].y), fmaxf(v[h].z, v[h].w)));
   }
   bool out;
   int ilo, ihi;
   {
     const double y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
     const double yz2 = y0 * y0 + z0 * z0;
-    const double xa = ix0 * a.dx - a.ox, xb = xa + (kCellsPerLane - 1) * a.dx;
+    const double xa = ix0 * a.dx - a.ox, xb = xa + (kCountLaneCells - 1) * a.dx;
     const double ra2 = xa * xa + yz2, rb2 = xb * xb + yz2;
     const double lo2 = (xa <= 0 && xb >= 0) ? yz2 : fmin(ra2, rb2);
     const float rlo = sqrtf((float)lo2) * (1.f - 1e-5f);
@@ -379,36 +383,41 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(8))) voi
       }
     }
     if (__ballot(hcrit < 65536) != 0ull) {
-      const u32x4 rr = draw(cell0 >> 3, stream_poisson(P.ipop), 0u, a.seed);
 #pragma unroll
-      for (int c = 0; c < kCellsPerLane; c++) {
-        const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
-        const bool open = (int)hi >= hcrit;
-        const unsigned long long m = __ballot(open);
-        if (m != 0ull) {
-          if (open)
-            queue[qn + __popcll(m & below)] = (uint32_t)(kCellsPerLane * lane + c) | ((uint32_t)ip << 9) | (hi << 16);
-          qn += __popcll(m);
+      for (int g = 0; g < kCountLaneCells / 8; g++) {
+        // the high 16 bits of 8 cells come from one draw at site cell/8
+        const u32x4 rr = draw((cell0 >> 3) + g, stream_poisson(P.ipop), 0u, a.seed);
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const uint32_t hi = poisson_hi16(rr, (uint64_t)c);
+          const bool open = (int)hi >= hcrit;
+          const unsigned long long m = __ballot(open);
+          if (m != 0ull) {
+            if (open)
+              queue[qn + __popcll(m & below)] =
+                  (uint32_t)(kCountLaneCells * lane + 8 * g + c) | ((uint32_t)ip << 10) | (hi << 16);
+            qn += __popcll(m);
+          }
         }
       }
     }
     // --- B: the queue, all lanes busy; run when another population might
     // not fit, and after the last -------------------------------------------
-    if (qn > kQueueCap - kBlockCells || ip == a.npop - 1) {
+    if (qn > kQueueCap - kCountCells || ip == a.npop - 1) {
       __syncthreads();   // one wavefront: orders the LDS writes above before the reads below
       for (int i = lane; i < qn; i += kLanes) {
         const uint32_t e = queue[i];
-        const int cb = (int)(e & (kBlockCells - 1));
-        const int q = (int)((e >> 9) & 15u);
+        const int cb = (int)(e & (kCountCells - 1));
+        const int q = (int)((e >> 10) & 15u);
         const uint32_t hi = e >> 16;
         const size_t cell = base + cb;
-        const uint2 gl = sgeo[cb / kCellsPerLane];
-        const int ix = (int)(gl.x & 0xFFFFu) + cb % kCellsPerLane, iy = (int)(gl.x >> 16), iz = (int)gl.y;
+        const uint2 gl = sgeo[cb / kCountLaneCells];
+        const int ix = (int)(gl.x & 0xFFFFu) + cb % kCountLaneCells, iy = (int)(gl.x >> 16), iz = (int)gl.y;
         const float dl = a.g[((size_t)iz * N + iy) * a.ngx + ix];
         const int k = settle_cell(a, stab[q], smodel[q], a.pop[0].ipop + q, q, cell, ix, iy, iz, dl, hi);
         ((uint8_t *)(sbytes + q * kLanes))[cb] =
             (uint8_t)(k < 0 ? kBytePending : (k < kBytePending ? k : kByteEscape));
-        if (k > 0) atomicAdd(&stot[q], (uint32_t)k);
+        if (k > 0) atomicAdd(&stot[q][cb / kBlockCells], (uint32_t)k);
         if (k >= kBytePending) {
           // the exact count of a saturated byte goes to the escape list
           const unsigned x = atomicAdd(a.esc_n, 1u);
@@ -420,11 +429,11 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(8))) voi
       qn = 0;
     }
   }
-  // --- the bytes of the lane's 8 cells and the block totals ------------------
+  // --- the bytes of the lane's 16 cells and the two block totals -------------
   for (int ip = 0; ip < a.npop; ip++) {
     const PopTab &P = a.pop[ip];
-    *(uint2 *)(P.cnt8 + cell0) = sbytes[ip * kLanes + lane];
-    if (lane == 0) P.btot[blk] = stot[ip];
+    *(uint4 *)(P.cnt8 + cell0) = sbytes[ip * kLanes + lane];
+    if (lane < kCountSubs) P.btot[blk * kCountSubs + lane] = stot[ip][lane];
   }
 }
 
@@ -698,8 +707,8 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   const int p0 = ipop < 0 ? 0 : ipop, p1 = ipop < 0 ? nall : ipop + 1;
   const int N = c->N, na = c->na;
   const size_t ncells = (size_t)N * N * N;
-  if (N % kCellsPerLane || ncells % kBlockCells)
-    return clr_fail(c, CLR_ERR_ARG, "grid of %d^3 cells is not a whole number of %d-cell blocks", N, kBlockCells);
+  if (N % kCountLaneCells || ncells % kCountCells)
+    return clr_fail(c, CLR_ERR_ARG, "grid of %d^3 cells is not a whole number of %d-cell blocks", N, kCountCells);
   for (int i = p0; i < p1; i++) {
     const TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
     if (!T.set || !T.norm_set) return clr_fail(c, CLR_ERR_STATE, "population %d has no norm table", i);
@@ -722,7 +731,8 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   const size_t nblocks = ncells / kBlockCells;
   const size_t nsb = (nblocks + kScanB - 1) / kScanB;
   // scratch per population: bytes (ncells), block totals, offsets, scan partials
-  const size_t per_pop = ncells + nblocks * 4 + (nblocks + 1) * 8 + (nsb + 1) * 8 + 64;
+  // (a multiple of 256 bytes: every population's bytes start aligned)
+  const size_t per_pop = (ncells + nblocks * 4 + (nblocks + 1) * 8 + (nsb + 1) * 8 + 64 + 255) / 256 * 256;
   rc = clr_ensure(c, (void **)&c->rowcnt, &c->rows_n, (size_t)np * per_pop);
   if (rc) return rc;
 
@@ -787,8 +797,8 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     CLR_HIP(c, hipMemsetAsync(a.esc_n, 0, sizeof(unsigned int), c->stream));
     CLR_HIP(c, hipMemsetAsync(a.exq_n, 0, sizeof(unsigned int), c->stream));
     CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
-    hipLaunchKernelGGL(k_count, dim3((unsigned)nblocks), dim3(kLanes), (size_t)np * kLanes * sizeof(uint2), c->stream,
-                       a);
+    hipLaunchKernelGGL(k_count, dim3((unsigned)(ncells / kCountCells)), dim3(kLanes),
+                       (size_t)np * kLanes * sizeof(uint4), c->stream, a);
     CLR_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_exact, dim3((a.exq_cap + 255) / 256), dim3(256), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
