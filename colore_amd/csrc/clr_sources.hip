@@ -529,33 +529,36 @@ __device__ inline int escaped_count(const ulonglong2 *esc, unsigned n, unsigned 
   return (lo < n && esc[lo].x == key) ? (int)esc[lo].y : 0;
 }
 
+// One wavefront per kCountCells cells (two 512-cell blocks of the scan): a lane
+// reads the 16 count bytes of its cells per population.
 __global__ __launch_bounds__(kLanes) void k_fill(SrcArgs a) {
-  constexpr int G = 4;   // populations whose loads are in flight together
+  constexpr int G = 2;   // populations whose loads are in flight together
   const int lane = threadIdx.x;
-  const size_t blk = blockIdx.x;
-  const uint64_t cell0 = (uint64_t)blk * kBlockCells + (uint64_t)kCellsPerLane * lane;
+  const size_t w0 = (size_t)blockIdx.x * kCountSubs;   // first 512-cell block
+  const uint64_t cell0 = (uint64_t)blockIdx.x * kCountCells + (uint64_t)kCountLaneCells * lane;
   for (int g0 = 0; g0 < a.npop; g0 += G) {
     long long b0[G], b1[G];
-    uint2 w[G];
+    uint4 w[G];
 #pragma unroll
     for (int q = 0; q < G; q++) {
       b0[q] = b1[q] = 0;
-      w[q] = make_uint2(0u, 0u);
+      w[q] = make_uint4(0u, 0u, 0u, 0u);
       if (g0 + q < a.npop) {
         const PopTab &P = a.pop[g0 + q];
-        b0[q] = P.boff[blk];
-        b1[q] = P.boff[blk + 1];
-        w[q] = *(const uint2 *)(P.cnt8 + cell0);
+        b0[q] = P.boff[w0];
+        b1[q] = P.boff[w0 + kCountSubs];
+        w[q] = *(const uint4 *)(P.cnt8 + cell0);
       }
     }
 #pragma unroll
     for (int q = 0; q < G; q++) {
-      if (g0 + q >= a.npop || b1[q] == b0[q]) continue;   // uniform: no sources in this block
-      int k[kCellsPerLane];
+      if (g0 + q >= a.npop || b1[q] == b0[q]) continue;   // uniform: no sources in these cells
+      const uint32_t wd[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
+      int k[kCountLaneCells];
       long long mine = 0;
 #pragma unroll
-      for (int c = 0; c < kCellsPerLane; c++) {
-        k[c] = (int)(((c < 4 ? w[q].x : w[q].y) >> (8 * (c & 3))) & 0xFFu);
+      for (int c = 0; c < kCountLaneCells; c++) {
+        k[c] = (int)((wd[c / 4] >> (8 * (c & 3))) & 0xFFu);
         if (k[c] == kByteEscape)
           k[c] = escaped_count(a.esc, a.esc_count, ((unsigned long long)(g0 + q) << 56) | (cell0 + c));
         mine += k[c];
@@ -569,7 +572,7 @@ __global__ __launch_bounds__(kLanes) void k_fill(SrcArgs a) {
       long long pid = b0[q] + incl - mine;
       uint4 *desc = a.pop[g0 + q].desc;
 #pragma unroll
-      for (int c = 0; c < kCellsPerLane; c++) {
+      for (int c = 0; c < kCountLaneCells; c++) {
         const uint64_t cell = cell0 + c;
         for (int is = 0; is < k[c]; is++, pid++)
           if (pid < b1[q]) desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
@@ -863,7 +866,7 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   }
   CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
   if (total > 0) {
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)nblocks), dim3(kLanes), 0, c->stream, a);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)(ncells / kCountCells)), dim3(kLanes), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
   }
   rc = clr_phi_ready(c, 0);   // k_write reads phi
