@@ -57,27 +57,38 @@ struct ShellArgs {
   int N, ngx, nz, ntr, fused, na;
   double dx, ox, oy, oz, idr, r_last, z_last, sigma2, idz;
   const double *r_arr, *zt;
-  const float4 *tab;   // per bin: D, z, b_0 .. b_{NT-1} in (NT+5)/4 float4s (k_pack_shell_table)
+  const float4 *tab;   // per bin: (value, slope) of D, z, b_0 .. b_{NT-1} (k_pack_shell_table)
   float d1_last_f, z_last_f, idr_f, idz_f;
   float bz0[kMaxTr];
   int model[kMaxTr];
 };
 
-// FP32 tables of one shell pass packed per bin: component 0 is D(r),
-// component 1 is z(r), component 2 + t is b_t(r); w float4s per bin.
+// FP32 tables of one shell pass packed per bin i as (f_i, f_{i+1} - f_i) for
+// quantity 0 = D(r), 1 = z(r), 2 + t = b_t(r): one gather per cell, and
+// f_i + slope * t is the same FP32 value as f_i + (f_{i+1} - f_i) * t.
+// shell_tab_w(ntr) float4s per bin.
 struct ShellTabSrc {
   const float *d1f;
   const double *zt;
   const float *bzf[kMaxTr];
 };
-__global__ void k_pack_shell_table(ShellTabSrc src, int ntr, int na, int w, float4 *__restrict__ out) {
+__host__ __device__ constexpr int shell_tab_w(int ntr) { return (2 * (2 + ntr) + 3) / 4; }
+__global__ void k_pack_shell_table(ShellTabSrc src, int ntr, int na, float4 *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= na) return;
-  float v[4 * 3];
-  for (int j = 0; j < 4 * w; j++) v[j] = 0.f;
-  v[0] = src.d1f[i];
-  v[1] = (float)src.zt[i];
-  for (int t = 0; t < ntr; t++) v[2 + t] = src.bzf[t][i];
+  const int w = shell_tab_w(ntr), j = i + 1 < na ? i + 1 : i;
+  float v[4 * shell_tab_w(kMaxTr)];
+  for (int k = 0; k < 4 * w; k++) v[k] = 0.f;
+  const float d0 = src.d1f[i], d1 = src.d1f[j], z0 = (float)src.zt[i], z1 = (float)src.zt[j];
+  v[0] = d0;
+  v[1] = d1 - d0;
+  v[2] = z0;
+  v[3] = z1 - z0;
+  for (int t = 0; t < ntr; t++) {
+    const float b0 = src.bzf[t][i], b1 = src.bzf[t][j];
+    v[4 + 2 * t] = b0;
+    v[5 + 2 * t] = b1 - b0;
+  }
   for (int k = 0; k < w; k++) out[(size_t)i * w + k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
 
@@ -178,12 +189,22 @@ constexpr size_t shell_lds_bytes(int nz) {
          (size_t)kWaves * kSlots * 64 * ((2 + NT) * sizeof(double) + sizeof(int));
 }
 
+// z(r) of cell (ix,iy,iz) by the reference's FP64 expressions (density.c:
+// 1084-1094, f_of_r_linear).
+__device__ inline double cell_z_f64(const ShellArgs &a, int ix, int iy, int iz) {
+#pragma clang fp contract(off)
+  const double obs[3] = {a.ox, a.oy, a.oz};
+  double x0, y0, z0;
+  const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
+  return f_of_r(a.r_arr, a.zt, a.idr, a.r_last, r, 0.0, a.z_last);
+}
+
 // M0: every tracer uses bias model 0 (no other model's code in the loop).
 template <int NT, bool M0>
 __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_shells(ShellArgs a,
                                                                                       double *__restrict__ partial) {
   constexpr int NQ = 2 + NT;
-  constexpr int W = (NT + 5) / 4;
+  constexpr int W = shell_tab_w(NT);
   extern __shared__ double acc[];
   const int nz = a.nz, N = a.N;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -192,7 +213,6 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
   int *logid = (int *)(acc + (size_t)kWaves * nz * NQ + (size_t)kWaves * kSlots * NQ * 64) + wave * kSlots * 64;
   for (int i = threadIdx.x; i < kWaves * nz * NQ; i += blockDim.x) acc[i] = 0;
   __syncthreads();
-  const double obs[3] = {a.ox, a.oy, a.oz};
   const float ox = (float)a.ox, oy = (float)a.oy, oz = (float)a.oz, dxf = (float)a.dx;
   const int ntx = (N + kTX - 1) / kTX, nty = N / (kTY * kSub), ntz = N / kTZ;
   const long ntiles = (long)ntx * nty * ntz;
@@ -219,7 +239,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
     // is processed.  Bin lookup: FP32 radius (absolute error ~1e-3 cell), one
     // index for D, z and every b_t (f_of_r32 semantics).
     struct Bin {
-      float4 A[W], B[W];
+      float4 C[W];
       float tq;
       bool e0, e1;
     };
@@ -233,10 +253,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
       const int i0 = (bn.e0 || bn.e1) ? 0 : (int)xq;
       bn.tq = xq - (float)i0;
 #pragma unroll
-      for (int k = 0; k < W; k++) {
-        bn.A[k] = a.tab[(size_t)i0 * W + k];
-        bn.B[k] = a.tab[(size_t)(i0 + 1) * W + k];
-      }
+      for (int k = 0; k < W; k++) bn.C[k] = a.tab[(size_t)i0 * W + k];
     };
     // the lane's cell in the tile's first row; rows at uniform offsets from it
     float *const tb = a.g + ((size_t)(tz * kTZ) * N + iy0) * a.ngx + (valid ? ix : 0);
@@ -261,18 +278,20 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
       if (rr + 1 < kTY * kTZ) fetch(rr + 1, bnext);
       if (rr + kPrefetch < kTY * kTZ && valid) dq[j] = tb[roff(rr + kPrefetch)];
       if (valid) {
-        float comp[4 * W];
+        float comp[NQ];
         const bool e0 = bn.e0, e1 = bn.e1;
         {
           const float tq = bn.tq;
+          float f[4 * W];
 #pragma unroll
           for (int k = 0; k < W; k++) {
-            const float4 A = bn.A[k], B = bn.B[k];
-            comp[4 * k + 0] = A.x + (B.x - A.x) * tq;
-            comp[4 * k + 1] = A.y + (B.y - A.y) * tq;
-            comp[4 * k + 2] = A.z + (B.z - A.z) * tq;
-            comp[4 * k + 3] = A.w + (B.w - A.w) * tq;
+            f[4 * k + 0] = bn.C[k].x;
+            f[4 * k + 1] = bn.C[k].y;
+            f[4 * k + 2] = bn.C[k].z;
+            f[4 * k + 3] = bn.C[k].w;
           }
+#pragma unroll
+          for (int q = 0; q < NQ; q++) comp[q] = f[2 * q] + f[2 * q + 1] * tq;
           if (e0 || e1) {
             comp[0] = e0 ? 1.f : a.d1_last_f;
             comp[1] = e0 ? 0.f : a.z_last_f;
@@ -294,10 +313,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
         int ind = (int)sf + 1;
         const float fr = sf - floorf(sf);
         if (e0 || e1 || !(fr > 5e-4f && fr < 1.f - 5e-4f)) {
-#pragma clang fp contract(off)
-          double x0, y0, z0;
-          const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
-          zz = f_of_r(a.r_arr, a.zt, a.idr, a.r_last, r, 0.0, a.z_last);
+          zz = cell_z_f64(a, ix, iy, iz);
           ind = (int)(zz * a.idz) + 1;
         }
         if (ind >= 0 && ind < nz) {
@@ -527,13 +543,13 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
   if (rc) return rc;
   double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals, *work = fin + nvals;
-  const int w = (ntr + 5) / 4;
+  const int w = shell_tab_w(ntr);
   rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
   if (rc) return rc;
   a.tab = (const float4 *)c->misc;
 
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, ntr, na, w,
+  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, ntr, na,
                      (float4 *)c->misc);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
