@@ -8,7 +8,7 @@ Tolerances (FP32 device grids against the FP64 oracle):
   * shell sums: counts exact; z and bias sums (FP32 per-cell terms summed
     in FP64) 2e-6 relative; norm tables from the same sums 1e-12 relative.
   * Poisson counts and the catalogue (positions, base pixels): bit-exact given
-    the same density grid and norm tables; dz_rsd within 1e-12 relative.
+    the same density grid and norm tables; dz_rsd within 1e-6 relative.
   * kappa / ISW maps from the same potential: 1e-5 of the map RMS.
 """
 import os
@@ -58,6 +58,19 @@ def test_gaussian_fields_philox(workdir, N):
         rms_e, max_e = _field_err(g, o, N)
         assert rms_e < 2e-6 and max_e < 5e-5, (N, rms_e, max_e)
     assert abs(s2 / os2 - 1) < 1e-5, (s2, os2)
+
+
+@pytest.mark.parametrize("N", [32, 192])
+def test_grid_copies_round_trip(workdir, N):
+    """Device rows are padded to 128-byte lines; uploads and fetches convert
+    to and from the host layout float[N][N][2(N/2+1)] exactly."""
+    from colore_amd import lib
+    ctx = lib.Context(N, 1000.0)
+    rng = np.random.default_rng(N)
+    for which in (0, 1):
+        g = rng.standard_normal(ctx.padded_shape).astype(np.float32)
+        ctx.upload_grid(which, g)
+        assert np.array_equal(ctx.fetch_grid(which), g)
 
 
 def test_gaussian_fields_unsmoothed_potential(workdir):
