@@ -5,8 +5,11 @@
 // v_r V1(r)/H(r) (imap.c:199-200), then nsub^3 sub-points at
 // x + (i+1/2) dx/nsub (imap.c:212-218); each sub-point whose shifted radius
 // falls in a shell adds T and 1 to its RING pixel (he_ang2pix,
-// healpix_extra.c:141-172) with device atomics, as the reference's
-// omp atomic does.  The shell of a radius is found by binary search over the
+// healpix_extra.c:141-172).  A cell's sub-points land in a few (shell,
+// pixel) bins: the thread counts them in a small register cache and adds
+// n T and n per bin with device atomics (the reference adds T and 1 per
+// sub-point with an omp atomic; the counts are the same, the float sums
+// differ only by rounding, whose order neither fixes).  The shell of a radius is found by binary search over the
 // sorted shells; for contiguous shells this is the shell the reference's walk
 // (get_r_index_imap, imap.c:76-103) returns.
 #include "clr_internal.h"
@@ -81,6 +84,21 @@ __global__ __launch_bounds__(256) void k_imap(ImapArgs a, const double *__restri
     int irad = shell_of(r0s, rfs, a.nr, r0);
     const int ns = irad < 0 ? nsub[0] : (irad >= a.nr ? nsub[a.nr - 1] : nsub[irad]);
     const double dxs = a.dx / ns;
+    // (bin, hits) of the cell's sub-points; a full cache flushes its oldest
+    // entry
+    constexpr int kCache = 8;
+    int64_t key[kCache];
+    int hits[kCache];
+#pragma unroll
+    for (int k = 0; k < kCache; k++) {
+      key[k] = -1;
+      hits[k] = 0;
+    }
+    int next = 0;
+    const auto flush = [&](int64_t o, int n) {
+      atomicAdd(&data[o], temp * (float)n);
+      atomicAdd(&nadd[o], n);
+    };
     for (int izz = 0; izz < ns; izz++) {
       const double z = z0 + (izz + 0.5) * dxs;
       for (int iyy = 0; iyy < ns; iyy++) {
@@ -98,13 +116,31 @@ __global__ __launch_bounds__(256) void k_imap(ImapArgs a, const double *__restri
           irad = shell_of(r0s, rfs, a.nr, r + dr_rsd);
           if (irad >= 0 && irad < a.nr) {
             const int64_t pix = ang2pix_ring(a.nside, cth, phi);
-            const size_t o = (size_t)irad * a.npix + pix;
-            atomicAdd(&data[o], temp);
-            atomicAdd(&nadd[o], 1);
+            const int64_t o = (int64_t)irad * a.npix + pix;
+            bool found = false;
+#pragma unroll
+            for (int k = 0; k < kCache; k++)
+              if (key[k] == o) {
+                hits[k]++;
+                found = true;
+              }
+            if (!found) {
+#pragma unroll
+              for (int k = 0; k < kCache; k++)
+                if (k == next) {
+                  if (key[k] >= 0) flush(key[k], hits[k]);
+                  key[k] = o;
+                  hits[k] = 1;
+                }
+              next = next == kCache - 1 ? 0 : next + 1;
+            }
           }
         }
       }
     }
+#pragma unroll
+    for (int k = 0; k < kCache; k++)
+      if (key[k] >= 0) flush(key[k], hits[k]);
   }
 }
 
