@@ -418,9 +418,14 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
         inj);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
-  // delta: y and x passes with the sigma^2 partials
+  // delta: y and x passes with the sigma^2 partials; phi's passes start on
+  // stream2 once delta's y pass is done (its y pass shares the memory system
+  // with delta's x pass, then both run under the instruction-bound stages
+  // that follow on stream); every reader of npot waits on ev_phi
   L.coly(dim3((unsigned)(ntile * N), 1, 1), dim3(L.colThreads), c->stream, d, d, c->tw);
   CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev_gen, c->stream));
+  CLR_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_gen, 0));
   hipEvent_t e2 = c->ev2, e3 = c->ev3;
   CLR_HIP(c, hipEventRecord(e2, c->stream));
   L.c2r(dim3((unsigned)nxb, 1), dim3(L.xThreads), c->stream, c->dens, c->dens, norm, c->red, nrows, c->tw);
@@ -428,11 +433,6 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, c->stream, c->red, nxb, c->red_out);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(e3, c->stream));
-  // phi: y and x passes on stream2 after the density's (memory-bound like
-  // them), overlapping the instruction-bound stages that follow on stream;
-  // every reader of npot waits on ev_phi
-  CLR_HIP(c, hipEventRecord(c->ev_gen, c->stream));
-  CLR_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_gen, 0));
   CLR_HIP(c, hipEventRecord(c->ev4, c->stream2));
   L.coly(dim3((unsigned)(ntile * N), 1, 1), dim3(L.colThreads), c->stream2, p, p, c->tw);
   CLR_HIP(c, hipGetLastError());
