@@ -104,6 +104,16 @@ def time_maps(run, ctx, config, sigma2):
         ms = sorted(ts)[1]
         out[which] = {"ms": ms, "nside": nside, "planes": n,
                       "pixel_steps_per_s": 12 * nside * nside * (N // 2) / (ms * 1e-3)}
+    # the sources' line-of-sight pass (velocity + lensing) on the first
+    # population's catalogue, re-sampled alone from the same density
+    n0 = ctx.sample_sources(0, int(run.scalar("seed")), 2)
+    a2r = run.r_a2r()
+    ts = []
+    for _ in range(3):
+        ctx.source_beams(0, n0, a2r, True)
+        ts.append(ctx.stage_times()["beams"])
+    ms = sorted(ts)[1]
+    out["src_beams"] = {"ms": ms, "sources": n0, "sources_per_s": n0 / (ms * 1e-3)}
     if int(run.scalar("n_imap")) > 0:
         # the intensity species joins the shell sums of the last step's density
         amp, bz, model = run.tracer(1, 0)

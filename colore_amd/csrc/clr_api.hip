@@ -306,6 +306,12 @@ int clr_isw(clr_ctx *c, int nside, int nplanes, const double *r_src, float *out)
   return clr_launch_los(c, 1, nside, nplanes, r_src, out);
 }
 
+int clr_source_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz_rsd, float *lens, long long n) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_src_beams(c, ipop, r_arr_a2r, dz_rsd, lens, n);
+}
+
 int clr_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf, float *data,
              int32_t *nadd) {
   if (!c) return CLR_ERR_ARG;

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(P *RPB) void k_c2r_x(float *__restrict__ g0, float 
                                                    double *__restrict__ partial, size_t nrows,
                                                    const float2 *__restrict__ tw) {
   using RL = RadixList<Rs...>;
-  constexpr int M = N / 2, NH = N / 2 + 1, E = M / P, R0 = RL::first, RZ = RL::last;
+  constexpr int M = N / 2, E = M / P, R0 = RL::first, RZ = RL::last;
   static_assert(RL::product == M, "radices must multiply to N/2");
   constexpr int LR = lds_row_len<M>();
   __shared__ float2 lds[RPB * LR];

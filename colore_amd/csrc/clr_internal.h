@@ -34,6 +34,7 @@ enum StageId {
   ST_SRC_WRITE,   // its k_fill and k_write launches
   ST_FFT_PHI,     // y and x passes of phi (second stream, overlapping later stages)
   ST_SHELLS,      // the k_shells launch of a normalisation call
+  ST_BEAMS,       // k_src_beams of a clr_source_beams call
   ST_COUNT
 };
 
@@ -128,5 +129,6 @@ int clr_phi_ready(clr_ctx *c, int host);
 int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc);
 int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
 int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out);
+int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz, float *lens, long long n);
 int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf,
                     float *data, int32_t *nadd);

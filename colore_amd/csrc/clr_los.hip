@@ -185,3 +185,271 @@ int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *
   c->stage_ms[which == 0 ? ST_KAPPA : ST_ISW] = ms;
   return CLR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Sources' line-of-sight properties: srcs_beams_preproc_single
+// (srcs.c:425-445), srcs_get_beam_properties_single (srcs.c:452-625, the
+// build without _USE_FAST_LENSING) and srcs_beams_postproc_single
+// (srcs.c:634-737) for one population.  One thread per source in catalogue
+// order: sources of one wavefront come from neighbouring cells, so their
+// rays run side by side and share the cache lines of the stencil reads.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BeamArgs {
+  const float *phi;
+  int N, ngx, nr, na;
+  double idx, dr, idr, ox, oy, oz, factor_vel;
+  double glob_idr, r_last, z_last, v1_0, v1_last;
+  const double *r_arr, *z_arr, *v1_arr;   // background r-tables
+  const double *a2r;                      // r(a) at a = i/(na-1) (r_of_z, cosmo.c:101-112)
+  const double *fac;                      // fac_r_0, fac_r_1, fac_r_2 (nr each)
+};
+
+// The reference wraps once (beaming.c:129-133), which covers every cell the
+// march and the source reads reach when r_max is below the box size; the
+// full modulo keeps other inputs inside the grid.
+__device__ inline int wrap1(int v, int N) {
+  if (v >= N || v < 0) {
+    v %= N;
+    if (v < 0) v += N;
+  }
+  return v;
+}
+
+// get_element (beaming.c:31-117), RETURN_VEL: unscaled central differences,
+// periodic in x, y and z on one GPU (fourier.c:412-413).
+__device__ inline void vel_at(const float *__restrict__ phi, int N, size_t ngx, size_t plane, int ix, int iy,
+                              int iz, double v[3]) {
+  const int xh = ix == N - 1 ? 0 : ix + 1, xl = ix == 0 ? N - 1 : ix - 1;
+  const int yh = iy == N - 1 ? 0 : iy + 1, yl = iy == 0 ? N - 1 : iy - 1;
+  const int zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
+  const size_t Z0 = (size_t)iz * plane, Y0 = (size_t)iy * ngx;
+  v[0] = (double)phi[xh + Y0 + Z0] - (double)phi[xl + Y0 + Z0];
+  v[1] = (double)phi[ix + (size_t)yh * ngx + Z0] - (double)phi[ix + (size_t)yl * ngx + Z0];
+  v[2] = (double)phi[ix + Y0 + (size_t)zh * plane] - (double)phi[ix + Y0 + (size_t)zl * plane];
+}
+
+// get_element, RETURN_VEL | RETURN_TID: velocity and the 19-point tidal
+// tensor (xx, xy, xz, yy, yz, zz; mixed terms with 1/4).
+__device__ inline void vel_tid_at(const float *__restrict__ phi, int N, size_t ngx, size_t plane, int ix, int iy,
+                                  int iz, double v[3], double t[6]) {
+#pragma clang fp contract(off)
+  const int xh = ix == N - 1 ? 0 : ix + 1, xl = ix == 0 ? N - 1 : ix - 1;
+  const int yh = iy == N - 1 ? 0 : iy + 1, yl = iy == 0 ? N - 1 : iy - 1;
+  const int zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
+  const size_t Z0 = (size_t)iz * plane, ZH = (size_t)zh * plane, ZL = (size_t)zl * plane;
+  const size_t Y0 = (size_t)iy * ngx, YH = (size_t)yh * ngx, YL = (size_t)yl * ngx;
+  const double c0 = phi[ix + Y0 + Z0];
+  const double pxh = phi[xh + Y0 + Z0], pxl = phi[xl + Y0 + Z0];
+  const double pyh = phi[ix + YH + Z0], pyl = phi[ix + YL + Z0];
+  const double pzh = phi[ix + Y0 + ZH], pzl = phi[ix + Y0 + ZL];
+  v[0] = pxh - pxl;
+  v[1] = pyh - pyl;
+  v[2] = pzh - pzl;
+  t[0] = pxh + pxl - 2 * c0;
+  t[3] = pyh + pyl - 2 * c0;
+  t[5] = pzh + pzl - 2 * c0;
+  t[1] = 0.25 * ((double)phi[xh + YH + Z0] + (double)phi[xl + YL + Z0] - (double)phi[xh + YL + Z0] -
+                 (double)phi[xl + YH + Z0]);
+  t[2] = 0.25 * ((double)phi[xh + Y0 + ZH] + (double)phi[xl + Y0 + ZL] - (double)phi[xh + Y0 + ZL] -
+                 (double)phi[xl + Y0 + ZH]);
+  t[4] = 0.25 * ((double)phi[ix + YH + ZH] + (double)phi[ix + YL + ZL] - (double)phi[ix + YH + ZL] -
+                 (double)phi[ix + YL + ZH]);
+}
+
+template <bool LENS>
+__global__ __launch_bounds__(256) void k_src_beams(BeamArgs a, const float4 *__restrict__ pos, long long n,
+                                                   float *__restrict__ dz, float *__restrict__ lens) {
+#pragma clang fp contract(off)
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pos[i];
+  const int N = a.N;
+  const size_t ngx = (size_t)a.ngx, plane = (size_t)N * ngx;
+  const float *__restrict__ phi = a.phi;
+  const double px = p.x, py = p.y, pz = p.z;
+  const double r = sqrt(px * px + py * py + pz * pz);
+  const double ir = 1. / (r > 0.001 ? r : 0.001);
+  const double u[3] = {px * ir, py * ir, pz * ir};
+  const double xn[3] = {(px + a.ox) * a.idx, (py + a.oy) * a.idx, (pz + a.oz) * a.idx};
+
+  // CIC velocity at the source (interpolate_from_grid with INTERP_TYPE_SKW,
+  // beaming.c:193-266)
+  int i0[3], i1[3];
+  double h0[3], h1[3];
+  for (int ax = 0; ax < 3; ax++) {
+    const int c = (int)xn[ax];   // (long)x: truncation toward zero
+    h0[ax] = xn[ax] - c;
+    h1[ax] = 1 - h0[ax];
+    i0[ax] = wrap1(c, N);
+    i1[ax] = wrap1(c + 1, N);
+  }
+  double v[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+    const double w = (bz ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
+    double vc[3];
+    vel_at(phi, N, ngx, plane, bx ? i1[0] : i0[0], by ? i1[1] : i0[1], bz ? i1[2] : i0[2], vc);
+    v[0] += vc[0] * w;
+    v[1] += vc[1] * w;
+    v[2] += vc[2] * w;
+  }
+  const float vr = (float)(0.5 * a.idx * (v[0] * u[0] + v[1] * u[1] + v[2] * u[2]));
+  // postproc: dz_rsd *= V1(r_of_z(z0)) factor_vel, z0 the catalogue's float z(r)
+  const float z0 = (float)f_of_r(a.r_arr, a.z_arr, a.glob_idr, a.r_last, r, 0.0, a.z_last);
+  double rz;
+  {
+    const double aa = 1. / (1 + (double)z0);
+    if (aa >= 1) rz = 0;
+    else if (aa <= 0) rz = a.a2r[0];
+    else {
+      const int ia = (int)(aa * (a.na - 1));
+      rz = a.a2r[ia] + (a.a2r[ia + 1] - a.a2r[ia]) * (aa - (double)ia / (a.na - 1)) * (a.na - 1.);
+    }
+  }
+  const double vg = f_of_r(a.r_arr, a.v1_arr, a.glob_idr, a.r_last, rz, a.v1_0, a.v1_last);
+  dz[i] = (float)((double)vr * (vg * a.factor_vel));
+  if (!LENS) return;
+
+  double cth = u[2];
+  if (cth >= 1) cth = 1;
+  if (cth <= -1) cth = -1;
+  const double sth = sqrt((1 - cth) * (1 + cth));
+  double cph = 1, sph = 0;
+  if (sth != 0) { cph = u[0] / sth; sph = u[1] / sth; }
+  const double pre = a.idx * a.idx * ir, prem = 0.5 * a.idx * ir;
+  const double ux[3] = {cth * cph * prem, cth * sph * prem, -sth * prem};
+  const double uy[2] = {-sph * prem, cph * prem};
+  const double rk[6] = {(cth * cth * cph * cph + sph * sph) * pre, (2 * cph * sph * (cth * cth - 1)) * pre,
+                        (-2 * cth * sth * cph) * pre,                (cth * cth * sph * sph + cph * cph) * pre,
+                        (-2 * cth * sth * sph) * pre,                (sth * sth) * pre};
+  const double re1[6] = {(cth * cth * cph * cph - sph * sph) * pre, (2 * cph * sph * (cth * cth + 1)) * pre,
+                         (-2 * cth * sth * cph) * pre,                (cth * cth * sph * sph - cph * cph) * pre,
+                         (-2 * cth * sth * sph) * pre,                (sth * sth) * pre};
+  const double re2[5] = {(-2 * cth * cph * sph) * pre, (2 * cth * (cph * cph - sph * sph)) * pre,
+                         (2 * sth * sph) * pre,        (2 * cth * sph * cph) * pre,
+                         (-2 * sth * cph) * pre};
+  const double *__restrict__ f0 = a.fac, *__restrict__ f1 = a.fac + a.nr, *__restrict__ f2 = a.fac + 2 * a.nr;
+  double dtx = 0, dty = 0, kp = 0, e1 = 0, e2 = 0;
+  const int irmax = min((int)(r * a.idr + 0.5), a.nr - 1);
+  for (int i_r = 0; i_r <= irmax; i_r++) {
+    const double rm = (i_r + 0.5) * a.dr;
+    int ix[3];
+    ix[0] = wrap1((int)((rm * u[0] + a.ox) * a.idx + 0.5), N);
+    ix[1] = wrap1((int)((rm * u[1] + a.oy) * a.idx + 0.5), N);
+    ix[2] = wrap1((int)((rm * u[2] + a.oz) * a.idx + 0.5), N);
+    double vp[3], tp[6];
+    vel_tid_at(phi, N, ngx, plane, ix[0], ix[1], ix[2], vp, tp);
+    const double fr = f1[i_r] * r - f2[i_r];
+    const double frm = 2 * (f0[i_r] * r - f1[i_r]);
+    double de1 = 0, de2 = 0, dk = 0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      de1 += re1[q] * tp[q];
+      de2 += (q < 5 ? re2[q] : 0.0) * tp[q];
+      dk += rk[q] * tp[q];
+    }
+    const double dvx = ux[0] * vp[0] + ux[1] * vp[1] + ux[2] * vp[2];
+    const double dvy = uy[0] * vp[0] + uy[1] * vp[1] + 0.0 * vp[2];
+    e1 += de1 * fr;
+    e2 += de2 * fr;
+    kp += dk * fr;
+    dtx += dvx * frm;
+    dty += dvy * frm;
+  }
+  const double RTOD = 57.2957795;
+  float *o = lens + 5 * i;
+  o[0] = (float)e1;
+  o[1] = (float)e2;
+  o[2] = (float)kp;
+  o[3] = (float)(dty * RTOD);
+  o[4] = (float)(dtx * RTOD);
+}
+
+}  // namespace
+
+int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz, float *lens, long long n) {
+  if (ipop < 0 || ipop >= c->n_tr[CLR_TRACER_SRCS]) return clr_fail(c, CLR_ERR_ARG, "no source population %d", ipop);
+  if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
+  const CatalogDev &C = c->cat[ipop];
+  if (n != C.n) return clr_fail(c, CLR_ERR_ARG, "population %d holds %lld sources, not %lld", ipop, C.n, n);
+  if (!r_arr_a2r || (n > 0 && !dz)) return clr_fail(c, CLR_ERR_ARG, "null r(a) table or output");
+  const int N = c->N, na = c->na;
+  const double r_max = c->r_max;
+  if (!(r_max > 0)) return clr_fail(c, CLR_ERR_STATE, "tables carry no r_max");
+  if (!(c->hubble_0 > 0) || !(c->omega_m > 0)) return clr_fail(c, CLR_ERR_STATE, "tables carry no H0 or Omega_M");
+  if (n == 0) return CLR_OK;
+  const int nr = N / 2;
+  const double dr = r_max / nr;
+  // fac_r_0/1/2 (srcs.c:470-480), FP64 on the host from the uploaded tables
+  std::vector<double> rr(na), zz(na), dd(na);
+  CLR_HIP(c, hipMemcpy(rr.data(), c->bg, na * sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(zz.data(), c->bg + na, na * sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(dd.data(), c->bg + 2 * (size_t)na, na * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> fac(3 * (size_t)nr);
+  for (int i = 0; i < nr; i++) {
+    const double rm = (i + 0.5) * dr;
+    const double pg = f_of_r(rr.data(), dd.data(), c->glob_idr, c->r_last, rm, 1.0, dd[na - 1]) *
+                      (1 + f_of_r(rr.data(), zz.data(), c->glob_idr, c->r_last, rm, 0.0, zz[na - 1]));
+    fac[i] = pg * dr;
+    fac[nr + i] = rm * pg * dr;
+    fac[2 * nr + i] = rm * rm * pg * dr;
+  }
+  const size_t tab_bytes = (3 * (size_t)nr + na) * sizeof(double);
+  int rc = clr_ensure(c, &c->misc, &c->misc_n, tab_bytes);
+  if (rc) return rc;
+  double *dfac = (double *)c->misc, *da2r = dfac + 3 * nr;
+  CLR_HIP(c, hipMemcpyAsync(dfac, fac.data(), 3 * (size_t)nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(da2r, r_arr_a2r, na * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  BeamArgs a{};
+  {
+    const int prc = clr_phi_ready(c, 0);
+    if (prc) return prc;
+  }
+  a.phi = c->npot;
+  a.N = N;
+  a.ngx = c->ngx;
+  a.nr = nr;
+  a.na = na;
+  a.idx = N / c->L;
+  a.dr = dr;
+  a.idr = 1. / dr;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.factor_vel = -c->fgrowth_0 / (1.5 * c->hubble_0 * c->omega_m);
+  a.glob_idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.z_last = c->h_bg_last[1];
+  a.v1_0 = c->h_bg_first[3];
+  a.v1_last = c->h_bg_last[3];
+  a.r_arr = c->bg;
+  a.z_arr = c->bg + na;
+  a.v1_arr = c->bg + 3 * (size_t)na;
+  a.a2r = da2r;
+  a.fac = dfac;
+  const int per = lens ? 6 : 1;
+  float *tmp = nullptr;
+  CLR_HIP(c, hipMalloc((void **)&tmp, (size_t)n * per * sizeof(float)));
+  CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (lens)
+    hipLaunchKernelGGL(k_src_beams<true>, grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp, tmp + n);
+  else
+    hipLaunchKernelGGL(k_src_beams<false>, grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp,
+                       (float *)nullptr);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dz, tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && lens)
+    e = hipMemcpyAsync(lens, tmp + n, (size_t)n * 5 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "source beams: %s", hipGetErrorString(e));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->stage_ms[ST_BEAMS] = ms;
+  return CLR_OK;
+}

@@ -369,6 +369,8 @@ double clr_run_get_bg(const clr_run *r, double rr, int tag, int ipop) {
   }
 }
 
+const double *clr_run_a2r(const clr_run *r) { return r->r_arr_a2r; }
+
 double clr_run_r_of_z(const clr_run *r, double z) {
   const double a = 1. / (1 + z);
   if (a >= 1) return 0;

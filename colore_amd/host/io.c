@@ -164,34 +164,45 @@ int clr_write_healpix_map(const char *fname, long long nside, const float *map) 
   return ok ? 0 : 1;
 }
 
-int clr_write_catalog_fits(const char *fname, int ipop, long long n, const float *rdz) {
+int clr_write_catalog_fits(const char *fname, int ipop, long long n, const float *rdz, const float *lens) {
   FILE *f = fopen(fname, "wb");
   if (!f) return 1;
+  const int nc = lens ? 9 : 4;
   int32_t *ty = malloc((n > 0 ? n : 1) * sizeof(int32_t));
-  float *c[4];
-  for (int k = 0; k < 4; k++) c[k] = malloc((n > 0 ? n : 1) * sizeof(float));
+  float *c[9];
+  for (int k = 0; k < nc; k++) c[k] = malloc((n > 0 ? n : 1) * sizeof(float));
   for (long long i = 0; i < n; i++) {
     ty[i] = ipop;
     for (int k = 0; k < 4; k++) c[k][i] = rdz[4 * i + k];
+    for (int k = 4; k < nc; k++) c[k][i] = lens[5 * i + k - 4];
   }
   hdr ex = {0};
   kv_str(&ex, "CONTENTS", "Source catalog", NULL);
-  const char *tt[] = {"TYPE", "RA", "DEC", "Z_COSMO", "DZ_RSD"}, *tu[] = {"NA", "DEG", "DEG", "NA", "NA"};
-  const void *cols[] = {ty, c[0], c[1], c[2], c[3]};
-  int ok = write_primary(f) && write_table(f, 5, tt, "JEEEE", tu, NULL, n, cols, &ex);
+  const char *tt[] = {"TYPE", "RA", "DEC", "Z_COSMO", "DZ_RSD", "E1", "E2", "KAPPA", "DRA", "DDEC"};
+  const char *tu[] = {"NA", "DEG", "DEG", "NA", "NA", "NA", "NA", "NA", "DEG", "DEG"};
+  const void *cols[] = {ty, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8]};
+  int ok = write_primary(f) && write_table(f, nc + 1, tt, "JEEEEEEEEE", tu, NULL, n, cols, &ex);
   free(ex.buf);
   free(ty);
-  for (int k = 0; k < 4; k++) free(c[k]);
+  for (int k = 0; k < nc; k++) free(c[k]);
   fclose(f);
   return ok ? 0 : 1;
 }
 
-int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const float *rdz) {
+int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const float *rdz, const float *lens) {
   FILE *f = fopen(fname, "w");
   if (!f) return 1;
-  fprintf(f, "#[1]type [2]RA, [3]dec, [4]z0, [5]dz_RSD \n");
-  for (long long i = 0; i < n; i++)
-    fprintf(f, "%d %E %E %E %E \n", ipop, rdz[4 * i], rdz[4 * i + 1], rdz[4 * i + 2], rdz[4 * i + 3]);
+  fprintf(f, "#[1]type [2]RA, [3]dec, [4]z0, [5]dz_RSD ");
+  fprintf(f, lens ? "#[6]e1, [7]e2 [8]kappa [9]d_ra [10]d_dec\n" : "\n");
+  for (long long i = 0; i < n; i++) {
+    fprintf(f, "%d %E %E %E %E ", ipop, rdz[4 * i], rdz[4 * i + 1], rdz[4 * i + 2], rdz[4 * i + 3]);
+    if (lens) {
+      const float *l = lens + 5 * i;
+      fprintf(f, "%E %E %E %E %E \n", l[0], l[1], l[2], l[3], l[4]);
+    } else {
+      fprintf(f, "\n");
+    }
+  }
   fclose(f);
   return 0;
 }

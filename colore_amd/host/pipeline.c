@@ -124,7 +124,10 @@ int clr_run_execute(clr_run *r, int device) {
 
   /* srcs_set_cartesian + srcs_get_local_properties */
   long long nsrc[CLR_NPOP_MAX] = {0};
-  float *props[CLR_NPOP_MAX] = {0};
+  float *props[CLR_NPOP_MAX] = {0}, *lens[CLR_NPOP_MAX] = {0};
+  /* need_beaming (io.c:460-461; fast lensing is not part of this build) */
+  int need_beaming = r->do_kappa || r->do_isw || r->n_tr[2] > 0;
+  for (int ip = 0; ip < r->n_tr[0]; ip++) need_beaming |= r->lensing_srcs[ip] || r->skw_srcs[ip];
   if (r->n_tr[0] > 0) {
     printf("*** Getting point sources\n");
     for (int ip = 0; ip < r->n_tr[0]; ip++) {
@@ -136,8 +139,18 @@ int clr_run_execute(clr_run *r, int device) {
       stage_report(ctx, 5);
       props[ip] = malloc((nsrc[ip] > 0 ? nsrc[ip] : 1) * 4 * sizeof(float));
       CHK(clr_source_properties(ctx, ip, props[ip], nsrc[ip]));
-      if (r->lensing_srcs[ip] || r->skw_srcs[ip])
-        fprintf(stderr, "MPI task 0, Warning: lensing/skewers at sources are not produced by this build (population %d)\n", ip + 1);
+      if (need_beaming) {
+        /* srcs_beams_preproc/get_beam_properties/postproc (srcs.c:425-737):
+         * the CIC velocity replaces dz_rsd; lensing when asked for */
+        printf("   Line-of-sight properties\n");
+        float *dz = malloc((nsrc[ip] > 0 ? nsrc[ip] : 1) * sizeof(float));
+        if (r->lensing_srcs[ip]) lens[ip] = malloc((nsrc[ip] > 0 ? nsrc[ip] : 1) * 5 * sizeof(float));
+        CHK(clr_source_beams(ctx, ip, clr_run_a2r(r), dz, lens[ip], nsrc[ip]));
+        for (long long i = 0; i < nsrc[ip]; i++) props[ip][4 * i + 3] = dz[i];
+        free(dz);
+      }
+      if (r->skw_srcs[ip])
+        fprintf(stderr, "MPI task 0, Warning: skewers at sources are not produced by this build (population %d)\n", ip + 1);
     }
     printf("\n");
   }
@@ -201,11 +214,11 @@ int clr_run_execute(clr_run *r, int device) {
       if (r->output_format == 1) {
         printf(" %d-th population (FITS)\n", ip);
         snprintf(fname, sizeof(fname), "%s_srcs_s%d_0.fits", r->prefix_out, ip + 1);
-        if (clr_write_catalog_fits(fname, ip, nsrc[ip], props[ip])) fprintf(stderr, "MPI task 0, Warning: could not write %s\n", fname);
+        if (clr_write_catalog_fits(fname, ip, nsrc[ip], props[ip], lens[ip])) fprintf(stderr, "MPI task 0, Warning: could not write %s\n", fname);
       } else {
         printf(" %d-th population (ASCII)\n", ip);
         snprintf(fname, sizeof(fname), "%s_srcs_s%d_0.txt", r->prefix_out, ip + 1);
-        if (clr_write_catalog_ascii(fname, ip, nsrc[ip], props[ip])) fprintf(stderr, "MPI task 0, Warning: could not write %s\n", fname);
+        if (clr_write_catalog_ascii(fname, ip, nsrc[ip], props[ip], lens[ip])) fprintf(stderr, "MPI task 0, Warning: could not write %s\n", fname);
       }
       clr_timer(2);
     }
@@ -231,7 +244,7 @@ int clr_run_execute(clr_run *r, int device) {
     clr_timer(2);
     printf("\n");
   }
-  for (int i = 0; i < CLR_NPOP_MAX; i++) { free(props[i]); free(imd[i]); free(imn[i]); }
+  for (int i = 0; i < CLR_NPOP_MAX; i++) { free(props[i]); free(lens[i]); free(imd[i]); free(imn[i]); }
   free(kmap);
   free(imap_isw);
   clr_destroy(ctx);

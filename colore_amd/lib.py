@@ -20,7 +20,7 @@ CLR_BIAS_EXPONENTIAL, CLR_BIAS_EXP_TRUNCATED, CLR_BIAS_TRUNCATED = 0, 1, 2
 CLR_DENS_LOGNORMAL, CLR_DENS_CLIP = 0, 3
 CLR_RNG_PHILOX, CLR_RNG_INJECTED = 0, 1
 STAGES = ["gen", "fft_yz", "fft_x", "density", "norm", "srcs", "kappa", "isw", "imap", "src_pass", "src_write",
-          "fft_phi", "shells"]
+          "fft_phi", "shells", "beams"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
@@ -80,6 +80,7 @@ def load_amd():
     _sig(lib, "clr_source_properties", ci, [_vp, ci, _f, ctypes.c_longlong])
     _sig(lib, "clr_kappa", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
+    _sig(lib, "clr_source_beams", ci, [_vp, ci, _d, _f, _f, ctypes.c_longlong])
     _sig(lib, "clr_imap", ci, [_vp, ci, ci, ci, _d, _d, _f, _i32])
     _sig(lib, "clr_fetch_grid", ci, [_vp, ci, _f])
     _sig(lib, "clr_upload_grid", ci, [_vp, ci, _f])
@@ -108,6 +109,7 @@ def load_host():
                                           ctypes.POINTER(_d)])
     _sig(lib, "clr_run_get_bg", cd, [_vp, cd, ci, ci])
     _sig(lib, "clr_run_r_of_z", cd, [_vp, cd])
+    _sig(lib, "clr_run_a2r", _d, [_vp])
     _sig(lib, "clr_hp_ang2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, cd, cd])
     _sig(lib, "clr_hp_vec2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, _d])
     _sig(lib, "clr_hp_ring2nest", ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong])
@@ -115,8 +117,8 @@ def load_host():
     _sig(lib, "clr_hp_pix2vec_nest", None, [ctypes.c_longlong, ctypes.c_longlong, _d])
     _sig(lib, "clr_write_density_grid", ci, [ctypes.c_char_p, ci, cd, _f])
     _sig(lib, "clr_write_healpix_map", ci, [ctypes.c_char_p, ctypes.c_longlong, _f])
-    _sig(lib, "clr_write_catalog_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f])
-    _sig(lib, "clr_write_catalog_ascii", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f])
+    _sig(lib, "clr_write_catalog_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
+    _sig(lib, "clr_write_catalog_ascii", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
     return lib
 
 
@@ -157,6 +159,10 @@ class Run:
                     logkmax=t.logkmax, idlogk=t.idlogk, ns=t.n_scal, fgrowth_0=t.fgrowth_0,
                     hubble_0=t.hubble_0, omega_m=t.omega_m, prefac_lensing=t.prefac_lensing,
                     r_max=t.r_max)
+
+    def r_a2r(self):
+        """r(a) at a = i/(NA-1): the table r_of_z interpolates (cosmo.c:101-112)."""
+        return np.ctypeslib.as_array(self.host.clr_run_a2r(self.h), shape=(CLR_NA,)).copy()
 
     def imap_shells(self, ipop):
         """(nside, r0, rf) of intensity species ipop."""
@@ -278,6 +284,15 @@ class Context:
         out = np.zeros((n, 4), dtype=np.float32)
         self._chk(self.lib.clr_source_properties(self.h, ipop, ptr(out, ctypes.c_float), n))
         return out
+
+    def source_beams(self, ipop, n, r_a2r, lensing=True):
+        """(dz_rsd[n], lens[n, 5] or None): the sources' line-of-sight pass."""
+        a2r = np.ascontiguousarray(r_a2r, dtype=np.float64)
+        dz = np.zeros(max(n, 1), dtype=np.float32)
+        lens = np.zeros((max(n, 1), 5), dtype=np.float32) if lensing else None
+        self._chk(self.lib.clr_source_beams(self.h, ipop, ptr(a2r), ptr(dz, ctypes.c_float),
+                                            ptr(lens, ctypes.c_float) if lensing else None, n))
+        return dz[:n], (lens[:n] if lensing else None)
 
     def kappa(self, nside, r_src):
         r = np.ascontiguousarray(r_src, dtype=np.float64)

@@ -171,6 +171,24 @@ int clr_kappa(clr_ctx *ctx, int nside, int nplanes, const double *r_src,
 /* isw_get_beam_properties (isw.c:78-147); same conventions as clr_kappa. */
 int clr_isw(clr_ctx *ctx, int nside, int nplanes, const double *r_src,
             float *out);
+/* srcs_beams_preproc_single, srcs_get_beam_properties_single and
+ * srcs_beams_postproc_single (srcs.c:425-737, the build without
+ * _USE_FAST_LENSING) for source population ipop after clr_sample_sources.
+ * The reference runs them whenever it marches lines of sight (need_beaming,
+ * io.c:460-461: lensing at sources, kappa, ISW, skewers or custom maps), and
+ * they replace the catalogue's dz_rsd.  Per source, in catalogue order:
+ *   dz_rsd[i]     0.5 (N/L) (v.u) V1(r_of_z(z0)) (-f0 / (1.5 H0 Om)), v the
+ *                 trilinear (CIC) interpolation of the central differences of
+ *                 phi at the source (srcs.c:496-504, 655-662);
+ *   lens[5i..5i+4] (e1, e2, kappa, dra, ddec) of the nearest-cell march of
+ *                 N/2 steps of r_max/(N/2) up to the source (srcs.c:533-614,
+ *                 dra and ddec in degrees, srcs.c:718-719), when lens is not
+ *                 NULL (include_lensing).  kappa, dra and ddec start from 0
+ *                 (the reference adds them onto unset memory, common.c:373).
+ * r_arr_a2r: CLR_NA values of r at a = i/(CLR_NA-1), the table of r_of_z
+ * (cosmo.c:101-112).  n must be the population's source count. */
+int clr_source_beams(clr_ctx *ctx, int ipop, const double *r_arr_a2r,
+                     float *dz_rsd, float *lens, long long n);
 /* imap_set_cartesian_single (imap.c:135-245) for intensity species ipop.
  * r0/rf: the nr shell edges (any order, sorted as imap_preproc does).
  * Outputs full-sky RING maps data[nr][12 nside^2] (sum of T) and
@@ -194,8 +212,8 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
  * the counting pass of the sources call, its catalogue writers (descriptors
  * and per-source positions), the y and x passes of phi (these run on a second
  * stream, overlapping the density's passes and the stages after them; every
- * call that reads phi waits for them), the shell-sum kernel alone.  Fills at
- * most n values; returns the number of stages. */
+ * call that reads phi waits for them), the shell-sum kernel alone, the
+ * source-beam kernel.  Fills at most n values; returns the number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 
 #ifdef __cplusplus

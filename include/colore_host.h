@@ -66,6 +66,8 @@ const char *clr_run_prefix(const clr_run *r);
 /* get_bg / r_of_z on the run's tables (cosmo.c:40-112). */
 double   clr_run_get_bg(const clr_run *r, double rr, int tag, int ipop);
 double   clr_run_r_of_z(const clr_run *r, double z);
+/* r_arr_a2r: CLR_NA values of r at a = i/(CLR_NA-1) (cosmo.c:669-675). */
+const double *clr_run_a2r(const clr_run *r);
 #define CLR_BG_Z  0
 #define CLR_BG_D1 1
 #define CLR_BG_V1 2
@@ -90,10 +92,13 @@ void      clr_hp_pix2vec_nest(long long nside, long long pix, double *v);
 int clr_write_density_grid(const char *fname, int N, double L, const float *padded);
 /* he_write_healpix_map (healpix_extra.c:4-57): one 1E column, RING */
 int clr_write_healpix_map(const char *fname, long long nside, const float *map_ring);
-/* write_catalog FITS / ASCII (io.c:1019-1236), 5 columns TYPE RA DEC
- * Z_COSMO DZ_RSD; radecz = float[4n] (RA, Dec, z0, dz_rsd) */
-int clr_write_catalog_fits(const char *fname, int ipop, long long n, const float *radecz);
-int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const float *radecz);
+/* write_catalog FITS / ASCII (io.c:1019-1236), columns TYPE RA DEC
+ * Z_COSMO DZ_RSD from radecz = float[4n] (RA, Dec, z0, dz_rsd), and, when
+ * lens is not NULL (has_lensing), E1 E2 KAPPA DRA DDEC from lens = float[5n]. */
+int clr_write_catalog_fits(const char *fname, int ipop, long long n, const float *radecz,
+                           const float *lens);
+int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const float *radecz,
+                            const float *lens);
 
 #ifdef __cplusplus
 }

@@ -762,6 +762,166 @@ void orc_kappa_isw(int which, int N, double L, const double obs[3], double r_max
   free(ord); free(irmax); free(irmin); free(inv); free(f1); free(f2);
 }
 
+/* get_element (beaming.c:31-117) on one rank: the unscaled central
+ * differences of phi (v) and its 19-point second differences (t: xx, xy, xz,
+ * yy, yz, zz), periodic in x, y and z (fourier.c:412-413 sets the z halo
+ * slices to the periodic neighbours). */
+static void grid_element(const double *phi, int N, long ix, long iy, long iz, double v[3], double t[6]) {
+  const long ngx = 2 * (N / 2 + 1);
+  const long xh = ix == N - 1 ? 0 : ix + 1, xl = ix == 0 ? N - 1 : ix - 1;
+  const long yh = iy == N - 1 ? 0 : iy + 1, yl = iy == 0 ? N - 1 : iy - 1;
+  const long zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
+  const long Z0 = iz * ngx * N, ZH = zh * ngx * N, ZL = zl * ngx * N;
+  const long Y0 = iy * ngx, YH = yh * ngx, YL = yl * ngx;
+  if (v) {
+    v[0] = phi[xh + Y0 + Z0] - phi[xl + Y0 + Z0];
+    v[1] = phi[ix + YH + Z0] - phi[ix + YL + Z0];
+    v[2] = phi[ix + Y0 + ZH] - phi[ix + Y0 + ZL];
+  }
+  if (t) {
+    const double c0 = phi[ix + Y0 + Z0];
+    t[0] = phi[xh + Y0 + Z0] + phi[xl + Y0 + Z0] - 2 * c0;
+    t[3] = phi[ix + YH + Z0] + phi[ix + YL + Z0] - 2 * c0;
+    t[5] = phi[ix + Y0 + ZH] + phi[ix + Y0 + ZL] - 2 * c0;
+    t[1] = 0.25 * (phi[xh + YH + Z0] + phi[xl + YL + Z0] - phi[xh + YL + Z0] - phi[xl + YH + Z0]);
+    t[2] = 0.25 * (phi[xh + Y0 + ZH] + phi[xl + Y0 + ZL] - phi[xh + Y0 + ZL] - phi[xl + Y0 + ZH]);
+    t[4] = 0.25 * (phi[ix + YH + ZH] + phi[ix + YL + ZL] - phi[ix + YH + ZL] - phi[ix + YL + ZH]);
+  }
+}
+
+static long wrap_cell(long i, int N) { return i >= N ? i - N : (i < 0 ? i + N : i); }
+
+/* r_of_z (cosmo.c:101-112): r(a) tabulated at a = i/(NA-1). */
+static double r_of_z_tab(int na, const double *r_a2r, double z) {
+  const double a = 1. / (1 + z);
+  if (a >= 1) return 0;
+  if (a <= 0) return r_a2r[0];
+  const int ia = (int)(a * (na - 1));
+  return r_a2r[ia] + (r_a2r[ia + 1] - r_a2r[ia]) * (a - (double)ia / (na - 1)) * (na - 1.);
+}
+
+/* srcs_beams_preproc_single (srcs.c:425-445), srcs_get_beam_properties_single
+ * (srcs.c:452-625, the build without _USE_FAST_LENSING) and
+ * srcs_beams_postproc_single (srcs.c:634-737) on one rank.  pos4[4i..4i+2]
+ * are the positions relative to the observer (CatalogCartesian).  dz[i] is
+ * the CIC radial velocity at the source (interpolate_from_grid with
+ * INTERP_TYPE_SKW = CIC, beaming.c:193-268) scaled by V1(r_of_z(z0)) times
+ * -f0/(1.5 H0 Om); with lens != NULL, lens[5i..5i+4] = (e1, e2, kappa, dra,
+ * ddec) of the NGP march of nr = N/2 steps of r_max/nr up to the source
+ * (kappa, dra and ddec start at 0, dra/ddec in degrees). */
+void orc_src_beams(int N, double L, const double obs[3], double r_max, const orc_bg *bg,
+                   const double *r_a2r, double fgrowth_0, double hubble_0, double omega_m,
+                   const double *npot, long long n, const float *pos4, double *dz, double *lens) {
+  const double idx = N / L;
+  const int nr = N / 2, na = bg->na;
+  const double dr = r_max / nr, idr = 1. / dr;
+  const double factor_vel = -fgrowth_0 / (1.5 * hubble_0 * omega_m);
+  const double RTOD = 57.2957795;
+  double *f0 = malloc(3 * (size_t)nr * sizeof(double)), *f1 = f0 + nr, *f2 = f0 + 2 * nr;
+  for (int i = 0; i < nr; i++) {
+    const double rm = (i + 0.5) * dr;
+    const double pg = orc_f_of_r(bg, bg->d1, 1, bg->d1[na - 1], rm) * (1 + orc_f_of_r(bg, bg->z, 0, bg->z[na - 1], rm));
+    f0[i] = pg * dr;
+    f1[i] = rm * pg * dr;
+    f2[i] = rm * rm * pg * dr;
+  }
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long ip = 0; ip < n; ip++) {
+    const float *pos = pos4 + 4 * ip;
+    const double r = sqrt((double)pos[0] * pos[0] + (double)pos[1] * pos[1] + (double)pos[2] * pos[2]);
+    const double ir = 1. / (r > 0.001 ? r : 0.001);
+    double xn[3], u[3];
+    for (int ax = 0; ax < 3; ax++) {
+      xn[ax] = (pos[ax] + obs[ax]) * idx;
+      u[ax] = pos[ax] * ir;
+    }
+    /* trilinear (CIC) velocity at the source, beaming.c:193-266 */
+    long i0[3], i1[3];
+    double h0[3], h1[3];
+    for (int ax = 0; ax < 3; ax++) {
+      i0[ax] = (long)xn[ax];
+      h0[ax] = xn[ax] - i0[ax];
+      h1[ax] = 1 - h0[ax];
+      i1[ax] = wrap_cell(i0[ax] + 1, N);
+      i0[ax] = wrap_cell(i0[ax], N);
+    }
+    double v[3] = {0, 0, 0};
+    for (int c = 0; c < 8; c++) {
+      const int bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+      const double w = (bz ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
+      double vc[3];
+      grid_element(npot, N, bx ? i1[0] : i0[0], by ? i1[1] : i0[1], bz ? i1[2] : i0[2], vc, NULL);
+      for (int ax = 0; ax < 3; ax++) v[ax] += vc[ax] * w;
+    }
+    const double vr = 0.5 * idx * (v[0] * u[0] + v[1] * u[1] + v[2] * u[2]);
+    /* postproc: z0 is the catalogue's float z(r) (srcs.c:409) */
+    const float z0 = (float)orc_f_of_r(bg, bg->z, 0, bg->z[na - 1], r);
+    const double vg = orc_f_of_r(bg, bg->v1, bg->v1[0], bg->v1[na - 1], r_of_z_tab(na, r_a2r, z0));
+    dz[ip] = (double)(float)vr * (vg * factor_vel);
+    if (!lens) continue;
+    double cth_h = u[2], sth_h, cph_h = 1, sph_h = 0;
+    if (cth_h >= 1) cth_h = 1;
+    if (cth_h <= -1) cth_h = -1;
+    sth_h = sqrt((1 - cth_h) * (1 + cth_h));
+    if (sth_h != 0) { cph_h = u[0] / sth_h; sph_h = u[1] / sth_h; }
+    const double pre = idx * idx * ir, prem = 0.5 * idx * ir;
+    const double ux[3] = {cth_h * cph_h * prem, cth_h * sph_h * prem, -sth_h * prem};
+    const double uy[3] = {-sph_h * prem, cph_h * prem, 0};
+    const double rk[6] = {(cth_h * cth_h * cph_h * cph_h + sph_h * sph_h) * pre,
+                          (2 * cph_h * sph_h * (cth_h * cth_h - 1)) * pre,
+                          (-2 * cth_h * sth_h * cph_h) * pre,
+                          (cth_h * cth_h * sph_h * sph_h + cph_h * cph_h) * pre,
+                          (-2 * cth_h * sth_h * sph_h) * pre,
+                          (sth_h * sth_h) * pre};
+    const double re1[6] = {(cth_h * cth_h * cph_h * cph_h - sph_h * sph_h) * pre,
+                           (2 * cph_h * sph_h * (cth_h * cth_h + 1)) * pre,
+                           (-2 * cth_h * sth_h * cph_h) * pre,
+                           (cth_h * cth_h * sph_h * sph_h - cph_h * cph_h) * pre,
+                           (-2 * cth_h * sth_h * sph_h) * pre,
+                           (sth_h * sth_h) * pre};
+    const double re2[6] = {(-2 * cth_h * cph_h * sph_h) * pre,
+                           (2 * cth_h * (cph_h * cph_h - sph_h * sph_h)) * pre,
+                           (2 * sth_h * sph_h) * pre,
+                           (2 * cth_h * sph_h * cph_h) * pre,
+                           (-2 * sth_h * cph_h) * pre,
+                           0};
+    double dtx = 0, dty = 0, kp = 0, e1 = 0, e2 = 0;
+    int irmax = (int)(r * idr + 0.5);
+    if (irmax > nr - 1) irmax = nr - 1;
+    for (int i_r = 0; i_r <= irmax; i_r++) {
+      const double rm = (i_r + 0.5) * dr;
+      long ix[3];
+      for (int ax = 0; ax < 3; ax++) ix[ax] = wrap_cell((long)((rm * u[ax] + obs[ax]) * idx + 0.5), N);
+      double vp[3], tp[6];
+      grid_element(npot, N, ix[0], ix[1], ix[2], vp, tp);
+      const double fr = f1[i_r] * r - f2[i_r];
+      const double frm = 2 * (f0[i_r] * r - f1[i_r]);
+      double de1 = 0, de2 = 0, dk = 0, dvx = 0, dvy = 0;
+      for (int q = 0; q < 6; q++) {
+        de1 += re1[q] * tp[q];
+        de2 += re2[q] * tp[q];
+        dk += rk[q] * tp[q];
+      }
+      for (int ax = 0; ax < 3; ax++) {
+        dvx += ux[ax] * vp[ax];
+        dvy += uy[ax] * vp[ax];
+      }
+      e1 += de1 * fr;
+      e2 += de2 * fr;
+      kp += dk * fr;
+      dtx += dvx * frm;
+      dty += dvy * frm;
+    }
+    double *o = lens + 5 * ip;
+    o[0] = e1;
+    o[1] = e2;
+    o[2] = kp;
+    o[3] = dty * RTOD;
+    o[4] = dtx * RTOD;
+  }
+  free(f0);
+}
+
 /* get_r_index_imap (imap.c:76-103), walk bounded to 2 nr steps */
 static int r_index_imap(int nr, const double *r0, const double *rf, double r, int start) {
   int ir0 = start < 0 ? 0 : (start >= nr ? nr - 1 : start);

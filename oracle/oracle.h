@@ -101,6 +101,11 @@ int orc_poisson(double lam, uint64_t cell, uint32_t stream, uint32_t seed);
 void orc_kappa_isw(int which, int N, double L, const double obs[3], double r_max,
                    const orc_bg *bg, const double *npot, int nside, int nplanes,
                    const double *r_src, double *out);
+/* srcs.c:425-737 (beams without _USE_FAST_LENSING): dz[n] and, when lens is
+ * not NULL, lens[5n] = (e1, e2, kappa, dra, ddec); r_a2r = r(a = i/(na-1)). */
+void orc_src_beams(int N, double L, const double obs[3], double r_max, const orc_bg *bg,
+                   const double *r_a2r, double fgrowth_0, double hubble_0, double omega_m,
+                   const double *npot, long long n, const float *pos4, double *dz, double *lens);
 void orc_imap(int N, double L, const double obs[3], const orc_bg *bg, const double *dens,
               const double *npot, const double *tz, const double *bz, double bz0,
               const double *norm, double norm0, double normf, int model,

@@ -74,6 +74,8 @@ def load(build=True):
     _sig(lib, "orc_source_props", None, [ctypes.POINTER(Bg), ctypes.c_longlong, _f, _f])
     _sig(lib, "orc_poisson", ci, [cd, ctypes.c_uint64, cu, cu])
     _sig(lib, "orc_kappa_isw", None, [ci, ci, cd, _d, cd, ctypes.POINTER(Bg), _d, ci, ci, _d, _d])
+    _sig(lib, "orc_src_beams", None, [ci, cd, _d, cd, ctypes.POINTER(Bg), _d, cd, cd, cd, _d, ctypes.c_longlong,
+                                       _f, _d, _d])
     _sig(lib, "orc_imap", None, [ci, cd, _d, ctypes.POINTER(Bg), _d, _d, _d, _d, cd, _d, cd, cd, ci, cd, cd, cd,
                                  ci, ci, _d, _d, _d, _i])
     _sig(lib, "orc_ang2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, cd, cd])
@@ -186,6 +188,20 @@ class Oracle:
         self.lib.orc_kappa_isw(which, N, L, p(obs), self.ta["r_max"], ctypes.byref(self.bg),
                                p(np.ascontiguousarray(npot, dtype=np.float64)), nside, len(r), p(r), p(out))
         return out.reshape(len(r), -1)
+
+    def src_beams(self, N, L, npot, pos4, r_a2r, lensing=True):
+        """(dz[n], lens[n, 5] or None) of srcs.c:425-737 for sources at pos4."""
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        n = len(pos4)
+        dz = np.zeros(max(n, 1))
+        lens = np.zeros((max(n, 1), 5)) if lensing else None
+        obs = np.full(3, 0.5 * L)
+        t = self.ta
+        self.lib.orc_src_beams(N, L, p(obs), t["r_max"], ctypes.byref(self.bg),
+                               p(np.ascontiguousarray(r_a2r, dtype=np.float64)), t["fgrowth_0"], t["hubble_0"],
+                               t["omega_m"], p(np.ascontiguousarray(npot, dtype=np.float64)), n, p(pos4, ctypes.c_float),
+                               p(dz), p(lens) if lensing else None)
+        return dz[:n], (lens[:n] if lensing else None)
 
     def imap(self, N, L, dens, npot, tz, bz, norm, norm0, normf, model, nside, r0, rf):
         nr = len(r0)

@@ -235,6 +235,28 @@ def test_sources_bit_exact(workdir, N, ipop):
     np.testing.assert_allclose(props, oprops, rtol=1e-6, atol=1e-5)
 
 
+@pytest.mark.parametrize("N,ipop,lensing", [(64, 0, True), (128, 1, True), (64, 2, False)])
+def test_source_beams(workdir, N, ipop, lensing):
+    """The sources' line-of-sight pass on the sampled catalogue and phi: the
+    CIC radial velocity (dz_rsd with the V1(r_of_z(z0)) scaling) and the
+    nearest-cell march (e1, e2, kappa, dra, ddec) against the oracle's FP64
+    restatement: every value within 1e-5 of its column's RMS."""
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx, orc, pos, _ = _sources_vs_oracle(run, N, ipop, 3003)
+    npot = ctx.fetch_grid(1).astype(np.float64)
+    a2r = run.r_a2r()
+    dz, lens = ctx.source_beams(ipop, len(pos), a2r, lensing)
+    odz, olens = orc.src_beams(N, run.scalar("l_box"), npot, pos, a2r, lensing)
+    cols = [(dz, odz)] + ([(lens[:, k], olens[:, k]) for k in range(5)] if lensing else [])
+    for k, (g, o) in enumerate(cols):
+        rms = np.sqrt(np.mean(o * o))
+        assert rms > 0, k
+        err = np.abs(g.astype(np.float64) - o).max()
+        assert err <= 1e-5 * rms, (k, err, rms)
+    # the pass replaces the catalogue's dz_rsd, it does not add to it
+    assert not np.allclose(dz, pos[:, 3])
+
+
 def test_sources_all_populations_one_pass(workdir):
     """clr_sample_sources(ipop=-1) gives the catalogues of the per-population
     calls (same streams, same order)."""

@@ -224,7 +224,7 @@ def test_fits_writers(tmp_path, host):
     np.testing.assert_array_equal(np.frombuffer(tab, dtype=">f4"), m)
     c = str(tmp_path / "c.fits")
     rdz = np.random.default_rng(0).random((100, 4)).astype(np.float32)
-    assert host.clr_write_catalog_fits(c.encode(), 2, 100, lib.ptr(rdz, ctypes.c_float)) == 0
+    assert host.clr_write_catalog_fits(c.encode(), 2, 100, lib.ptr(rdz, ctypes.c_float), None) == 0
     (_, _), (h1, tab) = _read_fits_table(c)
     assert [h1[f"TTYPE{i}"] for i in range(1, 6)] == ["TYPE", "RA", "DEC", "Z_COSMO", "DZ_RSD"]
     assert [h1[f"TFORM{i}"] for i in range(1, 6)] == ["1J", "1E", "1E", "1E", "1E"]
@@ -232,6 +232,27 @@ def test_fits_writers(tmp_path, host):
     rec = np.frombuffer(tab, dtype=[("t", ">i4"), ("ra", ">f4"), ("dec", ">f4"), ("z", ">f4"), ("dz", ">f4")])
     assert np.all(rec["t"] == 2)
     np.testing.assert_array_equal(np.stack([rec["ra"], rec["dec"], rec["z"], rec["dz"]], 1), rdz)
+    # with lensing columns (has_lensing): 10 fields
+    lens = np.random.default_rng(1).random((100, 5)).astype(np.float32)
+    assert host.clr_write_catalog_fits(c.encode(), 1, 100, lib.ptr(rdz, ctypes.c_float),
+                                       lib.ptr(lens, ctypes.c_float)) == 0
+    (_, _), (h1, tab) = _read_fits_table(c)
+    assert [h1[f"TTYPE{i}"] for i in range(1, 11)] == ["TYPE", "RA", "DEC", "Z_COSMO", "DZ_RSD", "E1", "E2",
+                                                        "KAPPA", "DRA", "DDEC"]
+    assert h1["TFIELDS"] == "10" and h1["TUNIT9"] == "DEG" and h1["TUNIT10"] == "DEG"
+    rec = np.frombuffer(tab, dtype=[("t", ">i4")] + [(f"c{k}", ">f4") for k in range(9)])
+    assert np.all(rec["t"] == 1)
+    np.testing.assert_array_equal(np.stack([rec[f"c{k}"] for k in range(4)], 1), rdz)
+    np.testing.assert_array_equal(np.stack([rec[f"c{k}"] for k in range(4, 9)], 1), lens)
+    a = str(tmp_path / "c.txt")
+    assert host.clr_write_catalog_ascii(a.encode(), 1, 100, lib.ptr(rdz, ctypes.c_float),
+                                        lib.ptr(lens, ctypes.c_float)) == 0
+    with open(a) as fh:
+        head = fh.readline()
+    assert head.startswith("#[1]type") and "[10]d_dec" in head
+    rows = np.loadtxt(a)
+    assert rows.shape == (100, 10)
+    np.testing.assert_allclose(rows[:, 5:], lens, rtol=1e-6)
 
 
 def test_density_grid_writer(tmp_path, host):

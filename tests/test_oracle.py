@@ -396,6 +396,79 @@ def test_kappa_closed_form():
     np.testing.assert_allclose(iw[0], want, rtol=1e-12)
 
 
+def _beam_positions(rng, n, rlo, rhi):
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    u = u[np.abs(u[:, 2]) < 0.95][: n // 2]
+    r = rng.uniform(rlo, rhi, len(u))
+    pos = np.zeros((len(u), 4), dtype=np.float32)
+    pos[:, :3] = u * r[:, None]
+    return pos
+
+
+def test_source_beams_closed_form():
+    """phi = 1/2 x.M.x + g.x about the observer, D(1+z) = 1 (to 1e-7) and
+    z = r/3000.  The CIC central difference of phi at a source is 2 dx (M x +
+    g) exactly, so dz = (x.M.u + g.u) V1(r) (-f0 / (1.5 H0 Om)).  The tidal
+    tensor is M dx^2 exactly, so kappa, e1 and e2 are (M_tt + M_pp),
+    (M_tt - M_pp) and 2 M_tp, the tensor projected on the unit vectors
+    e_theta, e_phi, times S = sum dr (rm - rm^2/r) over the march; with M = 0
+    the deflections are (e_theta.g, e_phi.g) times sum 2 dr (1 - rm/r), in
+    degrees."""
+    ta = _tables()
+    N, Lb = 32, 1000.0
+    dx = Lb / N
+    ta["r_max"] = Lb / (2 * (1 + 2.0 / N))
+    ta["v1"] = 1 + ta["r"] / 1000.0
+    orc = Oracle(ta)
+    na = len(ta["r"])
+    a = np.arange(na) / (na - 1.0)
+    r_a2r = np.empty(na)
+    r_a2r[1:] = 3000.0 * (1 / a[1:] - 1)
+    r_a2r[0] = 1e9
+    rng = np.random.default_rng(21)
+    M = rng.normal(size=(3, 3)) * 1e-7
+    M = M + M.T
+    g = rng.normal(size=3) * 1e-5
+    i = (np.arange(N) - N // 2) * dx
+
+    def field(M, g):
+        X = np.stack(np.meshgrid(i, i, i, indexing="ij"), -1)   # [z][y][x] -> (z, y, x) coordinates
+        X = X[..., ::-1]                                          # (x, y, z)
+        phi = np.zeros((N, N, 2 * (N // 2 + 1)))
+        phi[:, :, :N] = 0.5 * np.einsum("...i,ij,...j->...", X, M, X) + X @ g
+        return phi
+
+    pos = _beam_positions(rng, 400, 3 * dx, 12 * dx)
+    x = pos[:, :3].astype(np.float64)
+    r = np.linalg.norm(x, axis=1)
+    u = x / r[:, None]
+    fv = -ta["fgrowth_0"] / (1.5 * ta["hubble_0"] * ta["omega_m"])
+    dz, lens = orc.src_beams(N, Lb, field(M, g), pos, r_a2r, lensing=True)
+    want = (np.einsum("ni,ij,nj->n", x, M, u) + u @ g) * (1 + r / 1000.0) * fv
+    np.testing.assert_allclose(dz, want, rtol=2e-6, atol=1e-6 * np.abs(want).max())
+    cth = u[:, 2]
+    sth = np.sqrt(1 - cth ** 2)
+    cph, sph = u[:, 0] / sth, u[:, 1] / sth
+    et = np.stack([cth * cph, cth * sph, -sth], 1)
+    ep = np.stack([-sph, cph, np.zeros_like(sph)], 1)
+    Mtt, Mpp, Mtp = (np.einsum("ni,ij,nj->n", e, M, f) for e, f in ((et, et), (ep, ep), (et, ep)))
+    nr = N // 2
+    dr = ta["r_max"] / nr
+    S, D = np.zeros(len(r)), np.zeros(len(r))
+    for k in range(len(r)):
+        rm = (np.arange(min(int(r[k] / dr + 0.5), nr - 1) + 1) + 0.5) * dr
+        S[k] = np.sum(dr * (rm - rm * rm / r[k]))
+        D[k] = np.sum(2 * dr * (1 - rm / r[k]))
+    for col, w in ((0, (Mtt - Mpp) * S), (1, 2 * Mtp * S), (2, (Mtt + Mpp) * S)):
+        np.testing.assert_allclose(lens[:, col], w, rtol=1e-6, atol=1e-6 * np.abs(w).max())
+    _, lens0 = orc.src_beams(N, Lb, field(np.zeros((3, 3)), g), pos, r_a2r, lensing=True)
+    RTOD = 57.2957795
+    np.testing.assert_allclose(lens0[:, 3], (ep @ g) * D * RTOD, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(lens0[:, 4], (et @ g) * D * RTOD, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(lens0[:, :3], 0, atol=1e-12)
+
+
 def test_imap_constant_temperature():
     """T = const, b = 0, norm = 1, no RSD: every painted pixel averages to T."""
     ta = _tables()
