@@ -453,3 +453,149 @@ int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *d
   c->stage_ms[ST_BEAMS] = ms;
   return CLR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Skewers (srcs.c:507-529) with their postproc scaling (srcs.c:725-733): one
+// thread per (source, radial step), steps of a source in consecutive lanes,
+// so a wavefront covers one or a few rays and writes whole rows.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SkwArgs {
+  const float *phi, *dens;
+  int N, ngx, nr, na, gaussian;
+  double idx, dr, idr, ox, oy, oz, factor_vel, sigma2;
+  double glob_idr, r_last, v1_0, v1_last, d1_last;
+  float dxf;
+  const double *r_arr, *d1_arr, *v1_arr;
+};
+
+__global__ __launch_bounds__(256) void k_src_skewers(SkwArgs a, const float4 *__restrict__ pos, long long n,
+                                                     float *__restrict__ dg, float *__restrict__ vs) {
+#pragma clang fp contract(off)
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * a.nr) return;
+  const long long i = t / a.nr;
+  const int i_r = (int)(t - i * a.nr);
+  const float4 p = pos[i];
+  const int N = a.N;
+  const size_t ngx = (size_t)a.ngx, plane = (size_t)N * ngx;
+  const double px = p.x, py = p.y, pz = p.z;
+  const double r = sqrt(px * px + py * py + pz * pz);
+  const double ir = 1. / (r > 0.001 ? r : 0.001);
+  const double u[3] = {px * ir, py * ir, pz * ir};
+  const int irmax = min((int)(r * a.idr + 0.5), a.nr - 1);
+  if (i_r > irmax) {   // calloc'd rows (common.c:381-387)
+    dg[t] = 0;
+    vs[t] = 0;
+    return;
+  }
+  const double rm = (i_r + 0.5) * a.dr;
+  const double xn[3] = {(rm * u[0] + a.ox) * a.idx, (rm * u[1] + a.oy) * a.idx, (rm * u[2] + a.oz) * a.idx};
+  int i0[3], i1[3];
+  double h0[3], h1[3];
+  for (int ax = 0; ax < 3; ax++) {
+    const int c = (int)xn[ax];
+    h0[ax] = xn[ax] - c;
+    h1[ax] = 1 - h0[ax];
+    i0[ax] = wrap1(c, N);
+    i1[ax] = wrap1(c + 1, N);
+  }
+  double v[3] = {0, 0, 0}, d = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+    const double w = (bz ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
+    const int cx = bx ? i1[0] : i0[0], cy = by ? i1[1] : i0[1], cz = bz ? i1[2] : i0[2];
+    double vc[3];
+    vel_at(a.phi, N, ngx, plane, cx, cy, cz, vc);
+    v[0] += vc[0] * w;
+    v[1] += vc[1] * w;
+    v[2] += vc[2] * w;
+    const float dc = a.dens[(size_t)cz * plane + (size_t)cy * ngx + cx];
+    if (a.gaussian) {
+      // get_element RETURN_GAUSS (beaming.c:46-57): the cell's offsets and
+      // radius in float, D(r) from the FP64 table
+      const float x0 = (float)((cx + 0.0) * a.dxf - a.ox), y0 = (float)((cy + 0.0) * a.dxf - a.oy);
+      const float z0 = (float)((cz + 0.0) * a.dxf - a.oz);
+      const double rc = sqrt((double)(x0 * x0 + y0 * y0 + z0 * z0));
+      const double D = f_of_r(a.r_arr, a.d1_arr, a.glob_idr, a.r_last, rc, 1.0, a.d1_last);
+      d += (log((double)(1.0f + dc)) / D + D * a.sigma2 * 0.5) * w;
+    } else {
+      d += (double)dc * w;
+    }
+  }
+  const float vr = (float)(0.5 * a.idx * (v[0] * u[0] + v[1] * u[1] + v[2] * u[2]));
+  const double vg = f_of_r(a.r_arr, a.v1_arr, a.glob_idr, a.r_last, rm, a.v1_0, a.v1_last);
+  dg[t] = (float)d;
+  vs[t] = (float)((double)vr * (vg * a.factor_vel));
+}
+
+}  // namespace
+
+int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, float *dg, float *vs, long long n) {
+  if (ipop < 0 || ipop >= c->n_tr[CLR_TRACER_SRCS]) return clr_fail(c, CLR_ERR_ARG, "no source population %d", ipop);
+  if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
+  if (!c->density_done) return clr_fail(c, CLR_ERR_STATE, "density field not computed");
+  const CatalogDev &C = c->cat[ipop];
+  if (n != C.n) return clr_fail(c, CLR_ERR_ARG, "population %d holds %lld sources, not %lld", ipop, C.n, n);
+  if (n > 0 && (!dg || !vs)) return clr_fail(c, CLR_ERR_ARG, "null skewer output");
+  if (gaussian && !(sigma2 >= 0)) return clr_fail(c, CLR_ERR_ARG, "bad sigma2 %g", sigma2);
+  const int N = c->N, na = c->na;
+  if (!(c->r_max > 0)) return clr_fail(c, CLR_ERR_STATE, "tables carry no r_max");
+  if (!(c->hubble_0 > 0) || !(c->omega_m > 0)) return clr_fail(c, CLR_ERR_STATE, "tables carry no H0 or Omega_M");
+  if (n == 0) return CLR_OK;
+  {
+    const int prc = clr_phi_ready(c, 0);
+    if (prc) return prc;
+  }
+  SkwArgs a{};
+  a.phi = c->npot;
+  a.dens = c->dens;
+  a.N = N;
+  a.ngx = c->ngx;
+  a.nr = N / 2;
+  a.na = na;
+  a.gaussian = gaussian ? 1 : 0;
+  a.idx = N / c->L;
+  a.dr = c->r_max / a.nr;
+  a.idr = 1. / a.dr;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.factor_vel = -c->fgrowth_0 / (1.5 * c->hubble_0 * c->omega_m);
+  a.sigma2 = sigma2;
+  a.glob_idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.v1_0 = c->h_bg_first[3];
+  a.v1_last = c->h_bg_last[3];
+  a.d1_last = c->h_bg_last[2];
+  a.dxf = (float)(c->L / N);
+  a.r_arr = c->bg;
+  a.d1_arr = c->bg + 2 * (size_t)na;
+  a.v1_arr = c->bg + 3 * (size_t)na;
+  // sources per launch: at most 2^26 skewer values (512 MiB of staging)
+  const long long chunk = std::max(1LL, (1LL << 26) / a.nr);
+  const long long cap = std::min(n, chunk);
+  float *tmp = nullptr;
+  CLR_HIP(c, hipMalloc((void **)&tmp, (size_t)cap * a.nr * 2 * sizeof(float)));
+  hipError_t e = hipEventRecord(c->ev0, c->stream);
+  for (long long s0 = 0; s0 < n && e == hipSuccess; s0 += cap) {
+    const long long m = std::min(cap, n - s0);
+    const long long tot = m * a.nr;
+    hipLaunchKernelGGL(k_src_skewers, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
+                       (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dg + (size_t)s0 * a.nr, tmp, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(vs + (size_t)s0 * a.nr, tmp + (size_t)cap * a.nr, (size_t)tot * sizeof(float),
+                         hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "source skewers: %s", hipGetErrorString(e));
+  return CLR_OK;
+}

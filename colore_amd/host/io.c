@@ -206,3 +206,49 @@ int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const floa
   fclose(f);
   return 0;
 }
+
+/* FITS image extension of ny rows of nx floats (fits_create_img FLOAT_IMG) */
+static int write_image(FILE *f, long long nx, long long ny, const float *data, const char *contents) {
+  hdr h = {0};
+  kv_str(&h, "XTENSION", "IMAGE", "IMAGE extension");
+  kv_int(&h, "BITPIX", -32, "number of bits per data pixel");
+  kv_int(&h, "NAXIS", 2, "number of data axes");
+  kv_int(&h, "NAXIS1", nx, "length of data axis 1");
+  kv_int(&h, "NAXIS2", ny, "length of data axis 2");
+  kv_int(&h, "PCOUNT", 0, "required keyword; must = 0");
+  kv_int(&h, "GCOUNT", 1, "required keyword; must = 1");
+  kv_str(&h, "CONTENTS", contents, NULL);
+  if (!flush_hdr(f, &h)) return 0;
+  const size_t tot = (size_t)nx * ny;
+  uint8_t buf[4 * 4096];
+  for (size_t i0 = 0; i0 < tot; i0 += 4096) {
+    const size_t m = tot - i0 < 4096 ? tot - i0 : 4096;
+    for (size_t i = 0; i < m; i++) be32(buf + 4 * i, data + i0 + i);
+    if (fwrite(buf, 4, m, f) != m) return 0;
+  }
+  const size_t pad = (2880 - (4 * tot) % 2880) % 2880;
+  char z[2880];
+  memset(z, 0, sizeof(z));
+  return fwrite(z, 1, pad, f) == pad;
+}
+
+int clr_append_skewers_fits(const char *fname, int gaussian, long long n, int nr, const float *dens_or_gauss,
+                            const float *v, const float *bg4, float sigma_g) {
+  FILE *f = fopen(fname, "ab");
+  if (!f) return 1;
+  int ok = write_image(f, nr, n, dens_or_gauss, gaussian ? "gaussian skewers" : "density skewers") &&
+           write_image(f, nr, n, v, "velocity skewers");
+  hdr ex = {0};
+  kv_str(&ex, "CONTENTS", "Background cosmology", NULL);
+  {
+    char s[100];
+    snprintf(s, sizeof(s), "%-8.8s= %20.7G", "SIGMA_G", (double)sigma_g);
+    card(&ex, s);
+  }
+  const char *tt[] = {"R", "Z", "D", "V"}, *tu[] = {"MPC_H", "NA", "NA", "NA"};
+  const void *cols[] = {bg4, bg4 + nr, bg4 + 2 * (size_t)nr, bg4 + 3 * (size_t)nr};
+  ok = ok && write_table(f, 4, tt, "EEEE", tu, NULL, nr, cols, &ex);
+  free(ex.buf);
+  fclose(f);
+  return ok ? 0 : 1;
+}

@@ -125,6 +125,7 @@ int clr_run_execute(clr_run *r, int device) {
   /* srcs_set_cartesian + srcs_get_local_properties */
   long long nsrc[CLR_NPOP_MAX] = {0};
   float *props[CLR_NPOP_MAX] = {0}, *lens[CLR_NPOP_MAX] = {0};
+  float *skw_d[CLR_NPOP_MAX] = {0}, *skw_v[CLR_NPOP_MAX] = {0};
   /* need_beaming (io.c:460-461; fast lensing is not part of this build) */
   int need_beaming = r->do_kappa || r->do_isw || r->n_tr[2] > 0;
   for (int ip = 0; ip < r->n_tr[0]; ip++) need_beaming |= r->lensing_srcs[ip] || r->skw_srcs[ip];
@@ -149,8 +150,17 @@ int clr_run_execute(clr_run *r, int device) {
         for (long long i = 0; i < nsrc[ip]; i++) props[ip][4 * i + 3] = dz[i];
         free(dz);
       }
-      if (r->skw_srcs[ip])
-        fprintf(stderr, "MPI task 0, Warning: skewers at sources are not produced by this build (population %d)\n", ip + 1);
+      if (r->skw_srcs[ip]) {
+        if (r->skw_gauss[ip] && r->dens_type != CLR_DENS_LOGNORMAL) {
+          fprintf(stderr, "MPI task 0, Fatal: Cannot write Gaussian skewers with density type %d\n", r->dens_type);
+          clr_destroy(ctx);
+          return 1;
+        }
+        const size_t ns = (size_t)(nsrc[ip] > 0 ? nsrc[ip] : 1) * (r->n_grid / 2);
+        skw_d[ip] = malloc(ns * sizeof(float));
+        skw_v[ip] = malloc(ns * sizeof(float));
+        CHK(clr_source_skewers(ctx, ip, r->skw_gauss[ip], sigma2, skw_d[ip], skw_v[ip], nsrc[ip]));
+      }
     }
     printf("\n");
   }
@@ -215,6 +225,22 @@ int clr_run_execute(clr_run *r, int device) {
         printf(" %d-th population (FITS)\n", ip);
         snprintf(fname, sizeof(fname), "%s_srcs_s%d_0.fits", r->prefix_out, ip + 1);
         if (clr_write_catalog_fits(fname, ip, nsrc[ip], props[ip], lens[ip])) fprintf(stderr, "MPI task 0, Warning: could not write %s\n", fname);
+        if (skw_d[ip]) {
+          /* slicing information, io.c:1164-1180 */
+          const int nr = r->n_grid / 2;
+          const double dr = r->r_max / nr;
+          float *bg4 = malloc(4 * (size_t)nr * sizeof(float));
+          for (int ir = 0; ir < nr; ir++) {
+            const double rr = dr * (ir + 0.5);
+            bg4[ir] = (float)rr;
+            bg4[nr + ir] = (float)clr_run_get_bg(r, rr, CLR_BG_Z, 0);
+            bg4[2 * nr + ir] = (float)clr_run_get_bg(r, rr, CLR_BG_D1, 0);
+            bg4[3 * nr + ir] = (float)clr_run_get_bg(r, rr, CLR_BG_V1, 0);
+          }
+          if (clr_append_skewers_fits(fname, r->skw_gauss[ip], nsrc[ip], nr, skw_d[ip], skw_v[ip], bg4, (float)sqrt(sigma2)))
+            fprintf(stderr, "MPI task 0, Warning: could not write skewers to %s\n", fname);
+          free(bg4);
+        }
       } else {
         printf(" %d-th population (ASCII)\n", ip);
         snprintf(fname, sizeof(fname), "%s_srcs_s%d_0.txt", r->prefix_out, ip + 1);
@@ -244,7 +270,9 @@ int clr_run_execute(clr_run *r, int device) {
     clr_timer(2);
     printf("\n");
   }
-  for (int i = 0; i < CLR_NPOP_MAX; i++) { free(props[i]); free(lens[i]); free(imd[i]); free(imn[i]); }
+  for (int i = 0; i < CLR_NPOP_MAX; i++) {
+    free(props[i]); free(lens[i]); free(skw_d[i]); free(skw_v[i]); free(imd[i]); free(imn[i]);
+  }
   free(kmap);
   free(imap_isw);
   clr_destroy(ctx);

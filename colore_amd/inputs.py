@@ -114,10 +114,12 @@ CONFIGS = {
 
 def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_format="ASCII",
                  output_density=False, with_maps=True, seed=None, imap_bands=None, imap_nside=None,
-                 kappa_nside=None):
+                 kappa_nside=None, lensing=(), skewers=()):
     """Writes P(k), n(z), b(z) (+T(z), nu table) and the param file of config
     `name` into outdir; returns the param-file path.  n_grid/z_max override
-    the config's values (small variants for tests)."""
+    the config's values (small variants for tests); `lensing` lists the
+    source populations with include_lensing, `skewers` maps populations to
+    store_skewers ("density" or "gaussian")."""
     cfg = dict(CONFIGS[name])
     os.makedirs(outdir, exist_ok=True)
     n = int(n_grid or cfg["n_grid"])
@@ -149,8 +151,12 @@ def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_form
             b = 0.95 / np.array([_growth(z) for z in zs])
         write_two_col(os.path.join(outdir, f"bz{i + 1}.txt"), zs, b)
         lines += [f"srcs{i + 1}: {{", f'  nz_filename = "{outdir}/nz{i + 1}.txt";',
-                  f'  bias_filename = "{outdir}/bz{i + 1}.txt";', "  include_lensing = false;",
-                  "  store_skewers = false;", f'  bias_model = "{model}";', "};"]
+                  f'  bias_filename = "{outdir}/bz{i + 1}.txt";',
+                  f"  include_lensing = {'true' if i in lensing else 'false'};",
+                  f"  store_skewers = {'true' if i in skewers else 'false'};"]
+        if i in skewers:
+            lines.append(f"  gaussian_skewers = {'true' if skewers[i] == 'gaussian' else 'false'};")
+        lines += [f'  bias_model = "{model}";', "};"]
     if with_maps and "imap" in cfg:
         im = cfg["imap"]
         nb = imap_bands or im["nbands"]

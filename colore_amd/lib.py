@@ -81,6 +81,7 @@ def load_amd():
     _sig(lib, "clr_kappa", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_source_beams", ci, [_vp, ci, _d, _f, _f, ctypes.c_longlong])
+    _sig(lib, "clr_source_skewers", ci, [_vp, ci, ci, ctypes.c_double, _f, _f, ctypes.c_longlong])
     _sig(lib, "clr_imap", ci, [_vp, ci, ci, ci, _d, _d, _f, _i32])
     _sig(lib, "clr_fetch_grid", ci, [_vp, ci, _f])
     _sig(lib, "clr_upload_grid", ci, [_vp, ci, _f])
@@ -119,6 +120,7 @@ def load_host():
     _sig(lib, "clr_write_healpix_map", ci, [ctypes.c_char_p, ctypes.c_longlong, _f])
     _sig(lib, "clr_write_catalog_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
     _sig(lib, "clr_write_catalog_ascii", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
+    _sig(lib, "clr_append_skewers_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, ci, _f, _f, _f, ctypes.c_float])
     return lib
 
 
@@ -293,6 +295,15 @@ class Context:
         self._chk(self.lib.clr_source_beams(self.h, ipop, ptr(a2r), ptr(dz, ctypes.c_float),
                                             ptr(lens, ctypes.c_float) if lensing else None, n))
         return dz[:n], (lens[:n] if lensing else None)
+
+    def source_skewers(self, ipop, n, gaussian=False, sigma2=0.0):
+        """(dens_or_gauss[n, N/2], v[n, N/2]): the sources' skewers."""
+        nr = self.N // 2
+        dg = np.zeros((max(n, 1), nr), dtype=np.float32)
+        vs = np.zeros((max(n, 1), nr), dtype=np.float32)
+        self._chk(self.lib.clr_source_skewers(self.h, ipop, int(gaussian), float(sigma2), ptr(dg, ctypes.c_float),
+                                              ptr(vs, ctypes.c_float), n))
+        return dg[:n], vs[:n]
 
     def kappa(self, nside, r_src):
         r = np.ascontiguousarray(r_src, dtype=np.float64)

@@ -189,6 +189,22 @@ int clr_isw(clr_ctx *ctx, int nside, int nplanes, const double *r_src,
  * (cosmo.c:101-112).  n must be the population's source count. */
 int clr_source_beams(clr_ctx *ctx, int ipop, const double *r_arr_a2r,
                      float *dz_rsd, float *lens, long long n);
+/* The skewers of the same pass (store_skewers, srcs.c:507-529) with their
+ * postproc scaling (srcs.c:725-733), for population ipop after
+ * clr_sample_sources and clr_density (or the fused shell sums).  Rows of
+ * nr = N/2 radial steps per source, in catalogue order (the layout of
+ * d_skw/g_skw and v_skw, common.c:381-387): at step j, rm = (j+1/2) r_max/nr,
+ *   dens_or_gauss[i nr + j]  trilinear (CIC) delta, or with gaussian != 0 the
+ *                            Gaussian field ln(1+delta)/D + D sigma2/2, D at
+ *                            the grid cell's radius (beaming.c:46-57);
+ *   v[i nr + j]              CIC radial velocity 0.5 (N/L)(v.u) times
+ *                            V1(rm) (-f0/(1.5 H0 Om));
+ * steps past min((int)(r/dr + 1/2), nr-1) hold 0.  Each row is scaled once
+ * (the reference's MAX loop bound, srcs.c:725, runs into the next row for
+ * sources beyond r_max - dr/2).  sigma2: the Gaussian variance
+ * (sigma2_gauss) that clr_gaussian_fields returned. */
+int clr_source_skewers(clr_ctx *ctx, int ipop, int gaussian, double sigma2,
+                       float *dens_or_gauss, float *v, long long n);
 /* imap_set_cartesian_single (imap.c:135-245) for intensity species ipop.
  * r0/rf: the nr shell edges (any order, sorted as imap_preproc does).
  * Outputs full-sky RING maps data[nr][12 nside^2] (sum of T) and

@@ -99,6 +99,13 @@ int clr_write_catalog_fits(const char *fname, int ipop, long long n, const float
                            const float *lens);
 int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const float *radecz,
                             const float *lens);
+/* The skewer HDUs of a FITS catalogue (io.c:1139-1187), appended to fname:
+ * an n x nr image of densities (or Gaussian fields), one of velocities, and
+ * a table R Z D V (bg4 = float[4 nr]: r, z, D, V1 at each step) whose
+ * header holds SIGMA_G = sqrt(sigma2_gauss). */
+int clr_append_skewers_fits(const char *fname, int gaussian, long long n, int nr,
+                            const float *dens_or_gauss, const float *v, const float *bg4,
+                            float sigma_g);
 
 #ifdef __cplusplus
 }

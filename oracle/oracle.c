@@ -922,6 +922,76 @@ void orc_src_beams(int N, double L, const double obs[3], double r_max, const orc
   free(f0);
 }
 
+/* The skewers of srcs_get_beam_properties_single (srcs.c:507-529) and their
+ * postproc scaling (srcs.c:725-733) on one rank.  Per source i and step
+ * j < nr = N/2 (rows of nr): dg[i nr + j] is the trilinear (CIC) density
+ * (RETURN_DENS) or, with gaussian != 0, the Gaussian field ln(1+delta)/D +
+ * D sigma2/2 with D at the grid cell's radius (RETURN_GAUSS,
+ * beaming.c:46-57); v[i nr + j] is the CIC radial velocity 0.5 (N/L)(v.u)
+ * times V1(rm) (-f0/(1.5 H0 Om)).  Steps past min((int)(r/dr+0.5), nr-1)
+ * stay 0 (calloc, common.c:381-387).  Each row's nr values are scaled once
+ * (the reference's loop bound MAX(..., nr-1), srcs.c:725, runs past the row
+ * for sources beyond r_max - dr/2). */
+void orc_src_skewers(int N, double L, const double obs[3], double r_max, const orc_bg *bg,
+                     double fgrowth_0, double hubble_0, double omega_m, const double *dens,
+                     const double *npot, int gaussian, double sigma2, long long n, const float *pos4,
+                     double *dg, double *vs) {
+  const long ngx = 2 * (N / 2 + 1);
+  const double idx = N / L;
+  const int nr = N / 2, na = bg->na;
+  const double dr = r_max / nr, idr = 1. / dr;
+  const double factor_vel = -fgrowth_0 / (1.5 * hubble_0 * omega_m);
+  const float dxf = (float)(L / N);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long ip = 0; ip < n; ip++) {
+    const float *pos = pos4 + 4 * ip;
+    const double r = sqrt((double)pos[0] * pos[0] + (double)pos[1] * pos[1] + (double)pos[2] * pos[2]);
+    const double ir = 1. / (r > 0.001 ? r : 0.001);
+    const double u[3] = {pos[0] * ir, pos[1] * ir, pos[2] * ir};
+    int irmax = (int)(r * idr + 0.5);
+    if (irmax > nr - 1) irmax = nr - 1;
+    double *drow = dg + (size_t)ip * nr, *vrow = vs + (size_t)ip * nr;
+    for (int j = 0; j < nr; j++) drow[j] = vrow[j] = 0;
+    for (int i_r = 0; i_r <= irmax; i_r++) {
+      const double rm = (i_r + 0.5) * dr;
+      long i0[3], i1[3];
+      double h0[3], h1[3];
+      for (int ax = 0; ax < 3; ax++) {
+        const double xn = (rm * u[ax] + obs[ax]) * idx;
+        i0[ax] = (long)xn;
+        h0[ax] = xn - i0[ax];
+        h1[ax] = 1 - h0[ax];
+        i1[ax] = wrap_cell(i0[ax] + 1, N);
+        i0[ax] = wrap_cell(i0[ax], N);
+      }
+      double v[3] = {0, 0, 0}, d = 0;
+      for (int c = 0; c < 8; c++) {
+        const int bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+        const double w = (bz ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
+        const long cx = bx ? i1[0] : i0[0], cy = by ? i1[1] : i0[1], cz = bz ? i1[2] : i0[2];
+        double vc[3];
+        grid_element(npot, N, cx, cy, cz, vc, NULL);
+        for (int ax = 0; ax < 3; ax++) v[ax] += vc[ax] * w;
+        const double dc = dens[cx + cy * ngx + cz * ngx * N];
+        if (gaussian) {
+          /* beaming.c:51-56: the cell's offsets in float */
+          const float x0 = (float)((cx + 0.0) * dxf - obs[0]), y0 = (float)((cy + 0.0) * dxf - obs[1]);
+          const float z0 = (float)((cz + 0.0) * dxf - obs[2]);
+          const double rc = sqrt((double)(x0 * x0 + y0 * y0 + z0 * z0));
+          const double D = orc_f_of_r(bg, bg->d1, 1, bg->d1[na - 1], rc);
+          d += (log((double)(1.0f + (float)dc)) / D + D * sigma2 * 0.5) * w;
+        } else {
+          d += dc * w;
+        }
+      }
+      const double vr = 0.5 * idx * (v[0] * u[0] + v[1] * u[1] + v[2] * u[2]);
+      const double vg = orc_f_of_r(bg, bg->v1, bg->v1[0], bg->v1[na - 1], rm);
+      drow[i_r] = d;
+      vrow[i_r] = (double)(float)vr * (vg * factor_vel);
+    }
+  }
+}
+
 /* get_r_index_imap (imap.c:76-103), walk bounded to 2 nr steps */
 static int r_index_imap(int nr, const double *r0, const double *rf, double r, int start) {
   int ir0 = start < 0 ? 0 : (start >= nr ? nr - 1 : start);

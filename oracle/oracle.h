@@ -106,6 +106,12 @@ void orc_kappa_isw(int which, int N, double L, const double obs[3], double r_max
 void orc_src_beams(int N, double L, const double obs[3], double r_max, const orc_bg *bg,
                    const double *r_a2r, double fgrowth_0, double hubble_0, double omega_m,
                    const double *npot, long long n, const float *pos4, double *dz, double *lens);
+/* srcs.c:507-529, 725-733 (skewers): dg[n nr] density or Gaussian field,
+ * v[n nr] scaled radial velocity, nr = N/2. */
+void orc_src_skewers(int N, double L, const double obs[3], double r_max, const orc_bg *bg,
+                     double fgrowth_0, double hubble_0, double omega_m, const double *dens,
+                     const double *npot, int gaussian, double sigma2, long long n, const float *pos4,
+                     double *dg, double *vs);
 void orc_imap(int N, double L, const double obs[3], const orc_bg *bg, const double *dens,
               const double *npot, const double *tz, const double *bz, double bz0,
               const double *norm, double norm0, double normf, int model,

@@ -74,6 +74,8 @@ def load(build=True):
     _sig(lib, "orc_source_props", None, [ctypes.POINTER(Bg), ctypes.c_longlong, _f, _f])
     _sig(lib, "orc_poisson", ci, [cd, ctypes.c_uint64, cu, cu])
     _sig(lib, "orc_kappa_isw", None, [ci, ci, cd, _d, cd, ctypes.POINTER(Bg), _d, ci, ci, _d, _d])
+    _sig(lib, "orc_src_skewers", None, [ci, cd, _d, cd, ctypes.POINTER(Bg), cd, cd, cd, _d, _d, ci, cd,
+                                         ctypes.c_longlong, _f, _d, _d])
     _sig(lib, "orc_src_beams", None, [ci, cd, _d, cd, ctypes.POINTER(Bg), _d, cd, cd, cd, _d, ctypes.c_longlong,
                                        _f, _d, _d])
     _sig(lib, "orc_imap", None, [ci, cd, _d, ctypes.POINTER(Bg), _d, _d, _d, _d, cd, _d, cd, cd, ci, cd, cd, cd,
@@ -202,6 +204,21 @@ class Oracle:
                                t["omega_m"], p(np.ascontiguousarray(npot, dtype=np.float64)), n, p(pos4, ctypes.c_float),
                                p(dz), p(lens) if lensing else None)
         return dz[:n], (lens[:n] if lensing else None)
+
+    def src_skewers(self, N, L, dens, npot, pos4, gaussian=False, sigma2=0.0):
+        """(dg[n, N/2], v[n, N/2]) of the skewers (srcs.c:507-529, 725-733)."""
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        n = len(pos4)
+        nr = N // 2
+        dg = np.zeros((max(n, 1), nr))
+        vs = np.zeros((max(n, 1), nr))
+        obs = np.full(3, 0.5 * L)
+        t = self.ta
+        self.lib.orc_src_skewers(N, L, p(obs), t["r_max"], ctypes.byref(self.bg), t["fgrowth_0"], t["hubble_0"],
+                                 t["omega_m"], p(np.ascontiguousarray(dens, dtype=np.float64)),
+                                 p(np.ascontiguousarray(npot, dtype=np.float64)), int(gaussian), float(sigma2), n,
+                                 p(pos4, ctypes.c_float), p(dg), p(vs))
+        return dg[:n], vs[:n]
 
     def imap(self, N, L, dens, npot, tz, bz, norm, norm0, normf, model, nside, r0, rf):
         nr = len(r0)

@@ -161,6 +161,7 @@ def _sources_vs_oracle(run, N, ipop, seed, edit=None):
         amp, bz, model = edit(amp.copy(), bz.copy(), model)
         ctx.set_tracer(0, ipop, model, amp, bz)
     s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
+    ctx.sigma2_gauss = s2
     ntr = int(run.scalar("n_srcs"))
     ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=ntr)
     dens = ctx.fetch_grid(0).astype(np.float64)
@@ -257,6 +258,26 @@ def test_source_beams(workdir, N, ipop, lensing):
     assert not np.allclose(dz, pos[:, 3])
 
 
+@pytest.mark.parametrize("gaussian", [False, True])
+def test_source_skewers(workdir, gaussian):
+    """Skewers of every source (CIC density or Gaussian field, scaled radial
+    velocity, N/2 steps) against the oracle: within 1e-5 of the RMS, zeros
+    past each source's last step."""
+    N = 64
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx, orc, pos, _ = _sources_vs_oracle(run, N, 0, 3003)
+    dens = ctx.fetch_grid(0).astype(np.float64)
+    npot = ctx.fetch_grid(1).astype(np.float64)
+    dg, vs = ctx.source_skewers(0, len(pos), gaussian, ctx.sigma2_gauss)
+    odg, ovs = orc.src_skewers(N, run.scalar("l_box"), dens, npot, pos, gaussian, ctx.sigma2_gauss)
+    for g, o in ((dg, odg), (vs, ovs)):
+        rms = np.sqrt(np.mean(o * o))
+        assert rms > 0
+        err = np.abs(g.astype(np.float64) - o).max()
+        assert err <= 1e-5 * rms, (err, rms)
+        assert np.array_equal(g == 0, o == 0)
+
+
 def test_sources_all_populations_one_pass(workdir):
     """clr_sample_sources(ipop=-1) gives the catalogues of the per-population
     calls (same streams, same order)."""
@@ -347,3 +368,52 @@ def test_cli_end_to_end(tmp_path):
             head = f.read(2880 * 4).decode("ascii", "replace")
         k = head.index("NAXIS2")
         assert int(head[k + 10:k + 30].strip()) == n, (i, head[k:k + 30], n)
+
+
+def test_cli_lensing_catalogue(tmp_path):
+    """`CoLoRe param.cfg` with include_lensing on the first population: its
+    FITS catalogue carries E1 E2 KAPPA DRA DDEC, and every column equals the
+    library's source properties and line-of-sight pass for the same seed; the
+    other populations' DZ_RSD is the line-of-sight value too (the reference
+    replaces it whenever it marches lines of sight), and their files carry the
+    density (second) and Gaussian (third) skewers of the library."""
+    import subprocess
+    from conftest import ROOT
+    from colore_amd import inputs, lib
+    from test_host import _read_fits_table
+    N = 64
+    p = inputs.write_config("C3", str(tmp_path), n_grid=N, z_max=1.6, output_format="FITS", with_maps=False,
+                            lensing=(0,), skewers={1: "density", 2: "gaussian"})
+    exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
+    r = subprocess.run([exe, p], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    run = lib.Run(p)
+    assert run.scalar("n_srcs") == 3
+    ctx = _ctx_for(run, N)
+    seed = int(run.scalar("seed"))
+    s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
+    ctx.density(0, s2)
+    ctx.shell_norms(lognormal_fused=False, sigma2=s2, ntracers=3)
+    a2r = run.r_a2r()
+    for i in range(3):
+        n = ctx.sample_sources(i, seed, int(run.scalar("nside_base")))
+        props = ctx.source_properties(i, n)
+        dz, lens = ctx.source_beams(i, n, a2r, lensing=(i == 0))
+        hdus = _read_fits_table(str(tmp_path / f"out_srcs_s{i + 1}_0.fits"))
+        assert len(hdus) == (2 if i == 0 else 5)
+        (_, _), (h1, tab) = hdus[:2]
+        nf = 10 if i == 0 else 5
+        assert h1["TFIELDS"] == str(nf) and int(h1["NAXIS2"]) == n
+        rec = np.frombuffer(tab, dtype=[("t", ">i4")] + [(f"c{k}", ">f4") for k in range(nf - 1)])
+        assert np.all(rec["t"] == i)
+        np.testing.assert_array_equal(np.stack([rec[f"c{k}"] for k in range(3)], 1), props[:, :3])
+        np.testing.assert_array_equal(rec["c3"], dz)
+        if i == 0:
+            np.testing.assert_array_equal(np.stack([rec[f"c{k}"] for k in range(4, 9)], 1), lens)
+        else:
+            dg, vs = ctx.source_skewers(i, n, gaussian=(i == 2), sigma2=s2)
+            (hd, dd), (hv, dv), (hb, _) = hdus[2:]
+            assert hd["CONTENTS"] == ("gaussian skewers" if i == 2 else "density skewers")
+            np.testing.assert_array_equal(np.frombuffer(dd, dtype=">f4").reshape(n, N // 2), dg)
+            np.testing.assert_array_equal(np.frombuffer(dv, dtype=">f4").reshape(n, N // 2), vs)
+            assert abs(float(hb["SIGMA_G"]) - np.sqrt(s2)) < 1e-6 * np.sqrt(s2)

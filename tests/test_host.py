@@ -207,6 +207,7 @@ def _read_fits_table(path):
             if done:
                 break
         size = int(hdr.get("NAXIS1", 0)) * int(hdr.get("NAXIS2", 0)) if hdr.get("NAXIS") == "2" else 0
+        size *= abs(int(hdr.get("BITPIX", 8))) // 8
         hdus.append((hdr, data[pos:pos + size]))
         pos += (size + 2879) // 2880 * 2880
     return hdus
@@ -253,6 +254,31 @@ def test_fits_writers(tmp_path, host):
     rows = np.loadtxt(a)
     assert rows.shape == (100, 10)
     np.testing.assert_allclose(rows[:, 5:], lens, rtol=1e-6)
+
+
+def test_skewer_hdus(tmp_path, host):
+    """clr_append_skewers_fits: two n x nr images and the R Z D V table with
+    SIGMA_G, after the catalogue table (io.c:1139-1187)."""
+    c = str(tmp_path / "s.fits")
+    rng = np.random.default_rng(2)
+    n, nr = 7, 16
+    rdz = rng.random((n, 4)).astype(np.float32)
+    assert host.clr_write_catalog_fits(c.encode(), 0, n, lib.ptr(rdz, ctypes.c_float), None) == 0
+    d, v = rng.random((n, nr)).astype(np.float32), rng.random((n, nr)).astype(np.float32)
+    bg4 = rng.random((4, nr)).astype(np.float32)
+    assert host.clr_append_skewers_fits(c.encode(), 1, n, nr, lib.ptr(d, ctypes.c_float), lib.ptr(v, ctypes.c_float),
+                                        lib.ptr(bg4, ctypes.c_float), ctypes.c_float(0.75)) == 0
+    hdus = _read_fits_table(c)
+    assert len(hdus) == 5
+    (hi, di), (hv, dv), (hb, db) = hdus[2:]
+    assert hi["XTENSION"] == "IMAGE" and hi["BITPIX"] == "-32" and hi["NAXIS1"] == str(nr) and hi["NAXIS2"] == str(n)
+    assert hi["CONTENTS"] == "gaussian skewers" and hv["CONTENTS"] == "velocity skewers"
+    np.testing.assert_array_equal(np.frombuffer(di, dtype=">f4").reshape(n, nr), d)
+    np.testing.assert_array_equal(np.frombuffer(dv, dtype=">f4").reshape(n, nr), v)
+    assert hb["CONTENTS"] == "Background cosmology" and float(hb["SIGMA_G"]) == 0.75
+    assert [hb[f"TTYPE{i}"] for i in range(1, 5)] == ["R", "Z", "D", "V"] and hb["TUNIT1"] == "MPC_H"
+    rec = np.frombuffer(db, dtype=">f4").reshape(nr, 4)
+    np.testing.assert_array_equal(rec.T, bg4)
 
 
 def test_density_grid_writer(tmp_path, host):

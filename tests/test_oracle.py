@@ -469,6 +469,58 @@ def test_source_beams_closed_form():
     np.testing.assert_allclose(lens0[:, :3], 0, atol=1e-12)
 
 
+def test_source_skewers_closed_form():
+    """Linear fields about the observer, phi = g.x and ln(1+delta) = a + b.x,
+    D = 1/2 (1 at r = 0, as get_bg returns there): the CIC reads are exact, so along the skewer of a source at r,
+    step j (rm = (j+1/2) dr, j <= min(int(r/dr+1/2), nr-1)) holds
+    delta = exp(a + b.u rm) - 1 to CIC accuracy, the Gaussian field
+    2 (a + b.u rm) + sigma2/4, and the velocity (g.u) V1(rm) (-f0/(1.5 H0 Om));
+    later steps are 0."""
+    ta = _tables()
+    N, Lb = 32, 1000.0
+    dx = Lb / N
+    ta["r_max"] = Lb / (2 * (1 + 2.0 / N))
+    ta["v1"] = 1 + ta["r"] / 1000.0
+    ta["d1"] = np.full_like(ta["r"], 0.5)
+    orc = Oracle(ta)
+    rng = np.random.default_rng(5)
+    g = rng.normal(size=3) * 1e-5
+    a, b = 0.1, rng.normal(size=3) * 2e-3
+    i = (np.arange(N) - N // 2) * dx
+    X = np.stack(np.meshgrid(i, i, i, indexing="ij"), -1)[..., ::-1]
+    phi = np.zeros((N, N, 2 * (N // 2 + 1)))
+    phi[:, :, :N] = X @ g
+    dens = np.zeros_like(phi)
+    dens[:, :, :N] = np.expm1(a + X @ b)
+    dens = dens.astype(np.float32).astype(np.float64)
+    pos = _beam_positions(rng, 200, 3 * dx, 12 * dx)
+    x = pos[:, :3].astype(np.float64)
+    r = np.linalg.norm(x, axis=1)
+    u = x / r[:, None]
+    fv = -ta["fgrowth_0"] / (1.5 * ta["hubble_0"] * ta["omega_m"])
+    nr = N // 2
+    dr = ta["r_max"] / nr
+    rm = (np.arange(nr) + 0.5) * dr
+    live = np.arange(nr)[None, :] <= np.minimum((r / dr + 0.5).astype(int), nr - 1)[:, None]
+    lin = a + (u @ b)[:, None] * rm[None, :]
+    s2 = 0.37
+    dg, vs = orc.src_skewers(N, Lb, dens, phi, pos, gaussian=True, sigma2=s2)
+    want_g = np.where(live, 2 * lin + 0.25 * s2, 0)
+    # get_bg(r <= 0, BG_D1) is 1, not the table: skip the steps whose CIC
+    # corners include the observer's cell
+    far = rm > 2 * dx
+    np.testing.assert_allclose(dg[:, far], want_g[:, far], rtol=0, atol=1e-6)
+    assert np.all(np.abs(dg[:, 0] - want_g[:, 0])[live[:, 0]] > 1e-4)
+    want_v = np.where(live, (u @ g)[:, None] * (1 + rm / 1000.0)[None, :] * fv, 0)
+    np.testing.assert_allclose(vs, want_v, rtol=1e-6, atol=1e-9 * np.abs(want_v).max())
+    dd, vs2 = orc.src_skewers(N, Lb, dens, phi, pos, gaussian=False)
+    np.testing.assert_array_equal(vs2, vs)
+    # CIC of exp(linear): within the trilinear error (dx |b|)^2 / 8 of exp
+    want_d = np.where(live, np.expm1(lin), 0)
+    np.testing.assert_allclose(dd, want_d, rtol=0, atol=3 * (dx * np.abs(b).max()) ** 2 / 8 * np.exp(lin.max()))
+    assert np.abs(dd - want_d).max() > 0   # nonlinear field: CIC is not exact here
+
+
 def test_imap_constant_temperature():
     """T = const, b = 0, norm = 1, no RSD: every painted pixel averages to T."""
     ta = _tables()
