@@ -35,6 +35,7 @@ enum StageId {
   ST_FFT_PHI,     // y and x passes of phi (second stream, overlapping later stages)
   ST_SHELLS,      // the k_shells launch of a normalisation call
   ST_BEAMS,       // k_src_beams of a clr_source_beams call
+  ST_CSTM,        // k_cstm of a clr_custom_map call
   ST_COUNT
 };
 
@@ -130,6 +131,7 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
 int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
 int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out);
 int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz, float *lens, long long n);
+int clr_launch_cstm(clr_ctx *c, int ipop, int nside, float *out);
 int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, float *dg, float *vs, long long n);
 int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf,
                     float *data, int32_t *nadd);

@@ -599,3 +599,122 @@ int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, fl
   if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "source skewers: %s", hipGetErrorString(e));
   return CLR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Custom projected maps: cstm_beams_preproc_single +
+// cstm_get_beam_properties_single (cstm.c:39-150).  One thread per pixel
+// centre (NEST order), the same radial march as kappa with a trilinear
+// density read per step.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CstmArgs {
+  const float *dens;
+  int N, ngx, ir_min, ir_max, model;
+  double idx, dr, ox, oy, oz;
+};
+
+__global__ __launch_bounds__(256) void k_cstm(CstmArgs a, const double *__restrict__ kz,
+                                              const double *__restrict__ bz, const double *__restrict__ nz,
+                                              int64_t nside, int64_t npix, float *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= npix) return;
+  double u[3];
+  pix2vec_nest(nside, ip, u);
+  const int N = a.N;
+  const size_t ngx = (size_t)a.ngx, plane = (size_t)N * ngx;
+  double cval = 0;
+  for (int irr = a.ir_min; irr <= a.ir_max; irr++) {
+    const double rm = (irr + 0.5) * a.dr;
+    const double xn[3] = {(rm * u[0] + a.ox) * a.idx, (rm * u[1] + a.oy) * a.idx, (rm * u[2] + a.oz) * a.idx};
+    int i0[3], i1[3];
+    double h0[3], h1[3];
+    for (int ax = 0; ax < 3; ax++) {
+      const int c = (int)xn[ax];
+      h0[ax] = xn[ax] - c;
+      h1[ax] = 1 - h0[ax];
+      i0[ax] = wrap1(c, N);
+      i1[ax] = wrap1(c + 1, N);
+    }
+    double d = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int bx = c & 1, by = (c >> 1) & 1, bzc = c >> 2;
+      const double w = (bzc ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
+      d += (double)a.dens[(size_t)(bzc ? i1[2] : i0[2]) * plane + (size_t)(by ? i1[1] : i0[1]) * ngx +
+                          (bx ? i1[0] : i0[0])] * w;
+    }
+    // the reference holds d as flouble before bias_model
+    cval += kz[irr] * (bias_model_f64((double)(float)d, bz[irr], a.model) * nz[irr] - 1);
+  }
+  out[ip] = (float)(cval * a.dr);
+}
+
+}  // namespace
+
+int clr_launch_cstm(clr_ctx *c, int ipop, int nside, float *out) {
+  if (ipop < 0 || ipop >= c->n_tr[CLR_TRACER_CSTM]) return clr_fail(c, CLR_ERR_ARG, "no custom map %d", ipop);
+  if (!c->density_done) return clr_fail(c, CLR_ERR_STATE, "density field not computed");
+  if (nside < 1 || nside > 8192 || (nside & (nside - 1))) return clr_fail(c, CLR_ERR_ARG, "bad nside %d", nside);
+  if (!out) return clr_fail(c, CLR_ERR_ARG, "null output");
+  const TracerDev &T = c->tr[CLR_TRACER_CSTM][ipop];
+  if (!T.set || !T.norm_set) return clr_fail(c, CLR_ERR_STATE, "custom map %d has no norm table", ipop);
+  const int N = c->N, na = c->na;
+  if (!(c->r_max > 0)) return clr_fail(c, CLR_ERR_STATE, "tables carry no r_max");
+  const int nr = N / 2;
+  const double dr = c->r_max / nr;
+  // kernel K/IH, bias and norm at each step (cstm.c:87-99), FP64 on the host
+  std::vector<double> rr(na), ih(na), kt(na), bt(na), nt(na);
+  CLR_HIP(c, hipMemcpy(rr.data(), c->bg, na * sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(ih.data(), c->bg + 5 * (size_t)na, na * sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(kt.data(), T.amp, na * sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(bt.data(), T.bz, na * sizeof(double), hipMemcpyDeviceToHost));
+  CLR_HIP(c, hipMemcpy(nt.data(), T.norm, na * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> tab(3 * (size_t)nr);
+  double kmax = -1e100;
+  for (int ir = 0; ir < nr; ir++) {
+    const double rm = (ir + 0.5) * dr;
+    tab[ir] = f_of_r(rr.data(), kt.data(), c->glob_idr, c->r_last, rm, 0.0, 0.0) /
+              f_of_r(rr.data(), ih.data(), c->glob_idr, c->r_last, rm, ih[0], ih[na - 1]);
+    tab[nr + ir] = f_of_r(rr.data(), bt.data(), c->glob_idr, c->r_last, rm, bt[0], 1.0);
+    tab[2 * nr + ir] = f_of_r(rr.data(), nt.data(), c->glob_idr, c->r_last, rm, T.norm0, T.normf);
+    kmax = std::max(kmax, fabs(tab[ir]));
+  }
+  int ir_min = 0, ir_max = nr - 1;
+  for (int ir = 0; ir < nr; ir++)
+    if (fabs(tab[ir]) > 1e-4 * kmax) { ir_min = ir; break; }
+  for (int ir = nr - 1; ir >= 0; ir--)
+    if (fabs(tab[ir]) > 1e-4 * kmax) { ir_max = ir; break; }
+  if (ir_max < ir_min) return clr_fail(c, CLR_ERR_ARG, "Custom kernel has no suppport");
+  const int64_t npix = 12LL * nside * nside;
+  const size_t need = 3 * (size_t)nr * sizeof(double) + (size_t)npix * sizeof(float);
+  int rc = clr_ensure(c, &c->misc, &c->misc_n, need);
+  if (rc) return rc;
+  double *dt = (double *)c->misc;
+  float *dout = (float *)(dt + 3 * nr);
+  CLR_HIP(c, hipMemcpyAsync(dt, tab.data(), 3 * (size_t)nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  CstmArgs a{};
+  a.dens = c->dens;
+  a.N = N;
+  a.ngx = c->ngx;
+  a.ir_min = ir_min;
+  a.ir_max = ir_max;
+  a.model = T.model;
+  a.idx = N / c->L;
+  a.dr = dr;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_cstm, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, a, dt, dt + nr,
+                     dt + 2 * nr, (int64_t)nside, npix, dout);
+  CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(out, dout, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->stage_ms[ST_CSTM] = ms;
+  return CLR_OK;
+}

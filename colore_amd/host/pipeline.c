@@ -180,13 +180,16 @@ int clr_run_execute(clr_run *r, int device) {
     }
     printf("\n");
   }
-  if (r->n_tr[2] > 0) fprintf(stderr, "MPI task 0, Warning: custom maps are not produced by this build\n");
 
-  /* get_beam_properties: kappa, isw */
-  float *kmap = NULL, *imap_isw = NULL;
-  if (r->do_kappa || r->do_isw) {
+  /* get_beam_properties: kappa, isw, custom maps */
+  float *kmap = NULL, *imap_isw = NULL, *cmap[CLR_NPOP_MAX] = {0};
+  if (r->do_kappa || r->do_isw || r->n_tr[2] > 0) {
     printf("*** Getting LOS information\n");
     clr_timer(0);
+    for (int ip = 0; ip < r->n_tr[2]; ip++) {
+      cmap[ip] = malloc((size_t)12 * r->nside_cstm[ip] * r->nside_cstm[ip] * sizeof(float));
+      CHK(clr_custom_map(ctx, ip, r->nside_cstm[ip], cmap[ip]));
+    }
     if (r->do_kappa) {
       kmap = malloc((size_t)12 * r->nside_kappa * r->nside_kappa * r->n_kappa * sizeof(float));
       CHK(clr_kappa(ctx, r->nside_kappa, r->n_kappa, r->r_kappa, kmap));
@@ -270,8 +273,21 @@ int clr_run_execute(clr_run *r, int device) {
     clr_timer(2);
     printf("\n");
   }
+  if (r->n_tr[2] > 0) {
+    /* write_cstm (io.c:754-806): nadd = 1, NEST converted to RING */
+    clr_timer(0);
+    printf("*** Writing custom maps\n");
+    for (int ip = 0; ip < r->n_tr[2]; ip++) {
+      char pat[CLR_PATH + 64];
+      printf(" %d-th species\n", ip);
+      snprintf(pat, sizeof(pat), "%%s_custom_s%d.fits", ip + 1);
+      write_maps(pat, r, r->nside_cstm[ip], 1, cmap[ip]);
+    }
+    clr_timer(2);
+    printf("\n");
+  }
   for (int i = 0; i < CLR_NPOP_MAX; i++) {
-    free(props[i]); free(lens[i]); free(skw_d[i]); free(skw_v[i]); free(imd[i]); free(imn[i]);
+    free(props[i]); free(lens[i]); free(skw_d[i]); free(skw_v[i]); free(imd[i]); free(imn[i]); free(cmap[i]);
   }
   free(kmap);
   free(imap_isw);

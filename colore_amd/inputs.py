@@ -114,12 +114,13 @@ CONFIGS = {
 
 def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_format="ASCII",
                  output_density=False, with_maps=True, seed=None, imap_bands=None, imap_nside=None,
-                 kappa_nside=None, lensing=(), skewers=()):
+                 kappa_nside=None, lensing=(), skewers=(), custom=()):
     """Writes P(k), n(z), b(z) (+T(z), nu table) and the param file of config
     `name` into outdir; returns the param-file path.  n_grid/z_max override
     the config's values (small variants for tests); `lensing` lists the
     source populations with include_lensing, `skewers` maps populations to
-    store_skewers ("density" or "gaussian")."""
+    store_skewers ("density" or "gaussian"); `custom` lists (z0, sigma_z,
+    nside, bias) of custom maps with a Gaussian kernel K(z)."""
     cfg = dict(CONFIGS[name])
     os.makedirs(outdir, exist_ok=True)
     n = int(n_grid or cfg["n_grid"])
@@ -157,6 +158,12 @@ def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_form
         if i in skewers:
             lines.append(f"  gaussian_skewers = {'true' if skewers[i] == 'gaussian' else 'false'};")
         lines += [f'  bias_model = "{model}";', "};"]
+    for i, (z0, sz, ns, bc) in enumerate(custom):
+        write_two_col(os.path.join(outdir, f"kz_cstm{i + 1}.txt"), zs, np.exp(-0.5 * ((zs - z0) / sz) ** 2))
+        write_two_col(os.path.join(outdir, f"bz_cstm{i + 1}.txt"), zs, np.full_like(zs, bc))
+        lines += [f"custom{i + 1}: {{", f'  kz_filename = "{outdir}/kz_cstm{i + 1}.txt";',
+                  f'  bias_filename = "{outdir}/bz_cstm{i + 1}.txt";', f"  nside = {ns};",
+                  '  bias_model = "exponential";', "};"]
     if with_maps and "imap" in cfg:
         im = cfg["imap"]
         nb = imap_bands or im["nbands"]

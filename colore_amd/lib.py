@@ -20,7 +20,7 @@ CLR_BIAS_EXPONENTIAL, CLR_BIAS_EXP_TRUNCATED, CLR_BIAS_TRUNCATED = 0, 1, 2
 CLR_DENS_LOGNORMAL, CLR_DENS_CLIP = 0, 3
 CLR_RNG_PHILOX, CLR_RNG_INJECTED = 0, 1
 STAGES = ["gen", "fft_yz", "fft_x", "density", "norm", "srcs", "kappa", "isw", "imap", "src_pass", "src_write",
-          "fft_phi", "shells", "beams"]
+          "fft_phi", "shells", "beams", "cstm"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
@@ -81,6 +81,7 @@ def load_amd():
     _sig(lib, "clr_kappa", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_source_beams", ci, [_vp, ci, _d, _f, _f, ctypes.c_longlong])
+    _sig(lib, "clr_custom_map", ci, [_vp, ci, ci, _f])
     _sig(lib, "clr_source_skewers", ci, [_vp, ci, ci, ctypes.c_double, _f, _f, ctypes.c_longlong])
     _sig(lib, "clr_imap", ci, [_vp, ci, ci, ci, _d, _d, _f, _i32])
     _sig(lib, "clr_fetch_grid", ci, [_vp, ci, _f])
@@ -304,6 +305,12 @@ class Context:
         self._chk(self.lib.clr_source_skewers(self.h, ipop, int(gaussian), float(sigma2), ptr(dg, ctypes.c_float),
                                               ptr(vs, ctypes.c_float), n))
         return dg[:n], vs[:n]
+
+    def custom_map(self, ipop, nside):
+        """Custom projected map ipop (NEST, float32[12 nside^2])."""
+        out = np.zeros(12 * nside * nside, dtype=np.float32)
+        self._chk(self.lib.clr_custom_map(self.h, ipop, nside, ptr(out, ctypes.c_float)))
+        return out
 
     def kappa(self, nside, r_src):
         r = np.ascontiguousarray(r_src, dtype=np.float64)

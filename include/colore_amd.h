@@ -171,6 +171,15 @@ int clr_kappa(clr_ctx *ctx, int nside, int nplanes, const double *r_src,
 /* isw_get_beam_properties (isw.c:78-147); same conventions as clr_kappa. */
 int clr_isw(clr_ctx *ctx, int nside, int nplanes, const double *r_src,
             float *out);
+/* cstm_beams_preproc_single + cstm_get_beam_properties_single (cstm.c:39-150)
+ * for custom map ipop (customN, kernel K(r) set with clr_set_tracer kind
+ * CLR_TRACER_CSTM; its norm table from clr_shell_norms): per pixel of a
+ * HEALPix map at nside (NEST order, out[12 nside^2]), dr = r_max/(N/2) times
+ * the sum over the steps rm = (i+1/2) dr where |K/IH| exceeds 1e-4 of its
+ * largest value of K(rm)/IH(rm) (bias_model(d, b(rm)) norm(rm) - 1), d the
+ * trilinear (CIC) density at rm u.  Needs clr_density (or the fused shell
+ * sums) first. */
+int clr_custom_map(clr_ctx *ctx, int ipop, int nside, float *out);
 /* srcs_beams_preproc_single, srcs_get_beam_properties_single and
  * srcs_beams_postproc_single (srcs.c:425-737, the build without
  * _USE_FAST_LENSING) for source population ipop after clr_sample_sources.
@@ -229,7 +238,8 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
  * and per-source positions), the y and x passes of phi (these run on a second
  * stream, overlapping the density's passes and the stages after them; every
  * call that reads phi waits for them), the shell-sum kernel alone, the
- * source-beam kernel.  Fills at most n values; returns the number of stages. */
+ * source-beam kernel, the custom-map kernel.  Fills at most n values; returns
+ * the number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 
 #ifdef __cplusplus

@@ -992,6 +992,67 @@ void orc_src_skewers(int N, double L, const double obs[3], double r_max, const o
   }
 }
 
+/* cstm_beams_preproc_single + cstm_get_beam_properties_single (cstm.c:39-150)
+ * on one rank: per HEALPix pixel centre (NEST order), the march over
+ * rm = (ir+1/2) dr, dr = r_max/(N/2), between the first and last step where
+ * |K(rm)/IH(rm)| exceeds 1e-4 of its maximum, of K/IH (bias_model(d, b)
+ * norm - 1) with d the trilinear (CIC) density (as float), times dr.  kz, bz,
+ * norm are the tracer's r-tables (get_bg edges: K 0/0, b b[0]/1, norm
+ * norm0/normf).  Returns 1 when the kernel has no support. */
+int orc_cstm(int N, double L, const double obs[3], double r_max, const orc_bg *bg, const double *dens,
+             const double *kz_arr, const double *bz_arr, const double *norm_arr, double norm0, double normf,
+             int model, int nside, double *out) {
+  const long ngx = 2 * (N / 2 + 1);
+  const double idx = N / L;
+  const int nr = N / 2, na = bg->na;
+  const double dr = r_max / nr;
+  double *kz = malloc(3 * (size_t)nr * sizeof(double)), *bz = kz + nr, *nz = kz + 2 * nr;
+  double kmax = -1e100;
+  for (int ir = 0; ir < nr; ir++) {
+    const double rm = (ir + 0.5) * dr;
+    kz[ir] = orc_f_of_r(bg, kz_arr, 0, 0, rm) / orc_f_of_r(bg, bg->ih, bg->ih[0], bg->ih[na - 1], rm);
+    bz[ir] = orc_f_of_r(bg, bz_arr, bz_arr[0], 1, rm);
+    nz[ir] = orc_f_of_r(bg, norm_arr, norm0, normf, rm);
+    if (fabs(kz[ir]) > kmax) kmax = fabs(kz[ir]);
+  }
+  int irmin = 0, irmax = nr - 1;
+  for (int ir = 0; ir < nr; ir++)
+    if (fabs(kz[ir]) > 1e-4 * kmax) { irmin = ir; break; }
+  for (int ir = nr - 1; ir >= 0; ir--)
+    if (fabs(kz[ir]) > 1e-4 * kmax) { irmax = ir; break; }
+  if (irmax < irmin) { free(kz); return 1; }
+  const long long npix = 12LL * nside * nside;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long ip = 0; ip < npix; ip++) {
+    double u[3];
+    orc_pix2vec_nest(nside, ip, u);
+    double cval = 0;
+    for (int irr = irmin; irr <= irmax; irr++) {
+      const double rm = (irr + 0.5) * dr;
+      long i0[3], i1[3];
+      double h0[3], h1[3];
+      for (int ax = 0; ax < 3; ax++) {
+        const double xn = (rm * u[ax] + obs[ax]) * idx;
+        i0[ax] = (long)xn;
+        h0[ax] = xn - i0[ax];
+        h1[ax] = 1 - h0[ax];
+        i1[ax] = wrap_cell(i0[ax] + 1, N);
+        i0[ax] = wrap_cell(i0[ax], N);
+      }
+      double d = 0;
+      for (int c = 0; c < 8; c++) {
+        const int bx = c & 1, by = (c >> 1) & 1, bzc = c >> 2;
+        const double w = (bzc ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
+        d += dens[(bx ? i1[0] : i0[0]) + (by ? i1[1] : i0[1]) * ngx + (bzc ? i1[2] : i0[2]) * ngx * N] * w;
+      }
+      cval += kz[irr] * (orc_bias_model((float)d, bz[irr], model) * nz[irr] - 1);
+    }
+    out[ip] = cval * dr;
+  }
+  free(kz);
+  return 0;
+}
+
 /* get_r_index_imap (imap.c:76-103), walk bounded to 2 nr steps */
 static int r_index_imap(int nr, const double *r0, const double *rf, double r, int start) {
   int ir0 = start < 0 ? 0 : (start >= nr ? nr - 1 : start);

@@ -112,6 +112,11 @@ void orc_src_skewers(int N, double L, const double obs[3], double r_max, const o
                      double fgrowth_0, double hubble_0, double omega_m, const double *dens,
                      const double *npot, int gaussian, double sigma2, long long n, const float *pos4,
                      double *dg, double *vs);
+/* cstm.c:39-150: one custom map (NEST) of the tracer's K, b and norm tables;
+ * returns 1 when the kernel has no support. */
+int orc_cstm(int N, double L, const double obs[3], double r_max, const orc_bg *bg, const double *dens,
+             const double *kz_arr, const double *bz_arr, const double *norm_arr, double norm0, double normf,
+             int model, int nside, double *out);
 void orc_imap(int N, double L, const double obs[3], const orc_bg *bg, const double *dens,
               const double *npot, const double *tz, const double *bz, double bz0,
               const double *norm, double norm0, double normf, int model,

@@ -76,6 +76,7 @@ def load(build=True):
     _sig(lib, "orc_kappa_isw", None, [ci, ci, cd, _d, cd, ctypes.POINTER(Bg), _d, ci, ci, _d, _d])
     _sig(lib, "orc_src_skewers", None, [ci, cd, _d, cd, ctypes.POINTER(Bg), cd, cd, cd, _d, _d, ci, cd,
                                          ctypes.c_longlong, _f, _d, _d])
+    _sig(lib, "orc_cstm", ci, [ci, cd, _d, cd, ctypes.POINTER(Bg), _d, _d, _d, _d, cd, cd, ci, ci, _d])
     _sig(lib, "orc_src_beams", None, [ci, cd, _d, cd, ctypes.POINTER(Bg), _d, cd, cd, cd, _d, ctypes.c_longlong,
                                        _f, _d, _d])
     _sig(lib, "orc_imap", None, [ci, cd, _d, ctypes.POINTER(Bg), _d, _d, _d, _d, cd, _d, cd, cd, ci, cd, cd, cd,
@@ -219,6 +220,15 @@ class Oracle:
                                  p(np.ascontiguousarray(npot, dtype=np.float64)), int(gaussian), float(sigma2), n,
                                  p(pos4, ctypes.c_float), p(dg), p(vs))
         return dg[:n], vs[:n]
+
+    def cstm(self, N, L, dens, kz, bz, norm, norm0, normf, model, nside):
+        """One custom map (NEST) of cstm.c:39-150; None when the kernel has no support."""
+        out = np.zeros(12 * nside * nside)
+        obs = np.full(3, 0.5 * L)
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (dens, kz, bz, norm)]
+        rc = self.lib.orc_cstm(N, L, p(obs), self.ta["r_max"], ctypes.byref(self.bg), p(keep[0]), p(keep[1]),
+                               p(keep[2]), p(keep[3]), norm0, normf, model, nside, p(out))
+        return None if rc else out
 
     def imap(self, N, L, dens, npot, tz, bz, norm, norm0, normf, model, nside, r0, rf):
         nr = len(r0)

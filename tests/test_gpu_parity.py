@@ -278,6 +278,29 @@ def test_source_skewers(workdir, gaussian):
         assert np.array_equal(g == 0, o == 0)
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_custom_map(workdir, fused):
+    """A custom projected map (Gaussian K(z) at z = 0.5, b = 1.2, nside 16)
+    from the device density against the oracle's march on the same grid and
+    norm table: within 1e-5 of the map's RMS."""
+    N, nside = 64, 16
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False, custom=((0.5, 0.1, nside, 1.2),))
+    assert run.scalar("n_cstm") == 1
+    ctx = _ctx_for(run, N)
+    s2 = ctx.gaussian_fields(int(run.scalar("seed")), run.scalar("r_smooth"), True)
+    if not fused:
+        ctx.density(0, s2)
+    ctx.shell_norms(lognormal_fused=fused, sigma2=s2, ntracers=int(run.scalar("n_srcs")) + 1)
+    dens = ctx.fetch_grid(0).astype(np.float64)
+    out = ctx.custom_map(0, nside)
+    kz, bz, model = run.tracer(2, 0)
+    tab, n0, nf = ctx.norm_table(2, 0)
+    o = _oracle(run).cstm(N, run.scalar("l_box"), dens, kz, bz, tab, n0, nf, model, nside)
+    rms = np.sqrt(np.mean(o * o))
+    assert rms > 0
+    assert np.abs(out - o).max() <= 1e-5 * rms, (np.abs(out - o).max(), rms)
+
+
 def test_sources_all_populations_one_pass(workdir):
     """clr_sample_sources(ipop=-1) gives the catalogues of the per-population
     calls (same streams, same order)."""
@@ -376,14 +399,15 @@ def test_cli_lensing_catalogue(tmp_path):
     library's source properties and line-of-sight pass for the same seed; the
     other populations' DZ_RSD is the line-of-sight value too (the reference
     replaces it whenever it marches lines of sight), and their files carry the
-    density (second) and Gaussian (third) skewers of the library."""
+    density (second) and Gaussian (third) skewers of the library; the custom
+    map file holds the library's map in RING order."""
     import subprocess
     from conftest import ROOT
     from colore_amd import inputs, lib
     from test_host import _read_fits_table
     N = 64
     p = inputs.write_config("C3", str(tmp_path), n_grid=N, z_max=1.6, output_format="FITS", with_maps=False,
-                            lensing=(0,), skewers={1: "density", 2: "gaussian"})
+                            lensing=(0,), skewers={1: "density", 2: "gaussian"}, custom=((0.6, 0.15, 8, 1.0),))
     exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
     r = subprocess.run([exe, p], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -393,8 +417,16 @@ def test_cli_lensing_catalogue(tmp_path):
     seed = int(run.scalar("seed"))
     s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
     ctx.density(0, s2)
-    ctx.shell_norms(lognormal_fused=False, sigma2=s2, ntracers=3)
+    ctx.shell_norms(lognormal_fused=False, sigma2=s2, ntracers=4)
     a2r = run.r_a2r()
+    # the custom map, written in RING order
+    cm = ctx.custom_map(0, 8)
+    (_, _), (h1, tab) = _read_fits_table(str(tmp_path / "out_custom_s1.fits"))
+    assert h1["ORDERING"] == "RING" and h1["NSIDE"] == "8"
+    host = lib.load_host()
+    ring = np.zeros_like(cm)
+    ring[[host.clr_hp_nest2ring(8, i) for i in range(len(cm))]] = cm
+    np.testing.assert_array_equal(np.frombuffer(tab, dtype=">f4"), ring)
     for i in range(3):
         n = ctx.sample_sources(i, seed, int(run.scalar("nside_base")))
         props = ctx.source_properties(i, n)

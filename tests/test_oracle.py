@@ -521,6 +521,43 @@ def test_source_skewers_closed_form():
     assert np.abs(dd - want_d).max() > 0   # nonlinear field: CIC is not exact here
 
 
+def test_custom_map_closed_form(L):
+    """delta = c.x about the observer, the truncated model with b = 1.5 and
+    norm = 1: CIC is exact on linear fields, so the map at pixel u is
+    b (c.u) sum K(rm)/IH rm dr over the steps where |K/IH| exceeds 1e-4 of its
+    maximum (np.interp of the tables; IH = 3000)."""
+    ta = _tables()
+    N, Lb = 32, 1000.0
+    ta["r_max"] = Lb / (2 * (1 + 2.0 / N))
+    orc = Oracle(ta)
+    dx = Lb / N
+    rng = np.random.default_rng(8)
+    c = rng.normal(size=3) * 3e-4
+    i = (np.arange(N) - N // 2) * dx
+    X = np.stack(np.meshgrid(i, i, i, indexing="ij"), -1)[..., ::-1]
+    dens = np.zeros((N, N, 2 * (N // 2 + 1)))
+    dens[:, :, :N] = X @ c
+    r = ta["r"]
+    kz = np.exp(-((r - 250.0) / 40.0) ** 2)
+    bz = np.full_like(r, 1.5)
+    norm = np.ones_like(r)
+    nside = 4
+    out = orc.cstm(N, Lb, dens, kz, bz, norm, 1.0, 1.0, 2, nside)
+    nr = N // 2
+    dr = ta["r_max"] / nr
+    rm = (np.arange(nr) + 0.5) * dr
+    k = np.interp(rm, r, kz) / 3000.0
+    on = np.nonzero(np.abs(k) > 1e-4 * np.abs(k).max())[0]
+    sel = np.arange(on[0], on[-1] + 1)
+    u = np.zeros((12 * nside * nside, 3))
+    for ip in range(len(u)):
+        L.orc_pix2vec_nest(nside, ip, p(u[ip]))
+    want = 1.5 * (u @ c) * np.sum(k[sel] * rm[sel]) * dr
+    np.testing.assert_allclose(out, want, rtol=0, atol=2e-6 * np.abs(want).max())
+    # a kernel that is 0 everywhere keeps the default edges (the whole march) and gives 0
+    assert np.all(orc.cstm(N, Lb, dens, np.zeros_like(r), bz, norm, 1.0, 1.0, 2, nside) == 0)
+
+
 def test_imap_constant_temperature():
     """T = const, b = 0, norm = 1, no RSD: every painted pixel averages to T."""
     ta = _tables()
