@@ -358,13 +358,19 @@ static double f_of_r_h(const clr_run *r, const double *f, double f0, double ff, 
 }
 
 double clr_run_get_bg(const clr_run *r, double rr, int tag, int ipop) {
-  (void)ipop;
   switch (tag) {
     case CLR_BG_Z: return f_of_r_h(r, r->z_arr_r2z, 0, r->z_arr_r2z[CLR_NA - 1], rr);
     case CLR_BG_D1: return f_of_r_h(r, r->growth_d_arr, 1, r->growth_d_arr[CLR_NA - 1], rr);
     case CLR_BG_V1: return f_of_r_h(r, r->growth_v_arr, r->growth_v_arr[0], r->growth_v_arr[CLR_NA - 1], rr);
     case CLR_BG_PD: return f_of_r_h(r, r->growth_pd_arr, r->growth_pd_arr[0], r->growth_pd_arr[CLR_NA - 1], rr);
     case CLR_BG_IH: return f_of_r_h(r, r->ihub_arr, r->ihub_arr[0], r->ihub_arr[CLR_NA - 1], rr);
+    case CLR_BG_BZ_SRCS:
+    case CLR_BG_BZ_IMAP:
+    case CLR_BG_BZ_CSTM: {
+      const int k = tag - CLR_BG_BZ_SRCS;
+      if (ipop < 0 || ipop >= r->n_tr[k]) return NAN;
+      return f_of_r_h(r, r->bz_arr[k][ipop], r->bz_arr[k][ipop][0], 1, rr);
+    }
     default: return NAN;
   }
 }

@@ -54,7 +54,7 @@ int clr_run_execute(clr_run *r, int device) {
   clr_ctx *ctx = NULL;
   char err[1024];
   printf("Seed : %u\n", (unsigned)r->seed);
-  if (r->do_pred) fprintf(stderr, "MPI task 0, Warning: theory predictions (write_pred) are not produced by this build\n");
+  if (r->do_pred && clr_write_predictions(r)) fprintf(stderr, "MPI task 0, Warning: could not write the predictions\n");
   if (r->just_do_pred) return 0;
   if (clr_run_device(r, device, &ctx, err, sizeof(err))) {
     fprintf(stderr, "MPI task 0, Fatal: %s\n", err);

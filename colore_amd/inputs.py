@@ -114,13 +114,14 @@ CONFIGS = {
 
 def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_format="ASCII",
                  output_density=False, with_maps=True, seed=None, imap_bands=None, imap_nside=None,
-                 kappa_nside=None, lensing=(), skewers=(), custom=()):
+                 kappa_nside=None, lensing=(), skewers=(), custom=(), pred_dz=None, just_pred=False):
     """Writes P(k), n(z), b(z) (+T(z), nu table) and the param file of config
     `name` into outdir; returns the param-file path.  n_grid/z_max override
     the config's values (small variants for tests); `lensing` lists the
     source populations with include_lensing, `skewers` maps populations to
     store_skewers ("density" or "gaussian"); `custom` lists (z0, sigma_z,
-    nside, bias) of custom maps with a Gaussian kernel K(z)."""
+    nside, bias) of custom maps with a Gaussian kernel K(z); pred_dz turns on
+    write_pred (just_pred: just_write_pred)."""
     cfg = dict(CONFIGS[name])
     os.makedirs(outdir, exist_ok=True)
     n = int(n_grid or cfg["n_grid"])
@@ -133,8 +134,10 @@ def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_form
     lines = ["global: {", f'  prefix_out = "{outdir}/out";', f'  output_format = "{output_format}";',
              f"  output_density = {'true' if output_density else 'false'};",
              f'  pk_filename = "{outdir}/pk.txt";', f"  z_min = {zmin};", f"  z_max = {zmax};",
-             f"  seed = {int(seed if seed is not None else cfg['seed'])};", "  write_pred = false;",
-             "  just_write_pred = false;", "};", "field_par: {", f"  r_smooth = {cfg['r_smooth']};",
+             f"  seed = {int(seed if seed is not None else cfg['seed'])};"] + \
+            ([f"  write_pred = true;", f"  just_write_pred = {'true' if just_pred else 'false'};",
+              f"  pred_dz = {pred_dz};"] if pred_dz else ["  write_pred = false;", "  just_write_pred = false;"]) + \
+            ["};", "field_par: {", f"  r_smooth = {cfg['r_smooth']};",
              "  smooth_potential = true;", f"  n_grid = {n};", f"  dens_type = {dens_type};",
              "  lpt_buffer_fraction = 0.6;", "  lpt_interp_type = 1;", "  output_lpt = 0;", "};",
              "cosmo_par: {"] + [f"  {kk} = {vv};" for kk, vv in COSMO.items()] + ["};"]

@@ -73,6 +73,9 @@ const double *clr_run_a2r(const clr_run *r);
 #define CLR_BG_V1 2
 #define CLR_BG_PD 3
 #define CLR_BG_IH 4
+#define CLR_BG_BZ_SRCS 5   /* tracer bias tables: edges b[0] and 1 (cosmo.c:60-79) */
+#define CLR_BG_BZ_IMAP 6
+#define CLR_BG_BZ_CSTM 7
 
 /* Creates the device context and uploads every table of the run. */
 int      clr_run_device(clr_run *r, int device, clr_ctx **ctx, char *err, int errlen);
@@ -86,6 +89,15 @@ long long clr_hp_vec2pix_ring(long long nside, const double *v);
 long long clr_hp_ring2nest(long long nside, long long pix);
 long long clr_hp_nest2ring(long long nside, long long pix);
 void      clr_hp_pix2vec_nest(long long nside, long long pix, double *v);
+
+/* ---- theory predictions (predictions.c, fftlog.c) ------------------------ */
+/* FFTLog (Hamilton 2000) of l = 0 on n log-spaced points: xi(r) from P(k) and
+ * back (pk2xi / xi2pk, fftlog.c:137-157); the output grid r (or k) is written. */
+void clr_pk2xi(int n, const double *k, const double *pk, double *r, double *xi);
+void clr_xi2pk(int n, const double *r, const double *xi, double *k, double *pk);
+/* write_predictions (predictions.c:26-172): <prefix>_gbias.txt and the
+ * lognormal P(k) / xi(r) predictions of every tracer every pred_dz in z. */
+int  clr_write_predictions(const clr_run *r);
 
 /* ---- writers ------------------------------------------------------------ */
 /* write_density_grid native format (io.c:565-595) from a padded grid */

@@ -331,3 +331,115 @@ def test_cli_usage_and_test_memory(tmp_path):
     assert "L_box = 2726.703 Mpc/h, N_grid = 64" in r.stdout   # 2 r_max (1 + 2/64), r_max = chi(0.5)
     r = subprocess.run([exe, str(tmp_path / "missing.cfg")], capture_output=True, text=True)
     assert r.returncode == 1 and "Fatal: Couldn't open file" in r.stderr
+
+
+# ------------------------------------------------------------ predictions ----
+def _fftlog(host, name, x, y):
+    n = len(x)
+    xo, yo = np.zeros(n), np.zeros(n)
+    fn = getattr(host, name)
+    fn.restype = None
+    fn.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 4
+    x, y = np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(y, dtype=np.float64)
+    fn(n, lib.ptr(x), lib.ptr(y), lib.ptr(xo), lib.ptr(yo))
+    return xo, yo
+
+
+def _fht_numpy(r, a, mu):
+    """fht (fftlog.c, q = 0, kcrc = 1, low-ringing kcrc) from numpy's FFT and
+    scipy's complex log-Gamma."""
+    from scipy.special import loggamma
+    N = len(r)
+    L = np.log(r[-1] / r[0]) * N / (N - 1.0)
+    x, kcrc = (mu + 1) / 2, 1.0
+    arg = np.log(2 / kcrc) * N / L + 2 * loggamma(x + 1j * np.pi * N / (2 * L)).imag / np.pi
+    if arg != np.round(arg):
+        kcrc *= np.exp((arg - np.round(arg)) * L / N)
+    y, k0r0 = np.pi / L, kcrc * np.exp(-L)
+    m = np.arange(N)
+    mm = np.where(m <= N // 2, m, N - m)
+    u = np.exp(1j * (mm * (-2 * y * np.log(k0r0 / 2)) + 2 * loggamma(x + 1j * mm * y).imag))
+    u = np.where(m <= N // 2, u, np.conj(u))
+    if N % 2 == 0:
+        u[N // 2] = u[N // 2].real
+    b = np.fft.ifft(np.fft.fft(a) * u)[::-1]
+    return k0r0 / r[0] * np.exp(np.arange(N) * L / N), b
+
+
+@pytest.mark.parametrize("n", [4096, 10000, 3 * 7 * 97])
+def test_fftlog_matches_library_routines(host, n):
+    """The host's FFTLog (its own mixed-radix FFT for lengths with factors
+    2, 3, 5, 7, 97 and a Stirling-series log-Gamma) against the same
+    transform built on numpy.fft and scipy.special.loggamma."""
+    R = 10.0
+    k = 1e-4 * (1e6) ** (np.arange(n) / (n - 1))
+    pk = (2 * np.pi * R * R) ** 1.5 * np.exp(-0.5 * (k * R) ** 2) * (1 + 0.3 * np.sin(3 * np.log(k)))
+    r, xi = _fftlog(host, "clr_pk2xi", k, pk)
+    r2, b = _fht_numpy(k, k ** 1.5 * pk, 0.5)
+    np.testing.assert_allclose(r, r2, rtol=1e-13)
+    want = ((2 * np.pi * r2) ** -1.5 * b).real
+    np.testing.assert_allclose(xi, want, rtol=0, atol=1e-9 * np.abs(want).max())
+    k3, pk3 = _fftlog(host, "clr_xi2pk", r, xi)
+    r4, b4 = _fht_numpy(r, r ** 1.5 * xi, 0.5)
+    want = 8 * np.pi ** 3 * ((2 * np.pi * r4) ** -1.5 * b4).real
+    np.testing.assert_allclose(k3, r4, rtol=1e-13)
+    np.testing.assert_allclose(pk3, want, rtol=0, atol=1e-9 * np.abs(want).max())
+
+
+def test_fftlog_gaussian_pair(host):
+    """P(k) = (2 pi R^2)^(3/2) exp(-k^2 R^2 / 2) and xi(r) = exp(-r^2 / (2 R^2))
+    are a 3-D Fourier pair.  The reference's fht labels its output one grid
+    step low (the reversed inverse FFT is the forward one at n+1, while
+    k[n] = k0 e^(n L/N)), so value n is the transform at r_n e^D, D = L/N,
+    times e^(1.5 D) from the prefactor at r_n: the closed form holds in that
+    form to 1e-3 and misses by ~1.5 D without the shift."""
+    R, n = 10.0, 4096
+    k = 1e-4 * (1e6) ** (np.arange(n) / (n - 1))
+    pk = (2 * np.pi * R * R) ** 1.5 * np.exp(-0.5 * (k * R) ** 2)
+    r, xi = _fftlog(host, "clr_pk2xi", k, pk)
+    D = np.log(r[1] / r[0])
+    mid = (r > 0.5) & (r < 60)
+    exact = np.exp(-0.5 * (r[mid] * np.exp(D) / R) ** 2)
+    assert np.abs(xi[mid] * np.exp(-1.5 * D) - exact).max() < 1e-3
+    assert np.abs(xi[mid] - np.exp(-0.5 * (r[mid] / R) ** 2)).max() > 0.5 * 1.5 * D
+
+
+def test_write_predictions(tmp_path):
+    """just_write_pred: the driver writes <prefix>_gbias.txt (z, r(z), D(z),
+    one bias per tracer, every pred_dz up to z_max) and per tracer and z the
+    P(k) and xi(r) files; P_tl / P_ll = b exp(-(rsm_gm - rsm_mm) k^2) and, on
+    linear scales, P_tt -> b^2 P_ll exp(-(rsm_gg - rsm_mm) k^2)."""
+    from colore_amd import inputs
+    N = 64
+    p = inputs.write_config("C3", str(tmp_path), n_grid=N, z_max=1.6, with_maps=False, pred_dz=0.5, just_pred=True)
+    exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
+    r = subprocess.run([exe, p], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "*** Writing predictions" in r.stdout
+    assert not os.path.exists(tmp_path / "out_srcs_s1_0.txt")   # just_write_pred stops there
+    run = lib.Run(p)
+    gb = np.loadtxt(tmp_path / "out_gbias.txt")
+    assert gb.shape == (4, 6)
+    np.testing.assert_allclose(gb[:, 0], [0, 0.5, 1.0, 1.5])
+    for row in gb:
+        rr = run.host.clr_run_r_of_z(run.h, row[0])
+        assert abs(row[1] - rr) <= 1e-5 * max(rr, 1)
+        assert abs(row[2] - run.host.clr_run_get_bg(run.h, rr, 1, 0)) < 1e-5
+        for ip in range(3):
+            assert abs(row[3 + ip] - run.host.clr_run_get_bg(run.h, rr, 5, ip)) <= 1e-5 * abs(row[3 + ip])
+    L, dx2 = run.scalar("l_box"), (run.scalar("l_box") / N) ** 2
+    r2 = run.scalar("r_smooth") ** 2
+    gg, gm, mm = r2 + 0.9 * dx2 / 12, r2 + 2.0 * dx2 / 6, r2 + 1.88 * dx2 / 6
+    for ip in range(3):
+        b = gb[2, 3 + ip]
+        pk = np.loadtxt(tmp_path / f"out_pk_srcs_pop{ip}_z1.000.txt")
+        xi = np.loadtxt(tmp_path / f"out_xi_srcs_pop{ip}_z1.000.txt")
+        assert xi[:, 0].min() >= 0.5 and xi[:, 0].max() <= 300
+        k = pk[:, 0]
+        ok = (pk[:, 2] > 1e-200) & (pk[:, 3] > 1e-200)   # the grid smoothing underflows at high k
+        assert ok.sum() > 1000
+        np.testing.assert_allclose(pk[ok, 2] / pk[ok, 3], b * np.exp(-(gm - mm) * k[ok] ** 2), rtol=1e-4)
+        lin = (k > 0.01) & (k < 0.03)
+        ratio = pk[lin, 1] / (b * b * pk[lin, 3] * np.exp(-(gg - mm) * k[lin] ** 2))
+        assert np.all(np.abs(ratio - 1) < 0.05), ratio
+        np.testing.assert_allclose(xi[:, 2], b * xi[:, 3], rtol=1e-4)
