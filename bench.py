@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="C3")
     ap.add_argument("--n-grid", type=int, default=None)
-    ap.add_argument("--cpu-grid", type=int, default=256)
+    ap.add_argument("--cpu-grid", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--maps", action="store_true",
                     help="also time the kappa/ISW line-of-sight maps and the 21cm painting of the workload")
