@@ -1053,6 +1053,168 @@ int orc_cstm(int N, double L, const double obs[3], double r_max, const orc_bg *b
   return 0;
 }
 
+/* ========================================================================
+ * 1LPT density (density.c:37-175, 376-645)
+ * ====================================================================== */
+
+/* 1-D forward DFT (sign -1) in place by conjugation around the inverse */
+static void cfft_fwd(int n, double *d) {
+  for (int i = 0; i < n; i++) d[2 * i + 1] = -d[2 * i + 1];
+  orc_cfft_inv(n, d);
+  for (int i = 0; i < n; i++) d[2 * i + 1] = -d[2 * i + 1];
+}
+
+/* fftw_wrap_r2c: unnormalised forward transform of the padded real grid,
+ * half spectrum out[kz][ky][kx <= N/2] (interleaved complex). */
+void orc_r2c_3d(int N, const double *in, double *out) {
+  const int NH = N / 2 + 1, ngx = 2 * NH;
+  double *line = malloc(2 * (size_t)N * sizeof(double));
+  for (int z = 0; z < N; z++)
+    for (int y = 0; y < N; y++) {
+      const double *row = in + ((size_t)z * N + y) * ngx;
+      for (int x = 0; x < N; x++) { line[2 * x] = row[x]; line[2 * x + 1] = 0; }
+      cfft_fwd(N, line);
+      for (int kx = 0; kx < NH; kx++) {
+        const size_t o = 2 * ((size_t)kx + (size_t)NH * ((size_t)y + (size_t)N * z));
+        out[o] = line[2 * kx];
+        out[o + 1] = line[2 * kx + 1];
+      }
+    }
+  for (int pass = 0; pass < 2; pass++)   /* y, then z */
+    for (int a = 0; a < N; a++)
+      for (int kx = 0; kx < NH; kx++) {
+        for (int b = 0; b < N; b++) {
+          const size_t o = pass == 0 ? 2 * ((size_t)kx + (size_t)NH * ((size_t)b + (size_t)N * a))
+                                     : 2 * ((size_t)kx + (size_t)NH * ((size_t)a + (size_t)N * b));
+          line[2 * b] = out[o];
+          line[2 * b + 1] = out[o + 1];
+        }
+        cfft_fwd(N, line);
+        for (int b = 0; b < N; b++) {
+          const size_t o = pass == 0 ? 2 * ((size_t)kx + (size_t)NH * ((size_t)b + (size_t)N * a))
+                                     : 2 * ((size_t)kx + (size_t)NH * ((size_t)a + (size_t)N * b));
+          out[o] = line[2 * b];
+          out[o + 1] = line[2 * b + 1];
+        }
+      }
+  free(line);
+}
+
+/* lpt_1 on one rank: the displacement i k delta_k / (N^3 k^2) of the
+ * Gaussian field dg (padded real), particles at the cell corners moved by
+ * D(r) psi (positions in float, wrapped into [0, L)), deposited with NGP,
+ * CIC or TSC (pos_2_ngp/cic/tsc, float weights) and out = rho - 1 (padded
+ * real, padding 0).  pos (3 N^3 floats, optional) receives the positions. */
+int orc_lpt1(int N, double L, const double obs[3], const orc_bg *bg, const double *dg, int interp, double *out,
+             float *pos) {
+  if (interp < 0 || interp > 2) return 1;
+  const int NH = N / 2 + 1, ngx = 2 * NH, na = bg->na;
+  const size_t nh = (size_t)N * N * NH, ng = (size_t)N * N * ngx;
+  double *H = malloc(2 * nh * sizeof(double)), *S = malloc(2 * nh * sizeof(double));
+  double *disp = malloc(3 * ng * sizeof(double));
+  orc_r2c_3d(N, dg, H);
+  const double dk = 2 * M_PI / L, fftnorm = (double)N * ((long)N * N);
+  for (int ax = 0; ax < 3; ax++) {
+    for (int ii = 0; ii < N; ii++) {
+      const double kz = 2 * ii <= N ? ii * dk : -(N - ii) * dk;
+      for (int jj = 0; jj < N; jj++) {
+        const double ky = 2 * jj <= N ? jj * dk : -(N - jj) * dk;
+        for (int kk = 0; kk < NH; kk++) {
+          const double kx = kk * dk;
+          const size_t o = 2 * ((size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * ii));
+          const double km2 = fftnorm * (kx * kx + ky * ky + kz * kz);
+          const double kv = ax == 0 ? kx : (ax == 1 ? ky : kz);
+          if (km2 <= 0) { S[o] = S[o + 1] = 0; continue; }
+          /* i kv (re + i im) / km2 */
+          S[o] = -kv * H[o + 1] / km2;
+          S[o + 1] = kv * H[o] / km2;
+        }
+      }
+    }
+    orc_c2r_3d(N, S, disp + ax * ng);
+  }
+  for (size_t i = 0; i < ng; i++) out[i] = 0;
+  const float dxf = (float)(L / N), iag = (float)(N / L);
+  for (int iz = 0; iz < N; iz++)
+    for (int iy = 0; iy < N; iy++)
+      for (int ix = 0; ix < N; ix++) {
+        const size_t idx = (size_t)ix + (size_t)ngx * ((size_t)iy + (size_t)N * iz);
+        float xv[3];
+        xv[0] = (float)((ix + 0.0) * dxf - obs[0]);
+        xv[1] = (float)((iy + 0.0) * dxf - obs[1]);
+        xv[2] = (float)((iz + 0.0) * dxf - obs[2]);
+        const double r = sqrt((double)(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]));
+        const double d1 = orc_f_of_r(bg, bg->d1, 1, bg->d1[na - 1], r);
+        float p[3];
+        for (int ax = 0; ax < 3; ax++) {
+          float q = (float)(xv[ax] + d1 * (float)disp[ax * ng + idx] + obs[ax]);
+          if (q < 0) q = (float)(q + L);
+          if (q >= L) q = (float)(q - L);
+          p[ax] = q;
+        }
+        if (pos) {
+          float *pp = pos + 3 * ((size_t)ix + (size_t)N * ((size_t)iy + (size_t)N * iz));
+          pp[0] = p[0]; pp[1] = p[1]; pp[2] = p[2];
+        }
+        if (interp == 0) {
+          int i0[3];
+          for (int ax = 0; ax < 3; ax++) {
+            i0[ax] = (int)((double)(p[ax] * iag) + 0.5);
+            if (i0[ax] >= N) i0[ax] -= N;
+            if (i0[ax] < 0) i0[ax] += N;
+          }
+          out[i0[0] + ngx * ((size_t)i0[1] + (size_t)N * i0[2])] += 1.;
+        } else if (interp == 1) {
+          int i0[3], i1[3];
+          float a0[3], a1[3];
+          for (int ax = 0; ax < 3; ax++) {
+            i0[ax] = (int)(p[ax] * iag);
+            a1[ax] = p[ax] * iag - i0[ax];
+            a0[ax] = 1 - a1[ax];
+            i1[ax] = i0[ax] + 1;
+            if (i0[ax] < 0) i0[ax] += N;
+            if (i1[ax] < 0) i1[ax] += N;
+            if (i0[ax] >= N) i0[ax] -= N;
+            if (i1[ax] >= N) i1[ax] -= N;
+          }
+          for (int c = 0; c < 8; c++) {
+            const int bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+            const float w = (bx ? a1[0] : a0[0]) * (by ? a1[1] : a0[1]) * (bz ? a1[2] : a0[2]);
+            out[(bx ? i1[0] : i0[0]) + ngx * ((size_t)(by ? i1[1] : i0[1]) + (size_t)N * (bz ? i1[2] : i0[2]))] += w;
+          }
+        } else {
+          int ic[3][3];
+          float wt[3][3];
+          for (int ax = 0; ax < 3; ax++) {
+            const int i0 = (int)floorf((float)((double)(p[ax] * iag) + 0.5));
+            const float a = p[ax] * iag - i0;
+            wt[ax][0] = (float)(0.5 * (0.5 - a) * (0.5 - a));
+            wt[ax][2] = (float)(0.5 * (0.5 + a) * (0.5 + a));
+            wt[ax][1] = (float)(0.75 - a * a);
+            for (int s = 0; s < 3; s++) {
+              int v = i0 + s - 1;
+              if (v < 0) v += N;
+              if (v >= N) v -= N;
+              ic[ax][s] = v;
+            }
+          }
+          for (int sz = 0; sz < 3; sz++)
+            for (int sy = 0; sy < 3; sy++)
+              for (int sx = 0; sx < 3; sx++)
+                out[ic[0][sx] + ngx * ((size_t)ic[1][sy] + (size_t)N * ic[2][sz])] +=
+                    wt[0][sx] * wt[1][sy] * wt[2][sz];
+        }
+      }
+  for (int iz = 0; iz < N; iz++)
+    for (int iy = 0; iy < N; iy++)
+      for (int ix = 0; ix < N; ix++) {
+        const size_t idx = (size_t)ix + (size_t)ngx * ((size_t)iy + (size_t)N * iz);
+        out[idx] = out[idx] - 1.;
+      }
+  free(H); free(S); free(disp);
+  return 0;
+}
+
 /* get_r_index_imap (imap.c:76-103), walk bounded to 2 nr steps */
 static int r_index_imap(int nr, const double *r0, const double *rf, double r, int start) {
   int ir0 = start < 0 ? 0 : (start >= nr ? nr - 1 : start);

@@ -558,6 +558,74 @@ def test_custom_map_closed_form(L):
     assert np.all(orc.cstm(N, Lb, dens, np.zeros_like(r), bz, norm, 1.0, 1.0, 2, nside) == 0)
 
 
+def test_r2c_matches_numpy(L):
+    N = 12
+    rng = np.random.default_rng(4)
+    g = np.zeros((N, N, 2 * (N // 2 + 1)))
+    g[:, :, :N] = rng.normal(size=(N, N, N))
+    out = np.zeros(2 * N * N * (N // 2 + 1))
+    L.orc_r2c_3d(N, p(g), p(out))
+    ref = np.fft.rfftn(g[:, :, :N])
+    np.testing.assert_allclose(out[0::2] + 1j * out[1::2], ref.ravel(), atol=1e-10)
+
+
+def _deposit_numpy(N, pos, L, interp):
+    """NGP / CIC / TSC assignment of unit masses at pos (float32 [n, 3]) from
+    the kernels' definitions, with np.add.at."""
+    x = pos.astype(np.float32) * np.float32(N / L)
+    rho = np.zeros((N, N, N))
+    if interp == 0:
+        i = np.floor(x.astype(np.float64) + 0.5).astype(int) % N
+        np.add.at(rho, (i[:, 2], i[:, 1], i[:, 0]), 1.0)
+        return rho
+    if interp == 1:
+        i0 = np.floor(x).astype(int)
+        t = (x - i0).astype(np.float64)
+        offs, w = [0, 1], [1 - t, t]
+    else:
+        i0 = np.floor(x.astype(np.float64) + 0.5).astype(int)
+        t = (x - i0).astype(np.float64)
+        offs, w = [-1, 0, 1], [0.5 * (0.5 - t) ** 2, 0.75 - t * t, 0.5 * (0.5 + t) ** 2]
+    for a, oz in enumerate(offs):
+        for b, oy in enumerate(offs):
+            for c, ox in enumerate(offs):
+                np.add.at(rho, ((i0[:, 2] + oz) % N, (i0[:, 1] + oy) % N, (i0[:, 0] + ox) % N),
+                          w[c][:, 0] * w[b][:, 1] * w[a][:, 2])
+    return rho
+
+
+@pytest.mark.parametrize("interp", [0, 1, 2])
+def test_lpt1_closed_form(interp):
+    """delta_G = A cos(2 pi m x / L) and D = 1: the 1LPT displacement is
+    psi_x = -(A L / (2 pi m)) sin(2 pi m x / L), psi_y = psi_z = 0, so the
+    particle of cell i sits at i dx + psi; with A = 0 every scheme returns
+    delta = 0 exactly; with A != 0 the density is the NGP/CIC/TSC assignment of
+    those positions (np.add.at from the kernels' definitions) minus 1, and the
+    mass sums to N^3 (to the float weights' rounding)."""
+    ta = _tables()
+    ta["d1"] = np.ones_like(ta["d1"])
+    orc = Oracle(ta)
+    N, Lb, m = 16, 400.0, 2
+    dx = Lb / N
+    g = np.zeros((N, N, 2 * (N // 2 + 1)))
+    zero, _ = orc.lpt1(N, Lb, g, interp)
+    assert np.all(zero == 0)
+    A = 1.0
+    ix = np.arange(N)
+    g[:, :, :N] = A * np.cos(2 * np.pi * m * ix / N)[None, None, :]
+    d, pos = orc.lpt1(N, Lb, g, interp, with_pos=True)
+    k0 = 2 * np.pi * m / Lb
+    want_x = ix * dx - (A / k0) * np.sin(k0 * ix * dx)
+    np.testing.assert_allclose(pos[:, :, :, 0], np.broadcast_to(want_x % Lb, (N, N, N)), atol=2e-4 * dx)
+    iz, iy = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    np.testing.assert_allclose(pos[:, :, :, 1], np.broadcast_to((iy * dx)[:, :, None], (N, N, N)), atol=1e-4)
+    np.testing.assert_allclose(pos[:, :, :, 2], np.broadcast_to((iz * dx)[:, :, None], (N, N, N)), atol=1e-4)
+    assert abs(d[:, :, :N].sum()) < 1e-6 * N ** 3   # float weights
+    rho = _deposit_numpy(N, pos.reshape(-1, 3), Lb, interp)
+    np.testing.assert_allclose(d[:, :, :N], rho - 1, atol=1e-6)
+    assert np.abs(d[:, :, :N]).max() > 0.05
+
+
 def test_imap_constant_temperature():
     """T = const, b = 0, norm = 1, no RSD: every painted pixel averages to T."""
     ta = _tables()
