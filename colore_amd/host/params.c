@@ -152,8 +152,12 @@ clr_run *clr_run_read(const char *fname, char *err, int errlen) {
   clr_cfg_free(cfg);
   cfg = NULL;
 
-  if (r->dens_type != CLR_DENS_LOGNORMAL && r->dens_type != CLR_DENS_CLIP) {
-    snprintf(err, errlen, "Density type %d not supported (lognormal 0 and clipped 3 are)", r->dens_type);
+  if (r->dens_type != CLR_DENS_LOGNORMAL && r->dens_type != CLR_DENS_CLIP && r->dens_type != 1) {
+    snprintf(err, errlen, "Density type %d not supported (lognormal 0, 1LPT 1 and clipped 3 are)", r->dens_type);
+    goto fail;
+  }
+  if (r->dens_type == 1 && (r->lpt_interp_type < 0 || r->lpt_interp_type > 2)) {
+    snprintf(err, errlen, "Wrong interpolation type");
     goto fail;
   }
   const int nsides[3] = {r->nside_kappa, r->nside_isw, 0};

@@ -78,7 +78,14 @@ int clr_run_execute(clr_run *r, int device) {
   /* compute_physical_density_field */
   printf("*** Creating physical matter density\n");
   printf("Lognormalizin'\n");
-  CHK(clr_density(ctx, r->dens_type, sigma2));
+  if (r->dens_type == 1) {
+    /* lpt_1 (density.c:376-645) */
+    printf(" 1LPT\n");
+    if (r->output_lpt) fprintf(stderr, "MPI task 0, Warning: LPT particle positions (output_lpt) are not written by this build\n");
+    CHK(clr_lpt_density(ctx, 1, r->lpt_interp_type));
+  } else {
+    CHK(clr_density(ctx, r->dens_type, sigma2));
+  }
   stage_report(ctx, 3);
   printf("\n");
   if (r->output_density && dump_grid(ctx, r, "lightcone")) fprintf(stderr, "MPI task 0, Warning: grid dump failed\n");

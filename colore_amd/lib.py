@@ -82,6 +82,7 @@ def load_amd():
     _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_source_beams", ci, [_vp, ci, _d, _f, _f, ctypes.c_longlong])
     _sig(lib, "clr_custom_map", ci, [_vp, ci, ci, _f])
+    _sig(lib, "clr_lpt_density", ci, [_vp, ci, ci])
     _sig(lib, "clr_source_skewers", ci, [_vp, ci, ci, ctypes.c_double, _f, _f, ctypes.c_longlong])
     _sig(lib, "clr_imap", ci, [_vp, ci, ci, ci, _d, _d, _f, _i32])
     _sig(lib, "clr_fetch_grid", ci, [_vp, ci, _f])
@@ -305,6 +306,10 @@ class Context:
         self._chk(self.lib.clr_source_skewers(self.h, ipop, int(gaussian), float(sigma2), ptr(dg, ctypes.c_float),
                                               ptr(vs, ctypes.c_float), n))
         return dg[:n], vs[:n]
+
+    def lpt_density(self, interp, order=1):
+        """1LPT density (rho - 1) in the density grid, replacing clr_density."""
+        self._chk(self.lib.clr_lpt_density(self.h, order, interp))
 
     def custom_map(self, ipop, nside):
         """Custom projected map ipop (NEST, float32[12 nside^2])."""

@@ -124,6 +124,18 @@ int clr_gaussian_fields(clr_ctx *ctx, uint32_t seed, double r_smooth,
  * (density.c:1070-1103) or densclip (density.c:1034-1067), in place. */
 int clr_density(clr_ctx *ctx, int dens_type, double sigma2);
 
+/* The Lagrangian alternative of clr_density (dens_type 1, density.c:376-645
+ * lpt_1): the displacement i k delta_k / (N^3 k^2) of the Gaussian field,
+ * one particle per cell moved by D(r) psi and assigned to the grid with
+ * interp CLR_INTERP_NGP, _CIC or _TSC (lpt_interp_type, pos_2_dens,
+ * density.c:37-189); the grid then holds rho - 1.  order must be 1 (2LPT,
+ * dens_type 2, returns CLR_ERR_ARG).  Needs three grid-sized buffers of
+ * device memory for the call. */
+#define CLR_INTERP_NGP 0
+#define CLR_INTERP_CIC 1
+#define CLR_INTERP_TSC 2
+int clr_lpt_density(clr_ctx *ctx, int order, int interp);
+
 /* compute_density_normalization (density.c:1227-1393).  Computes the FP64
  * per-shell sums on the device and builds each tracer's NA norm table on the
  * device.  Optional outputs (may be NULL): nz_out, counts[nz], zsum[nz],

@@ -301,6 +301,34 @@ def test_custom_map(workdir, fused):
     assert np.abs(out - o).max() <= 1e-5 * rms, (np.abs(out - o).max(), rms)
 
 
+@pytest.mark.parametrize("interp", [0, 1, 2])
+def test_lpt_density(workdir, interp):
+    """1LPT density from a Gaussian field on the device against the oracle's
+    FP64 transforms and deposit of the same field: CIC and TSC within 1e-4
+    (FP32 FFT and atomics), NGP counts equal but in at most 1e-4 of the cells
+    (a particle within rounding of a cell boundary), the same mass.  The
+    field is the generated one times 25, so that displacements cross cells."""
+    N = 64
+    path, run = make_run("C1", N, workdir, with_maps=False)
+    ctx = _ctx_for(run, N)
+    s2 = ctx.gaussian_fields(int(run.scalar("seed")), run.scalar("r_smooth"), True)
+    g = ctx.fetch_grid(0) * np.float32(25)
+    ctx.upload_grid(0, g)
+    ctx.declare_grids(s2, False)
+    dg = g.astype(np.float64)
+    ctx.lpt_density(interp)
+    d = ctx.fetch_grid(0)[:, :, :N].astype(np.float64)
+    o = _oracle(run).lpt1(N, run.scalar("l_box"), dg, interp)[0][:, :, :N]
+    assert abs(d.sum()) < 1e-3 * N ** 3 and abs(o.sum()) < 1e-6 * N ** 3
+    assert np.abs(o).max() > 0.1
+    if interp == 0:
+        diff = np.abs(d - o)
+        assert np.all((diff < 1e-6) | (np.abs(diff - np.round(diff)) < 1e-6))
+        assert np.count_nonzero(diff > 0.5) <= 1e-4 * N ** 3 + 2
+    else:
+        assert np.abs(d - o).max() < 1e-4, np.abs(d - o).max()
+
+
 def test_sources_all_populations_one_pass(workdir):
     """clr_sample_sources(ipop=-1) gives the catalogues of the per-population
     calls (same streams, same order)."""
