@@ -477,3 +477,31 @@ def test_cli_lensing_catalogue(tmp_path):
             np.testing.assert_array_equal(np.frombuffer(dd, dtype=">f4").reshape(n, N // 2), dg)
             np.testing.assert_array_equal(np.frombuffer(dv, dtype=">f4").reshape(n, N // 2), vs)
             assert abs(float(hb["SIGMA_G"]) - np.sqrt(s2)) < 1e-6 * np.sqrt(s2)
+
+
+def test_cli_lpt(tmp_path):
+    """`CoLoRe param.cfg` with dens_type = 1 (1LPT, CIC): the run completes and
+    its lightcone density dump holds the library's 1LPT density of the same
+    seed (FP32 atomics: equal to 1e-5)."""
+    import subprocess
+    from conftest import ROOT
+    from colore_amd import inputs, lib
+    N = 64
+    p = inputs.write_config("C1", str(tmp_path), n_grid=N, dens_type=1, output_density=True, with_maps=False)
+    exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
+    r = subprocess.run([exe, p], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "1LPT" in r.stdout
+    raw = open(tmp_path / "out_dens_lightcone_0.dat", "rb").read()
+    hdr = np.frombuffer(raw[:8], dtype=np.int32)
+    assert hdr[1] == 4
+    n_grid, nz = np.frombuffer(raw[16:28], dtype=np.int32)[:2]
+    assert n_grid == N and nz == N
+    grid = np.frombuffer(raw[28:], dtype=np.float32).reshape(N, N, N)
+    run = lib.Run(p)
+    ctx = _ctx_for(run, N)
+    ctx.gaussian_fields(int(run.scalar("seed")), run.scalar("r_smooth"), True)
+    ctx.lpt_density(1)
+    d = ctx.fetch_grid(0)[:, :, :N]
+    np.testing.assert_allclose(grid, d, rtol=0, atol=1e-5)
+    assert np.abs(d).max() > 0.01
