@@ -82,9 +82,10 @@ def traffic_per_launch(kernel, config, N):
 
 
 def time_maps(run, ctx, config, sigma2):
-    """kappa and ISW at the workload's source planes and the 21cm shells of its
-    first intensity species, on the last step's grids: device ms per call
-    (median of 3) and work rates."""
+    """kappa and ISW at the workload's source planes, the first population's
+    line-of-sight pass and the 21cm shells of the first intensity species, on
+    the last step's grids, then a 1LPT (CIC) density of a new realisation:
+    device ms per call (median of 3; best of 2 for 1LPT) and work rates."""
     import numpy as np
     from colore_amd import lib
     out = {}
@@ -126,6 +127,14 @@ def time_maps(run, ctx, config, sigma2):
             ts.append(ctx.stage_times()["imap"])
         ms = sorted(ts)[1]
         out["imap"] = {"ms": ms, "nside": nside, "shells": len(r0), "cells_per_s": N ** 3 / (ms * 1e-3)}
+    # 1LPT density (CIC) of a fresh realisation, in place of the lognormal transform
+    ts = []
+    for _ in range(2):
+        ctx.gaussian_fields(int(run.scalar("seed")), run.scalar("r_smooth"), True)
+        ctx.lpt_density(1)
+        ts.append(ctx.stage_times()["density"])
+    ms = min(ts)
+    out["lpt1_cic"] = {"ms": ms, "cells_per_s": N ** 3 / (ms * 1e-3)}
     return out
 
 
