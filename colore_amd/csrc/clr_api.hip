@@ -63,7 +63,7 @@ int clr_ring2nest_table(clr_ctx *c, int nside) {
 static void free_all(clr_ctx *c) {
   auto f = [](void *p) { if (p) hipFree(p); };
   f(c->dens); f(c->npot); f(c->bg); f(c->pk); f(c->tw); f(c->amp); f(c->red); f(c->red_out);
-  f(c->rowcnt); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d1f); f(c->r2n);
+  f(c->rowcnt); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d2); f(c->d1f); f(c->r2n);
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < CLR_NPOP_MAX; i++) {
       f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].ptab);
@@ -169,6 +169,14 @@ int clr_upload_tables(clr_ctx *c, const clr_tables *t) {
   for (int i = 0; i < 6; i++) {
     c->h_bg_first[i] = src[i][0];
     c->h_bg_last[i] = src[i][na - 1];
+  }
+  if (c->d2) hipFree(c->d2);
+  c->d2 = nullptr;
+  if (t->growth_d2_arr) {
+    CLR_HIP(c, hipMalloc((void **)&c->d2, na * sizeof(double)));
+    CLR_HIP(c, hipMemcpy(c->d2, t->growth_d2_arr, na * sizeof(double), hipMemcpyHostToDevice));
+    c->d2_first = t->growth_d2_arr[0];
+    c->d2_last = t->growth_d2_arr[na - 1];
   }
   // get_bg(L/2, BG_Z): the top of the normalisation shells (density.c:1233)
   c->zmax_half_box = clr::f_of_r(t->r_arr_r2z, t->z_arr_r2z, t->glob_idr, c->r_last, 0.5 * c->L, 0.0,

@@ -95,6 +95,8 @@ struct clr_ctx {
   size_t hp_n = 0;
   void *esc = nullptr;      // sources: escape list of saturated count bytes (+ counter)
   size_t esc_bytes = 0;
+  double *d2 = nullptr;     // D2(r) table of 2LPT (NA), when the tables carry it
+  double d2_first = 0, d2_last = 0;
   void *misc = nullptr;     // generic scratch
   size_t misc_n = 0;
 

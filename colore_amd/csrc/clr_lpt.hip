@@ -1,15 +1,17 @@
-// clr_lpt.hip -- first-order Lagrangian perturbation theory density
-// (lpt_1, density.c:376-645, with pos_2_ngp / pos_2_cic / pos_2_tsc,
-// density.c:37-175) on gfx950.
+// clr_lpt.hip -- Lagrangian perturbation theory densities (lpt_1,
+// density.c:376-645, and lpt_2, density.c:646-1031, with pos_2_ngp /
+// pos_2_cic / pos_2_tsc, density.c:37-175) on gfx950.
 //
 // The Gaussian field in the density grid is transformed forward (r2c, in
 // place on the padded rows), each displacement component i k_a delta_k /
 // (N^3 k^2) is written to its own half spectrum and transformed back (c2r, in
-// place); the three FFT kinds are rocFFT's (through hipFFT): this is a
+// place); 2LPT adds the six digradients, the second-order source and its
+// displacement.  The FFTs are rocFFT's (through hipFFT): this is a
 // secondary path, not the per-step hot path.  One thread per grid cell then
-// moves the cell's particle by D(r) psi (positions in FP32 as the reference
-// stores them, wrapped into the box) and adds its NGP, CIC or TSC weights to
-// the zeroed density grid with FP32 atomics; a last pass subtracts 1.
+// computes the cell's particle position (FP32 as the reference stores it,
+// wrapped into the box), one thread per particle adds its NGP, CIC or TSC
+// weights to the zeroed density grid with FP32 atomics, and a last pass
+// subtracts 1.
 #include "clr_internal.h"
 #include <hipfft/hipfft.h>
 #include <math.h>
@@ -45,13 +47,90 @@ __global__ __launch_bounds__(256) void k_lpt_disp(const float2 *__restrict__ dk,
   p2[o] = make_float2((float)(-kz * d.y / km2), (float)(kz * d.x / km2));
 }
 
-struct MoveArgs {
-  const float *ps0, *ps1, *ps2;
-  float *dens;
-  const double *r_arr, *d1;
-  double idr, r_last, d1_last, ox, oy, oz, L;
-  float dxf, iag;
-  int N, ngx, interp;
+// 2LPT (density.c:683-722): the displacement spectra and the six
+// "digradient" spectra i k_b psi_a of the stored (FP32) psi_a, for
+// (a,b) = xx, xy, xz, yy, yz, zz; psi_z is written over delta_k itself
+__global__ __launch_bounds__(256) void k_lpt2_grad(float2 *__restrict__ dk, float2 *__restrict__ p0,
+                                                   float2 *__restrict__ p1, float2 *__restrict__ g0,
+                                                   float2 *__restrict__ g1, float2 *__restrict__ g2,
+                                                   float2 *__restrict__ g3, float2 *__restrict__ g4,
+                                                   float2 *__restrict__ g5, int N, int nhp, double dkf,
+                                                   double fftnorm) {
+#pragma clang fp contract(off)
+  const int NH = N / 2 + 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)N * N * NH) return;
+  const int kk = (int)(t % NH);
+  const long long rw = t / NH;
+  const int jj = (int)(rw % N), ii = (int)(rw / N);
+  const size_t o = (size_t)kk + (size_t)nhp * ((size_t)jj + (size_t)N * ii);
+  const double kv[3] = {kk * dkf, 2 * jj <= N ? jj * dkf : -(N - jj) * dkf, 2 * ii <= N ? ii * dkf : -(N - ii) * dkf};
+  const double km2 = fftnorm * (kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+  const float2 d = dk[o];
+  float2 ps[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++)
+    ps[ax] = km2 <= 0 ? make_float2(0.f, 0.f)
+                      : make_float2((float)(-kv[ax] * d.y / km2), (float)(kv[ax] * d.x / km2));
+  // i kb (re + i im) = (-kb im, kb re)
+  auto ik = [](double kb, float2 v) { return make_float2((float)(-kb * v.y), (float)(kb * v.x)); };
+  p0[o] = ps[0];
+  p1[o] = ps[1];
+  dk[o] = ps[2];
+  g0[o] = ik(kv[0], ps[0]);
+  g1[o] = ik(kv[1], ps[0]);
+  g2[o] = ik(kv[2], ps[0]);
+  g3[o] = ik(kv[1], ps[1]);
+  g4[o] = ik(kv[2], ps[1]);
+  g5[o] = ik(kv[2], ps[2]);
+}
+
+// density.c:732-752: the second-order source xx yy + xx zz + yy zz - xy^2 -
+// xz^2 - yz^2 (FP32, the reference's order), into g5 on the real cells
+__global__ __launch_bounds__(256) void k_lpt2_source(const float *__restrict__ g0, const float *__restrict__ g1,
+                                                     const float *__restrict__ g2, const float *__restrict__ g3,
+                                                     const float *__restrict__ g4, float *__restrict__ g5, int N,
+                                                     int ngx) {
+#pragma clang fp contract(off)
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)N * N * N) return;
+  const int ix = (int)(t % N);
+  const size_t idx = (size_t)ix + (size_t)ngx * (size_t)(t / N);
+  const float xx = g0[idx], xy = g1[idx], xz = g2[idx], yy = g3[idx], yz = g4[idx], zz = g5[idx];
+  g5[idx] = xx * yy + xx * zz + yy * zz - xy * xy - xz * xz - yz * yz;
+}
+
+// density.c:778-805: psi2_a = -i k_a S_k / (N^3 k^2)
+__global__ __launch_bounds__(256) void k_lpt2_disp2(const float2 *__restrict__ sk, float2 *__restrict__ q0,
+                                                    float2 *__restrict__ q1, float2 *__restrict__ q2, int N,
+                                                    int nhp, double dkf, double fftnorm) {
+#pragma clang fp contract(off)
+  const int NH = N / 2 + 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)N * N * NH) return;
+  const int kk = (int)(t % NH);
+  const long long rw = t / NH;
+  const int jj = (int)(rw % N), ii = (int)(rw / N);
+  const size_t o = (size_t)kk + (size_t)nhp * ((size_t)jj + (size_t)N * ii);
+  const double kv[3] = {kk * dkf, 2 * jj <= N ? jj * dkf : -(N - jj) * dkf, 2 * ii <= N ? ii * dkf : -(N - ii) * dkf};
+  const double km2 = fftnorm * (kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+  const float2 s = sk[o];
+  float2 *q[3] = {q0, q1, q2};
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++)
+    // -i kv (re + i im) / km2 = (kv im, -kv re) / km2
+    q[ax][o] = km2 <= 0 ? make_float2(0.f, 0.f)
+                        : make_float2((float)(kv[ax] * s.y / km2), (float)(-kv[ax] * s.x / km2));
+}
+
+struct PosArgs {
+  const float *d0, *d1, *d2;      // first-order displacement
+  const float *s0, *s1, *s2;      // second-order displacement (2LPT) or null
+  float *p0, *p1, *p2;            // positions out (may alias d*)
+  const double *r_arr, *dd1, *dd2;
+  double idr, r_last, d1_last, d2_first, d2_last, ox, oy, oz, L;
+  float dxf;
+  int N, ngx;
 };
 
 // The reference wraps once (density.c:47-50, 80-85, 125-130), enough for
@@ -64,9 +143,9 @@ __device__ inline int wrapn(int v, int N) {
   return v;
 }
 
-// one thread per cell: the particle's position (density.c:481-505) and its
-// mass assignment (density.c:37-175)
-__global__ __launch_bounds__(256) void k_lpt_move(MoveArgs a) {
+// one thread per cell: the particle's position x + D1 psi1 (+ D2 psi2) +
+// x_obs, FP32, wrapped into [0, L) (density.c:481-505, 852-875)
+__global__ __launch_bounds__(256) void k_lpt_pos(PosArgs a) {
 #pragma clang fp contract(off)
   const int N = a.N;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,39 +153,59 @@ __global__ __launch_bounds__(256) void k_lpt_move(MoveArgs a) {
   const int ix = (int)(t % N);
   const long long rw = t / N;
   const int iy = (int)(rw % N), iz = (int)(rw / N);
-  const size_t ngx = (size_t)a.ngx;
-  const size_t idx = (size_t)ix + ngx * ((size_t)iy + (size_t)N * iz);
+  const size_t idx = (size_t)ix + (size_t)a.ngx * (size_t)rw;
   float xv[3];
   xv[0] = (float)((ix + 0.0) * (double)a.dxf - a.ox);
   xv[1] = (float)((iy + 0.0) * (double)a.dxf - a.oy);
   xv[2] = (float)((iz + 0.0) * (double)a.dxf - a.oz);
   const float r2 = xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2];
-  const double dg = f_of_r(a.r_arr, a.d1, a.idr, a.r_last, sqrt((double)r2), 1.0, a.d1_last);
+  const double r = sqrt((double)r2);
+  const double dg = f_of_r(a.r_arr, a.dd1, a.idr, a.r_last, r, 1.0, a.d1_last);
   const double obs[3] = {a.ox, a.oy, a.oz};
-  const float disp[3] = {a.ps0[idx], a.ps1[idx], a.ps2[idx]};
+  const float d[3] = {a.d0[idx], a.d1[idx], a.d2[idx]};
   float p[3];
+  if (a.s0) {
+    const double d2g = f_of_r(a.r_arr, a.dd2, a.idr, a.r_last, r, a.d2_first, a.d2_last);
+    const float s[3] = {a.s0[idx], a.s1[idx], a.s2[idx]};
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) p[ax] = (float)(xv[ax] + dg * d[ax] + d2g * s[ax] + obs[ax]);
+  } else {
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) p[ax] = (float)(xv[ax] + dg * d[ax] + obs[ax]);
+  }
 #pragma unroll
   for (int ax = 0; ax < 3; ax++) {
-    float q = (float)(xv[ax] + dg * disp[ax] + obs[ax]);
-    if (q < 0) q = (float)(q + a.L);
-    if (q >= a.L) q = (float)(q - a.L);
-    p[ax] = q;
+    if (p[ax] < 0) p[ax] = (float)(p[ax] + a.L);
+    if (p[ax] >= a.L) p[ax] = (float)(p[ax] - a.L);
   }
-  float *__restrict__ g = a.dens;
+  a.p0[idx] = p[0];
+  a.p1[idx] = p[1];
+  a.p2[idx] = p[2];
+}
+
+// one thread per particle: NGP, CIC or TSC weights (density.c:37-175) added
+// to the zeroed grid with FP32 atomics
+__global__ __launch_bounds__(256) void k_lpt_deposit(const float *__restrict__ px, const float *__restrict__ py,
+                                                     const float *__restrict__ pz, float *__restrict__ g, int N,
+                                                     int ngxi, float iag, int interp) {
+#pragma clang fp contract(off)
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)N * N * N) return;
+  const size_t ngx = (size_t)ngxi;
+  const size_t idx = (size_t)(t % N) + ngx * (size_t)(t / N);
+  const float p[3] = {px[idx], py[idx], pz[idx]};
   const size_t pl = (size_t)N * ngx;
-  if (a.interp == 0) {
+  if (interp == 0) {
     int i0[3];
 #pragma unroll
-    for (int ax = 0; ax < 3; ax++) {
-      i0[ax] = wrapn((int)((double)(p[ax] * a.iag) + 0.5), N);
-    }
+    for (int ax = 0; ax < 3; ax++) i0[ax] = wrapn((int)((double)(p[ax] * iag) + 0.5), N);
     atomicAdd(g + i0[0] + ngx * i0[1] + pl * i0[2], 1.0f);
-  } else if (a.interp == 1) {
+  } else if (interp == 1) {
     int i0[3], i1[3];
     float a0[3], a1[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
-      const float xs = p[ax] * a.iag;
+      const float xs = p[ax] * iag;
       const int c = (int)xs;
       a1[ax] = xs - c;
       a0[ax] = 1 - a1[ax];
@@ -124,7 +223,7 @@ __global__ __launch_bounds__(256) void k_lpt_move(MoveArgs a) {
     float wt[3][3];
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
-      const float xs = p[ax] * a.iag;
+      const float xs = p[ax] * iag;
       const int i0 = (int)floorf((float)((double)xs + 0.5));
       const float d = xs - i0;
       wt[ax][0] = (float)(0.5 * (0.5 - d) * (0.5 - d));
@@ -161,15 +260,20 @@ int fft_fail(clr_ctx *c, hipfftResult r, const char *what) {
 int clr_launch_lpt(clr_ctx *c, int order, int interp) {
   if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
   if (c->density_done) return clr_fail(c, CLR_ERR_STATE, "the density grid is already transformed");
-  if (order != 1) return clr_fail(c, CLR_ERR_ARG, "Density type %d not supported", order == 2 ? 2 : order);
+  if (order != 1 && order != 2) return clr_fail(c, CLR_ERR_ARG, "Density type %d not supported", order);
+  if (order == 2 && !c->d2) return clr_fail(c, CLR_ERR_STATE, "2LPT needs the D2 growth table (clr_tables.growth_d2_arr)");
   if (interp < 0 || interp > 2) return clr_fail(c, CLR_ERR_ARG, "Wrong interpolation type");
   const int N = c->N, ngx = c->ngx, nhp = ngx / 2, NH = N / 2 + 1;
   const size_t gbytes = (size_t)ngx * N * N * sizeof(float);
-  float *ps[3] = {nullptr, nullptr, nullptr};
-  for (int i = 0; i < 3; i++) {
+  // order 1: psi_x, psi_y, psi_z; order 2: psi_x, psi_y (psi_z over delta) and
+  // the six digradients, whose first three become psi2 and last three hold
+  // the positions
+  const int nbuf = order == 1 ? 3 : 8;
+  float *ps[8] = {nullptr};
+  for (int i = 0; i < nbuf; i++) {
     if (hipMalloc((void **)&ps[i], gbytes) != hipSuccess) {
       for (int j = 0; j < i; j++) (void)hipFree(ps[j]);
-      return clr_fail(c, CLR_ERR_HIP, "out of device memory for the displacement grids");
+      return clr_fail(c, CLR_ERR_HIP, "out of device memory for the LPT grids");
     }
   }
   hipfftHandle fwd = 0, inv = 0;
@@ -186,55 +290,101 @@ int clr_launch_lpt(clr_ctx *c, int order, int interp) {
   if (fr == HIPFFT_SUCCESS) fr = hipfftSetStream(fwd, c->stream);
   if (fr == HIPFFT_SUCCESS) fr = hipfftSetStream(inv, c->stream);
   hipError_t e = hipSuccess;
-  if (fr == HIPFFT_SUCCESS) e = hipEventRecord(c->ev0, c->stream);
-  if (fr == HIPFFT_SUCCESS && e == hipSuccess)
-    fr = hipfftExecR2C(fwd, (hipfftReal *)c->dens, (hipfftComplex *)c->dens);
-  if (fr == HIPFFT_SUCCESS && e == hipSuccess) {
-    const long long modes = (long long)N * N * NH;
-    hipLaunchKernelGGL(k_lpt_disp, dim3((unsigned)((modes + 255) / 256)), dim3(256), 0, c->stream,
-                       (const float2 *)c->dens, (float2 *)ps[0], (float2 *)ps[1], (float2 *)ps[2], N, nhp,
-                       2 * M_PI / c->L, (double)N * ((double)N * N));
-    e = hipGetLastError();
+  const long long modes = (long long)N * N * NH, cells = (long long)N * N * N;
+  const dim3 gm((unsigned)((modes + 255) / 256)), gc((unsigned)((cells + 255) / 256)), blk(256);
+  const double dkf = 2 * M_PI / c->L, fftnorm = (double)N * ((double)N * N);
+  auto ok = [&]() { return fr == HIPFFT_SUCCESS && e == hipSuccess; };
+  auto r2c = [&](float *g) { if (ok()) fr = hipfftExecR2C(fwd, (hipfftReal *)g, (hipfftComplex *)g); };
+  auto c2r = [&](float *g) { if (ok()) fr = hipfftExecC2R(inv, (hipfftComplex *)g, (hipfftReal *)g); };
+  if (ok()) e = hipEventRecord(c->ev0, c->stream);
+  r2c(c->dens);
+  const float *dsp[3], *sec[3] = {nullptr, nullptr, nullptr};
+  float *pos[3];
+  if (order == 1) {
+    if (ok()) {
+      hipLaunchKernelGGL(k_lpt_disp, gm, blk, 0, c->stream, (const float2 *)c->dens, (float2 *)ps[0],
+                         (float2 *)ps[1], (float2 *)ps[2], N, nhp, dkf, fftnorm);
+      e = hipGetLastError();
+    }
+    for (int i = 0; i < 3; i++) c2r(ps[i]);
+    for (int i = 0; i < 3; i++) dsp[i] = pos[i] = ps[i];
+  } else {
+    float **g = ps + 2;   // g[0..5]: xx, xy, xz, yy, yz, zz
+    if (ok()) {
+      hipLaunchKernelGGL(k_lpt2_grad, gm, blk, 0, c->stream, (float2 *)c->dens, (float2 *)ps[0], (float2 *)ps[1],
+                         (float2 *)g[0], (float2 *)g[1], (float2 *)g[2], (float2 *)g[3], (float2 *)g[4],
+                         (float2 *)g[5], N, nhp, dkf, fftnorm);
+      e = hipGetLastError();
+    }
+    for (int i = 0; i < 6; i++) c2r(g[i]);
+    if (ok()) {
+      hipLaunchKernelGGL(k_lpt2_source, gc, blk, 0, c->stream, g[0], g[1], g[2], g[3], g[4], g[5], N, ngx);
+      e = hipGetLastError();
+    }
+    r2c(g[5]);
+    if (ok()) {
+      hipLaunchKernelGGL(k_lpt2_disp2, gm, blk, 0, c->stream, (const float2 *)g[5], (float2 *)g[0], (float2 *)g[1],
+                         (float2 *)g[2], N, nhp, dkf, fftnorm);
+      e = hipGetLastError();
+    }
+    c2r(ps[0]);
+    c2r(ps[1]);
+    c2r(c->dens);
+    for (int i = 0; i < 3; i++) c2r(g[i]);
+    dsp[0] = ps[0];
+    dsp[1] = ps[1];
+    dsp[2] = c->dens;
+    for (int i = 0; i < 3; i++) {
+      sec[i] = g[i];
+      pos[i] = g[3 + i];
+    }
   }
-  for (int i = 0; i < 3 && fr == HIPFFT_SUCCESS && e == hipSuccess; i++)
-    fr = hipfftExecC2R(inv, (hipfftComplex *)ps[i], (hipfftReal *)ps[i]);
-  if (fr == HIPFFT_SUCCESS && e == hipSuccess) e = hipMemsetAsync(c->dens, 0, gbytes, c->stream);
-  if (fr == HIPFFT_SUCCESS && e == hipSuccess) {
-    MoveArgs a{};
-    a.ps0 = ps[0];
-    a.ps1 = ps[1];
-    a.ps2 = ps[2];
-    a.dens = c->dens;
+  if (ok()) {
+    PosArgs a{};
+    a.d0 = dsp[0];
+    a.d1 = dsp[1];
+    a.d2 = dsp[2];
+    a.s0 = sec[0];
+    a.s1 = sec[1];
+    a.s2 = sec[2];
+    a.p0 = pos[0];
+    a.p1 = pos[1];
+    a.p2 = pos[2];
     a.r_arr = c->bg;
-    a.d1 = c->bg + 2 * (size_t)c->na;
+    a.dd1 = c->bg + 2 * (size_t)c->na;
+    a.dd2 = c->d2;
     a.idr = c->glob_idr;
     a.r_last = c->r_last;
     a.d1_last = c->h_bg_last[2];
+    a.d2_first = c->d2_first;
+    a.d2_last = c->d2_last;
     a.ox = c->obs[0];
     a.oy = c->obs[1];
     a.oz = c->obs[2];
     a.L = c->L;
     a.dxf = (float)(c->L / N);
-    a.iag = (float)(N / c->L);
     a.N = N;
     a.ngx = ngx;
-    a.interp = interp;
-    const long long cells = (long long)N * N * N;
-    hipLaunchKernelGGL(k_lpt_move, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_lpt_pos, gc, blk, 0, c->stream, a);
     e = hipGetLastError();
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_lpt_norm, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, c->dens, N,
-                         ngx);
-      e = hipGetLastError();
-    }
   }
-  if (fr == HIPFFT_SUCCESS && e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+  if (ok()) e = hipMemsetAsync(c->dens, 0, gbytes, c->stream);
+  if (ok()) {
+    hipLaunchKernelGGL(k_lpt_deposit, gc, blk, 0, c->stream, pos[0], pos[1], pos[2], c->dens, N, ngx,
+                       (float)(N / c->L), interp);
+    e = hipGetLastError();
+  }
+  if (ok()) {
+    hipLaunchKernelGGL(k_lpt_norm, gc, blk, 0, c->stream, c->dens, N, ngx);
+    e = hipGetLastError();
+  }
+  if (ok()) e = hipEventRecord(c->ev1, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (fr != HIPFFT_SUCCESS) rc = fft_fail(c, fr, "plan or transform");
-  else if (e != hipSuccess) rc = clr_fail(c, CLR_ERR_HIP, "1LPT: %s", hipGetErrorString(e));
+  else if (e != hipSuccess) rc = clr_fail(c, CLR_ERR_HIP, "LPT: %s", hipGetErrorString(e));
   if (fwd) hipfftDestroy(fwd);
   if (inv) hipfftDestroy(inv);
-  for (int i = 0; i < 3; i++) (void)hipFree(ps[i]);
+  for (int i = 0; i < nbuf; i++) (void)hipFree(ps[i]);
   if (rc) return rc;
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);

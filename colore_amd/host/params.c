@@ -152,11 +152,11 @@ clr_run *clr_run_read(const char *fname, char *err, int errlen) {
   clr_cfg_free(cfg);
   cfg = NULL;
 
-  if (r->dens_type != CLR_DENS_LOGNORMAL && r->dens_type != CLR_DENS_CLIP && r->dens_type != 1) {
-    snprintf(err, errlen, "Density type %d not supported (lognormal 0, 1LPT 1 and clipped 3 are)", r->dens_type);
+  if (r->dens_type < 0 || r->dens_type > 3) {
+    snprintf(err, errlen, "Density type %d not supported", r->dens_type);
     goto fail;
   }
-  if (r->dens_type == 1 && (r->lpt_interp_type < 0 || r->lpt_interp_type > 2)) {
+  if ((r->dens_type == 1 || r->dens_type == 2) && (r->lpt_interp_type < 0 || r->lpt_interp_type > 2)) {
     snprintf(err, errlen, "Wrong interpolation type");
     goto fail;
   }
@@ -232,6 +232,7 @@ int clr_run_tables(const clr_run *r, clr_tables *t) {
   t->omega_m = r->OmegaM;
   t->prefac_lensing = r->prefac_lensing;
   t->r_max = r->r_max;
+  t->growth_d2_arr = r->growth_d2_arr;
   return 0;
 }
 

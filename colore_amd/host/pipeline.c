@@ -78,11 +78,11 @@ int clr_run_execute(clr_run *r, int device) {
   /* compute_physical_density_field */
   printf("*** Creating physical matter density\n");
   printf("Lognormalizin'\n");
-  if (r->dens_type == 1) {
-    /* lpt_1 (density.c:376-645) */
-    printf(" 1LPT\n");
+  if (r->dens_type == 1 || r->dens_type == 2) {
+    /* lpt_1 / lpt_2 (density.c:376-1031) */
+    printf(" %dLPT\n", r->dens_type);
     if (r->output_lpt) fprintf(stderr, "MPI task 0, Warning: LPT particle positions (output_lpt) are not written by this build\n");
-    CHK(clr_lpt_density(ctx, 1, r->lpt_interp_type));
+    CHK(clr_lpt_density(ctx, r->dens_type, r->lpt_interp_type));
   } else {
     CHK(clr_density(ctx, r->dens_type, sigma2));
   }

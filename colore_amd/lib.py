@@ -39,7 +39,7 @@ class Tables(ctypes.Structure):
         ("numk", ctypes.c_int), ("logkmin", ctypes.c_double), ("logkmax", ctypes.c_double),
         ("idlogk", ctypes.c_double), ("logkarr", _d), ("pkarr", _d), ("n_scal", ctypes.c_double),
         ("fgrowth_0", ctypes.c_double), ("hubble_0", ctypes.c_double), ("omega_m", ctypes.c_double),
-        ("prefac_lensing", ctypes.c_double), ("r_max", ctypes.c_double),
+        ("prefac_lensing", ctypes.c_double), ("r_max", ctypes.c_double), ("growth_d2_arr", _d),
     ]
 
 
@@ -162,7 +162,7 @@ class Run:
                     pk=g(t.pkarr, t.numk), glob_idr=t.glob_idr, numk=t.numk, logkmin=t.logkmin,
                     logkmax=t.logkmax, idlogk=t.idlogk, ns=t.n_scal, fgrowth_0=t.fgrowth_0,
                     hubble_0=t.hubble_0, omega_m=t.omega_m, prefac_lensing=t.prefac_lensing,
-                    r_max=t.r_max)
+                    r_max=t.r_max, d2=g(t.growth_d2_arr))
 
     def r_a2r(self):
         """r(a) at a = i/(NA-1): the table r_of_z interpolates (cosmo.c:101-112)."""
@@ -308,7 +308,7 @@ class Context:
         return dg[:n], vs[:n]
 
     def lpt_density(self, interp, order=1):
-        """1LPT density (rho - 1) in the density grid, replacing clr_density."""
+        """1LPT / 2LPT density (rho - 1) in the density grid, replacing clr_density."""
         self._chk(self.lib.clr_lpt_density(self.h, order, interp))
 
     def custom_map(self, ipop, nside):

@@ -97,6 +97,7 @@ typedef struct {
   double        omega_m;
   double        prefac_lensing;/* 1.5 H0^2 Om                             */
   double        r_max;         /* chi(z_max): sets the LOS step r_max/(N/2)*/
+  const double *growth_d2_arr; /* D2(r) of 2LPT (cosmo.c:689); NULL: none */
 } clr_tables;
 int clr_upload_tables(clr_ctx *ctx, const clr_tables *tab);
 
@@ -124,13 +125,15 @@ int clr_gaussian_fields(clr_ctx *ctx, uint32_t seed, double r_smooth,
  * (density.c:1070-1103) or densclip (density.c:1034-1067), in place. */
 int clr_density(clr_ctx *ctx, int dens_type, double sigma2);
 
-/* The Lagrangian alternative of clr_density (dens_type 1, density.c:376-645
- * lpt_1): the displacement i k delta_k / (N^3 k^2) of the Gaussian field,
- * one particle per cell moved by D(r) psi and assigned to the grid with
- * interp CLR_INTERP_NGP, _CIC or _TSC (lpt_interp_type, pos_2_dens,
- * density.c:37-189); the grid then holds rho - 1.  order must be 1 (2LPT,
- * dens_type 2, returns CLR_ERR_ARG).  Needs three grid-sized buffers of
- * device memory for the call. */
+/* The Lagrangian alternatives of clr_density: order 1 = dens_type 1
+ * (lpt_1, density.c:376-645), the displacement i k delta_k / (N^3 k^2) of
+ * the Gaussian field; order 2 = dens_type 2 (lpt_2, density.c:646-1031), plus
+ * D2(r) times the displacement -i k S_k / (N^3 k^2) of the second-order
+ * source S (needs clr_tables.growth_d2_arr).  One particle per cell is moved
+ * and assigned to the grid with interp CLR_INTERP_NGP, _CIC or _TSC
+ * (lpt_interp_type, pos_2_dens, density.c:37-189); the grid then holds
+ * rho - 1.  Needs 3 (order 1) or 8 (order 2) grid-sized buffers of device
+ * memory for the call. */
 #define CLR_INTERP_NGP 0
 #define CLR_INTERP_CIC 1
 #define CLR_INTERP_TSC 2

@@ -1100,21 +1100,90 @@ void orc_r2c_3d(int N, const double *in, double *out) {
   free(line);
 }
 
-/* lpt_1 on one rank: the displacement i k delta_k / (N^3 k^2) of the
- * Gaussian field dg (padded real), particles at the cell corners moved by
- * D(r) psi (positions in float, wrapped into [0, L)), deposited with NGP,
- * CIC or TSC (pos_2_ngp/cic/tsc, float weights) and out = rho - 1 (padded
- * real, padding 0).  pos (3 N^3 floats, optional) receives the positions. */
-int orc_lpt1(int N, double L, const double obs[3], const orc_bg *bg, const double *dg, int interp, double *out,
-             float *pos) {
-  if (interp < 0 || interp > 2) return 1;
+/* k vector of half-spectrum mode (kk, jj, ii) (density.c:405-426) */
+static void lpt_kvec(int N, double dk, int kk, int jj, int ii, double kv[3]) {
+  kv[0] = kk * dk;
+  kv[1] = 2 * jj <= N ? jj * dk : -(N - jj) * dk;
+  kv[2] = 2 * ii <= N ? ii * dk : -(N - ii) * dk;
+}
+
+/* lpt_1 / lpt_2 on one rank: the displacement i k delta_k / (N^3 k^2) of
+ * the Gaussian field dg (padded real) and, for order 2, the second-order
+ * displacement -i k S_k / (N^3 k^2) of S = xx yy + xx zz + yy zz - xy^2 -
+ * xz^2 - yz^2 (the digradients i k_b psi_a, density.c:714-752); particles at
+ * the cell corners moved by D1 psi (+ D2 psi2, d2 = the D2(r) table), positions
+ * in float wrapped into [0, L), deposited with NGP, CIC or TSC
+ * (pos_2_ngp/cic/tsc, float weights) and out = rho - 1 (padded real, padding
+ * 0).  Grids pass through float as the reference stores them.  pos (3 N^3
+ * floats, optional) receives the positions. */
+int orc_lpt(int order, int N, double L, const double obs[3], const orc_bg *bg, const double *d2, const double *dg,
+            int interp, double *out, float *pos) {
+  if (interp < 0 || interp > 2 || order < 1 || order > 2 || (order == 2 && !d2)) return 1;
   const int NH = N / 2 + 1, ngx = 2 * NH, na = bg->na;
   const size_t nh = (size_t)N * N * NH, ng = (size_t)N * N * ngx;
   double *H = malloc(2 * nh * sizeof(double)), *S = malloc(2 * nh * sizeof(double));
-  double *disp = malloc(3 * ng * sizeof(double));
+  double *disp = malloc(3 * ng * sizeof(double)), *sec = order == 2 ? calloc(3 * ng, sizeof(double)) : NULL;
   orc_r2c_3d(N, dg, H);
   const double dk = 2 * M_PI / L, fftnorm = (double)N * ((long)N * N);
-  for (int ax = 0; ax < 3; ax++) {
+  if (order == 2) {
+    /* the six digradients from the FP32-stored first-order spectra */
+    double *P = malloc(6 * nh * sizeof(double)), *G = malloc(6 * ng * sizeof(double));
+    for (int ii = 0; ii < N; ii++)
+      for (int jj = 0; jj < N; jj++)
+        for (int kk = 0; kk < NH; kk++) {
+          double kv[3];
+          lpt_kvec(N, dk, kk, jj, ii, kv);
+          const size_t o = 2 * ((size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * ii));
+          const double km2 = fftnorm * (kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+          for (int ax = 0; ax < 3; ax++) {
+            P[2 * ax * nh + o] = km2 <= 0 ? 0 : (float)(-kv[ax] * H[o + 1] / km2);
+            P[2 * ax * nh + o + 1] = km2 <= 0 ? 0 : (float)(kv[ax] * H[o] / km2);
+          }
+        }
+    static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; q++) {
+      for (int ii = 0; ii < N; ii++)
+        for (int jj = 0; jj < N; jj++)
+          for (int kk = 0; kk < NH; kk++) {
+            double kv[3];
+            lpt_kvec(N, dk, kk, jj, ii, kv);
+            const size_t o = 2 * ((size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * ii));
+            const double *pp = P + 2 * pa[q] * nh + o;
+            S[o] = (float)(-kv[pb[q]] * pp[1]);
+            S[o + 1] = (float)(kv[pb[q]] * pp[0]);
+          }
+      orc_c2r_3d(N, S, G + q * ng);
+      for (size_t i = 0; i < ng; i++) G[q * ng + i] = (float)G[q * ng + i];
+    }
+    double *src = malloc(ng * sizeof(double));
+    for (size_t i = 0; i < ng; i++) src[i] = 0;
+    for (int iz = 0; iz < N; iz++)
+      for (int iy = 0; iy < N; iy++)
+        for (int ix = 0; ix < N; ix++) {
+          const size_t i = (size_t)ix + (size_t)ngx * ((size_t)iy + (size_t)N * iz);
+          const float xx = G[i], xy = G[ng + i], xz = G[2 * ng + i], yy = G[3 * ng + i], yz = G[4 * ng + i],
+                      zz = G[5 * ng + i];
+          src[i] = xx * yy + xx * zz + yy * zz - xy * xy - xz * xz - yz * yz;
+        }
+    orc_r2c_3d(N, src, H);   /* H now holds S_k; the first-order spectra are in P */
+    for (int ax = 0; ax < 3; ax++) {
+      for (int ii = 0; ii < N; ii++)
+        for (int jj = 0; jj < N; jj++)
+          for (int kk = 0; kk < NH; kk++) {
+            double kv[3];
+            lpt_kvec(N, dk, kk, jj, ii, kv);
+            const size_t o = 2 * ((size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * ii));
+            const double km2 = fftnorm * (kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+            const double hr = (float)H[o], hi = (float)H[o + 1];
+            S[o] = km2 <= 0 ? 0 : (float)(kv[ax] * hi / km2);
+            S[o + 1] = km2 <= 0 ? 0 : (float)(-kv[ax] * hr / km2);
+          }
+      orc_c2r_3d(N, S, sec + ax * ng);
+      orc_c2r_3d(N, P + 2 * ax * nh, disp + ax * ng);
+    }
+    free(P); free(G); free(src);
+  }
+  for (int ax = 0; ax < 3 && order == 1; ax++) {
     for (int ii = 0; ii < N; ii++) {
       const double kz = 2 * ii <= N ? ii * dk : -(N - ii) * dk;
       for (int jj = 0; jj < N; jj++) {
@@ -1145,9 +1214,12 @@ int orc_lpt1(int N, double L, const double obs[3], const orc_bg *bg, const doubl
         xv[2] = (float)((iz + 0.0) * dxf - obs[2]);
         const double r = sqrt((double)(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]));
         const double d1 = orc_f_of_r(bg, bg->d1, 1, bg->d1[na - 1], r);
+        const double d2g = order == 2 ? orc_f_of_r(bg, d2, d2[0], d2[na - 1], r) : 0;
         float p[3];
         for (int ax = 0; ax < 3; ax++) {
-          float q = (float)(xv[ax] + d1 * (float)disp[ax * ng + idx] + obs[ax]);
+          float q = order == 1 ? (float)(xv[ax] + d1 * (float)disp[ax * ng + idx] + obs[ax])
+                               : (float)(xv[ax] + d1 * (float)disp[ax * ng + idx] +
+                                         d2g * (float)sec[ax * ng + idx] + obs[ax]);
           if (q < 0) q = (float)(q + L);
           if (q >= L) q = (float)(q - L);
           p[ax] = q;
@@ -1211,7 +1283,7 @@ int orc_lpt1(int N, double L, const double obs[3], const orc_bg *bg, const doubl
         const size_t idx = (size_t)ix + (size_t)ngx * ((size_t)iy + (size_t)N * iz);
         out[idx] = out[idx] - 1.;
       }
-  free(H); free(S); free(disp);
+  free(H); free(S); free(disp); free(sec);
   return 0;
 }
 

@@ -119,10 +119,11 @@ int orc_cstm(int N, double L, const double obs[3], double r_max, const orc_bg *b
              int model, int nside, double *out);
 /* fftw_wrap_r2c: forward unnormalised half spectrum of a padded real grid. */
 void orc_r2c_3d(int N, const double *in, double *out);
-/* lpt_1 (density.c:376-645) with pos_2_ngp/cic/tsc (interp 0/1/2):
- * out = deposited density - 1 (padded); pos (optional) 3 floats per cell. */
-int orc_lpt1(int N, double L, const double obs[3], const orc_bg *bg, const double *dg, int interp, double *out,
-             float *pos);
+/* lpt_1 / lpt_2 (density.c:376-1031) with pos_2_ngp/cic/tsc (interp 0/1/2):
+ * out = deposited density - 1 (padded); d2 = D2(r) table (order 2); pos
+ * (optional) 3 floats per cell. */
+int orc_lpt(int order, int N, double L, const double obs[3], const orc_bg *bg, const double *d2, const double *dg,
+            int interp, double *out, float *pos);
 void orc_imap(int N, double L, const double obs[3], const orc_bg *bg, const double *dens,
               const double *npot, const double *tz, const double *bz, double bz0,
               const double *norm, double norm0, double normf, int model,

@@ -78,7 +78,7 @@ def load(build=True):
                                          ctypes.c_longlong, _f, _d, _d])
     _sig(lib, "orc_cstm", ci, [ci, cd, _d, cd, ctypes.POINTER(Bg), _d, _d, _d, _d, cd, cd, ci, ci, _d])
     _sig(lib, "orc_r2c_3d", None, [ci, _d, _d])
-    _sig(lib, "orc_lpt1", ci, [ci, cd, _d, ctypes.POINTER(Bg), _d, ci, _d, _f])
+    _sig(lib, "orc_lpt", ci, [ci, ci, cd, _d, ctypes.POINTER(Bg), _d, _d, ci, _d, _f])
     _sig(lib, "orc_src_beams", None, [ci, cd, _d, cd, ctypes.POINTER(Bg), _d, cd, cd, cd, _d, ctypes.c_longlong,
                                        _f, _d, _d])
     _sig(lib, "orc_imap", None, [ci, cd, _d, ctypes.POINTER(Bg), _d, _d, _d, _d, cd, _d, cd, cd, ci, cd, cd, cd,
@@ -232,14 +232,16 @@ class Oracle:
                                p(keep[2]), p(keep[3]), norm0, normf, model, nside, p(out))
         return None if rc else out
 
-    def lpt1(self, N, L, dg, interp, with_pos=False):
-        """(delta[N][N][2(N/2+1)], positions[N][N][N][3] or None) of lpt_1."""
+    def lpt1(self, N, L, dg, interp, with_pos=False, order=1, d2=None):
+        """(delta[N][N][2(N/2+1)], positions[N][N][N][3] or None) of lpt_1
+        (order 2: lpt_2 with the D2 table d2)."""
         out = np.zeros((N, N, 2 * (N // 2 + 1)))
         pos = np.zeros((N, N, N, 3), dtype=np.float32) if with_pos else None
         obs = np.full(3, 0.5 * L)
         dg = np.ascontiguousarray(dg, dtype=np.float64)
-        rc = self.lib.orc_lpt1(N, L, p(obs), ctypes.byref(self.bg), p(dg), interp, p(out),
-                               p(pos, ctypes.c_float) if with_pos else None)
+        d2a = np.ascontiguousarray(d2, dtype=np.float64) if d2 is not None else None
+        rc = self.lib.orc_lpt(order, N, L, p(obs), ctypes.byref(self.bg), p(d2a) if d2a is not None else None, p(dg),
+                              interp, p(out), p(pos, ctypes.c_float) if with_pos else None)
         assert rc == 0
         return out, pos
 

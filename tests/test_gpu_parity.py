@@ -301,10 +301,10 @@ def test_custom_map(workdir, fused):
     assert np.abs(out - o).max() <= 1e-5 * rms, (np.abs(out - o).max(), rms)
 
 
-@pytest.mark.parametrize("interp", [0, 1, 2])
-def test_lpt_density(workdir, interp):
-    """1LPT density from a Gaussian field on the device against the oracle's
-    FP64 transforms and deposit of the same field: CIC and TSC within 1e-4
+@pytest.mark.parametrize("order,interp", [(1, 0), (1, 1), (1, 2), (2, 1), (2, 2)])
+def test_lpt_density(workdir, order, interp):
+    """1LPT and 2LPT densities from a Gaussian field on the device against the
+    oracle's FP64 transforms and deposit of the same field: CIC and TSC within 1e-4
     (FP32 FFT and atomics), NGP counts equal but in at most 1e-4 of the cells
     (a particle within rounding of a cell boundary), the same mass.  The
     field is the generated one times 25, so that displacements cross cells."""
@@ -316,9 +316,10 @@ def test_lpt_density(workdir, interp):
     ctx.upload_grid(0, g)
     ctx.declare_grids(s2, False)
     dg = g.astype(np.float64)
-    ctx.lpt_density(interp)
+    ctx.lpt_density(interp, order)
     d = ctx.fetch_grid(0)[:, :, :N].astype(np.float64)
-    o = _oracle(run).lpt1(N, run.scalar("l_box"), dg, interp)[0][:, :, :N]
+    ta = run.table_arrays()
+    o = _oracle(run).lpt1(N, run.scalar("l_box"), dg, interp, order=order, d2=ta["d2"])[0][:, :, :N]
     assert abs(d.sum()) < 1e-3 * N ** 3 and abs(o.sum()) < 1e-6 * N ** 3
     assert np.abs(o).max() > 0.1
     if interp == 0:

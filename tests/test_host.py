@@ -150,7 +150,7 @@ def test_param_errors(tmp_path):
     assert "HDF5 format not supported" in err_of(text.replace('"ASCII"', '"HDF5"'))
     assert "Unrecognized format" in err_of(text.replace('"ASCII"', '"XML"'))
     assert "Unknown bias model" in err_of(text.replace('"exponential"', '"quadratic"'))
-    assert "Density type 2 not supported" in err_of(text.replace("dens_type = 0", "dens_type = 2"))
+    assert "Density type 4 not supported" in err_of(text.replace("dens_type = 0", "dens_type = 4"))
     assert "Wrong interpolation type" in err_of(text.replace("dens_type = 0", "dens_type = 1")
                                                 .replace("lpt_interp_type = 1", "lpt_interp_type = 3"))
     assert "Source plane 1 outside redshift range" in err_of(text + "kappa: { z_out = [0.9]; nside = 8; };\n")

@@ -626,6 +626,44 @@ def test_lpt1_closed_form(interp):
     assert np.abs(d[:, :, :N]).max() > 0.05
 
 
+@pytest.mark.parametrize("interp", [1, 2])
+def test_lpt2_closed_form(interp):
+    """delta_G = A cos(k1 x) + B cos(k2 y), D1 = 1, D2 = c: the digradients
+    are -A cos(k1 x) (xx) and -B cos(k2 y) (yy), the second-order source is
+    A B cos(k1 x) cos(k2 y), and psi2 = -grad S / (k1^2 + k2^2), so the
+    particle of cell (x, y, z) sits at x - (A/k1) sin(k1 x) + c k1 A B
+    sin(k1 x) cos(k2 y) / (k1^2 + k2^2), y - (B/k2) sin(k2 y) + c k2 A B
+    cos(k1 x) sin(k2 y) / (k1^2 + k2^2), z; the deposit of those positions is
+    the density.  A single plane wave has S = 0: 2LPT = 1LPT."""
+    ta = _tables()
+    ta["d1"] = np.ones_like(ta["d1"])
+    orc = Oracle(ta)
+    N, Lb = 16, 400.0
+    dx = Lb / N
+    c2 = -0.4
+    d2 = np.full_like(ta["r"], c2)
+    A, B, m1, m2 = 0.8, 0.6, 1, 2
+    k1, k2 = 2 * np.pi * m1 / Lb, 2 * np.pi * m2 / Lb
+    i = np.arange(N) * dx
+    Z, Y, X = np.meshgrid(i, i, i, indexing="ij")
+    g = np.zeros((N, N, 2 * (N // 2 + 1)))
+    g[:, :, :N] = A * np.cos(k1 * X) + B * np.cos(k2 * Y)
+    d, pos = orc.lpt1(N, Lb, g, interp, with_pos=True, order=2, d2=d2)
+    kk = k1 * k1 + k2 * k2
+    wx = X - (A / k1) * np.sin(k1 * X) + c2 * k1 * A * B * np.sin(k1 * X) * np.cos(k2 * Y) / kk
+    wy = Y - (B / k2) * np.sin(k2 * Y) + c2 * k2 * A * B * np.cos(k1 * X) * np.sin(k2 * Y) / kk
+    np.testing.assert_allclose(pos[..., 0], wx % Lb, atol=5e-4 * dx)
+    np.testing.assert_allclose(pos[..., 1], wy % Lb, atol=5e-4 * dx)
+    np.testing.assert_allclose(pos[..., 2], Z, atol=1e-4)
+    rho = _deposit_numpy(N, pos.reshape(-1, 3), Lb, interp)
+    np.testing.assert_allclose(d[:, :, :N], rho - 1, atol=1e-6)
+    g1 = np.zeros_like(g)
+    g1[:, :, :N] = A * np.cos(k1 * X)
+    d1o, p1 = orc.lpt1(N, Lb, g1, interp, with_pos=True)
+    d2o, p2 = orc.lpt1(N, Lb, g1, interp, with_pos=True, order=2, d2=d2)
+    np.testing.assert_allclose(p2, p1, atol=1e-4)
+
+
 def test_imap_constant_temperature():
     """T = const, b = 0, norm = 1, no RSD: every painted pixel averages to T."""
     ta = _tables()
