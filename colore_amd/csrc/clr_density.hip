@@ -391,12 +391,21 @@ __global__ __launch_bounds__(256) void k_shell_final(const double *__restrict__ 
 
 // Per tracer t: norm table on the NA r-grid.  sums = [nz][nq] from
 // k_shell_final; work = scratch of 2*nz doubles.
-__global__ void k_norm_table(const double *__restrict__ sums, int nz, int nq, int t, double zmax,
-                             const double *__restrict__ r_arr, const double *__restrict__ zt,
-                             double idr, double r_last, double z_last, int na, double *__restrict__ work,
-                             double *__restrict__ norm_out, double *__restrict__ edges) {
+constexpr int kNormMaxShells = 1024;
+struct NormOuts {
+  double *p[3 * CLR_NPOP_MAX];
+};
+
+// Every tracer's norm table in one launch: blockIdx.y is the tracer, the x
+// blocks split the NA entries; each block rebuilds the tracer's (z, norm)
+// nodes in LDS from the FP64 shell sums.
+__global__ __launch_bounds__(256) void k_norm_table(const double *__restrict__ sums, int nz, int nq, double zmax,
+                                                   const double *__restrict__ r_arr, const double *__restrict__ zt,
+                                                   double idr, double r_last, double z_last, int na, NormOuts outs,
+                                                   double *__restrict__ edges) {
 #pragma clang fp contract(off)
-  double *zarr = work, *nrm = work + nz;
+  __shared__ double zarr[kNormMaxShells], nrm[kNormMaxShells];
+  const int t = blockIdx.y;
   if (threadIdx.x == 0) {
     for (int iz = 0; iz < nz; iz++) {
       const double n = sums[iz * nq + 0];
@@ -411,13 +420,18 @@ __global__ void k_norm_table(const double *__restrict__ sums, int nz, int nq, in
     zarr[nz - 1] = zmax;
     nrm[0] = nrm[1];
     nrm[nz - 1] = nrm[nz - 2];
-    edges[0] = nrm[0];
-    edges[1] = nrm[nz - 1];
+    if (blockIdx.x == 0) {
+      edges[2 * t] = nrm[0];
+      edges[2 * t + 1] = nrm[nz - 1];
+    }
   }
   __syncthreads();
   const double z0 = zarr[0], zf = zarr[nz - 1];
   const double n0 = nrm[0], nf = nrm[nz - 1];
-  for (int ii = threadIdx.x; ii < na; ii += blockDim.x) {
+  double *__restrict__ norm_out = outs.p[t];
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ii >= na) return;
+  {
     const double z = f_of_r(r_arr, zt, idr, r_last, r_arr[ii], 0.0, z_last);
     double nm;
     if (z < z0) nm = n0;
@@ -486,6 +500,7 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   // shells: zmax = z(L/2), nz = (int)(zmax/DZ_SIGMA)+2, idz = (nz-2)/zmax
   const double zmax = c->zmax_half_box;
   const int nz = (int)(zmax / 0.05) + 2;
+  if (nz > kNormMaxShells) return clr_fail(c, CLR_ERR_ARG, "%d redshift shells (at most %d)", nz, kNormMaxShells);
   const double idz = (nz - 2) / zmax;
 
   ShellArgs a{};
@@ -559,10 +574,11 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,
                      (int)nvals, fin);
   CLR_HIP(c, hipGetLastError());
-  for (int t = 0; t < ntr; t++) {
-    TracerDev &T = c->tr[tk[t]][tp[t]];
-    hipLaunchKernelGGL(k_norm_table, dim3(1), dim3(256), 0, c->stream, fin, nz, nq, t, zmax, r, zt,
-                       c->glob_idr, c->r_last, z_last, na, work, T.norm, work + 2 * nz + 2 * t);
+  if (ntr > 0) {
+    NormOuts outs{};
+    for (int t = 0; t < ntr; t++) outs.p[t] = c->tr[tk[t]][tp[t]].norm;
+    hipLaunchKernelGGL(k_norm_table, dim3((unsigned)((na + 255) / 256), (unsigned)ntr), dim3(256), 0, c->stream,
+                       fin, nz, nq, zmax, r, zt, c->glob_idr, c->r_last, z_last, na, outs, work + 2 * nz);
     CLR_HIP(c, hipGetLastError());
   }
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
