@@ -462,12 +462,18 @@ __global__ __launch_bounds__(256) void k_exact(SrcArgs a) {
   }
 }
 
-// exclusive scan of n uint32 -> int64 offsets (out[n] = total).  Three
-// kernels: per-block sums, scan of the block sums (one block), block-local scan.
+// exclusive scan of n uint32 -> int64 offsets (out[n] = total), for every
+// population at once (blockIdx.y).  Three kernels: per-block sums, scan of
+// the block sums (one block per population), block-local scan.
 constexpr int kScanB = 1024;
-__global__ __launch_bounds__(kScanB) void k_scan_blocks(const uint32_t *__restrict__ in, size_t n,
-                                                        long long *__restrict__ bsum) {
+struct ScanPtrs {
+  const uint32_t *in[CLR_NPOP_MAX];
+  long long *bsum[CLR_NPOP_MAX];
+  long long *out[CLR_NPOP_MAX];
+};
+__global__ __launch_bounds__(kScanB) void k_scan_blocks(ScanPtrs p, size_t n) {
   __shared__ long long s[kScanB];
+  const uint32_t *__restrict__ in = p.in[blockIdx.y];
   const size_t i = (size_t)blockIdx.x * kScanB + threadIdx.x;
   s[threadIdx.x] = i < n ? in[i] : 0;
   __syncthreads();
@@ -475,11 +481,12 @@ __global__ __launch_bounds__(kScanB) void k_scan_blocks(const uint32_t *__restri
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) p.bsum[blockIdx.y][blockIdx.x] = s[0];
 }
-__global__ __launch_bounds__(kScanB) void k_scan_top(long long *__restrict__ bsum, size_t nb) {
+__global__ __launch_bounds__(kScanB) void k_scan_top(ScanPtrs p, size_t nb) {
   __shared__ long long s[kScanB];
   __shared__ long long carry;
+  long long *__restrict__ bsum = p.bsum[blockIdx.y];
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   for (size_t base = 0; base < nb; base += kScanB) {
@@ -500,10 +507,11 @@ __global__ __launch_bounds__(kScanB) void k_scan_top(long long *__restrict__ bsu
   }
   if (threadIdx.x == 0) bsum[nb] = carry;
 }
-__global__ __launch_bounds__(kScanB) void k_scan_apply(const uint32_t *__restrict__ in, size_t n,
-                                                       const long long *__restrict__ bsum,
-                                                       long long *__restrict__ out) {
+__global__ __launch_bounds__(kScanB) void k_scan_apply(ScanPtrs p, size_t n) {
   __shared__ long long s[kScanB];
+  const uint32_t *__restrict__ in = p.in[blockIdx.y];
+  const long long *__restrict__ bsum = p.bsum[blockIdx.y];
+  long long *__restrict__ out = p.out[blockIdx.y];
   const size_t i = (size_t)blockIdx.x * kScanB + threadIdx.x;
   const long long v = i < n ? in[i] : 0;
   s[threadIdx.x] = v;
@@ -806,15 +814,19 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     hipLaunchKernelGGL(k_exact, dim3((a.exq_cap + 255) / 256), dim3(256), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());
     CLR_HIP(c, hipEventRecord(c->ev3, c->stream));
+    ScanPtrs sp{};
     for (int i = 0; i < np; i++) {
-      const PopTab &P = a.pop[i];
-      hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i]);
-      hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanB), 0, c->stream, bsum[i], nsb);
-      hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsb), dim3(kScanB), 0, c->stream, P.btot, nblocks, bsum[i],
-                         (long long *)P.boff);
-      CLR_HIP(c, hipGetLastError());
-      CLR_HIP(c, hipMemcpyAsync(&tot[i], P.boff + nblocks, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+      sp.in[i] = a.pop[i].btot;
+      sp.bsum[i] = bsum[i];
+      sp.out[i] = (long long *)a.pop[i].boff;
     }
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb, (unsigned)np), dim3(kScanB), 0, c->stream, sp, nblocks);
+    hipLaunchKernelGGL(k_scan_top, dim3(1, (unsigned)np), dim3(kScanB), 0, c->stream, sp, nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsb, (unsigned)np), dim3(kScanB), 0, c->stream, sp, nblocks);
+    CLR_HIP(c, hipGetLastError());
+    for (int i = 0; i < np; i++)
+      CLR_HIP(c, hipMemcpyAsync(&tot[i], a.pop[i].boff + nblocks, sizeof(long long), hipMemcpyDeviceToHost,
+                                c->stream));
     unsigned int nexq = 0;
     CLR_HIP(c, hipMemcpyAsync(&nesc, a.esc_n, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
     CLR_HIP(c, hipMemcpyAsync(&nexq, a.exq_n, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
