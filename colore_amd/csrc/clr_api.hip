@@ -320,10 +320,10 @@ int clr_source_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz_rs
   return clr_launch_src_beams(c, ipop, r_arr_a2r, dz_rsd, lens, n);
 }
 
-int clr_lpt_density(clr_ctx *c, int order, int interp) {
+int clr_lpt_density(clr_ctx *c, int order, int interp, float *pos_out) {
   if (!c) return CLR_ERR_ARG;
   CLR_HIP(c, hipSetDevice(c->device));
-  return clr_launch_lpt(c, order, interp);
+  return clr_launch_lpt(c, order, interp, pos_out);
 }
 
 int clr_custom_map(clr_ctx *c, int ipop, int nside, float *out) {

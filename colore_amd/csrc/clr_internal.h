@@ -134,7 +134,7 @@ int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
 int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out);
 int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz, float *lens, long long n);
 int clr_launch_cstm(clr_ctx *c, int ipop, int nside, float *out);
-int clr_launch_lpt(clr_ctx *c, int order, int interp);
+int clr_launch_lpt(clr_ctx *c, int order, int interp, float *pos_out);
 int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, float *dg, float *vs, long long n);
 int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf,
                     float *data, int32_t *nadd);

@@ -15,6 +15,7 @@
 #include "clr_internal.h"
 #include <hipfft/hipfft.h>
 #include <math.h>
+#include <vector>
 
 using namespace clr;
 
@@ -257,7 +258,7 @@ int fft_fail(clr_ctx *c, hipfftResult r, const char *what) {
 
 }  // namespace
 
-int clr_launch_lpt(clr_ctx *c, int order, int interp) {
+int clr_launch_lpt(clr_ctx *c, int order, int interp, float *pos_out) {
   if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
   if (c->density_done) return clr_fail(c, CLR_ERR_STATE, "the density grid is already transformed");
   if (order != 1 && order != 2) return clr_fail(c, CLR_ERR_ARG, "Density type %d not supported", order);
@@ -367,6 +368,17 @@ int clr_launch_lpt(clr_ctx *c, int order, int interp) {
     a.ngx = ngx;
     hipLaunchKernelGGL(k_lpt_pos, gc, blk, 0, c->stream, a);
     e = hipGetLastError();
+  }
+  if (ok() && pos_out) {
+    // the particle positions (write_lpt, io.c:619-695): x, y, z per particle in cell order
+    std::vector<float> plane((size_t)N * N * N);
+    for (int ax = 0; ax < 3 && ok(); ax++) {
+      e = hipMemcpy2DAsync(plane.data(), (size_t)N * sizeof(float), pos[ax], (size_t)ngx * sizeof(float),
+                           (size_t)N * sizeof(float), (size_t)N * N, hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess)
+        for (size_t i = 0; i < plane.size(); i++) pos_out[3 * i + ax] = plane[i];
+    }
   }
   if (ok()) e = hipMemsetAsync(c->dens, 0, gbytes, c->stream);
   if (ok()) {

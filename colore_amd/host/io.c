@@ -252,3 +252,46 @@ int clr_append_skewers_fits(const char *fname, int gaussian, long long n, int nr
   fclose(f);
   return ok ? 0 : 1;
 }
+
+/* write_lpt (io.c:597-695): a Gadget-2 snapshot of the LPT particles
+ * (header with the reference's extra flag word, positions, zero velocities,
+ * 64-bit IDs 0..n-1) */
+int clr_write_lpt_gadget(const char *fname, long long n, double l_box, double omega_m, double omega_l,
+                         double hhub, const float *xyz) {
+  FILE *f = fopen(fname, "wb");
+  if (!f) return 1;
+  unsigned char h[256];
+  memset(h, 0, sizeof(h));
+  int np1 = (int)n;
+  double mass1 = 27.7455 * omega_m * l_box * l_box * l_box / (double)n, one = 1.0;
+  unsigned int lo = (unsigned int)n, hi = (unsigned int)((unsigned long long)n >> 32);
+  int nfiles = 1, gflag = 1;
+  memcpy(h + 4, &np1, 4);            /* np[1] */
+  memcpy(h + 24 + 8, &mass1, 8);     /* mass[1] */
+  memcpy(h + 72, &one, 8);           /* time (redshift 0 at 80) */
+  memcpy(h + 96 + 4, &lo, 4);        /* np_total[1] */
+  memcpy(h + 124, &nfiles, 4);       /* num_files */
+  memcpy(h + 128, &l_box, 8);        /* boxsize */
+  memcpy(h + 136, &omega_m, 8);
+  memcpy(h + 144, &omega_l, 8);
+  memcpy(h + 152, &hhub, 8);
+  memcpy(h + 168 + 4, &hi, 4);       /* np_total_highword[1] */
+  memcpy(h + 196, &gflag, 4);        /* flag_gadgetformat */
+  int blk = 256, ok = 1;
+  ok = ok && fwrite(&blk, 4, 1, f) == 1 && fwrite(h, 1, 256, f) == 256 && fwrite(&blk, 4, 1, f) == 1;
+  blk = (int)(n * 12);
+  ok = ok && fwrite(&blk, 4, 1, f) == 1 && fwrite(xyz, 12, (size_t)n, f) == (size_t)n && fwrite(&blk, 4, 1, f) == 1;
+  const float zero3[3] = {0, 0, 0};
+  ok = ok && fwrite(&blk, 4, 1, f) == 1;
+  for (long long i = 0; ok && i < n; i++) ok = fwrite(zero3, 12, 1, f) == 1;
+  ok = ok && fwrite(&blk, 4, 1, f) == 1;
+  blk = (int)(n * 8);
+  ok = ok && fwrite(&blk, 4, 1, f) == 1;
+  for (long long i = 0; ok && i < n; i++) {
+    const unsigned long long id = (unsigned long long)i;
+    ok = fwrite(&id, 8, 1, f) == 1;
+  }
+  ok = ok && fwrite(&blk, 4, 1, f) == 1;
+  fclose(f);
+  return ok ? 0 : 1;
+}

@@ -81,8 +81,20 @@ int clr_run_execute(clr_run *r, int device) {
   if (r->dens_type == 1 || r->dens_type == 2) {
     /* lpt_1 / lpt_2 (density.c:376-1031) */
     printf(" %dLPT\n", r->dens_type);
-    if (r->output_lpt) fprintf(stderr, "MPI task 0, Warning: LPT particle positions (output_lpt) are not written by this build\n");
-    CHK(clr_lpt_density(ctx, r->dens_type, r->lpt_interp_type));
+    const long long np = (long long)r->n_grid * r->n_grid * r->n_grid;
+    float *xyz = r->output_lpt ? malloc((size_t)np * 3 * sizeof(float)) : NULL;
+    if (r->output_lpt && !xyz) { fprintf(stderr, "MPI task 0, Fatal: Ran out of memory\n"); clr_destroy(ctx); return 1; }
+    const int lrc = clr_lpt_density(ctx, r->dens_type, r->lpt_interp_type, xyz);
+    if (lrc) free(xyz);
+    CHK(lrc);
+    if (xyz) {
+      char lname[CLR_PATH + 32];
+      printf(" - Writing LPT positions\n");
+      snprintf(lname, sizeof(lname), "%s_lpt_out.0", r->prefix_out);
+      if (clr_write_lpt_gadget(lname, np, r->l_box, r->OmegaM, r->OmegaL, r->hhub, xyz))
+        fprintf(stderr, "MPI task 0, Warning: could not write %s\n", lname);
+      free(xyz);
+    }
   } else {
     CHK(clr_density(ctx, r->dens_type, sigma2));
   }

@@ -114,7 +114,8 @@ CONFIGS = {
 
 def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_format="ASCII",
                  output_density=False, with_maps=True, seed=None, imap_bands=None, imap_nside=None,
-                 kappa_nside=None, lensing=(), skewers=(), custom=(), pred_dz=None, just_pred=False):
+                 kappa_nside=None, lensing=(), skewers=(), custom=(), pred_dz=None, just_pred=False,
+                 output_lpt=False):
     """Writes P(k), n(z), b(z) (+T(z), nu table) and the param file of config
     `name` into outdir; returns the param-file path.  n_grid/z_max override
     the config's values (small variants for tests); `lensing` lists the
@@ -139,7 +140,7 @@ def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_form
               f"  pred_dz = {pred_dz};"] if pred_dz else ["  write_pred = false;", "  just_write_pred = false;"]) + \
             ["};", "field_par: {", f"  r_smooth = {cfg['r_smooth']};",
              "  smooth_potential = true;", f"  n_grid = {n};", f"  dens_type = {dens_type};",
-             "  lpt_buffer_fraction = 0.6;", "  lpt_interp_type = 1;", "  output_lpt = 0;", "};",
+             "  lpt_buffer_fraction = 0.6;", "  lpt_interp_type = 1;", f"  output_lpt = {int(output_lpt)};", "};",
              "cosmo_par: {"] + [f"  {kk} = {vv};" for kk, vv in COSMO.items()] + ["};"]
     for i, (nzs, bzs, model) in enumerate(cfg["srcs"]):
         if nzs[0] == "gauss":

@@ -82,7 +82,7 @@ def load_amd():
     _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_source_beams", ci, [_vp, ci, _d, _f, _f, ctypes.c_longlong])
     _sig(lib, "clr_custom_map", ci, [_vp, ci, ci, _f])
-    _sig(lib, "clr_lpt_density", ci, [_vp, ci, ci])
+    _sig(lib, "clr_lpt_density", ci, [_vp, ci, ci, _f])
     _sig(lib, "clr_source_skewers", ci, [_vp, ci, ci, ctypes.c_double, _f, _f, ctypes.c_longlong])
     _sig(lib, "clr_imap", ci, [_vp, ci, ci, ci, _d, _d, _f, _i32])
     _sig(lib, "clr_fetch_grid", ci, [_vp, ci, _f])
@@ -123,6 +123,8 @@ def load_host():
     _sig(lib, "clr_write_catalog_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
     _sig(lib, "clr_write_catalog_ascii", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
     _sig(lib, "clr_append_skewers_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, ci, _f, _f, _f, ctypes.c_float])
+    _sig(lib, "clr_write_lpt_gadget", ci, [ctypes.c_char_p, ctypes.c_longlong, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, _f])
     return lib
 
 
@@ -307,9 +309,12 @@ class Context:
                                               ptr(vs, ctypes.c_float), n))
         return dg[:n], vs[:n]
 
-    def lpt_density(self, interp, order=1):
-        """1LPT / 2LPT density (rho - 1) in the density grid, replacing clr_density."""
-        self._chk(self.lib.clr_lpt_density(self.h, order, interp))
+    def lpt_density(self, interp, order=1, positions=False):
+        """1LPT / 2LPT density (rho - 1) in the density grid, replacing
+        clr_density; with positions, returns the particles' float32 (N^3, 3)."""
+        pos = np.zeros((self.N ** 3, 3), dtype=np.float32) if positions else None
+        self._chk(self.lib.clr_lpt_density(self.h, order, interp, ptr(pos, ctypes.c_float) if positions else None))
+        return pos
 
     def custom_map(self, ipop, nside):
         """Custom projected map ipop (NEST, float32[12 nside^2])."""

@@ -133,11 +133,13 @@ int clr_density(clr_ctx *ctx, int dens_type, double sigma2);
  * and assigned to the grid with interp CLR_INTERP_NGP, _CIC or _TSC
  * (lpt_interp_type, pos_2_dens, density.c:37-189); the grid then holds
  * rho - 1.  Needs 3 (order 1) or 8 (order 2) grid-sized buffers of device
- * memory for the call. */
+ * memory for the call.  pos_out (NULL or 3 N^3 floats) receives the particle
+ * positions x, y, z in cell order (ix fastest), as write_lpt dumps them
+ * (output_lpt, io.c:619-695). */
 #define CLR_INTERP_NGP 0
 #define CLR_INTERP_CIC 1
 #define CLR_INTERP_TSC 2
-int clr_lpt_density(clr_ctx *ctx, int order, int interp);
+int clr_lpt_density(clr_ctx *ctx, int order, int interp, float *pos_out);
 
 /* compute_density_normalization (density.c:1227-1393).  Computes the FP64
  * per-shell sums on the device and builds each tracer's NA norm table on the

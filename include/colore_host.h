@@ -111,6 +111,11 @@ int clr_write_catalog_fits(const char *fname, int ipop, long long n, const float
                            const float *lens);
 int clr_write_catalog_ascii(const char *fname, int ipop, long long n, const float *radecz,
                             const float *lens);
+/* write_lpt (io.c:597-695): Gadget-2 snapshot <fname> of n LPT particles
+ * (xyz = 3 floats each; header mass 27.7455 Om L^3 / n, zero velocities,
+ * IDs 0..n-1). */
+int clr_write_lpt_gadget(const char *fname, long long n, double l_box, double omega_m, double omega_l,
+                         double hhub, const float *xyz);
 /* The skewer HDUs of a FITS catalogue (io.c:1139-1187), appended to fname:
  * an n x nr image of densities (or Gaussian fields), one of velocities, and
  * a table R Z D V (bg4 = float[4 nr]: r, z, D, V1 at each step) whose

@@ -316,10 +316,15 @@ def test_lpt_density(workdir, order, interp):
     ctx.upload_grid(0, g)
     ctx.declare_grids(s2, False)
     dg = g.astype(np.float64)
-    ctx.lpt_density(interp, order)
+    pos = ctx.lpt_density(interp, order, positions=True)
     d = ctx.fetch_grid(0)[:, :, :N].astype(np.float64)
     ta = run.table_arrays()
-    o = _oracle(run).lpt1(N, run.scalar("l_box"), dg, interp, order=order, d2=ta["d2"])[0][:, :, :N]
+    o, opos = _oracle(run).lpt1(N, run.scalar("l_box"), dg, interp, with_pos=True, order=order, d2=ta["d2"])
+    o = o[:, :, :N]
+    # particle positions (cell order), as write_lpt dumps them: FP32 FFTs
+    dp = np.abs(pos - opos.reshape(-1, 3))
+    dp = np.minimum(dp, run.scalar("l_box") - dp)   # across the periodic edge
+    assert dp.max() < 1e-3 * run.scalar("l_box") / N, dp.max()
     assert abs(d.sum()) < 1e-3 * N ** 3 and abs(o.sum()) < 1e-6 * N ** 3
     assert np.abs(o).max() > 0.1
     if interp == 0:
@@ -481,14 +486,16 @@ def test_cli_lensing_catalogue(tmp_path):
 
 
 def test_cli_lpt(tmp_path):
-    """`CoLoRe param.cfg` with dens_type = 1 (1LPT, CIC): the run completes and
-    its lightcone density dump holds the library's 1LPT density of the same
-    seed (FP32 atomics: equal to 1e-5)."""
+    """`CoLoRe param.cfg` with dens_type = 1 (1LPT, CIC) and output_lpt: the run
+    completes, its lightcone density dump holds the library's 1LPT density of
+    the same seed (FP32 atomics: equal to 1e-5) and its Gadget snapshot the
+    library's particle positions."""
     import subprocess
     from conftest import ROOT
     from colore_amd import inputs, lib
     N = 64
-    p = inputs.write_config("C1", str(tmp_path), n_grid=N, dens_type=1, output_density=True, with_maps=False)
+    p = inputs.write_config("C1", str(tmp_path), n_grid=N, dens_type=1, output_density=True, with_maps=False,
+                            output_lpt=True)
     exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
     r = subprocess.run([exe, p], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -502,7 +509,13 @@ def test_cli_lpt(tmp_path):
     run = lib.Run(p)
     ctx = _ctx_for(run, N)
     ctx.gaussian_fields(int(run.scalar("seed")), run.scalar("r_smooth"), True)
-    ctx.lpt_density(1)
+    pos = ctx.lpt_density(1, positions=True)
     d = ctx.fetch_grid(0)[:, :, :N]
     np.testing.assert_allclose(grid, d, rtol=0, atol=1e-5)
     assert np.abs(d).max() > 0.01
+    # output_lpt: the Gadget snapshot's position block
+    snap = open(tmp_path / "out_lpt_out.0", "rb").read()
+    o = 8 + 256
+    assert np.frombuffer(snap[o:o + 4], dtype="<i4")[0] == 12 * N ** 3
+    np.testing.assert_allclose(np.frombuffer(snap[o + 4:o + 4 + 12 * N ** 3], dtype="<f4").reshape(-1, 3), pos,
+                               rtol=1e-6)

@@ -445,3 +445,34 @@ def test_write_predictions(tmp_path):
         ratio = pk[lin, 1] / (b * b * pk[lin, 3] * np.exp(-(gg - mm) * k[lin] ** 2))
         assert np.all(np.abs(ratio - 1) < 0.05), ratio
         np.testing.assert_allclose(xi[:, 2], b * xi[:, 3], rtol=1e-4)
+
+
+def test_lpt_gadget_writer(tmp_path, host):
+    """write_lpt's Gadget-2 snapshot: a 256-byte header (np[1], mass[1] =
+    27.7455 Om L^3 / n, time 1, np_total[1], one file, box, Om, OL, h, the
+    format flag word), then positions, zero velocities and 64-bit IDs, each
+    block between its byte-count words."""
+    import struct
+    n, L = 27, 300.0
+    xyz = np.random.default_rng(3).random((n, 3)).astype(np.float32) * L
+    f = str(tmp_path / "out_lpt_out.0")
+    assert host.clr_write_lpt_gadget(f.encode(), n, L, 0.3, 0.7, 0.7, lib.ptr(xyz, ctypes.c_float)) == 0
+    raw = open(f, "rb").read()
+    assert len(raw) == 8 + 256 + 3 * 8 + n * (12 + 12 + 8)
+    assert struct.unpack("<i", raw[:4])[0] == 256 and struct.unpack("<i", raw[260:264])[0] == 256
+    h = raw[4:260]
+    assert struct.unpack("<6i", h[:24]) == (0, n, 0, 0, 0, 0)
+    mass = struct.unpack("<6d", h[24:72])
+    assert mass[1] == pytest.approx(27.7455 * 0.3 * L ** 3 / n, rel=1e-15) and mass[0] == 0
+    assert struct.unpack("<2d", h[72:88]) == (1.0, 0.0)
+    assert struct.unpack("<6I", h[96:120])[1] == n and struct.unpack("<i", h[124:128])[0] == 1
+    assert struct.unpack("<4d", h[128:160]) == (L, 0.3, 0.7, 0.7)
+    assert struct.unpack("<6I", h[168:192])[1] == 0 and struct.unpack("<i", h[196:200])[0] == 1
+    o = 264
+    assert struct.unpack("<i", raw[o:o + 4])[0] == 12 * n
+    np.testing.assert_array_equal(np.frombuffer(raw[o + 4:o + 4 + 12 * n], dtype="<f4").reshape(n, 3), xyz)
+    o += 8 + 12 * n
+    assert not np.any(np.frombuffer(raw[o + 4:o + 4 + 12 * n], dtype="<f4"))
+    o += 8 + 12 * n
+    assert struct.unpack("<i", raw[o:o + 4])[0] == 8 * n
+    np.testing.assert_array_equal(np.frombuffer(raw[o + 4:o + 4 + 8 * n], dtype="<u8"), np.arange(n))
