@@ -301,6 +301,40 @@ def test_custom_map(workdir, fused):
     assert np.abs(out - o).max() <= 1e-5 * rms, (np.abs(out - o).max(), rms)
 
 
+def test_line_of_sight_passes_with_no_sources(workdir):
+    """A population with n(z) = 0 samples no source; its line-of-sight pass and
+    skewers then return nothing, without error (empty catalogue)."""
+    N = 64
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx = _ctx_for(run, N)
+    amp, bz, model = run.tracer(0, 1)
+    ctx.set_tracer(0, 1, model, np.zeros_like(amp), bz)
+    s2 = ctx.gaussian_fields(3003, run.scalar("r_smooth"), True)
+    ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=int(run.scalar("n_srcs")))
+    assert ctx.sample_sources(1, 3003, 2) == 0
+    dz, lens = ctx.source_beams(1, 0, run.r_a2r(), True)
+    assert dz.shape == (0,) and lens.shape == (0, 5)
+    dg, vs = ctx.source_skewers(1, 0, True, s2)
+    assert dg.shape == (0, N // 2) and vs.shape == (0, N // 2)
+    # and a wrong count is refused
+    with pytest.raises(RuntimeError):
+        ctx.source_beams(1, 5, run.r_a2r(), False)
+
+
+def test_source_beams_192(workdir):
+    """The line-of-sight pass on a 3*2^n grid (192^3)."""
+    N = 192
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx, orc, pos, _ = _sources_vs_oracle(run, N, 2, 3003)
+    npot = ctx.fetch_grid(1).astype(np.float64)
+    a2r = run.r_a2r()
+    dz, lens = ctx.source_beams(2, len(pos), a2r, True)
+    odz, olens = orc.src_beams(N, run.scalar("l_box"), npot, pos, a2r, True)
+    for k, (g, o) in enumerate([(dz, odz)] + [(lens[:, j], olens[:, j]) for j in range(5)]):
+        rms = np.sqrt(np.mean(o * o))
+        assert np.abs(g.astype(np.float64) - o).max() <= 1e-5 * rms, k
+
+
 @pytest.mark.parametrize("order,interp", [(1, 0), (1, 1), (1, 2), (2, 1), (2, 2)])
 def test_lpt_density(workdir, order, interp):
     """1LPT and 2LPT densities from a Gaussian field on the device against the
