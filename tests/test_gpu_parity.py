@@ -553,3 +553,39 @@ def test_cli_lpt(tmp_path):
     assert np.frombuffer(snap[o:o + 4], dtype="<i4")[0] == 12 * N ** 3
     np.testing.assert_allclose(np.frombuffer(snap[o + 4:o + 4 + 12 * N ** 3], dtype="<f4").reshape(-1, 3), pos,
                                rtol=1e-6)
+
+
+def test_cli_reference_example_layout(tmp_path):
+    """A param file with the sections and switches of the reference's own
+    example configuration (1LPT with CIC, lensing and skewers on the first
+    population, skewers on the second, an intensity species, kappa and ISW
+    at z = 0.4, predictions every 0.1 in z, the fast-lensing section that a
+    build without it ignores), on synthetic inputs at 64^3: the run completes
+    and writes every file the reference writes for it."""
+    import glob
+    import subprocess
+    from conftest import ROOT
+    from colore_amd import inputs
+    from test_host import _read_fits_table
+    N = 64
+    p = inputs.write_config("C3", str(tmp_path), n_grid=N, z_max=0.45, dens_type=1, output_format="FITS",
+                            output_density=True, imap_bands=4, imap_nside=16, lensing=(0,),
+                            skewers={0: "density", 1: "density"}, pred_dz=0.1, just_pred=False)
+    with open(p, "a") as f:
+        f.write("kappa:\n{\n  z_out= [0.4]\n  nside= 16\n}\nisw:\n{\n  z_out= [0.4]\n  nside= 16\n}\n"
+                'lensing:\n{\n  nside= 256\n  n_lensing= 11\n  spacing_type= "r"\n  write=false\n}\n')
+    exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
+    r = subprocess.run([exe, p], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "1LPT" in r.stdout and "*** Writing predictions" in r.stdout
+    files = sorted(os.path.basename(f) for f in glob.glob(str(tmp_path / "out*")))
+    for want in ("out_gbias.txt", "out_pk_srcs_pop0_z0.100.txt", "out_xi_imap_pop0_z0.400.txt",
+                 "out_dens_gaussian_0.dat", "out_dens_lightcone_0.dat", "out_srcs_s1_0.fits", "out_srcs_s2_0.fits",
+                 "out_srcs_s3_0.fits", "out_kappa_z000.fits", "out_isw_z000.fits", "out_imap_s1_nu000.fits",
+                 "out_imap_s1_nu003.fits"):
+        assert want in files, (want, files)
+    shapes = []
+    for i in range(3):
+        hdus = _read_fits_table(str(tmp_path / f"out_srcs_s{i + 1}_0.fits"))
+        shapes.append((len(hdus), hdus[1][0]["TFIELDS"]))
+    assert shapes == [(5, "10"), (5, "5"), (2, "5")], shapes
