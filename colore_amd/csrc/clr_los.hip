@@ -82,19 +82,18 @@ __global__ __launch_bounds__(256) void k_los(LosArgs a, const double *__restrict
       const int zh = ix[2] == N - 1 ? 0 : ix[2] + 1, zl = ix[2] == 0 ? N - 1 : ix[2] - 1;
       const size_t Z0 = (size_t)ix[2] * plane, ZH = (size_t)zh * plane, ZL = (size_t)zl * plane;
       const size_t Y0 = (size_t)ix[1] * ngx, YH = (size_t)yh * ngx, YL = (size_t)yl * ngx;
-      const double c0 = phi[ix[0] + Y0 + Z0];
-      double t[6];
-      t[0] = ((double)phi[xh + Y0 + Z0] + (double)phi[xl + Y0 + Z0] - 2 * c0);
-      t[3] = ((double)phi[ix[0] + YH + Z0] + (double)phi[ix[0] + YL + Z0] - 2 * c0);
-      t[1] = 0.25 * ((double)phi[xh + YH + Z0] + (double)phi[xl + YL + Z0] - (double)phi[xh + YL + Z0] -
-                     (double)phi[xl + YH + Z0]);
-      t[5] = ((double)phi[ix[0] + Y0 + ZH] + (double)phi[ix[0] + Y0 + ZL] - 2 * c0);
-      t[2] = 0.25 * ((double)phi[xh + Y0 + ZH] + (double)phi[xl + Y0 + ZL] - (double)phi[xh + Y0 + ZL] -
-                     (double)phi[xl + Y0 + ZH]);
-      t[4] = 0.25 * ((double)phi[ix[0] + YH + ZH] + (double)phi[ix[0] + YL + ZL] -
-                     (double)phi[ix[0] + YH + ZL] - (double)phi[ix[0] + YL + ZH]);
+      // the second differences in FP32, as get_element evaluates them on
+      // flouble values (beaming.c:79-116)
+      const float c0 = phi[ix[0] + Y0 + Z0];
+      float t[6];
+      t[0] = phi[xh + Y0 + Z0] + phi[xl + Y0 + Z0] - 2 * c0;
+      t[3] = phi[ix[0] + YH + Z0] + phi[ix[0] + YL + Z0] - 2 * c0;
+      t[1] = 0.25f * (phi[xh + YH + Z0] + phi[xl + YL + Z0] - phi[xh + YL + Z0] - phi[xl + YH + Z0]);
+      t[5] = phi[ix[0] + Y0 + ZH] + phi[ix[0] + Y0 + ZL] - 2 * c0;
+      t[2] = 0.25f * (phi[xh + Y0 + ZH] + phi[xl + Y0 + ZL] - phi[xh + Y0 + ZL] - phi[xl + Y0 + ZH]);
+      t[4] = 0.25f * (phi[ix[0] + YH + ZH] + phi[ix[0] + YL + ZL] - phi[ix[0] + YH + ZL] - phi[ix[0] + YL + ZH]);
       double dotp = 0;
-      for (int q = 0; q < 6; q++) dotp += rot[q] * t[q];
+      for (int q = 0; q < 6; q++) dotp += rot[q] * (double)t[q];
       k1 += dotp * fac1[irr];
       k2 += dotp * fac2[irr];
     }
@@ -225,9 +224,9 @@ __device__ inline void vel_at(const float *__restrict__ phi, int N, size_t ngx, 
   const int yh = iy == N - 1 ? 0 : iy + 1, yl = iy == 0 ? N - 1 : iy - 1;
   const int zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
   const size_t Z0 = (size_t)iz * plane, Y0 = (size_t)iy * ngx;
-  v[0] = (double)phi[xh + Y0 + Z0] - (double)phi[xl + Y0 + Z0];
-  v[1] = (double)phi[ix + (size_t)yh * ngx + Z0] - (double)phi[ix + (size_t)yl * ngx + Z0];
-  v[2] = (double)phi[ix + Y0 + (size_t)zh * plane] - (double)phi[ix + Y0 + (size_t)zl * plane];
+  v[0] = phi[xh + Y0 + Z0] - phi[xl + Y0 + Z0];   // FP32 differences, as get_element
+  v[1] = phi[ix + (size_t)yh * ngx + Z0] - phi[ix + (size_t)yl * ngx + Z0];
+  v[2] = phi[ix + Y0 + (size_t)zh * plane] - phi[ix + Y0 + (size_t)zl * plane];
 }
 
 // get_element, RETURN_VEL | RETURN_TID: velocity and the 19-point tidal
@@ -240,22 +239,20 @@ __device__ inline void vel_tid_at(const float *__restrict__ phi, int N, size_t n
   const int zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
   const size_t Z0 = (size_t)iz * plane, ZH = (size_t)zh * plane, ZL = (size_t)zl * plane;
   const size_t Y0 = (size_t)iy * ngx, YH = (size_t)yh * ngx, YL = (size_t)yl * ngx;
-  const double c0 = phi[ix + Y0 + Z0];
-  const double pxh = phi[xh + Y0 + Z0], pxl = phi[xl + Y0 + Z0];
-  const double pyh = phi[ix + YH + Z0], pyl = phi[ix + YL + Z0];
-  const double pzh = phi[ix + Y0 + ZH], pzl = phi[ix + Y0 + ZL];
+  // FP32 differences, as get_element evaluates them on flouble values
+  const float c0 = phi[ix + Y0 + Z0];
+  const float pxh = phi[xh + Y0 + Z0], pxl = phi[xl + Y0 + Z0];
+  const float pyh = phi[ix + YH + Z0], pyl = phi[ix + YL + Z0];
+  const float pzh = phi[ix + Y0 + ZH], pzl = phi[ix + Y0 + ZL];
   v[0] = pxh - pxl;
   v[1] = pyh - pyl;
   v[2] = pzh - pzl;
   t[0] = pxh + pxl - 2 * c0;
   t[3] = pyh + pyl - 2 * c0;
   t[5] = pzh + pzl - 2 * c0;
-  t[1] = 0.25 * ((double)phi[xh + YH + Z0] + (double)phi[xl + YL + Z0] - (double)phi[xh + YL + Z0] -
-                 (double)phi[xl + YH + Z0]);
-  t[2] = 0.25 * ((double)phi[xh + Y0 + ZH] + (double)phi[xl + Y0 + ZL] - (double)phi[xh + Y0 + ZL] -
-                 (double)phi[xl + Y0 + ZH]);
-  t[4] = 0.25 * ((double)phi[ix + YH + ZH] + (double)phi[ix + YL + ZL] - (double)phi[ix + YH + ZL] -
-                 (double)phi[ix + YL + ZH]);
+  t[1] = 0.25f * (phi[xh + YH + Z0] + phi[xl + YL + Z0] - phi[xh + YL + Z0] - phi[xl + YH + Z0]);
+  t[2] = 0.25f * (phi[xh + Y0 + ZH] + phi[xl + Y0 + ZL] - phi[xh + Y0 + ZL] - phi[xl + Y0 + ZH]);
+  t[4] = 0.25f * (phi[ix + YH + ZH] + phi[ix + YL + ZL] - phi[ix + YH + ZL] - phi[ix + YL + ZH]);
 }
 
 template <bool LENS>
