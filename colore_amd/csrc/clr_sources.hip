@@ -613,9 +613,10 @@ __device__ inline double get_rvel(const WriteArgs &a, int ix, int iy, int iz, do
   const int izh = iz == N - 1 ? 0 : iz + 1, izl = iz == 0 ? N - 1 : iz - 1;
   const size_t z0o = (size_t)iz * N * ngx, y0o = (size_t)iy * ngx;
   const double u0 = x0 / rr, u1 = y0 / rr, u2 = z0 / rr;
-  const double v0 = (double)phi[ixh + y0o + z0o] - (double)phi[ixl + y0o + z0o];
-  const double v1 = (double)phi[ix + (size_t)iyh * ngx + z0o] - (double)phi[ix + (size_t)iyl * ngx + z0o];
-  const double v2 = (double)phi[ix + y0o + (size_t)izh * N * ngx] - (double)phi[ix + y0o + (size_t)izl * N * ngx];
+  // FP32 differences of the stored potential, as get_rvel takes them
+  const double v0 = phi[ixh + y0o + z0o] - phi[ixl + y0o + z0o];
+  const double v1 = phi[ix + (size_t)iyh * ngx + z0o] - phi[ix + (size_t)iyl * ngx + z0o];
+  const double v2 = phi[ix + y0o + (size_t)izh * N * ngx] - phi[ix + y0o + (size_t)izl * N * ngx];
   return 0.5 * a.idx * (v0 * u0 + v1 * u1 + v2 * u2);
 }
 

@@ -572,9 +572,10 @@ static double rvel(int N, double L, const double *phi, int ix, int iy, int iz, d
   iz_0 *= ngx * N; iz_lo *= ngx * N; iz_hi *= ngx * N;
   iy_0 *= ngx; iy_lo *= ngx; iy_hi *= ngx;
   double u[3] = {x0 / rr, y0 / rr, z0 / rr}, v[3];
-  v[0] = phi[ix_hi + iy_0 + iz_0] - phi[ix_lo + iy_0 + iz_0];
-  v[1] = phi[ix_0 + iy_hi + iz_0] - phi[ix_0 + iy_lo + iz_0];
-  v[2] = phi[ix_0 + iy_0 + iz_hi] - phi[ix_0 + iy_0 + iz_lo];
+  /* flouble differences, rounded to float as the reference computes them */
+  v[0] = (float)phi[ix_hi + iy_0 + iz_0] - (float)phi[ix_lo + iy_0 + iz_0];
+  v[1] = (float)phi[ix_0 + iy_hi + iz_0] - (float)phi[ix_0 + iy_lo + iz_0];
+  v[2] = (float)phi[ix_0 + iy_0 + iz_hi] - (float)phi[ix_0 + iy_0 + iz_lo];
   return 0.5 * idx * (v[0] * u[0] + v[1] * u[1] + v[2] * u[2]);
 }
 
