@@ -9,6 +9,17 @@
  * either Philox4x32-10 counters (the production streams of the GPU path) or,
  * for the Fourier modes, a caller-supplied buffer (e.g. the reference's
  * per-thread MT19937 streams, orc_ref_mode_stream).
+ *
+ * Pins (tests/test_oracle.py, tests/test_host.py): known-answer vectors
+ * (Philox, MT19937 raw stream), numpy/scipy routines (FFTs, r2c, splines,
+ * quadrature), closed forms (modes, lognormal/clip, bias models, kappa/ISW,
+ * source beams, skewers, custom maps, 1LPT/2LPT displacements, imap at
+ * constant T), invariants and brute force (shell sums, HEALPix, deposits).
+ * Parity unpinned (no library or reference build here to pin against):
+ * GSL's large-lambda Poisson sampler (orc_poisson uses inversion, pinned
+ * statistically), GSL's MT19937 uniform mapping, FFTW's treatment of the
+ * non-Hermitian kx = 0 and N/2 planes (numpy's convention only), chealpix's
+ * exact code paths (closed forms only).
  */
 #ifndef COLORE_ORACLE_H
 #define COLORE_ORACLE_H
