@@ -242,6 +242,96 @@ __global__ __launch_bounds__(256) void k_lpt_deposit(const float *__restrict__ p
   }
 }
 
+// The same CIC / TSC assignment with the weights first summed in LDS: a
+// block takes a 32 x 16 x 16 tile of particles (32 per thread) and holds the
+// grid region of the tile grown by 4 cells on every side (40 x 24 x 24
+// floats, 90 KB); weights landing in the region go to LDS, the rare ones
+// further out straight to the grid, and the region is then added to the grid
+// once (about 2.8 global atomics per particle instead of 8 for CIC, 27 for
+// TSC).  Needs N >= 64 and N a multiple of 32, so that the region never wraps
+// onto itself.
+constexpr int kDTX = 32, kDTY = 16, kDTZ = 16, kDHalo = 4;
+constexpr int kDRX = kDTX + 2 * kDHalo, kDRY = kDTY + 2 * kDHalo, kDRZ = kDTZ + 2 * kDHalo;
+constexpr size_t kDepositLds = (size_t)kDRX * kDRY * kDRZ * sizeof(float);
+
+__global__ __launch_bounds__(256) void k_lpt_deposit_tiled(const float *__restrict__ px, const float *__restrict__ py,
+                                                           const float *__restrict__ pz, float *__restrict__ g,
+                                                           int N, int ngxi, float iag, int interp) {
+#pragma clang fp contract(off)
+  extern __shared__ float reg[];
+  const size_t ngx = (size_t)ngxi, pl = (size_t)N * ngx;
+  const int ntx = N / kDTX, nty = N / kDTY;
+  const int b = blockIdx.x;
+  const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
+  const int ox = tx * kDTX - kDHalo, oy = ty * kDTY - kDHalo, oz = tz * kDTZ - kDHalo;
+  for (int i = threadIdx.x; i < kDRX * kDRY * kDRZ; i += blockDim.x) reg[i] = 0.f;
+  __syncthreads();
+  // relative coordinate of cell c in the region along one axis (c in [0, N))
+  auto rel = [N](int c, int o) {
+    int d = c - o;
+    if (d < 0) d += N;
+    if (d >= N) d -= N;
+    return d;
+  };
+  auto add = [&](int cx, int cy, int cz, float w) {
+    const int rx = rel(cx, ox), ry = rel(cy, oy), rz = rel(cz, oz);
+    if (rx < kDRX && ry < kDRY && rz < kDRZ) atomicAdd(&reg[rx + kDRX * (ry + kDRY * rz)], w);
+    else atomicAdd(g + cx + ngx * cy + pl * cz, w);
+  };
+  for (int q = threadIdx.x; q < kDTX * kDTY * kDTZ; q += blockDim.x) {
+    const int lx = q % kDTX, ly = (q / kDTX) % kDTY, lz = q / (kDTX * kDTY);
+    const size_t idx = (size_t)(tx * kDTX + lx) + ngx * ((size_t)(ty * kDTY + ly) + (size_t)N * (tz * kDTZ + lz));
+    const float p[3] = {px[idx], py[idx], pz[idx]};
+    if (interp == 1) {
+      int i0[3], i1[3];
+      float a0[3], a1[3];
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++) {
+        const float xs = p[ax] * iag;
+        const int c = (int)xs;
+        a1[ax] = xs - c;
+        a0[ax] = 1 - a1[ax];
+        i0[ax] = wrapn(c, N);
+        i1[ax] = wrapn(c + 1, N);
+      }
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+        add(bx ? i1[0] : i0[0], by ? i1[1] : i0[1], bz ? i1[2] : i0[2],
+            (bx ? a1[0] : a0[0]) * (by ? a1[1] : a0[1]) * (bz ? a1[2] : a0[2]));
+      }
+    } else {
+      int ic[3][3];
+      float wt[3][3];
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++) {
+        const float xs = p[ax] * iag;
+        const int i0 = (int)floorf((float)((double)xs + 0.5));
+        const float d = xs - i0;
+        wt[ax][0] = (float)(0.5 * (0.5 - d) * (0.5 - d));
+        wt[ax][2] = (float)(0.5 * (0.5 + d) * (0.5 + d));
+        wt[ax][1] = (float)(0.75 - d * d);
+        ic[ax][0] = wrapn(i0 - 1, N);
+        ic[ax][1] = wrapn(i0, N);
+        ic[ax][2] = wrapn(i0 + 1, N);
+      }
+      for (int sz = 0; sz < 3; sz++)
+        for (int sy = 0; sy < 3; sy++)
+#pragma unroll
+          for (int sx = 0; sx < 3; sx++)
+            add(ic[0][sx], ic[1][sy], ic[2][sz], wt[0][sx] * wt[1][sy] * wt[2][sz]);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kDRX * kDRY * kDRZ; i += blockDim.x) {
+    const float v = reg[i];
+    if (v == 0.f) continue;
+    const int rx = i % kDRX, ry = (i / kDRX) % kDRY, rz = i / (kDRX * kDRY);
+    const int cx = wrapn(ox + rx, N), cy = wrapn(oy + ry, N), cz = wrapn(oz + rz, N);
+    atomicAdd(g + cx + ngx * cy + pl * cz, v);
+  }
+}
+
 // density.c:609-622: delta = rho * 1 - 1 on the real cells
 __global__ __launch_bounds__(256) void k_lpt_norm(float *__restrict__ g, int N, int ngx) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,9 +472,18 @@ int clr_launch_lpt(clr_ctx *c, int order, int interp, float *pos_out) {
   }
   if (ok()) e = hipMemsetAsync(c->dens, 0, gbytes, c->stream);
   if (ok()) {
-    hipLaunchKernelGGL(k_lpt_deposit, gc, blk, 0, c->stream, pos[0], pos[1], pos[2], c->dens, N, ngx,
-                       (float)(N / c->L), interp);
-    e = hipGetLastError();
+    if (interp != 0 && N >= 64 && N % kDTX == 0) {
+      e = hipFuncSetAttribute((const void *)k_lpt_deposit_tiled, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kDepositLds);
+      const unsigned tiles = (unsigned)((N / kDTX) * (N / kDTY) * (N / kDTZ));
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_lpt_deposit_tiled, dim3(tiles), blk, kDepositLds, c->stream, pos[0], pos[1], pos[2],
+                           c->dens, N, ngx, (float)(N / c->L), interp);
+    } else {
+      hipLaunchKernelGGL(k_lpt_deposit, gc, blk, 0, c->stream, pos[0], pos[1], pos[2], c->dens, N, ngx,
+                         (float)(N / c->L), interp);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
   }
   if (ok()) {
     hipLaunchKernelGGL(k_lpt_norm, gc, blk, 0, c->stream, c->dens, N, ngx);
