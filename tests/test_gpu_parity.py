@@ -335,14 +335,15 @@ def test_source_beams_192(workdir):
         assert np.abs(g.astype(np.float64) - o).max() <= 1e-5 * rms, k
 
 
-@pytest.mark.parametrize("order,interp", [(1, 0), (1, 1), (1, 2), (2, 1), (2, 2)])
-def test_lpt_density(workdir, order, interp):
+@pytest.mark.parametrize("order,interp,N", [(1, 0, 64), (1, 1, 64), (1, 2, 64), (2, 1, 64), (2, 2, 64),
+                                            (1, 1, 128), (1, 2, 128)])
+def test_lpt_density(workdir, order, interp, N):
     """1LPT and 2LPT densities from a Gaussian field on the device against the
     oracle's FP64 transforms and deposit of the same field: CIC and TSC within 1e-4
     (FP32 FFT and atomics), NGP counts equal but in at most 1e-4 of the cells
     (a particle within rounding of a cell boundary), the same mass.  The
-    field is the generated one times 25, so that displacements cross cells."""
-    N = 64
+    field is the generated one times 25, so that displacements cross cells.
+    At 128^3 the LDS-staged deposit has interior tiles as well as wrapped ones."""
     path, run = make_run("C1", N, workdir, with_maps=False)
     ctx = _ctx_for(run, N)
     s2 = ctx.gaussian_fields(int(run.scalar("seed")), run.scalar("r_smooth"), True)
