@@ -115,6 +115,13 @@ def time_maps(run, ctx, config, sigma2):
         ts.append(ctx.stage_times()["beams"])
     ms = sorted(ts)[1]
     out["src_beams"] = {"ms": ms, "sources": n0, "sources_per_s": n0 / (ms * 1e-3)}
+    # skewers of the smallest population (N/2 density and velocity values per source)
+    counts = [ctx.sample_sources(i, int(run.scalar("seed")), 2) for i in range(int(run.scalar("n_srcs")))]
+    ism = int(min(range(len(counts)), key=lambda i: counts[i]))
+    ctx.source_skewers(ism, counts[ism], False)
+    ms = ctx.stage_times()["skewers"]
+    out["src_skewers"] = {"ms": ms, "population": ism, "sources": counts[ism],
+                          "values_per_s": counts[ism] * (N // 2) / (ms * 1e-3)}
     if int(run.scalar("n_imap")) > 0:
         # the intensity species joins the shell sums of the last step's density
         amp, bz, model = run.tracer(1, 0)

@@ -36,6 +36,7 @@ enum StageId {
   ST_SHELLS,      // the k_shells launch of a normalisation call
   ST_BEAMS,       // k_src_beams of a clr_source_beams call
   ST_CSTM,        // k_cstm of a clr_custom_map call
+  ST_SKW,         // the k_src_skewers launches of a clr_source_skewers call (kernels only)
   ST_COUNT
 };
 

@@ -577,19 +577,26 @@ int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, fl
   float *tmp = nullptr;
   CLR_HIP(c, hipMalloc((void **)&tmp, (size_t)cap * a.nr * 2 * sizeof(float)));
   hipError_t e = hipEventRecord(c->ev0, c->stream);
+  float kms = 0;
   for (long long s0 = 0; s0 < n && e == hipSuccess; s0 += cap) {
     const long long m = std::min(cap, n - s0);
     const long long tot = m * a.nr;
-    hipLaunchKernelGGL(k_src_skewers, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
-                       (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
-    e = hipGetLastError();
+    e = hipEventRecord(c->ev2, c->stream);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(k_src_skewers, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
+                         (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(c->ev3, c->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(dg + (size_t)s0 * a.nr, tmp, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(vs + (size_t)s0 * a.nr, tmp + (size_t)cap * a.nr, (size_t)tot * sizeof(float),
                          hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float cms = 0;
+    if (e == hipSuccess && hipEventElapsedTime(&cms, c->ev2, c->ev3) == hipSuccess) kms += cms;
   }
+  c->stage_ms[ST_SKW] = kms;
   if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);

@@ -20,7 +20,7 @@ CLR_BIAS_EXPONENTIAL, CLR_BIAS_EXP_TRUNCATED, CLR_BIAS_TRUNCATED = 0, 1, 2
 CLR_DENS_LOGNORMAL, CLR_DENS_CLIP = 0, 3
 CLR_RNG_PHILOX, CLR_RNG_INJECTED = 0, 1
 STAGES = ["gen", "fft_yz", "fft_x", "density", "norm", "srcs", "kappa", "isw", "imap", "src_pass", "src_write",
-          "fft_phi", "shells", "beams", "cstm"]
+          "fft_phi", "shells", "beams", "cstm", "skewers"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)

@@ -255,8 +255,8 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
  * and per-source positions), the y and x passes of phi (these run on a second
  * stream, overlapping the density's passes and the stages after them; every
  * call that reads phi waits for them), the shell-sum kernel alone, the
- * source-beam kernel, the custom-map kernel.  Fills at most n values; returns
- * the number of stages. */
+ * source-beam kernel, the custom-map kernel, the skewer kernels (summed over
+ * their chunks).  Fills at most n values; returns the number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 
 #ifdef __cplusplus
