@@ -83,9 +83,10 @@ def traffic_per_launch(kernel, config, N):
 
 def time_maps(run, ctx, config, sigma2):
     """kappa and ISW at the workload's source planes, the first population's
-    line-of-sight pass and the 21cm shells of the first intensity species, on
-    the last step's grids, then a 1LPT (CIC) density of a new realisation:
-    device ms per call (median of 3; best of 2 for 1LPT) and work rates."""
+    line-of-sight pass, the smallest population's skewers, the 21cm shells of
+    the first intensity species and a custom map, on the last step's grids,
+    then a 1LPT (CIC) density of a new realisation: device ms per call
+    (median of 3; one call for skewers, best of 2 for 1LPT) and work rates."""
     import numpy as np
     from colore_amd import lib
     out = {}
@@ -134,6 +135,17 @@ def time_maps(run, ctx, config, sigma2):
             ts.append(ctx.stage_times()["imap"])
         ms = sorted(ts)[1]
         out["imap"] = {"ms": ms, "nside": nside, "shells": len(r0), "cells_per_s": N ** 3 / (ms * 1e-3)}
+    # a custom projected map at nside 512 whose kernel K(r) is the first population's n(r)
+    amp, bz, model = run.tracer(0, 0)
+    ctx.set_tracer(2, 0, model, amp, bz)
+    ctx.shell_norms(lognormal_fused=False, sigma2=sigma2,
+                    ntracers=int(run.scalar("n_srcs")) + int(run.scalar("n_imap")) + 1)
+    ts = []
+    for _ in range(3):
+        ctx.custom_map(0, 512)
+        ts.append(ctx.stage_times()["cstm"])
+    ms = sorted(ts)[1]
+    out["custom_map"] = {"ms": ms, "nside": 512, "pixels_per_s": 12 * 512 * 512 / (ms * 1e-3)}
     # 1LPT density (CIC) of a fresh realisation, in place of the lognormal transform
     ts = []
     for _ in range(2):
