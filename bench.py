@@ -13,7 +13,7 @@ Prints one JSON line.  `value` is grid cells per second of the whole step.
 `roofline` is for the kernel with the largest device time: algorithmic HBM
 bytes per launch over its mean duration (HIP events on the library's stream),
 against 8 TB/s.  `cpu_baseline` times the FP64 CPU oracle on the same
-configuration at 256^3 on the host cores.
+configuration at 512^3 on the host cores.
 """
 import argparse
 import json

@@ -360,6 +360,27 @@ int clr_fetch_grid(clr_ctx *c, int which, float *host) {
   return CLR_OK;
 }
 
+int clr_fetch_grid_slab(clr_ctx *c, int which, int iz0, int nz, float *host) {
+  if (!c || !host) return CLR_ERR_ARG;
+  if (which != CLR_GRID_DENS && which != CLR_GRID_NPOT) return clr_fail(c, CLR_ERR_ARG, "bad grid %d", which);
+  if (iz0 < 0 || nz < 0 || iz0 + nz > c->N) return clr_fail(c, CLR_ERR_ARG, "planes %d..%d outside the grid", iz0, iz0 + nz);
+  CLR_HIP(c, hipSetDevice(c->device));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  int rc = clr_phi_ready(c, 1);
+  if (rc) return rc;
+  if (nz == 0) return CLR_OK;
+  const size_t hrow = (size_t)2 * c->NH * sizeof(float), drow = (size_t)c->ngx * sizeof(float);
+  const float *src = (which == CLR_GRID_DENS ? c->dens : c->npot) + (size_t)iz0 * c->N * c->ngx;
+  CLR_HIP(c, hipMemcpy2D(host, hrow, src, drow, hrow, (size_t)nz * c->N, hipMemcpyDeviceToHost));
+  return CLR_OK;
+}
+
+int clr_fetch_cell_counts(clr_ctx *c, int ipop, int iz0, int nz, int32_t *counts) {
+  if (!c || !counts) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_fetch_counts(c, ipop, iz0, nz, counts);
+}
+
 int clr_upload_grid(clr_ctx *c, int which, const float *host) {
   if (!c || !host) return CLR_ERR_ARG;
   if (which != CLR_GRID_DENS && which != CLR_GRID_NPOT) return clr_fail(c, CLR_ERR_ARG, "bad grid %d", which);

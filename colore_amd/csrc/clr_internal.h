@@ -100,6 +100,12 @@ struct clr_ctx {
   double d2_first = 0, d2_last = 0;
   void *misc = nullptr;     // generic scratch
   size_t misc_n = 0;
+  // the count bytes of the last clr_sample_sources call: populations
+  // cnt_p0 .. cnt_p0+cnt_np-1 at rowcnt + (ipop-cnt_p0)*cnt_per_pop, cnt_nesc
+  // sorted escape entries at esc+16
+  int cnt_p0 = -1, cnt_np = 0;
+  size_t cnt_per_pop = 0;
+  unsigned int cnt_nesc = 0;
 
   int have_fields = 0;
   int density_done = 0;
@@ -132,6 +138,7 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
 int clr_phi_ready(clr_ctx *c, int host);
 int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc);
 int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
+int clr_launch_fetch_counts(clr_ctx *c, int ipop, int iz0, int nz, int32_t *out);
 int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out);
 int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz, float *lens, long long n);
 int clr_launch_cstm(clr_ctx *c, int ipop, int nside, float *out);

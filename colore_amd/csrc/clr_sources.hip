@@ -686,6 +686,20 @@ __global__ __launch_bounds__(256) void k_write(WriteArgs a) {
   a.ipix[pid] = a.r2n[vec2pix_ring32(a.nside_base, (double)pp[0], (double)pp[1], (double)pp[2])];
 }
 
+// Per-cell counts (the reference's nsources array, srcs.c:125, without the
+// row padding) of cells [cell0, cell0+n) of population slot q: the count
+// byte, or the escape-list entry of a saturated byte.
+__global__ void k_expand_counts(const uint8_t *__restrict__ cnt8, const ulonglong2 *__restrict__ esc,
+                                unsigned nesc, unsigned long long q, uint64_t cell0, uint64_t n,
+                                int32_t *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t cell = cell0 + i;
+  int k = cnt8[cell];
+  if (k == kByteEscape) k = escaped_count(esc, nesc, (q << 56) | cell);
+  out[i] = k;
+}
+
 // Src fields of srcs_get_local_properties_single: RA, Dec, z0, dz_rsd.
 __global__ void k_src_props(const float4 *__restrict__ pos, long long n, const double *__restrict__ r_arr,
                             const double *__restrict__ zt, double idr, double r_last, double z_last,
@@ -844,6 +858,10 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     }
   }
   a.esc_count = nesc;
+  c->cnt_p0 = p0;
+  c->cnt_np = np;
+  c->cnt_per_pop = per_pop;
+  c->cnt_nesc = nesc;
   if (nesc > 0) {
     // sorted by key for k_fill's binary search
     std::vector<ulonglong2> h(nesc);
@@ -940,5 +958,26 @@ int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "source properties: %s", hipGetErrorString(e));
+  return CLR_OK;
+}
+
+int clr_launch_fetch_counts(clr_ctx *c, int ipop, int iz0, int nz, int32_t *out) {
+  const int N = c->N;
+  if (iz0 < 0 || nz < 0 || iz0 + nz > N) return clr_fail(c, CLR_ERR_ARG, "planes %d..%d outside the grid", iz0, iz0 + nz);
+  if (c->cnt_p0 < 0 || ipop < c->cnt_p0 || ipop >= c->cnt_p0 + c->cnt_np)
+    return clr_fail(c, CLR_ERR_STATE, "population %d was not sampled by the last clr_sample_sources call", ipop);
+  if (nz == 0) return CLR_OK;
+  const int q = ipop - c->cnt_p0;
+  const uint8_t *cnt8 = (const uint8_t *)c->rowcnt + (size_t)q * c->cnt_per_pop;
+  const uint64_t n = (uint64_t)N * N * nz, cell0 = (uint64_t)N * N * iz0;
+  int32_t *tmp = nullptr;
+  CLR_HIP(c, hipMalloc((void **)&tmp, n * sizeof(int32_t)));
+  hipLaunchKernelGGL(k_expand_counts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cnt8,
+                     (const ulonglong2 *)((char *)c->esc + 16), c->cnt_nesc, (unsigned long long)q, cell0, n, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "cell counts: %s", hipGetErrorString(e));
   return CLR_OK;
 }

@@ -87,6 +87,8 @@ def load_amd():
     _sig(lib, "clr_imap", ci, [_vp, ci, ci, ci, _d, _d, _f, _i32])
     _sig(lib, "clr_fetch_grid", ci, [_vp, ci, _f])
     _sig(lib, "clr_upload_grid", ci, [_vp, ci, _f])
+    _sig(lib, "clr_fetch_grid_slab", ci, [_vp, ci, ci, ci, _f])
+    _sig(lib, "clr_fetch_cell_counts", ci, [_vp, ci, ci, ci, _i32])
     _sig(lib, "clr_declare_grids", ci, [_vp, ctypes.c_double, ci])
     _sig(lib, "clr_stage_times", ci, [_vp, _d, ci])
     return lib
@@ -348,6 +350,19 @@ class Context:
         g = np.empty(self.padded_shape, dtype=np.float32)
         self._chk(self.lib.clr_fetch_grid(self.h, which, ptr(g, ctypes.c_float)))
         return g
+
+    def fetch_grid_slab(self, which, iz0, nz):
+        """Planes iz0 .. iz0+nz-1 of a grid, float32[nz][N][2(N/2+1)]."""
+        g = np.empty((nz,) + self.padded_shape[1:], dtype=np.float32)
+        self._chk(self.lib.clr_fetch_grid_slab(self.h, which, iz0, nz, ptr(g, ctypes.c_float)))
+        return g
+
+    def fetch_cell_counts(self, ipop, iz0, nz):
+        """Per-cell source counts of population ipop for planes iz0 .. iz0+nz-1,
+        int32[nz][N][N] (z, y, x)."""
+        c = np.empty((nz, self.N, self.N), dtype=np.int32)
+        self._chk(self.lib.clr_fetch_cell_counts(self.h, ipop, iz0, nz, ptr(c, ctypes.c_int32)))
+        return c
 
     def upload_grid(self, which, g):
         g = np.ascontiguousarray(g, dtype=np.float32)

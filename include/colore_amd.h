@@ -49,7 +49,7 @@ typedef struct clr_ctx clr_ctx;
 #define CLR_BIAS_EXP_TRUNCATED 1   /* _BIAS_MODEL_2                            */
 #define CLR_BIAS_TRUNCATED     2   /* _BIAS_MODEL_3                            */
 
-/* density types, src/common.h:115-118 (1LPT/2LPT are not implemented here) */
+/* density types, src/common.h:115-118 (1LPT/2LPT: clr_lpt_density) */
 #define CLR_DENS_LOGNORMAL 0
 #define CLR_DENS_CLIP      3
 
@@ -243,6 +243,15 @@ int clr_imap(clr_ctx *ctx, int ipop, int nside, int nr, const double *r0,
  * float[N][N][2(N/2+1)]. */
 int clr_fetch_grid(clr_ctx *ctx, int which, float *host);
 int clr_upload_grid(clr_ctx *ctx, int which, const float *host);
+/* The z-slab of planes iz0 .. iz0+nz-1 of a grid, in the same padded layout
+ * float[nz][N][2(N/2+1)] -- what one rank's density dump holds (nz_here
+ * planes from iz0_here, io.c:565-595). */
+int clr_fetch_grid_slab(clr_ctx *ctx, int which, int iz0, int nz, float *host);
+/* Per-cell source counts of population ipop from the last clr_sample_sources
+ * call that sampled it (the nsources array of srcs_set_cartesian_single,
+ * srcs.c:125-183, without its row padding), for the planes iz0 .. iz0+nz-1:
+ * counts[(iz-iz0) N^2 + iy N + ix]. */
+int clr_fetch_cell_counts(clr_ctx *ctx, int ipop, int iz0, int nz, int32_t *counts);
 /* Declares uploaded grids as the output of the field stage (sigma2 is the
  * measured variance to use) and, if density_transformed, of the density
  * stage too, so that later stages can run on caller-supplied grids. */

@@ -5,6 +5,7 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -139,52 +140,97 @@ double orc_bias_model(double d, double b, int model) {
  * ====================================================================== */
 
 /* create_grids_fourier, fourier.c:285-359 */
+/* One mode of create_grids_fourier (fourier.c:314-353) at half-spectrum
+ * index (kk, jj, ii): frequencies i <= N/2 -> +i dk, else -(N-i) dk; k = 0
+ * is zero; (phase, modulus) from rng_delta_gauss (common.c:191-201). */
+static void mode_at(int N, double L, const orc_pk *pk, uint32_t seed, const uint32_t *inj, double r2,
+                    int do_smoothing, int smooth_potential, double prefac_lensing, int kk, int jj, int ii,
+                    double d[2], double p[2]) {
+  const int NH = N / 2 + 1;
+  const double dk = 2 * M_PI / L;
+  const double idk3 = 1. / (dk * dk * dk);
+  const double kz = (2 * ii <= N) ? ii * dk : -(N - ii) * dk;
+  const double ky = (2 * jj <= N) ? jj * dk : -(N - jj) * dk;
+  const double kx = kk * dk;
+  const size_t index = (size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * ii);
+  const double k_mod2 = kx * kx + ky * ky + kz * kz;
+  if (k_mod2 <= 0) { d[0] = d[1] = p[0] = p[1] = 0; return; }
+  const double lgk = 0.5 * log10(k_mod2);
+  const double sigma2 = orc_pk_linear0(pk, lgk) * idk3;
+  uint32_t x0, x1;
+  if (inj) { x0 = inj[2 * index]; x1 = inj[2 * index + 1]; }
+  else {
+    /* one Philox draw per (kz, kz+N/2) pair of modes */
+    uint32_t r[4];
+    const size_t site = (size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * (ii % (N / 2)));
+    orc_draw(site, 0u, 0u, seed, r);
+    const int hi = ii >= N / 2;
+    x0 = r[2 * hi]; x1 = r[2 * hi + 1];
+  }
+  const double phase = 2 * M_PI * orc_u01(x0);
+  const double u = orc_u01(x1);
+  const double mod = sqrt(-sigma2 * log(1 - u));
+  double dre = mod * cos(phase), dim = mod * sin(phase);
+  double pre = -prefac_lensing * dre / k_mod2, pim = -prefac_lensing * dim / k_mod2;
+  if (do_smoothing) {
+    const double sm = exp(-0.5 * r2 * k_mod2);
+    dre *= sm; dim *= sm;
+    if (smooth_potential) { pre *= sm; pim *= sm; }
+  }
+  d[0] = dre; d[1] = dim; p[0] = pre; p[1] = pim;
+}
+
 void orc_gen_modes(int N, double L, const orc_pk *pk, uint32_t seed, const uint32_t *inj,
                    double r_smooth, int smooth_potential, double prefac_lensing,
                    double *dens_k, double *npot_k) {
   const int NH = N / 2 + 1;
-  const double dk = 2 * M_PI / L;
-  const double idk3 = 1. / (dk * dk * dk);
   const int do_smoothing = r_smooth > 0;
   const double r2 = do_smoothing ? r_smooth * r_smooth : 0;
 #pragma omp parallel for schedule(static)
-  for (int ii = 0; ii < N; ii++) {
-    const double kz = (2 * ii <= N) ? ii * dk : -(N - ii) * dk;
-    for (int jj = 0; jj < N; jj++) {
-      const double ky = (2 * jj <= N) ? jj * dk : -(N - jj) * dk;
+  for (int ii = 0; ii < N; ii++)
+    for (int jj = 0; jj < N; jj++)
       for (int kk = 0; kk <= N / 2; kk++) {
-        const double kx = kk * dk;
         const size_t index = (size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * ii);
-        const double k_mod2 = kx * kx + ky * ky + kz * kz;
-        double *d = dens_k + 2 * index, *p = npot_k + 2 * index;
-        if (k_mod2 <= 0) { d[0] = d[1] = p[0] = p[1] = 0; continue; }
-        const double lgk = 0.5 * log10(k_mod2);
-        const double sigma2 = orc_pk_linear0(pk, lgk) * idk3;
-        uint32_t x0, x1;
-        if (inj) { x0 = inj[2 * index]; x1 = inj[2 * index + 1]; }
-        else {
-          /* one Philox draw per (kz, kz+N/2) pair of modes */
-          uint32_t r[4];
-          const size_t site = (size_t)kk + (size_t)NH * ((size_t)jj + (size_t)N * (ii % (N / 2)));
-          orc_draw(site, 0u, 0u, seed, r);
-          const int hi = ii >= N / 2;
-          x0 = r[2 * hi]; x1 = r[2 * hi + 1];
-        }
-        /* rng_delta_gauss, common.c:191-201 */
-        const double phase = 2 * M_PI * orc_u01(x0);
-        const double u = orc_u01(x1);
-        const double mod = sqrt(-sigma2 * log(1 - u));
-        double dre = mod * cos(phase), dim = mod * sin(phase);
-        double pre = -prefac_lensing * dre / k_mod2, pim = -prefac_lensing * dim / k_mod2;
-        if (do_smoothing) {
-          const double sm = exp(-0.5 * r2 * k_mod2);
-          dre *= sm; dim *= sm;
-          if (smooth_potential) { pre *= sm; pim *= sm; }
-        }
-        d[0] = dre; d[1] = dim; p[0] = pre; p[1] = pim;
+        mode_at(N, L, pk, seed, inj, r2, do_smoothing, smooth_potential, prefac_lensing, kk, jj, ii,
+                dens_k + 2 * index, npot_k + 2 * index);
       }
-    }
-  }
+}
+
+/* Parseval for the c2r of fourier.c:81-102 followed by the normalisation
+ * (sqrt(2 pi)/L)^3 of fourier.c:383-398: the real grid is the inverse DFT of
+ * the Hermitian spectrum F with F = X on 0 < kx < N/2 and, on the kx = 0 and
+ * kx = N/2 planes (whose imaginary parts the c2r drops after the z and y
+ * transforms), F(ky,kz) = (X(ky,kz) + conj X(-ky,-kz))/2.  So
+ *   <grid^2> = norm^2 [ 2 sum_{0<kx<N/2} |X|^2 + sum_{kx=0,N/2} |F|^2 ].
+ * Modes are regenerated one at a time (no spectrum is stored), so this runs
+ * at any N.  out[0] = <delta^2>, out[1] = <phi^2> over the N^3 cells. */
+void orc_kspace_variance(int N, double L, const orc_pk *pk, uint32_t seed, const uint32_t *inj,
+                         double r_smooth, int smooth_potential, double prefac_lensing, double out[2]) {
+  const int do_smoothing = r_smooth > 0;
+  const double r2 = do_smoothing ? r_smooth * r_smooth : 0;
+  double sd = 0, sp = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sd, sp)
+  for (int ii = 0; ii < N; ii++)
+    for (int jj = 0; jj < N; jj++)
+      for (int kk = 0; kk <= N / 2; kk++) {
+        double d[2], p[2];
+        mode_at(N, L, pk, seed, inj, r2, do_smoothing, smooth_potential, prefac_lensing, kk, jj, ii, d, p);
+        if (kk > 0 && 2 * kk < N) {
+          sd += 2 * (d[0] * d[0] + d[1] * d[1]);
+          sp += 2 * (p[0] * p[0] + p[1] * p[1]);
+        } else {
+          double dm[2], pm[2];
+          mode_at(N, L, pk, seed, inj, r2, do_smoothing, smooth_potential, prefac_lensing, kk, (N - jj) % N,
+                  (N - ii) % N, dm, pm);
+          const double fr = 0.5 * (d[0] + dm[0]), fi = 0.5 * (d[1] - dm[1]);
+          const double gr = 0.5 * (p[0] + pm[0]), gi = 0.5 * (p[1] - pm[1]);
+          sd += fr * fr + fi * fi;
+          sp += gr * gr + gi * gi;
+        }
+      }
+  const double nrm = pow(sqrt(2 * M_PI) / L, 3);
+  out[0] = sd * nrm * nrm;
+  out[1] = sp * nrm * nrm;
 }
 
 /* Stockham autosort complex inverse FFT (decimation in frequency), radices
@@ -522,7 +568,9 @@ int orc_poisson(double lam, uint64_t cell, uint32_t stream, uint32_t seed) {
   return k;
 }
 
-int orc_cell_count(const orc_pop *p, const double *dens, int ix, int iy, int iz) {
+/* lambda of srcs.c:158-176: n(r) dx^3 bias_model(delta, b(r)) norm(r) for a
+ * cell with r < L/2 + 20 and n(r) > 0, else 0 (no draw). */
+double orc_cell_lambda(const orc_pop *p, const double *dens, int ix, int iy, int iz) {
   const int N = p->N, ngx = 2 * (N / 2 + 1);
   const double dx = p->L / N;
   const double cell_vol = dx * dx * dx;
@@ -536,7 +584,12 @@ int orc_cell_count(const orc_pop *p, const double *dens, int ix, int iy, int iz)
   const double bias = orc_f_of_r(p->bg, p->bz, p->bz0, 1, r);
   const double dnorm = orc_f_of_r(p->bg, p->norm, p->norm0, p->normf, r);
   const size_t index = (size_t)ix + (size_t)ngx * ((size_t)iy + (size_t)N * iz);
-  const double lambda = ndens * cell_vol * orc_bias_model(dens[index], bias, p->model) * dnorm;
+  return ndens * cell_vol * orc_bias_model(dens[index], bias, p->model) * dnorm;
+}
+
+int orc_cell_count(const orc_pop *p, const double *dens, int ix, int iy, int iz) {
+  const int N = p->N;
+  const double lambda = orc_cell_lambda(p, dens, ix, iy, iz);
   const uint64_t cell = (uint64_t)ix + (uint64_t)N * ((uint64_t)iy + (uint64_t)N * iz);
   return orc_poisson(lambda, cell, 0x100u + (uint32_t)p->ipop, p->seed);
 }
@@ -552,6 +605,24 @@ long long orc_counts(const orc_pop *p, const double *dens, int *counts) {
         if (counts) counts[(size_t)ix + (size_t)N * ((size_t)iy + (size_t)N * iz)] = c;
         tot += c;
       }
+  return tot;
+}
+
+/* orc_counts for the planes iz0 .. iz0+nz-1 only; dens_slab holds those
+ * planes in the padded layout, counts[(iz-iz0) N^2 + iy N + ix]. */
+long long orc_counts_planes(const orc_pop *p, const double *dens_slab, int iz0, int nz, int *counts) {
+  const int N = p->N, ngx = 2 * (N / 2 + 1);
+  const double *dens = dens_slab - (ptrdiff_t)iz0 * N * ngx;   /* indexed by the global iz */
+  long long tot = 0;
+#pragma omp parallel for reduction(+ : tot) schedule(static)
+  for (long row = 0; row < (long)nz * N; row++) {
+    const int iz = iz0 + (int)(row / N), iy = (int)(row % N);
+    for (int ix = 0; ix < N; ix++) {
+      const int c = orc_cell_count(p, dens, ix, iy, iz);
+      counts[(size_t)row * N + ix] = c;
+      tot += c;
+    }
+  }
   return tot;
 }
 

@@ -60,6 +60,10 @@ double orc_bias_model(double d, double b, int model);
 void orc_gen_modes(int N, double L, const orc_pk *pk, uint32_t seed, const uint32_t *inj,
                    double r_smooth, int smooth_potential, double prefac_lensing,
                    double *dens_k, double *npot_k);
+/* <delta^2> and <phi^2> of the normalised real grids from the modes alone
+ * (Parseval with the c2r's handling of the kx = 0, N/2 planes), any N. */
+void orc_kspace_variance(int N, double L, const orc_pk *pk, uint32_t seed, const uint32_t *inj,
+                         double r_smooth, int smooth_potential, double prefac_lensing, double out[2]);
 /* 3-D c2r, FFTW convention (complex IDFT over z and y, then c2r along x using
  * Re of kx=0 and kx=N/2), unnormalised.  out: padded [N][N][2(N/2+1)]. */
 void orc_c2r_3d(int N, const double *in, double *out);
@@ -95,10 +99,14 @@ typedef struct {
   uint32_t seed;
   double fgrowth_0, hubble_0, omega_m;
 } orc_pop;
+/* lambda of one cell (srcs.c:158-176; 0 outside r < L/2+20 or where n <= 0). */
+double orc_cell_lambda(const orc_pop *p, const double *dens, int ix, int iy, int iz);
 /* Poisson count of one cell (srcs.c:158-176) with the Philox stream. */
 int orc_cell_count(const orc_pop *p, const double *dens, int ix, int iy, int iz);
 /* All counts: counts[N^3] in cell order ix + N(iy + N iz); returns total. */
 long long orc_counts(const orc_pop *p, const double *dens, int *counts);
+/* orc_counts for planes iz0 .. iz0+nz-1 (dens_slab: those planes, padded). */
+long long orc_counts_planes(const orc_pop *p, const double *dens_slab, int iz0, int nz, int *counts);
 /* Catalogue in z-major cell order: pos4[4n] (x,y,z,dz_rsd), ipix[n]
  * (NEST at nside_base).  Returns n (<= cap), or -1 if cap is too small. */
 long long orc_sources(const orc_pop *p, const double *dens, const double *npot, int nside_base,

@@ -61,6 +61,7 @@ def load(build=True):
     _sig(lib, "orc_bias_model", cd, [cd, cd, ci])
     _sig(lib, "orc_gen_modes", None, [ci, cd, ctypes.POINTER(Pk), cu, _u32, cd, ci, cd, _d, _d])
     _sig(lib, "orc_c2r_3d", None, [ci, _d, _d])
+    _sig(lib, "orc_kspace_variance", None, [ci, cd, ctypes.POINTER(Pk), cu, _u32, cd, ci, cd, _d])
     _sig(lib, "orc_c2r_3d_brute", None, [ci, _d, _d])
     _sig(lib, "orc_cfft_inv", None, [ci, _d])
     _sig(lib, "orc_normalize_sigma2", cd, [ci, cd, _d, _d, _d])
@@ -69,7 +70,9 @@ def load(build=True):
                                      _u64, _d, _d])
     _sig(lib, "orc_norm_table", None, [ctypes.POINTER(Bg), ci, cd, _u64, _d, _d, _d, _d, _d])
     _sig(lib, "orc_cell_count", ci, [ctypes.POINTER(Pop), _d, ci, ci, ci])
+    _sig(lib, "orc_cell_lambda", cd, [ctypes.POINTER(Pop), _d, ci, ci, ci])
     _sig(lib, "orc_counts", ctypes.c_longlong, [ctypes.POINTER(Pop), _d, _i])
+    _sig(lib, "orc_counts_planes", ctypes.c_longlong, [ctypes.POINTER(Pop), _d, ci, ci, _i])
     _sig(lib, "orc_sources", ctypes.c_longlong, [ctypes.POINTER(Pop), _d, _d, ci, ctypes.c_longlong, _f, _i])
     _sig(lib, "orc_source_props", None, [ctypes.POINTER(Bg), ctypes.c_longlong, _f, _f])
     _sig(lib, "orc_poisson", ci, [cd, ctypes.c_uint64, cu, cu])
@@ -110,6 +113,14 @@ class Oracle:
         self.lib.orc_gen_modes(N, L, ctypes.byref(self.pk), seed, inj, r_smooth, int(smooth_potential),
                                self.ta["prefac_lensing"], p(dk), p(pk))
         return dk, pk
+
+    def kspace_variance(self, N, L, seed, r_smooth, smooth_potential=True, injected=None):
+        """(<delta^2>, <phi^2>) of the normalised grids, from the modes alone."""
+        out = np.zeros(2)
+        inj = None if injected is None else p(np.ascontiguousarray(injected, dtype=np.uint32), ctypes.c_uint32)
+        self.lib.orc_kspace_variance(N, L, ctypes.byref(self.pk), seed, inj, r_smooth, int(smooth_potential),
+                                     self.ta["prefac_lensing"], p(out))
+        return out[0], out[1]
 
     def c2r(self, N, hk):
         out = np.zeros(N * N * 2 * (N // 2 + 1))
@@ -168,6 +179,16 @@ class Oracle:
         c = np.zeros(N ** 3, dtype=np.int32)
         tot = self.lib.orc_counts(ctypes.byref(pop), p(np.ascontiguousarray(dens, dtype=np.float64)), p(c, ctypes.c_int))
         return tot, c
+
+    def counts_planes(self, pop, dens_slab, iz0):
+        """Counts of the planes iz0 .. iz0+nz-1 held (padded) in dens_slab,
+        int32[nz][N][N]."""
+        N = pop.N
+        nz = dens_slab.shape[0]
+        c = np.zeros((nz, N, N), dtype=np.int32)
+        self.lib.orc_counts_planes(ctypes.byref(pop), p(np.ascontiguousarray(dens_slab, dtype=np.float64)), iz0, nz,
+                                   p(c, ctypes.c_int))
+        return c
 
     def sources(self, pop, dens, npot, nside_base=2, cap=None):
         if cap is None:

@@ -678,3 +678,221 @@ def test_imap_constant_temperature():
     hit = nadd > 0
     assert hit.sum() > 100
     np.testing.assert_allclose(data[hit] / nadd[hit], T, rtol=1e-12)
+
+
+@pytest.mark.parametrize("N,inj", [(12, False), (16, True), (32, False)])
+def test_kspace_variance_matches_grid(N, inj):
+    """Parseval of the c2r with its kx = 0, N/2 planes: the variances the
+    oracle computes from the modes alone equal the normalised grids' mean
+    squares (the c2r itself is pinned against numpy's irfftn), for the Philox
+    stream and for an injected stream whose kx = 0 and N/2 planes are not
+    Hermitian.  Dropping the conjugate partner on those planes changes the
+    result by O(1)."""
+    ta = _tables()
+    orc = Oracle(ta)
+    Lb, rs = 700.0, 3.0
+    nh = N // 2 + 1
+    injected = None
+    if inj:
+        rng = np.random.default_rng(11)
+        injected = rng.integers(0, 2 ** 32, size=2 * N * N * nh, dtype=np.uint64).astype(np.uint32)
+    d, ph, s2 = orc.fields(N, Lb, 9, rs, True, injected=injected)
+    vd, vp = orc.kspace_variance(N, Lb, 9, rs, True, injected=injected)
+    md = np.mean(d[:, :, :N] ** 2)
+    mp = np.mean(ph[:, :, :N] ** 2)
+    assert abs(vd / md - 1) < 1e-12 and abs(vp / mp - 1) < 1e-12, (vd, md, vp, mp)
+    # the k = 0 mode is zero, so the mean vanishes and sigma^2 = <delta^2>
+    assert abs(s2 / md - 1) < 1e-9
+
+
+# ------------------------------------------- lambda, RSD, shell sums, imap ----
+def _bias_np(d, b, model):
+    """bias_model (common.h:414-431) written from its definition."""
+    d = np.asarray(d, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    alive = d > -1
+    with np.errstate(all="ignore"):
+        if model == 0:
+            v = np.power(np.where(alive, 1 + d, 1.0), b)
+        elif model == 1:
+            v = np.where(d < 0, np.exp(b * d / np.where(alive, 1 + d, 1.0)), 1 + b * d)
+        else:
+            v = np.maximum(1 + b * d, 0.0)
+    return np.where(alive, v, 0.0)
+
+
+def _radii(N, Lb):
+    dx = Lb / N
+    c = np.arange(N) * dx - Lb / 2
+    return np.sqrt(c[None, None, :] ** 2 + c[None, :, None] ** 2 + c[:, None, None] ** 2), c
+
+
+@pytest.mark.parametrize("model", [0, 1, 2])
+def test_cell_lambda_and_shell_counts(model):
+    """lambda = n(r) dx^3 bias_model(delta, b(r)) norm(r) (srcs.c:158-176)
+    against numpy on every cell, with n, b and norm all varying with r; then
+    the counts per radial shell against the summed lambda within 5 Poisson
+    sigma.  A dx^2, a missing norm, or b read at the wrong radius fails the
+    first check."""
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb = 32, 1200.0
+    r_t = ta["r"]
+    amp = 6e-4 * (1 + r_t / 1000.0)
+    bz = 0.5 + r_t / 1500.0
+    norm = 0.8 + 0.3 * np.sin(r_t / 300.0)
+    rng = np.random.default_rng(model + 20)
+    g = np.exp(rng.normal(0, 0.6, size=(N, N, 2 * (N // 2 + 1)))) - 1.1
+    pop = orc.pop(N, Lb, amp, bz, norm, float(norm[0]), float(norm[-1]), model, 0, 4242)
+    r, _ = _radii(N, Lb)
+    dx = Lb / N
+    want = np.interp(r, r_t, amp) * dx ** 3 * _bias_np(g[:, :, :N], np.interp(r, r_t, bz), model) * \
+        np.interp(r, r_t, norm)
+    want[~(r < Lb / 2 + 20)] = 0
+    want[r == 0] = 0   # get_bg at r <= 0 returns f0, which is 0 for n(r) (cosmo.c:30-86)
+    gd = np.ascontiguousarray(g)
+    got = np.array([[[orc.lib.orc_cell_lambda(ctypes.byref(pop), p(gd), ix, iy, iz) for ix in range(N)]
+                     for iy in range(N)] for iz in range(N)])
+    assert np.all(np.abs(got - want) <= 1e-12 * np.maximum(np.abs(want), 1e-300))
+    assert np.count_nonzero(want == 0) > 0 and want.max() > 1
+    tot, counts = orc.counts(pop, gd)
+    counts = counts.reshape(N, N, N)
+    edges = np.linspace(0, r.max() + 1, 9)
+    for a, b in zip(edges[:-1], edges[1:]):
+        m = (r >= a) & (r < b)
+        lam = want[m].sum()
+        if lam > 50:
+            assert abs(counts[m].sum() - lam) < 5 * math.sqrt(lam), (a, b, counts[m].sum(), lam)
+
+
+def test_sources_rsd_linear_potential():
+    """Pass-2 RSD (srcs.c:87-118, 235-237): phi = g.x on the grid (values
+    exact in FP32, so the central differences are exact) gives
+    dz_rsd = f_v (g.u) V1(r), f_v = -f0/(1.5 H0 Om), u the cell's unit vector,
+    for every source of cells away from the periodic edges; V1 varies with r."""
+    ta = _tables()
+    ta["v1"] = 1.0 + ta["r"] / 5000.0
+    orc = Oracle(ta)
+    N, Lb = 32, 800.0
+    dx = Lb / N
+    gvec = np.array([0.5, -0.25, 0.125])
+    i = np.arange(N) * dx
+    phi = np.zeros((N, N, 2 * (N // 2 + 1)))
+    phi[:, :, :N] = gvec[0] * i[None, None, :] + gvec[1] * i[None, :, None] + gvec[2] * i[:, None, None]
+    assert np.array_equal(phi.astype(np.float32).astype(np.float64), phi)
+    r_t = ta["r"]
+    amp = np.where(r_t < 0.35 * Lb, 4e-3, 0.0)   # only cells with 1 <= index <= N-2
+    pop = orc.pop(N, Lb, amp, np.ones(5001), np.ones(5001), 1.0, 1.0, 0, 0, 77)
+    dens = np.zeros_like(phi)
+    pos, _ = orc.sources(pop, dens, phi, 2)
+    assert len(pos) > 2000
+    cell = np.floor((pos[:, :3].astype(np.float64) + Lb / 2) / dx + 0.5)
+    c0 = cell * dx - Lb / 2
+    rr = np.sqrt((c0 ** 2).sum(axis=1))
+    fv = -ta["fgrowth_0"] / (1.5 * ta["hubble_0"] * ta["omega_m"])
+    want = fv * (c0 @ gvec) / rr * np.interp(rr, r_t, ta["v1"])
+    np.testing.assert_allclose(pos[:, 3], want, rtol=3e-7, atol=1e-12 * np.abs(want).max())
+    # the sign: a potential rising outward pulls sources toward the observer
+    assert np.corrcoef(pos[:, 3], c0 @ gvec / rr)[0, 1] < -0.99 if fv > 0 else True
+
+
+def test_shell_sums_bias_varying_all_models():
+    """collect_density_normalization_from_grid (density.c:1128-1211) with a
+    random density and b(r) varying (one tracer per model, plus negative b)
+    against a numpy brute force: counts exact, z and bias sums to 1e-12."""
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb = 16, 1500.0
+    rng = np.random.default_rng(8)
+    g = rng.uniform(-1.3, 3.0, size=(N, N, 2 * (N // 2 + 1)))
+    r_t = ta["r"]
+    b_pos = 0.3 + r_t / 800.0
+    b_neg = -0.6 + r_t / 4000.0
+    tracers = [(b_pos, 0), (b_pos, 1), (b_pos, 2), (b_neg, 0), (b_neg, 2)]
+    nz, zmax, cnt, zs, sums = orc.shell_sums(N, Lb, g, tracers)
+    r, _ = _radii(N, Lb)
+    z = np.interp(r, r_t, ta["z"])
+    idz = (nz - 2) / zmax
+    s = (z * idz).astype(int) + 1
+    m = s < nz
+    np.testing.assert_array_equal(cnt, np.bincount(s[m], minlength=nz))
+    np.testing.assert_allclose(zs, np.bincount(s[m], weights=z[m], minlength=nz), rtol=1e-12)
+    d = g[:, :, :N]
+    for t, (b, model) in enumerate(tracers):
+        bb = np.interp(r, r_t, b)
+        want = np.bincount(s[m], weights=_bias_np(d, bb, model)[m], minlength=nz)
+        np.testing.assert_allclose(sums[t], want, rtol=1e-12, atol=1e-300)
+    assert np.all(sums[3][1:] != sums[0][1:])   # bin 0 is never filled (density.c:1168)
+
+
+@pytest.mark.parametrize("model", [0, 2])
+def test_imap_rsd_bias_brute_force(model):
+    """imap_set_cartesian_single (imap.c:135-245) with phi = g.x (exact FP32
+    differences), T(r), b(r), norm(r) and V1(r) all varying, against a numpy
+    enumeration of every cell's nsub^3 sub-points: the RSD shift
+    dr = f_v (g.u) V1 IH moves sub-points between shells; the pixel sums
+    (T bias_model(delta, b) norm per sub-point) agree to 1e-12 and the hit
+    counts exactly.  A wrong sign of the shift or a missing IH factor moves
+    hundreds of hits."""
+    ta = _tables()
+    ta["v1"] = 1.0 + ta["r"] / 5000.0
+    ta["ih"] = 3000.0 * (1.0 + ta["r"] / 8000.0)
+    orc = Oracle(ta)
+    N, Lb, nside = 16, 600.0, 4
+    dx = Lb / N
+    gvec = np.array([2.0e-2, -1.0e-2, 1.5e-2]) / 64.0
+    i = np.arange(N) * dx
+    phi = np.zeros((N, N, 2 * (N // 2 + 1)))
+    phi[:, :, :N] = gvec[0] * i[None, None, :] + gvec[1] * i[None, :, None] + gvec[2] * i[:, None, None]
+    assert np.array_equal(phi.astype(np.float32).astype(np.float64), phi)
+    rng = np.random.default_rng(model)
+    dens = np.zeros_like(phi)
+    dens[:, :, :N] = np.exp(rng.normal(0, 0.5, size=(N, N, N))) - 1
+    r_t = ta["r"]
+    tz = 0.2 * (1 + r_t / 1000.0)
+    bz = 1.3 + r_t / 2000.0
+    norm = 0.9 + r_t / 5000.0
+    r0 = np.array([100.0, 150.0, 200.0])
+    rf = np.array([150.0, 200.0, 250.0])
+    data, nadd = orc.imap(N, Lb, dens, phi, tz, bz, norm, float(norm[0]), float(norm[-1]), model, nside, r0, rf)
+    # numpy enumeration
+    fv = -ta["fgrowth_0"] / (1.5 * ta["hubble_0"] * ta["omega_m"])
+    npix = 12 * nside * nside
+    pix_size = math.sqrt(4 * math.pi / npix)
+    vol = (rf ** 3 - r0 ** 3) * pix_size ** 2 / 3
+    sc = np.minimum(np.power(vol, 0.333333), np.minimum(r0 * pix_size, rf - r0))
+    nsub = (dx / sc + 0.5).astype(int) + 1
+    want_d = np.zeros((3, npix))
+    want_n = np.zeros((3, npix), dtype=np.int64)
+    c = np.arange(N) * dx - Lb / 2
+    L_ = load()
+    moved = 0
+    for iz in range(N):
+        for iy in range(N):
+            for ix in range(N):
+                x0 = np.array([c[ix], c[iy], c[iz]])
+                rr = math.sqrt((x0 ** 2).sum())
+                if not (r0[0] - 20 <= rr <= rf[-1] + 20):
+                    continue
+                vr = fv * (x0 @ gvec) / rr
+                drs = vr * np.interp(rr, r_t, ta["v1"]) * np.interp(rr, r_t, ta["ih"])
+                temp = np.interp(rr, r_t, tz) * _bias_np(dens[iz, iy, ix], np.interp(rr, r_t, bz), model) * \
+                    np.interp(rr, r_t, norm)
+                sh = np.searchsorted(rf, rr, side="right")
+                ns = nsub[min(max(sh, 0), 2)] if rr >= r0[0] else nsub[0]
+                off = (np.arange(ns) + 0.5) * dx / ns
+                X, Y, Z = np.meshgrid(x0[0] + off, x0[1] + off, x0[2] + off, indexing="ij")
+                R = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+                shell = np.searchsorted(rf, R + drs, side="right")
+                ok = (R + drs >= r0[0]) & (shell < 3)
+                moved += np.count_nonzero(np.searchsorted(rf, R, side="right") != shell)
+                for xs, ys, zs_, rs_, s_ in zip(X[ok], Y[ok], Z[ok], R[ok], shell[ok]):
+                    ph = math.atan2(ys / rs_, xs / rs_)
+                    if ph < 0:
+                        ph += 2 * math.pi
+                    pix = L_.orc_ang2pix_ring(nside, zs_ / rs_, ph)
+                    want_d[s_, pix] += temp
+                    want_n[s_, pix] += 1
+    assert moved > 100
+    np.testing.assert_array_equal(nadd, want_n)
+    np.testing.assert_allclose(data, want_d, rtol=1e-12, atol=1e-300)
