@@ -63,7 +63,7 @@ int clr_ring2nest_table(clr_ctx *c, int nside) {
 static void free_all(clr_ctx *c) {
   auto f = [](void *p) { if (p) hipFree(p); };
   f(c->dens); f(c->npot); f(c->bg); f(c->pk); f(c->tw); f(c->amp); f(c->red); f(c->red_out);
-  f(c->rowcnt); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d2); f(c->d1f); f(c->r2n);
+  f(c->rowcnt); f(c->ref_cnt); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d2); f(c->d1f); f(c->r2n);
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < CLR_NPOP_MAX; i++) {
       f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].ptab);
@@ -282,6 +282,13 @@ int clr_sample_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   if (!c) return CLR_ERR_ARG;
   CLR_HIP(c, hipSetDevice(c->device));
   return clr_launch_sources(c, ipop, seed, nside_base, nsrc_out);
+}
+
+int clr_sample_sources_stream(clr_ctx *c, int ipop, int nthr, const int *iz_begin, const uint32_t *streams,
+                              long long stream_len, int nside_base, long long *nsrc_out) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  return clr_launch_sources_stream(c, ipop, nthr, iz_begin, streams, stream_len, nside_base, nsrc_out);
 }
 
 int clr_fetch_sources(clr_ctx *c, int ipop, float *pos4, int32_t *ipix, long long n) {

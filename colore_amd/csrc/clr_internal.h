@@ -106,6 +106,10 @@ struct clr_ctx {
   int cnt_p0 = -1, cnt_np = 0;
   size_t cnt_per_pop = 0;
   unsigned int cnt_nesc = 0;
+  // per-cell counts of the last reference-stream call (population ref_ipop)
+  void *ref_cnt = nullptr;
+  size_t ref_cnt_n = 0;
+  int ref_ipop = -1;
 
   int have_fields = 0;
   int density_done = 0;
@@ -139,6 +143,8 @@ int clr_phi_ready(clr_ctx *c, int host);
 int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc);
 int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
 int clr_launch_fetch_counts(clr_ctx *c, int ipop, int iz0, int nz, int32_t *out);
+int clr_launch_sources_stream(clr_ctx *c, int ipop, int nthr, const int *iz_begin, const uint32_t *streams,
+                              long long stream_len, int nside_base, long long *nsrc);
 int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out);
 int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz, float *lens, long long n);
 int clr_launch_cstm(clr_ctx *c, int ipop, int nside, float *out);

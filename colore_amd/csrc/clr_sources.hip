@@ -700,6 +700,200 @@ __global__ void k_expand_counts(const uint8_t *__restrict__ cnt8, const ulonglon
   out[i] = k;
 }
 
+// ---------------------------------------------------------------------------
+// Reference-stream mode: srcs_set_cartesian_single (srcs.c:120-283) fed the
+// reference's own per-thread uniform streams.  Thread t of the reference owns
+// the planes izb[t] .. izb[t+1]-1 (schedule(static)) and draws from its
+// MT19937 stream (seed + t + nthr (ipop + n_srcs IThread0), srcs.c:150);
+// the caller passes the streams' raw 32-bit outputs.
+//   k_ref_lambda: per cell the reference's FP64 lambda, or -1 where it draws
+//                 nothing (r >= L/2+20 or n(r) <= 0);
+//   k_ref_walk  : one lane per reference thread walks its cells in z, y, x
+//                 order through gsl_ran_poisson as GSL documents it (below),
+//                 consuming its stream exactly as the reference does;
+//   k_ref_write : pass 2 re-seeds (srcs.c:231): the k-th source of thread t
+//                 takes uniforms 3k, 3k+1, 3k+2 of stream t for x, y, z.
+// gsl_ran_poisson: while mu > 10, m = (unsigned)(7 mu/8), X = gamma_int(m)
+// (a < 12: -log of a product of uniform_pos; else Knuth's tan-based
+// rejection); X >= mu returns k + binomial(mu/X, m-1), else k += m,
+// mu -= X; then the multiplication method.  binomial: p > 1/2 mirrored,
+// inversion by sequential search (at most 111 terms per uniform) from
+// f(0) = q^n by binary powering.  GSL's BTPE branch (n p >= 14) is not
+// restated: inversion there too.
+// ---------------------------------------------------------------------------
+struct RefStream {
+  const uint32_t *s;
+  long long len, pos;
+  int overflow;
+};
+
+__device__ inline double rs_uniform(RefStream &r) {
+  if (r.pos >= r.len) { r.overflow = 1; return 0.5; }
+  return u01(r.s[r.pos++]);
+}
+__device__ inline double rs_uniform_pos(RefStream &r) {
+  double x;
+  do { x = rs_uniform(r); } while (x == 0 && !r.overflow);
+  return x;
+}
+__device__ inline double pow_uint(double x, unsigned n) {
+#pragma clang fp contract(off)
+  double v = 1.0;
+  do {
+    if (n & 1u) v *= x;
+    n >>= 1;
+    x *= x;
+  } while (n);
+  return v;
+}
+__device__ double ref_gamma_int(RefStream &r, unsigned a) {
+#pragma clang fp contract(off)
+  if (a < 12) {
+    double prod = 1;
+    for (unsigned i = 0; i < a; i++) prod *= rs_uniform_pos(r);
+    return -log(prod);
+  }
+  const double sqa = sqrt(2.0 * a - 1), am1 = (double)a - 1;
+  double x, y, v;
+  do {
+    do {
+      y = tan(M_PI * rs_uniform(r));
+      x = sqa * y + am1;
+    } while (x <= 0 && !r.overflow);
+    v = rs_uniform(r);
+  } while (v > (1 + y * y) * exp(am1 * log(x / am1) - sqa * y) && !r.overflow);
+  return x;
+}
+__device__ unsigned ref_binomial(RefStream &r, double p, unsigned n) {
+#pragma clang fp contract(off)
+  if (n == 0) return 0;
+  bool flipped = false;
+  if (p > 0.5) { p = 1.0 - p; flipped = true; }
+  const double q = 1 - p, s = p / q;
+  const double f0 = pow_uint(q, n);
+  unsigned ix = 0;
+  for (;;) {
+    double f = f0, u = rs_uniform(r);
+    bool done = false;
+    for (ix = 0; ix <= 110; ix++) {
+      if (u < f) { done = true; break; }
+      u -= f;
+      f *= s * (double)(n - ix) / (double)(ix + 1);
+    }
+    if (done || r.overflow) break;
+  }
+  return flipped ? n - ix : ix;
+}
+__device__ int ref_poisson(RefStream &r, double mu) {
+#pragma clang fp contract(off)
+  unsigned k = 0;
+  while (mu > 10 && !r.overflow) {
+    const unsigned m = (unsigned)(mu * (7.0 / 8.0));
+    const double X = ref_gamma_int(r, m);
+    if (X >= mu) return (int)(k + ref_binomial(r, mu / X, m - 1));
+    k += m;
+    mu -= X;
+  }
+  const double emu = exp(-mu);
+  double prod = 1.0;
+  do {
+    prod *= rs_uniform(r);
+    k++;
+  } while (prod > emu && !r.overflow);
+  return (int)(k - 1);
+}
+
+struct RefArgs {
+  const float *g;
+  const double *r_arr, *amp, *bz, *norm;
+  double idr, r_last, bz0, norm0, normf;
+  double dx, cell_vol, half_box_20, ox, oy, oz;
+  int N, ngx, model, nthr;
+  const int *izb;                 // nthr + 1 plane bounds
+  const uint32_t *streams;        // nthr x len
+  long long len;
+  double *lam;                    // N^3: lambda, or -1 (no draw)
+  uint32_t *cnt;                  // N^3 counts
+  int *overflow;                  // set when a stream runs out
+};
+
+__global__ void k_ref_lambda(RefArgs a) {
+#pragma clang fp contract(off)
+  const uint64_t cell = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = a.N;
+  if (cell >= (uint64_t)N * N * N) return;
+  const uint64_t rw = cell / (uint64_t)N;
+  const int ix = (int)(cell - rw * N), iy = (int)(rw % N), iz = (int)(rw / N);
+  const double obs[3] = {a.ox, a.oy, a.oz};
+  double x0, y0, z0;
+  const double r = cell_radius(ix, iy, iz, a.dx, obs, &x0, &y0, &z0);
+  double lam = -1.0;
+  if (r < a.half_box_20) {
+    const double nd = f_of_r(a.r_arr, a.amp, a.idr, a.r_last, r, 0.0, 0.0);
+    if (nd > 0) {
+      const double bias = f_of_r(a.r_arr, a.bz, a.idr, a.r_last, r, a.bz0, 1.0);
+      const double dnorm = f_of_r(a.r_arr, a.norm, a.idr, a.r_last, r, a.norm0, a.normf);
+      const double dl = a.g[(size_t)ix + (size_t)a.ngx * ((size_t)iy + (size_t)N * iz)];
+      lam = nd * a.cell_vol * bias_model_f64(dl, bias, a.model) * dnorm;
+    }
+  }
+  a.lam[cell] = lam;
+}
+
+// One lane per reference thread; the lanes of a wavefront walk different
+// threads' planes side by side.
+__global__ __launch_bounds__(64) void k_ref_walk(RefArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nthr) return;
+  const size_t plane = (size_t)a.N * a.N;
+  RefStream rs{a.streams + (size_t)t * a.len, a.len, 0, 0};
+  const size_t c0 = (size_t)a.izb[t] * plane, c1 = (size_t)a.izb[t + 1] * plane;
+  for (size_t c = c0; c < c1; c++) {
+    const double lam = a.lam[c];
+    a.cnt[c] = lam == -1.0 ? 0u : (uint32_t)ref_poisson(rs, lam);
+  }
+  if (rs.overflow) atomicOr(a.overflow, 1);
+}
+
+struct RefWriteExtra {
+  const uint32_t *cnt;
+  const long long *off;           // exclusive prefix of cnt (N^3 + 1)
+  const int *izb;
+  const uint32_t *streams;
+  long long len;
+  int nthr;
+  int *overflow;
+};
+
+__global__ __launch_bounds__(256) void k_ref_write(WriteArgs a, RefWriteExtra e) {
+#pragma clang fp contract(off)
+  const uint64_t cell = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = a.N;
+  if (cell >= (uint64_t)N * N * N) return;
+  const int k = (int)e.cnt[cell];
+  if (k == 0) return;
+  const uint64_t rw = cell / (uint64_t)N;
+  const int ix = (int)(cell - rw * N), iy = (int)(rw % N), iz = (int)(rw / N);
+  int t = 0;
+  while (t + 1 < e.nthr && iz >= e.izb[t + 1]) t++;
+  const long long first = e.off[(size_t)e.izb[t] * N * N];   // thread t's first source
+  const uint32_t *st = e.streams + (size_t)t * e.len;
+  const double x0 = (ix + 0.0) * a.dx - a.ox, y0 = (iy + 0.0) * a.dx - a.oy, z0 = (iz + 0.0) * a.dx - a.oz;
+  const double rr = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+  const double rvel = a.factor_vel * get_rvel(a, ix, iy, iz, x0, y0, z0, rr);
+  const double dz_rsd = rvel * f_of_r(a.r_arr, a.v1, a.idr, a.r_last, rr, a.v1_0, a.v1_last);
+  const double ctr[3] = {x0, y0, z0};
+  for (int is = 0; is < k; is++) {
+    const long long pid = e.off[cell] + is;
+    const long long j = 3 * (pid - first);
+    if (j + 2 >= e.len) { atomicOr(e.overflow, 2); return; }
+    float pp[3];
+    for (int ax = 0; ax < 3; ax++) pp[ax] = (float)(ctr[ax] + a.dx * (u01(st[j + ax]) - 0.5));
+    a.pos[pid] = make_float4(pp[0], pp[1], pp[2], (float)dz_rsd);
+    a.ipix[pid] = a.r2n[vec2pix_ring32(a.nside_base, (double)pp[0], (double)pp[1], (double)pp[2])];
+  }
+}
+
 // Src fields of srcs_get_local_properties_single: RA, Dec, z0, dz_rsd.
 __global__ void k_src_props(const float4 *__restrict__ pos, long long n, const double *__restrict__ r_arr,
                             const double *__restrict__ zt, double idr, double r_last, double z_last,
@@ -860,6 +1054,7 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   a.esc_count = nesc;
   c->cnt_p0 = p0;
   c->cnt_np = np;
+  c->ref_ipop = -1;
   c->cnt_per_pop = per_pop;
   c->cnt_nesc = nesc;
   if (nesc > 0) {
@@ -964,6 +1159,13 @@ int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n) {
 int clr_launch_fetch_counts(clr_ctx *c, int ipop, int iz0, int nz, int32_t *out) {
   const int N = c->N;
   if (iz0 < 0 || nz < 0 || iz0 + nz > N) return clr_fail(c, CLR_ERR_ARG, "planes %d..%d outside the grid", iz0, iz0 + nz);
+  if (c->ref_ipop >= 0 && ipop == c->ref_ipop) {
+    // counts of the last reference-stream call
+    const size_t n = (size_t)N * N * nz, c0 = (size_t)N * N * iz0;
+    if (nz == 0) return CLR_OK;
+    CLR_HIP(c, hipMemcpy(out, (const uint32_t *)c->ref_cnt + c0, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return CLR_OK;
+  }
   if (c->cnt_p0 < 0 || ipop < c->cnt_p0 || ipop >= c->cnt_p0 + c->cnt_np)
     return clr_fail(c, CLR_ERR_STATE, "population %d was not sampled by the last clr_sample_sources call", ipop);
   if (nz == 0) return CLR_OK;
@@ -979,5 +1181,162 @@ int clr_launch_fetch_counts(clr_ctx *c, int ipop, int iz0, int nz, int32_t *out)
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "cell counts: %s", hipGetErrorString(e));
+  return CLR_OK;
+}
+
+int clr_launch_sources_stream(clr_ctx *c, int ipop, int nthr, const int *iz_begin, const uint32_t *streams,
+                              long long stream_len, int nside_base, long long *nsrc) {
+  const int N = c->N, na = c->na;
+  if (ipop < 0 || ipop >= c->n_tr[CLR_TRACER_SRCS]) return clr_fail(c, CLR_ERR_ARG, "no source population %d", ipop);
+  if (!c->density_done) return clr_fail(c, CLR_ERR_STATE, "density field not computed");
+  if (N > 512) return clr_fail(c, CLR_ERR_ARG, "the reference-stream mode is for grids up to 512^3 (N=%d)", N);
+  if (nthr < 1 || nthr > 4096 || !iz_begin || !streams || stream_len < 3)
+    return clr_fail(c, CLR_ERR_ARG, "bad thread streams (nthr=%d, len=%lld)", nthr, stream_len);
+  if (iz_begin[0] != 0 || iz_begin[nthr] != N)
+    return clr_fail(c, CLR_ERR_ARG, "thread planes must cover 0..%d", N);
+  for (int t = 0; t < nthr; t++)
+    if (iz_begin[t + 1] < iz_begin[t]) return clr_fail(c, CLR_ERR_ARG, "thread planes must not decrease");
+  if (nside_base < 1 || nside_base > 64 || (nside_base & (nside_base - 1)))
+    return clr_fail(c, CLR_ERR_ARG, "nside_base=%d must be a power of two <= 64", nside_base);
+  const TracerDev &T = c->tr[CLR_TRACER_SRCS][ipop];
+  if (!T.set || !T.norm_set) return clr_fail(c, CLR_ERR_STATE, "population %d has no norm table", ipop);
+  int rc = clr_ring2nest_table(c, nside_base);
+  if (rc) return rc;
+  rc = clr_phi_ready(c, 0);   // k_ref_write reads phi
+  if (rc) return rc;
+  const size_t ncells = (size_t)N * N * N;
+  const size_t nsb = (ncells + kScanB - 1) / kScanB;
+  rc = clr_ensure(c, (void **)&c->ref_cnt, &c->ref_cnt_n, ncells * sizeof(uint32_t));
+  if (rc) return rc;
+  c->cnt_p0 = -1;
+  c->ref_ipop = -1;
+  double *lam = nullptr;
+  long long *off = nullptr, *bsum = nullptr;
+  uint32_t *dst = nullptr;
+  int *dizb = nullptr, *dflag = nullptr;
+  const double dx = c->L / N;
+  hipError_t e = hipMalloc((void **)&lam, ncells * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&off, (ncells + 1) * sizeof(long long));
+  if (e == hipSuccess) e = hipMalloc((void **)&bsum, (nsb + 1) * sizeof(long long));
+  if (e == hipSuccess) e = hipMalloc((void **)&dst, (size_t)nthr * stream_len * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&dizb, (size_t)(nthr + 1) * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&dflag, sizeof(int));
+  auto release = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    for (void *q : {(void *)lam, (void *)off, (void *)bsum, (void *)dst, (void *)dizb, (void *)dflag})
+      if (q) (void)hipFree(q);
+  };
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    release();
+    return clr_fail(c, CLR_ERR_OOM, "reference-stream buffers: %s", hipGetErrorString(e));
+  }
+  int flag = 0;
+  long long total = 0;
+  e = hipMemcpyAsync(dst, streams, (size_t)nthr * stream_len * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dizb, iz_begin, (size_t)(nthr + 1) * sizeof(int), hipMemcpyHostToDevice,
+                                          c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(dflag, 0, sizeof(int), c->stream);
+  RefArgs a{};
+  a.g = c->dens;
+  a.r_arr = c->bg;
+  a.amp = T.amp;
+  a.bz = T.bz;
+  a.norm = T.norm;
+  a.idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.bz0 = T.bz0;
+  a.norm0 = T.norm0;
+  a.normf = T.normf;
+  a.dx = dx;
+  a.cell_vol = dx * dx * dx;
+  a.half_box_20 = c->L / 2 + 20.;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.N = N;
+  a.ngx = c->ngx;
+  a.model = T.model;
+  a.nthr = nthr;
+  a.izb = dizb;
+  a.streams = dst;
+  a.len = stream_len;
+  a.lam = lam;
+  a.cnt = (uint32_t *)c->ref_cnt;
+  a.overflow = dflag;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_ref_lambda, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ref_walk, dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, c->stream, a);
+    ScanPtrs sp{};
+    sp.in[0] = a.cnt;
+    sp.bsum[0] = bsum;
+    sp.out[0] = off;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb, 1), dim3(kScanB), 0, c->stream, sp, ncells);
+    hipLaunchKernelGGL(k_scan_top, dim3(1, 1), dim3(kScanB), 0, c->stream, sp, nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsb, 1), dim3(kScanB), 0, c->stream, sp, ncells);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&total, off + ncells, sizeof(long long), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    release();
+    return clr_fail(c, CLR_ERR_HIP, "reference-stream counts: %s", hipGetErrorString(e));
+  }
+  if (flag) {
+    release();
+    return clr_fail(c, CLR_ERR_ARG, "a thread stream of %lld uniforms ran out in the counting pass", stream_len);
+  }
+  CatalogDev &C = c->cat[ipop];
+  if (total > C.cap) {
+    if (C.pos) (void)hipFree(C.pos);
+    if (C.ipix) (void)hipFree(C.ipix);
+    C.pos = nullptr;
+    C.ipix = nullptr;
+    C.cap = 0;
+    const long long cap = total + total / 16 + 1024;
+    if (hipMalloc((void **)&C.pos, (size_t)cap * 4 * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&C.ipix, (size_t)cap * sizeof(int32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      release();
+      return clr_fail(c, CLR_ERR_OOM, "cannot allocate catalogue of %lld sources", total);
+    }
+    C.cap = cap;
+  }
+  C.n = total;
+  if (total > 0) {
+    WriteArgs w{};
+    w.phi = c->npot;
+    w.r_arr = c->bg;
+    w.v1 = c->bg + 3 * (size_t)na;
+    w.idr = c->glob_idr;
+    w.r_last = c->r_last;
+    w.v1_0 = c->h_bg_first[3];
+    w.v1_last = c->h_bg_last[3];
+    w.dx = dx;
+    w.factor_vel = -c->fgrowth_0 / (1.5 * c->hubble_0 * c->omega_m);
+    w.idx = N / c->L;
+    w.ox = c->obs[0];
+    w.oy = c->obs[1];
+    w.oz = c->obs[2];
+    w.N = N;
+    w.ngx = c->ngx;
+    w.nside_base = nside_base;
+    w.ipop = ipop;
+    w.r2n = c->r2n;
+    w.pos = (float4 *)C.pos;
+    w.ipix = C.ipix;
+    w.n = total;
+    RefWriteExtra x{a.cnt, off, dizb, dst, stream_len, nthr, dflag};
+    hipLaunchKernelGGL(k_ref_write, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream, w, x);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  release();
+  if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "reference-stream catalogue: %s", hipGetErrorString(e));
+  if (flag) return clr_fail(c, CLR_ERR_ARG, "a thread stream of %lld uniforms ran out in the position pass", stream_len);
+  c->ref_ipop = ipop;
+  if (nsrc) *nsrc = total;
   return CLR_OK;
 }

@@ -77,6 +77,8 @@ def load_amd():
     _sig(lib, "clr_set_norm_table", ci, [_vp, ci, ci, _d, ctypes.c_double, ctypes.c_double])
     _sig(lib, "clr_sample_sources", ci, [_vp, ci, ctypes.c_uint32, ci, ctypes.POINTER(ctypes.c_longlong)])
     _sig(lib, "clr_fetch_sources", ci, [_vp, ci, _f, _i32, ctypes.c_longlong])
+    _sig(lib, "clr_sample_sources_stream", ci, [_vp, ci, ci, _i32, _u32, ctypes.c_longlong, ci,
+                                                ctypes.POINTER(ctypes.c_longlong)])
     _sig(lib, "clr_source_properties", ci, [_vp, ci, _f, ctypes.c_longlong])
     _sig(lib, "clr_kappa", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
@@ -280,6 +282,18 @@ class Context:
     def sample_sources(self, ipop, seed, nside_base=2):
         n = ctypes.c_longlong()
         self._chk(self.lib.clr_sample_sources(self.h, ipop, seed, nside_base, ctypes.byref(n)))
+        return n.value
+
+    def sample_sources_stream(self, ipop, nthr, iz_begin, streams, nside_base=2):
+        """Reference-stream mode: streams is uint32[nthr, len], the raw
+        outputs of each OpenMP thread's MT19937; iz_begin its plane split."""
+        streams = np.ascontiguousarray(streams, dtype=np.uint32)
+        izb = np.ascontiguousarray(iz_begin, dtype=np.int32)
+        assert streams.shape[0] == nthr and len(izb) == nthr + 1
+        n = ctypes.c_longlong()
+        self._chk(self.lib.clr_sample_sources_stream(self.h, ipop, nthr, ptr(izb, ctypes.c_int32),
+                                                     ptr(streams, ctypes.c_uint32), streams.shape[1], nside_base,
+                                                     ctypes.byref(n)))
         return n.value
 
     def fetch_sources(self, ipop, n):

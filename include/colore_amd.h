@@ -171,6 +171,22 @@ int clr_set_norm_table(clr_ctx *ctx, int kind, int ipop, const double *norm_arr,
  * returns the total. */
 int clr_sample_sources(clr_ctx *ctx, int ipop, uint32_t seed, int nside_base,
                        long long *nsrc_out);
+/* clr_sample_sources for population ipop fed the reference's own uniform
+ * streams instead of Philox (the reference-stream parity mode; grids up to
+ * 512^3).  srcs_set_cartesian_single (srcs.c:120-283) runs on nthr OpenMP
+ * threads; thread t owns the planes iz_begin[t] .. iz_begin[t+1]-1 (its
+ * schedule(static) share; iz_begin[0] = 0, iz_begin[nthr] = N) and draws from
+ * its gsl_rng_mt19937 seeded with seed + t + nthr (ipop + n_srcs IThread0)
+ * (srcs.c:150).  streams[t * stream_len + i] holds output i of thread t's
+ * generator (uniform = x / 2^32).  Pass 1 walks the thread's cells in z, y, x
+ * order and calls gsl_ran_poisson (common.c:185-189) for every cell with
+ * r < L/2+20 and n(r) > 0; pass 2 re-seeds the generator (srcs.c:231), so the
+ * k-th source of thread t takes outputs 3k, 3k+1, 3k+2 for x, y, z.  The
+ * catalogue order, RSD and base pixel are those of clr_sample_sources.
+ * Returns CLR_ERR_ARG when a stream is too short. */
+int clr_sample_sources_stream(clr_ctx *ctx, int ipop, int nthr, const int *iz_begin,
+                              const uint32_t *streams, long long stream_len, int nside_base,
+                              long long *nsrc_out);
 /* Copies population ipop's CatalogCartesian (common.h:162-167): pos[4n]
  * = (x, y, z, dz_rsd) relative to the observer, ipix[n]. */
 int clr_fetch_sources(clr_ctx *ctx, int ipop, float *pos4, int32_t *ipix,

@@ -708,6 +708,191 @@ long long orc_sources(const orc_pop *p, const double *dens, const double *npot, 
   return total;
 }
 
+/* ---- the reference's own streams (srcs.c:136-278 on OpenMP threads) ------
+ * gsl_rng_mt19937 output x maps to gsl_rng_uniform = x / 2^32 and
+ * gsl_rng_uniform_pos repeats while that is 0 (GSL's documented mapping).
+ * gsl_ran_poisson (called through rng_poisson, common.c:185-189), as GSL
+ * documents it:
+ *   while mu > 10: m = (unsigned)(mu 7/8); X = gamma_int(m);
+ *     X >= mu: return k + binomial(mu/X, m-1); else k += m, mu -= X;
+ *   then the multiplication method: prod of uniforms until prod <= e^-mu,
+ *     returning (number of uniforms) - 1.
+ * gamma_int(a): a < 12 -> -log(prod of a uniform_pos); else the rejection
+ * method of Knuth (TAOCP vol. 2, 3.4.1, algorithm A, tan-based).
+ * binomial(p, n): p > 1/2 uses 1-p and returns n - k; inversion by sequential
+ * search with f(x+1) = f(x) (n-x)/(x+1) p/q from f(0) = q^n (q^n by binary
+ * powering), at most 111 terms per uniform and a fresh uniform when they are
+ * exhausted.  GSL switches to the BTPE algorithm when n p >= 14; that branch
+ * is not restated (inversion is used there too): parity unpinned for cells
+ * that reach it. */
+typedef struct { const uint32_t *s; long long len, pos; int overflow; } ref_stream;
+
+static double rs_uniform(ref_stream *r) {
+  if (r->pos >= r->len) { r->overflow = 1; return 0.5; }
+  return r->s[r->pos++] / 4294967296.0;
+}
+static double rs_uniform_pos(ref_stream *r) {
+  double x;
+  do { x = rs_uniform(r); } while (x == 0 && !r->overflow);
+  return x;
+}
+static double pow_int_binary(double x, unsigned n) {
+  double v = 1.0;
+  do {
+    if (n & 1u) v *= x;
+    n >>= 1;
+    x *= x;
+  } while (n);
+  return v;
+}
+static double ref_gamma_int(ref_stream *r, unsigned a) {
+  if (a < 12) {
+    double prod = 1;
+    for (unsigned i = 0; i < a; i++) prod *= rs_uniform_pos(r);
+    return -log(prod);
+  }
+  const double sqa = sqrt(2.0 * a - 1), am1 = (double)a - 1;
+  double x, y, v;
+  do {
+    do {
+      y = tan(M_PI * rs_uniform(r));
+      x = sqa * y + am1;
+    } while (x <= 0 && !r->overflow);
+    v = rs_uniform(r);
+  } while (v > (1 + y * y) * exp(am1 * log(x / am1) - sqa * y) && !r->overflow);
+  return x;
+}
+static unsigned ref_binomial(ref_stream *r, double p, unsigned n) {
+  if (n == 0) return 0;
+  int flipped = 0;
+  if (p > 0.5) { p = 1.0 - p; flipped = 1; }
+  const double q = 1 - p, s = p / q;
+  const double f0 = pow_int_binary(q, n);
+  unsigned ix = 0;
+  for (;;) {
+    double f = f0, u = rs_uniform(r);
+    int done = 0;
+    for (ix = 0; ix <= 110; ix++) {
+      if (u < f) { done = 1; break; }
+      u -= f;
+      f *= s * (double)(n - ix) / (double)(ix + 1);
+    }
+    if (done || r->overflow) break;
+  }
+  return flipped ? n - ix : ix;
+}
+static int ref_poisson(ref_stream *r, double mu) {
+  unsigned k = 0;
+  while (mu > 10 && !r->overflow) {
+    const unsigned m = (unsigned)(mu * (7.0 / 8.0));
+    const double X = ref_gamma_int(r, m);
+    if (X >= mu) return (int)(k + ref_binomial(r, mu / X, m - 1));
+    k += m;
+    mu -= X;
+  }
+  const double emu = exp(-mu);
+  double prod = 1.0;
+  do {
+    prod *= rs_uniform(r);
+    k++;
+  } while (prod > emu && !r->overflow);
+  return (int)(k - 1);
+}
+
+int orc_ref_poisson(const uint32_t *stream, long long len, double mu, long long *used) {
+  ref_stream r = {stream, len, 0, 0};
+  const int k = ref_poisson(&r, mu);
+  if (used) *used = r.overflow ? -1 : r.pos;
+  return k;
+}
+
+void orc_mt_fill(uint32_t seed, long long n, uint32_t *out) {
+  orc_mt19937 mt;
+  orc_mt_seed(&mt, seed);
+  for (long long i = 0; i < n; i++) out[i] = orc_mt_next(&mt);
+}
+
+/* srcs_set_cartesian_single (srcs.c:120-283) on nthr OpenMP threads, thread t
+ * owning the planes iz_begin[t] .. iz_begin[t+1]-1 (schedule(static)) and
+ * drawing from streams[t*stream_len ...] (the MT19937 stream seeded with
+ * seed + t + nthr (ipop + n_srcs IThread0), srcs.c:150).  Pass 1: every cell
+ * with r < L/2+20 and n(r) > 0 draws its count with gsl_ran_poisson (even
+ * when lambda is 0, which takes one uniform).  Pass 2 re-seeds (srcs.c:231):
+ * the k-th source of thread t takes uniforms 3k, 3k+1, 3k+2 of the stream
+ * for x, y, z.  Returns the number of sources (<= cap), -1 when cap is too
+ * small, -2 when a stream runs out.  counts (N^3, cell order) may be NULL. */
+long long orc_sources_ref_stream(const orc_pop *p, const double *dens, const double *npot, int nside_base,
+                                 int nthr, const int *iz_begin, const uint32_t *streams, long long stream_len,
+                                 long long cap, float *pos4, int *ipix, int *counts) {
+  const int N = p->N;
+  const double dx = p->L / N;
+  const double factor_vel = -p->fgrowth_0 / (1.5 * p->hubble_0 * p->omega_m);
+  const double v1f0 = p->bg->v1[0], v1ff = p->bg->v1[p->bg->na - 1];
+  int *cnt = malloc((size_t)N * N * N * sizeof(int));
+  long long *tot = calloc((size_t)nthr + 1, sizeof(long long));
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int t = 0; t < nthr; t++) {
+    ref_stream rs = {streams + (size_t)t * stream_len, stream_len, 0, 0};
+    long long nt = 0;
+    for (int iz = iz_begin[t]; iz < iz_begin[t + 1]; iz++)
+      for (int iy = 0; iy < N; iy++)
+        for (int ix = 0; ix < N; ix++) {
+          const size_t c = (size_t)ix + (size_t)N * ((size_t)iy + (size_t)N * iz);
+          const double z0 = (iz + 0.0) * dx - p->obs[2], y0 = (iy + 0.0) * dx - p->obs[1];
+          const double x0 = (ix + 0.0) * dx - p->obs[0];
+          const double r = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+          int npp = 0;
+          if (r < p->L / 2 + 20. && orc_f_of_r(p->bg, p->amp, 0, 0, r) > 0)
+            npp = ref_poisson(&rs, orc_cell_lambda(p, dens, ix, iy, iz));
+          cnt[c] = npp;
+          nt += npp;
+        }
+    tot[t + 1] = nt;
+    if (rs.overflow) bad |= 1;
+  }
+  long long total = 0;
+  for (int t = 0; t < nthr; t++) { tot[t + 1] += tot[t]; }
+  total = tot[nthr];
+  if (counts) memcpy(counts, cnt, (size_t)N * N * N * sizeof(int));
+  if (bad) { free(cnt); free(tot); return -2; }
+  if (total > cap) { free(cnt); free(tot); return -1; }
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int t = 0; t < nthr; t++) {
+    const uint32_t *st = streams + (size_t)t * stream_len;
+    long long pid = tot[t], k = 0;
+    for (int iz = iz_begin[t]; iz < iz_begin[t + 1]; iz++)
+      for (int iy = 0; iy < N; iy++)
+        for (int ix = 0; ix < N; ix++) {
+          const size_t c = (size_t)ix + (size_t)N * ((size_t)iy + (size_t)N * iz);
+          const int npp = cnt[c];
+          if (npp <= 0) continue;
+          const double z0 = (iz + 0.0) * dx - p->obs[2], y0 = (iy + 0.0) * dx - p->obs[1];
+          const double x0 = (ix + 0.0) * dx - p->obs[0];
+          const double rr = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+          const double rv = factor_vel * rvel(N, p->L, npot, ix, iy, iz, x0, y0, z0, rr);
+          const double dz_rsd = rv * orc_f_of_r(p->bg, p->bg->v1, v1f0, v1ff, rr);
+          const double c0[3] = {x0, y0, z0};
+          for (int ip = 0; ip < npp; ip++, k++, pid++) {
+            float q[3];
+            for (int ax = 0; ax < 3; ax++) {
+              const long long j = 3 * k + ax;
+              if (j >= stream_len) { bad |= 1; q[ax] = 0; continue; }
+              q[ax] = (float)(c0[ax] + dx * (st[j] / 4294967296.0 - 0.5));
+            }
+            pos4[4 * pid + 0] = q[0];
+            pos4[4 * pid + 1] = q[1];
+            pos4[4 * pid + 2] = q[2];
+            pos4[4 * pid + 3] = (float)dz_rsd;
+            const double v[3] = {q[0], q[1], q[2]};
+            ipix[pid] = (int)orc_ring2nest(nside_base, orc_vec2pix_ring(nside_base, v));
+          }
+        }
+  }
+  free(cnt); free(tot);
+  return bad ? -2 : total;
+}
+
 /* srcs_get_local_properties_single, srcs.c:386-416, cart2sph srcs.c:68-85 */
 void orc_source_props(const orc_bg *bg, long long n, const float *pos4, float *out4) {
   const double RTOD = 57.2957795;

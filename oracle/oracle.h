@@ -111,6 +111,17 @@ long long orc_counts_planes(const orc_pop *p, const double *dens_slab, int iz0, 
  * (NEST at nside_base).  Returns n (<= cap), or -1 if cap is too small. */
 long long orc_sources(const orc_pop *p, const double *dens, const double *npot, int nside_base,
                       long long cap, float *pos4, int *ipix);
+/* gsl_ran_poisson restated on a caller-supplied MT19937 output stream
+ * (uniform = x/2^32); *used = uniforms consumed (-1 if the stream ran out). */
+int orc_ref_poisson(const uint32_t *stream, long long len, double mu, long long *used);
+/* n raw outputs of MT19937 seeded as gsl_rng_set does. */
+void orc_mt_fill(uint32_t seed, long long n, uint32_t *out);
+/* Sources from the reference's per-thread streams (srcs.c:120-283 on nthr
+ * threads with the static z split iz_begin[nthr+1]); streams[t*stream_len+i].
+ * Returns n, -1 (cap too small) or -2 (a stream ran out); counts optional. */
+long long orc_sources_ref_stream(const orc_pop *p, const double *dens, const double *npot, int nside_base,
+                                 int nthr, const int *iz_begin, const uint32_t *streams, long long stream_len,
+                                 long long cap, float *pos4, int *ipix, int *counts);
 /* srcs_get_local_properties_single: RA, Dec, z0, dz_rsd per source. */
 void orc_source_props(const orc_bg *bg, long long n, const float *pos4, float *out4);
 /* Poisson inversion used by both paths. */

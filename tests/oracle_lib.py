@@ -76,6 +76,10 @@ def load(build=True):
     _sig(lib, "orc_sources", ctypes.c_longlong, [ctypes.POINTER(Pop), _d, _d, ci, ctypes.c_longlong, _f, _i])
     _sig(lib, "orc_source_props", None, [ctypes.POINTER(Bg), ctypes.c_longlong, _f, _f])
     _sig(lib, "orc_poisson", ci, [cd, ctypes.c_uint64, cu, cu])
+    _sig(lib, "orc_ref_poisson", ci, [_u32, ctypes.c_longlong, cd, ctypes.POINTER(ctypes.c_longlong)])
+    _sig(lib, "orc_mt_fill", None, [cu, ctypes.c_longlong, _u32])
+    _sig(lib, "orc_sources_ref_stream", ctypes.c_longlong, [ctypes.POINTER(Pop), _d, _d, ci, ci, _i, _u32,
+                                                              ctypes.c_longlong, ctypes.c_longlong, _f, _i, _i])
     _sig(lib, "orc_kappa_isw", None, [ci, ci, cd, _d, cd, ctypes.POINTER(Bg), _d, ci, ci, _d, _d])
     _sig(lib, "orc_src_skewers", None, [ci, cd, _d, cd, ctypes.POINTER(Bg), cd, cd, cd, _d, _d, ci, cd,
                                          ctypes.c_longlong, _f, _d, _d])
@@ -201,6 +205,29 @@ class Oracle:
         assert n >= 0
         return pos[:n], ipix[:n]
 
+    def sources_ref_stream(self, pop, dens, npot, nthr, iz_begin, streams, nside_base=2, cap=None):
+        """(pos4, ipix, counts) of srcs.c:120-283 on nthr threads fed the
+        streams [nthr, len] (uint32 MT19937 outputs)."""
+        N = pop.N
+        streams = np.ascontiguousarray(streams, dtype=np.uint32)
+        izb = np.ascontiguousarray(iz_begin, dtype=np.int32)
+        counts = np.zeros(N ** 3, dtype=np.int32)
+        dd = np.ascontiguousarray(dens, dtype=np.float64)
+        pp = np.ascontiguousarray(npot, dtype=np.float64)
+        if cap is None:
+            n = self.lib.orc_sources_ref_stream(ctypes.byref(pop), p(dd), p(pp), nside_base, nthr, p(izb, ctypes.c_int),
+                                                p(streams, ctypes.c_uint32), streams.shape[1], 0, None, None,
+                                                p(counts, ctypes.c_int))
+            assert n != -2, "stream too short"
+            cap = int(counts.sum())
+        pos = np.zeros((max(cap, 1), 4), dtype=np.float32)
+        ipix = np.zeros(max(cap, 1), dtype=np.int32)
+        n = self.lib.orc_sources_ref_stream(ctypes.byref(pop), p(dd), p(pp), nside_base, nthr, p(izb, ctypes.c_int),
+                                            p(streams, ctypes.c_uint32), streams.shape[1], cap, p(pos, ctypes.c_float),
+                                            p(ipix, ctypes.c_int), p(counts, ctypes.c_int))
+        assert n >= 0, n
+        return pos[:n], ipix[:n], counts
+
     def source_props(self, pos4):
         out = np.zeros_like(pos4)
         self.lib.orc_source_props(ctypes.byref(self.bg), len(pos4), p(np.ascontiguousarray(pos4), ctypes.c_float),
@@ -281,3 +308,24 @@ class Oracle:
                           p(np.ascontiguousarray(r0, dtype=np.float64)), p(np.ascontiguousarray(rf, dtype=np.float64)),
                           p(data), p(nadd, ctypes.c_int))
         return data.reshape(nr, npix), nadd.reshape(nr, npix)
+
+
+def static_split(n, nthr):
+    """First iteration of each thread (and n) for libgomp's schedule(static)
+    over n iterations: the first n % nthr threads take one more."""
+    q, r = divmod(n, nthr)
+    b = [t * q + min(t, r) for t in range(nthr)] + [n]
+    return np.array(b, dtype=np.int32)
+
+
+def ref_streams(seed, ipop, nthr, length, n_srcs=1):
+    """The reference's per-thread MT19937 output streams of the source
+    sampler: thread t seeded with seed + t + nthr (ipop + n_srcs IThread0),
+    IThread0 = 0 on one rank (srcs.c:150)."""
+    lib = load()
+    out = np.zeros((nthr, length), dtype=np.uint32)
+    for t in range(nthr):
+        row = np.zeros(length, dtype=np.uint32)
+        lib.orc_mt_fill((seed + t + nthr * ipop) & 0xFFFFFFFF, length, p(row, ctypes.c_uint32))
+        out[t] = row
+    return out

@@ -590,3 +590,45 @@ def test_cli_reference_example_layout(tmp_path):
         hdus = _read_fits_table(str(tmp_path / f"out_srcs_s{i + 1}_0.fits"))
         shapes.append((len(hdus), hdus[1][0]["TFIELDS"]))
     assert shapes == [(5, "10"), (5, "5"), (2, "5")], shapes
+
+
+@pytest.mark.parametrize("N,nthr,ipop", [(128, 8, 0), (64, 13, 0)])
+def test_sources_reference_stream(workdir, N, nthr, ipop):
+    """Reference-stream mode: fed the reference's per-thread MT19937 streams
+    (seed + t + nthr ipop, srcs.c:150; libgomp's static plane split), the
+    device's Poisson counts, catalogue order, positions and base pixels equal
+    the oracle's restatement of srcs.c:120-283 with gsl_ran_poisson bit for
+    bit, on the device's density and norm tables; dz_rsd within 1e-6.  At
+    128^3 (z in [0.05, 0.5], n = 1e-3 peak) lambda reaches ~9 (1+delta), so
+    both of gsl_ran_poisson's branches run."""
+    from oracle_lib import ref_streams, static_split
+    path, run = make_run("C1", N, workdir, with_maps=False)
+    ctx = _ctx_for(run, N)
+    seed = int(run.scalar("seed"))
+    s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
+    ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=int(run.scalar("n_srcs")))
+    dens = ctx.fetch_grid(0).astype(np.float64)
+    npot = ctx.fetch_grid(1).astype(np.float64)
+    izb = static_split(N, nthr)
+    per_thread = int(4.0 * N ** 3 / nthr * 2.5) + 100000
+    st = ref_streams(seed, ipop, nthr, per_thread)
+    n = ctx.sample_sources_stream(ipop, nthr, izb, st)
+    pos, ipix = ctx.fetch_sources(ipop, n)
+    counts = ctx.fetch_cell_counts(ipop, 0, N).reshape(-1)
+    amp, bz, model = run.tracer(0, ipop)
+    tab, n0, nf = ctx.norm_table(0, ipop)
+    orc = _oracle(run)
+    pop = orc.pop(N, run.scalar("l_box"), amp, bz, tab, n0, nf, model, ipop, seed)
+    opos, oipix, ocounts = orc.sources_ref_stream(pop, dens, npot, nthr, izb, st)
+    assert n == len(opos) and n > 0
+    np.testing.assert_array_equal(counts, ocounts)
+    assert np.array_equal(pos[:, :3], opos[:, :3])
+    assert np.array_equal(ipix, oipix)
+    np.testing.assert_allclose(pos[:, 3], opos[:, 3], rtol=1e-6, atol=1e-12)
+    lam_peak = counts.max()
+    if N == 128:
+        assert lam_peak > 20, lam_peak
+    # a stream too short for the counting pass is refused
+    with pytest.raises(RuntimeError):
+        ctx.sample_sources_stream(ipop, nthr, izb, st[:, :1000])
+    ctx.close()

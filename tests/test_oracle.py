@@ -896,3 +896,139 @@ def test_imap_rsd_bias_brute_force(model):
     assert moved > 100
     np.testing.assert_array_equal(nadd, want_n)
     np.testing.assert_allclose(data, want_d, rtol=1e-12, atol=1e-300)
+
+
+# ------------------------------------------- the reference's own streams ----
+def _ref_poisson(L, stream, mu):
+    used = ctypes.c_longlong()
+    s = np.ascontiguousarray(stream, dtype=np.uint32)
+    k = L.orc_ref_poisson(p(s, ctypes.c_uint32), len(s), mu, ctypes.byref(used))
+    return k, used.value
+
+
+def test_ref_poisson_multiplication_method(L):
+    """mu <= 10 (gsl_ran_poisson's final loop): k = number of leading partial
+    products of the uniforms above e^-mu, k + 1 uniforms used; mu = 0 takes
+    one uniform and gives 0 (the reference calls the sampler for every cell
+    with n(r) > 0, whatever its lambda)."""
+    rng = np.random.default_rng(4)
+    for mu in [0.0, 1e-3, 0.37, 1.0, 2.5, 5.0, 9.2, 10.0]:
+        for _ in range(40):
+            st = rng.integers(1, 2 ** 32, size=64, dtype=np.uint64).astype(np.uint32)
+            u = st.astype(np.float64) / 2 ** 32
+            cp = np.cumprod(u)
+            kk = int(np.argmax(cp <= math.exp(-mu)))
+            assert _ref_poisson(L, st, mu) == (kk, kk + 1), (mu, kk)
+
+
+def test_ref_poisson_gamma_product_branch(L):
+    """10 < mu < 13.72: one reduction step with gamma_int(m), m = (int)(7 mu/8)
+    < 12, as -log of m uniforms; then either binomial(mu/X, m-1) by inversion
+    (k the first value whose CDF exceeds the next uniform, scipy's binomial CDF;
+    p > 1/2 counts from the other end) or k = m plus the multiplication method
+    on mu - X."""
+    rng = np.random.default_rng(6)
+    branches = set()
+    for it in range(400):
+        mu = 10.0 + 3.7 * rng.random()
+        st = rng.integers(1, 2 ** 32, size=80, dtype=np.uint64).astype(np.uint32)
+        m = int(mu * 7.0 / 8.0)
+        if it % 4 == 0:
+            # small uniforms: X > 2 mu, so p = mu/X < 1/2 (no flip)
+            st[:m] = rng.integers(1, 2 ** 32 // 20, size=m, dtype=np.uint64).astype(np.uint32)
+        u = st.astype(np.float64) / 2 ** 32
+        X = -math.log(np.prod(u[:m]))
+        if X >= mu:
+            pb, n = mu / X, m - 1
+            flip = pb > 0.5
+            pp = 1 - pb if flip else pb
+            cdf = stats.binom.cdf(np.arange(n + 1), n, pp)
+            ix = int(np.argmax(u[m] < cdf))
+            want = (n - ix if flip else ix, m + 1)
+            branches.add("binomial" + ("-flip" if flip else ""))
+        else:
+            rest = mu - X
+            cp = np.cumprod(u[m:])
+            kk = int(np.argmax(cp <= math.exp(-rest)))
+            want = (m + kk, m + kk + 1)
+            branches.add("reduce")
+        got = _ref_poisson(L, st, mu)
+        if got != want and X >= mu:
+            # inversion boundary: the running subtraction of the reference
+            # against the CDF; only within rounding of a CDF value
+            assert np.min(np.abs(u[m] - cdf)) < 1e-12, (mu, got, want)
+            continue
+        assert got == want, (mu, got, want)
+    assert branches == {"binomial", "binomial-flip", "reduce"}
+
+
+@pytest.mark.parametrize("mu", [9.2, 25.0, 80.0])
+def test_ref_poisson_distribution(L, mu):
+    """Across the branches (gamma_large for m >= 12): Poisson mean, variance
+    and chi^2 on MT19937 streams."""
+    from oracle_lib import ref_streams
+    st = ref_streams(1234, 0, 1, 4_000_000)[0]
+    ks, pos = [], 0
+    while len(ks) < 20000:
+        k, used = _ref_poisson(L, st[pos:], mu)
+        assert used > 0
+        pos += used
+        ks.append(k)
+    k = np.array(ks)
+    n = len(k)
+    assert abs(k.mean() - mu) < 5 * math.sqrt(mu / n)
+    assert abs(k.var() / mu - 1) < 6 * math.sqrt(2.0 / n) + 0.02
+    vals = np.arange(max(0, int(mu - 6 * math.sqrt(mu))), int(mu + 6 * math.sqrt(mu)) + 1)
+    obs = np.array([(k == v).sum() for v in vals])
+    exp = stats.poisson.pmf(vals, mu) * n
+    msk = exp > 5
+    chi2 = ((obs[msk] - exp[msk]) ** 2 / exp[msk]).sum()
+    assert stats.chi2.sf(chi2, msk.sum() - 1) > 1e-4
+
+
+def test_sources_ref_stream_walk():
+    """orc_sources_ref_stream against a driver of its own: every thread walks
+    its planes in z, y, x order and calls the restated gsl_ran_poisson on its
+    stream for each cell with r < L/2 + 20 and n(r) > 0; pass 2 restarts the
+    stream (srcs.c:231), the k-th source of a thread taking uniforms 3k..3k+2
+    for x, y, z.  MT19937 streams seeded seed + t + nthr ipop."""
+    from oracle_lib import ref_streams, static_split
+    ta = _tables()
+    orc = Oracle(ta)
+    N, Lb, nthr, ipop, seed = 16, 900.0, 3, 1, 555
+    r_t = ta["r"]
+    amp = np.where(r_t < 380, 6e-5, 0.0)   # lambda ~ 11 (1+delta)^1.4: both Poisson branches
+    rng = np.random.default_rng(1)
+    g = np.zeros((N, N, 2 * (N // 2 + 1)))
+    g[:, :, :N] = np.exp(rng.normal(0, 0.7, size=(N, N, N))) - 1.05
+    pop = orc.pop(N, Lb, amp, np.full(5001, 1.4), np.ones(5001), 1.0, 1.0, 0, ipop, seed)
+    izb = static_split(N, nthr)
+    assert list(izb) == [0, 6, 11, 16]
+    st = ref_streams(seed, ipop, nthr, 400000)
+    pos, ipix, counts = orc.sources_ref_stream(pop, g, g * 0, nthr, izb, st)
+    dx = Lb / N
+    want = np.zeros(N ** 3, dtype=np.int32)
+    src_cells = []
+    for t in range(nthr):
+        at = 0
+        for iz in range(izb[t], izb[t + 1]):
+            for iy in range(N):
+                for ix in range(N):
+                    r = math.sqrt(sum(((i * dx) - Lb / 2) ** 2 for i in (ix, iy, iz)))
+                    if r < Lb / 2 + 20 and r > 0 and np.interp(r, r_t, amp) > 0:
+                        lam = orc.lib.orc_cell_lambda(ctypes.byref(pop), p(np.ascontiguousarray(g)), ix, iy, iz)
+                        k, used = _ref_poisson(orc.lib, st[t][at:], lam)
+                        at += used
+                        c = ix + N * (iy + N * iz)
+                        want[c] = k
+                        src_cells += [(t, ix, iy, iz)] * k
+    np.testing.assert_array_equal(counts, want)
+    assert len(pos) == want.sum() > 500
+    # pass 2: positions from the start of each thread's stream
+    kt = {t: 0 for t in range(nthr)}
+    for i, (t, ix, iy, iz) in enumerate(src_cells):
+        k = kt[t]
+        kt[t] += 1
+        u = st[t][3 * k: 3 * k + 3].astype(np.float64) / 2 ** 32
+        c0 = np.array([ix, iy, iz]) * dx - Lb / 2
+        np.testing.assert_array_equal(pos[i, :3], (c0 + dx * (u - 0.5)).astype(np.float32))
