@@ -166,6 +166,10 @@ int clr_upload_tables(clr_ctx *c, const clr_tables *t) {
   c->na = na;
   c->glob_idr = t->glob_idr;
   c->r_last = t->r_arr_r2z[na - 1];
+  c->h_r.assign(t->r_arr_r2z, t->r_arr_r2z + na);
+  c->h_z.assign(t->z_arr_r2z, t->z_arr_r2z + na);
+  c->h_d1.assign(t->growth_d_arr, t->growth_d_arr + na);
+  c->geo_nz = 0;   // shell geometry depends on the tables
   for (int i = 0; i < 6; i++) {
     c->h_bg_first[i] = src[i][0];
     c->h_bg_last[i] = src[i][na - 1];
@@ -222,6 +226,7 @@ int clr_set_tracer(clr_ctx *c, int kind, int ipop, int bias_model, const double 
   if (!T.bzf) CLR_HIP(c, hipMalloc((void **)&T.bzf, (size_t)c->na * sizeof(float)));
   CLR_HIP(c, hipMemcpy(T.amp, amp_arr, b, hipMemcpyHostToDevice));
   CLR_HIP(c, hipMemcpy(T.bz, bz_arr, b, hipMemcpyHostToDevice));
+  T.h_bz.assign(bz_arr, bz_arr + c->na);
   std::vector<float> bzf(c->na);
   for (int i = 0; i < c->na; i++) bzf[i] = (float)bz_arr[i];
   CLR_HIP(c, hipMemcpy(T.bzf, bzf.data(), (size_t)c->na * sizeof(float), hipMemcpyHostToDevice));
@@ -281,7 +286,7 @@ int clr_set_norm_table(clr_ctx *c, int kind, int ipop, const double *norm_arr, d
 int clr_sample_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc_out) {
   if (!c) return CLR_ERR_ARG;
   CLR_HIP(c, hipSetDevice(c->device));
-  return clr_launch_sources(c, ipop, seed, nside_base, nsrc_out);
+  return clr_launch_sources(c, ipop, seed, nside_base, nsrc_out, 0);
 }
 
 int clr_sample_sources_stream(clr_ctx *c, int ipop, int nthr, const int *iz_begin, const uint32_t *streams,

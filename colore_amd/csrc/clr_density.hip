@@ -27,6 +27,7 @@
 //               all on the device.
 #include "clr_internal.h"
 #include <math.h>
+#include <vector>
 
 using namespace clr;
 
@@ -172,15 +173,6 @@ __device__ inline float bias_sum_term(float d, float l2, float q, float b, int m
   return d > -1.f ? (use_exp ? e : lin) : 0.f;
 }
 
-// e^x to about 2 ulp: the hardware exp2 of x log2(e) rounded, times the
-// first-order correction for that rounding (log2(e) = 1.44269502 + 1.925963e-8).
-__device__ inline float exp_lognormal(float x) {
-  const float t = x * 1.44269502f;
-  const float e = __builtin_fmaf(x, 1.44269502f, -t) + x * 1.925963e-8f;
-  const float r = __builtin_amdgcn_exp2f(t);
-  return __builtin_fmaf(r, e * 0.693147181f, r);
-}
-
 // LDS per block: kWaves x nz x NQ accumulators, then per wavefront kSlots x
 // NQ x 64 logged shell sums and kSlots x 64 shell ids.
 template <int NT>
@@ -234,7 +226,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
     double cacc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) cacc[q] = 0;
-    const float x32 = ix * dxf - ox;
+    const float x32 = coord_f32(ix, dxf, ox);
     // The next row's density and table bins are loaded while the current row
     // is processed.  Bin lookup: FP32 radius (absolute error ~1e-3 cell), one
     // index for D, z and every b_t (f_of_r32 semantics).
@@ -245,15 +237,13 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
     };
     auto fetch = [&](int rr2, Bin &bn) {
       const int iy2 = iy0 + (rr2 % kTY) * kSub, iz2 = tz * kTZ + (rr2 / kTY);
-      const float y32 = iy2 * dxf - oy, z32 = iz2 * dxf - oz;
-      const float rf = __builtin_amdgcn_sqrtf(x32 * x32 + y32 * y32 + z32 * z32);
-      const float xq = rf * a.idr_f;
-      bn.e0 = !(rf > 0.f);
-      bn.e1 = xq >= (float)(a.na - 1);
-      const int i0 = (bn.e0 || bn.e1) ? 0 : (int)xq;
-      bn.tq = xq - (float)i0;
+      const float y32 = coord_f32(iy2, dxf, oy), z32 = coord_f32(iz2, dxf, oz);
+      const TabPos tp = tab_pos_f32(radius_f32(x32, y32, z32), a.idr_f, a.na);
+      bn.e0 = tp.e0;
+      bn.e1 = tp.e1;
+      bn.tq = tp.tq;
 #pragma unroll
-      for (int k = 0; k < W; k++) bn.C[k] = a.tab[(size_t)i0 * W + k];
+      for (int k = 0; k < W; k++) bn.C[k] = a.tab[(size_t)tp.i0 * W + k];
     };
     // the lane's cell in the tile's first row; rows at uniform offsets from it
     float *const tb = a.g + ((size_t)(tz * kTZ) * N + iy0) * a.ngx + (valid ? ix : 0);
@@ -300,9 +290,10 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
           }
         }
         if (a.fused) {
-          const float dg = comp[0];
-          const float arg = dg * (dl - 0.5f * dg * (float)a.sigma2);
-          dl = exp_lognormal(arg) - 1.f;
+          // the same float as lognormal_f32 of the fused count pass
+          const TabPos tp{0, bn.tq, e0, e1};
+          const float D = growth_at(tp, bn.C[0].x, bn.C[0].y, a.d1_last_f);
+          dl = lognormal_f32(dl, D, (float)a.sigma2);
           tb[roff(rr)] = dl;
         }
         // shell index: FP32 z (error < 5e-5 shell) unless within 5e-4 shell of
@@ -554,18 +545,18 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   bool all0 = true;
   for (int t = 0; t < ntr; t++) all0 = all0 && a.model[t] == CLR_BIAS_EXPONENTIAL;
   const size_t nvals = (size_t)nz * nq;
-  size_t need = (size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr;
+  const int w = shell_tab_w(ntr);
+  const size_t need = (size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr;
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
   if (rc) return rc;
   double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals, *work = fin + nvals;
-  const int w = shell_tab_w(ntr);
   rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
   if (rc) return rc;
-  a.tab = (const float4 *)c->misc;
+  float4 *tab = (float4 *)c->misc;
+  a.tab = tab;
 
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, ntr, na,
-                     (float4 *)c->misc);
+  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, ntr, na, tab);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
   kShellFns[all0 ? 1 : 0][ntr](dim3(kShellBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
@@ -582,9 +573,10 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
     CLR_HIP(c, hipGetLastError());
   }
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
-  std::vector<double> h(nvals + 2 * ntr);
-  CLR_HIP(c, hipMemcpyAsync(h.data(), fin, nvals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  CLR_HIP(c, hipMemcpyAsync(h.data() + nvals, work + 2 * nz, 2 * ntr * sizeof(double), hipMemcpyDeviceToHost,
+  std::vector<double> hv(nvals + 2 * ntr);
+  double *h = hv.data();
+  CLR_HIP(c, hipMemcpyAsync(h, fin, nvals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(h + nvals, work + 2 * nz, 2 * ntr * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
   for (int t = 0; t < ntr; t++) {
@@ -611,3 +603,112 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   c->norms_done = 1;
   return CLR_OK;
 }
+
+// Cells and z sums per normalisation shell (the count and zsum columns of
+// collect_density_normalization_from_grid, density.c:1128-1211): the shell
+// pass without tracers; they depend only on the grid geometry and tables.
+int clr_launch_shell_geometry(clr_ctx *c) {
+  if (c->geo_nz > 0) return CLR_OK;
+  if (!c->tables_set || !c->d1f) return clr_fail(c, CLR_ERR_STATE, "clr_upload_tables must be called first");
+  const int N = c->N, na = c->na;
+  const double zmax = c->zmax_half_box;
+  const int nz = (int)(zmax / 0.05) + 2;
+  if (nz > kNormMaxShells) return clr_fail(c, CLR_ERR_ARG, "%d redshift shells (at most %d)", nz, kNormMaxShells);
+  ShellArgs a{};
+  ShellTabSrc tsrc{};
+  a.g = c->dens;
+  a.N = N;
+  a.ngx = c->ngx;
+  a.nz = nz;
+  a.ntr = 0;
+  a.fused = 0;
+  a.na = na;
+  a.dx = c->dx;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.z_last = c->h_bg_last[1];
+  a.idz = (nz - 2) / zmax;
+  a.r_arr = c->bg;
+  a.zt = c->bg + na;
+  tsrc.d1f = c->d1f;
+  tsrc.zt = c->bg + na;
+  a.z_last_f = (float)c->h_bg_last[1];
+  a.idz_f = (float)a.idz;
+  a.d1_last_f = (float)c->h_bg_last[2];
+  a.idr_f = (float)c->glob_idr;
+  const int nq = 2;
+  const size_t nvals = (size_t)nz * nq;
+  const int w = shell_tab_w(0);
+  int rc = clr_ensure(c, (void **)&c->red, &c->red_n, ((size_t)kShellBlocks * nvals + nvals) * sizeof(double));
+  if (rc) return rc;
+  double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals;
+  rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
+  if (rc) return rc;
+  a.tab = (const float4 *)c->misc;
+  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, 0, na,
+                     (float4 *)c->misc);
+  kShellFns[1][0](dim3(kShellBlocks), dim3(kShellThreads), shell_lds_bytes<0>(nz), c->stream, a, partial);
+  hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,
+                     (int)nvals, fin);
+  CLR_HIP(c, hipGetLastError());
+  std::vector<double> h(nvals);
+  CLR_HIP(c, hipMemcpyAsync(h.data(), fin, nvals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  c->geo_cnt.resize(nz);
+  c->geo_zsum.resize(nz);
+  for (int iz = 0; iz < nz; iz++) {
+    c->geo_cnt[iz] = h[(size_t)iz * nq];
+    c->geo_zsum[iz] = h[(size_t)iz * nq + 1];
+  }
+  c->geo_zmax = zmax;
+  c->geo_nz = nz;
+  return CLR_OK;
+}
+
+// Norm tables of the given tracers from per-shell norms norm_s[t][nz] on the
+// cached geometry: the sums are count/norm, and k_norm_table then applies the
+// reference's edge fill and interpolation in z (density.c:1272-1355).
+int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const int *ipops, const double *norm_s) {
+  if (c->geo_nz <= 0) return clr_fail(c, CLR_ERR_STATE, "no shell geometry");
+  if (ntr < 1 || ntr > 3 * CLR_NPOP_MAX) return clr_fail(c, CLR_ERR_ARG, "bad tracer count %d", ntr);
+  const int nz = c->geo_nz, na = c->na, nq = 2 + ntr;
+  std::vector<double> fin((size_t)nz * nq + 2 * (size_t)ntr, 0.0);
+  for (int iz = 0; iz < nz; iz++) {
+    const double n = c->geo_cnt[iz];
+    fin[(size_t)iz * nq] = n;
+    fin[(size_t)iz * nq + 1] = c->geo_zsum[iz];
+    for (int t = 0; t < ntr; t++) {
+      const double nm = norm_s[(size_t)t * nz + iz];
+      fin[(size_t)iz * nq + 2 + t] = n > 0 ? n / nm : 0.0;
+    }
+  }
+  const size_t bytes = fin.size() * sizeof(double);
+  int rc = clr_ensure(c, (void **)&c->red, &c->red_n, bytes);
+  if (rc) return rc;
+  double *dfin = c->red, *edges = c->red + (size_t)nz * nq;
+  CLR_HIP(c, hipMemcpyAsync(dfin, fin.data(), (size_t)nz * nq * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  NormOuts outs{};
+  for (int t = 0; t < ntr; t++) {
+    TracerDev &T = c->tr[kinds[t]][ipops[t]];
+    if (!T.set || !T.norm) return clr_fail(c, CLR_ERR_STATE, "tracer %d/%d has no tables", kinds[t], ipops[t]);
+    outs.p[t] = T.norm;
+  }
+  hipLaunchKernelGGL(k_norm_table, dim3((unsigned)((na + 255) / 256), (unsigned)ntr), dim3(256), 0, c->stream,
+                     dfin, nz, nq, c->geo_zmax, c->bg, c->bg + na, c->glob_idr, c->r_last, c->h_bg_last[1], na,
+                     outs, edges);
+  CLR_HIP(c, hipGetLastError());
+  std::vector<double> e(2 * (size_t)ntr);
+  CLR_HIP(c, hipMemcpyAsync(e.data(), edges, 2 * ntr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < ntr; t++) {
+    TracerDev &T = c->tr[kinds[t]][ipops[t]];
+    T.norm0 = e[2 * t];
+    T.normf = e[2 * t + 1];
+    T.norm_set = 1;
+  }
+  return CLR_OK;
+}
+

@@ -109,6 +109,59 @@ __host__ __device__ inline double bias_model_f64(double d, double b, int model) 
   return pow(1 + d, b);
 }
 
+// ---------------------------------------------------------------------------
+// The FP32 lognormal transform of one cell (lognormalize, density.c:1070-1103)
+// as the shell pass applies it in place and the surrogate-fused count pass
+// applies it on the fly: cell offsets and radius in FP32, D(r) by linear
+// interpolation of the FP32 growth table (D(0) = 1, D beyond the table = its
+// last value, as get_bg), delta = exp(D (dG - D sigma^2 / 2)) - 1.  Every
+// rounding is spelled out (explicit fma, contraction off), so both passes
+// produce the same float for the same cell.
+// ---------------------------------------------------------------------------
+__device__ inline float coord_f32(int i, float dxf, float o) { return __builtin_fmaf((float)i, dxf, -o); }
+__device__ inline float radius_f32(float x, float y, float z) {
+  return __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, __builtin_fmaf(y, y, z * z)));
+}
+// Table position of radius rf: bin i0 and fraction tq, with the two edge flags.
+struct TabPos {
+  int i0;
+  float tq;
+  bool e0, e1;
+};
+__device__ inline TabPos tab_pos_f32(float rf, float idr_f, int na) {
+  TabPos p;
+  const float xq = rf * idr_f;
+  p.e0 = !(rf > 0.f);
+  p.e1 = xq >= (float)(na - 1);
+  p.i0 = (p.e0 || p.e1) ? 0 : (int)xq;
+  p.tq = xq - (float)p.i0;
+  return p;
+}
+// D from the bin's (value, slope) pair.
+__device__ inline float growth_at(const TabPos &p, float d0, float slope, float d1_last_f) {
+  const float v = __builtin_fmaf(slope, p.tq, d0);
+  return p.e0 ? 1.f : (p.e1 ? d1_last_f : v);
+}
+__device__ inline float growth_f32(const float *__restrict__ d1f, const TabPos &p, float d1_last_f) {
+  const int j = p.i0 + 1;
+  const float d0 = d1f[p.i0], d1 = d1f[j];
+  return growth_at(p, d0, d1 - d0, d1_last_f);
+}
+// e^x to about 2 ulp: the hardware exp2 of x log2(e) rounded, times the
+// first-order correction for that rounding (log2(e) = 1.44269502 + 1.925963e-8).
+__device__ inline float exp_lognormal(float x) {
+  const float t = x * 1.44269502f;
+  const float e = __builtin_fmaf(x, 1.44269502f, -t) + x * 1.925963e-8f;
+  const float r = __builtin_amdgcn_exp2f(t);
+  return __builtin_fmaf(r, e * 0.693147181f, r);
+}
+__device__ inline float lognormal_f32(float dg, float D, float s2f) {
+#pragma clang fp contract(off)
+  const float hh = (0.5f * D) * s2f;
+  const float arg = D * (dg - hh);
+  return exp_lognormal(arg) - 1.f;
+}
+
 __device__ inline float bias_model_f32(float d, float b, int model) {
   if (d <= -1.f) return 0.f;
   if (model == 1) {

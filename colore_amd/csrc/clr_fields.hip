@@ -453,6 +453,7 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   const double mean = s[0] / ntot;
   const double sig2 = s[1] / ntot - mean * mean;
   c->sigma2 = sig2;
+  c->r_smooth = r_smooth;
   c->have_fields = 1;
   c->density_done = 0;
   c->norms_done = 0;

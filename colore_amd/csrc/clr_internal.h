@@ -19,6 +19,7 @@ struct TracerDev {
   double bz0 = 0;           // f0 of the bias table (its first entry)
   double norm0 = 1, normf = 1;
   int norm_set = 0;
+  std::vector<double> h_bz; // host copy of bz (surrogate inputs)
 };
 
 struct CatalogDev {
@@ -111,6 +112,14 @@ struct clr_ctx {
   size_t ref_cnt_n = 0;
   int ref_ipop = -1;
 
+  // host copies of r, z, D (surrogate inputs) and the shell geometry of the
+  // normalisation shells (cells and sum of z per shell; geo_nz = 0: not yet)
+  std::vector<double> h_r, h_z, h_d1;
+  int geo_nz = 0;
+  double geo_zmax = 0;
+  std::vector<double> geo_cnt, geo_zsum;
+  double r_smooth = 0;      // smoothing length of the last clr_gaussian_fields
+
   int have_fields = 0;
   int density_done = 0;
   int norms_done = 0;
@@ -140,7 +149,9 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
 // Makes npot safe to use: host = 0 orders c->stream after phi's passes,
 // host = 1 waits for them on the host.
 int clr_phi_ready(clr_ctx *c, int host);
-int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc);
+// fused: the grid holds the Gaussian field; counts use its lognormal
+// transform per cell (surrogate-fused sampling, norm tables already set)
+int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc, int fused);
 int clr_launch_source_props(clr_ctx *c, int ipop, float *out, long long n);
 int clr_launch_fetch_counts(clr_ctx *c, int ipop, int iz0, int nz, int32_t *out);
 int clr_launch_sources_stream(clr_ctx *c, int ipop, int nthr, const int *iz_begin, const uint32_t *streams,
@@ -152,3 +163,12 @@ int clr_launch_lpt(clr_ctx *c, int order, int interp, float *pos_out);
 int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, float *dg, float *vs, long long n);
 int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, const double *rf,
                     float *data, int32_t *nadd);
+
+// Shell geometry of the normalisation (cells and z sums per shell), cached in
+// the context; and norm tables built from given per-shell norms of tracers
+// (kind, ipop) as compute_density_normalization builds them from sums.
+int clr_launch_shell_geometry(clr_ctx *c);
+int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const int *ipops, const double *norm_s);
+
+// Prediction of the surrogate network on device buffers (clr_mlp.hip).
+int clr_mlp_predict_device(clr_mlp *net, const float *dx, int n, float *dy, int precision, hipStream_t stream);

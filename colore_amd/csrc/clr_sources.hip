@@ -66,7 +66,22 @@ struct SrcArgs {
   unsigned int *exq_n;
   unsigned int exq_cap;
   PopTab pop[CLR_NPOP_MAX];
+  // surrogate-fused sampling: the grid holds the Gaussian field and the
+  // lognormal transform is applied per cell as the shell pass would
+  int fused;
+  const float *d1f;
+  float s2f, d1_last_f, idr_f, dxf, oxf, oyf, ozf;
 };
+
+// The density of cell (ix,iy,iz) that the counts use: the stored value, or
+// with a.fused its FP32 lognormal transform (lognormal_f32), the same float
+// the shell pass would have stored.
+__device__ inline float cell_density(const SrcArgs &a, float g, int ix, int iy, int iz) {
+  if (!a.fused) return g;
+  const TabPos p = tab_pos_f32(radius_f32(coord_f32(ix, a.dxf, a.oxf), coord_f32(iy, a.dxf, a.oyf),
+                                          coord_f32(iz, a.dxf, a.ozf)), a.idr_f, a.na);
+  return lognormal_f32(g, growth_f32(a.d1f, p, a.d1_last_f), a.s2f);
+}
 
 // Per bin i (the interval [r_i, r_i+1] of the FP64 tables) one float4 that
 // the sampler gathers once per cell:
@@ -282,6 +297,7 @@ constexpr int kCountSubs = kCountCells / kBlockCells;
 // another population's 1024 entries might not fit, so usually once.
 constexpr int kQueueCap = kCountCells + kCountCells / 4;
 
+template <bool FUSED>
 __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(6))) void k_count(SrcArgs a) {
   __shared__ uint32_t queue[kQueueCap];
   extern __shared__ uint4 sbytes[];   // [npop][kLanes]: one byte per cell and population
@@ -350,6 +366,19 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(6))) voi
     out = rlo > hb;
     ilo = (int)(rlo * idr * (1.f - 1e-5f));
     ihi = (int)(fminf(rhi, hb) * idr * (1.f + 1e-5f));
+    if constexpr (FUSED) {
+      // dmax holds the largest Gaussian value g; every cell's density is
+      // exp(D (g_c - D s2/2)) - 1 with D in [D(rhi), D(rlo)] (D decreases with
+      // r), and D g - D^2 s2/2 is concave in D: its largest value on that
+      // range, widened for the FP32 roundings of the cell's own evaluation.
+      const TabPos plo = tab_pos_f32(rlo, a.idr_f, a.na), phi = tab_pos_f32(rhi, a.idr_f, a.na);
+      const float dhi = growth_f32(a.d1f, plo, a.d1_last_f) * (1.f + 1e-6f);
+      const float dlo = growth_f32(a.d1f, phi, a.d1_last_f) * (1.f - 1e-6f);
+      const float g = dmax, s2 = a.s2f;
+      const float dstar = s2 > 0.f ? fminf(fmaxf(g / s2, dlo), dhi) : (g > 0.f ? dhi : dlo);
+      const float fm = dstar * (g - 0.5f * dstar * s2);
+      dmax = expf(fm + 1e-5f * fabsf(fm) + 1e-6f) * (1.f + 1e-5f) - 1.f;
+    }
   }
 
   int qn = 0;   // uniform
@@ -413,7 +442,7 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(6))) voi
         const size_t cell = base + cb;
         const uint2 gl = sgeo[cb / kCountLaneCells];
         const int ix = (int)(gl.x & 0xFFFFu) + cb % kCountLaneCells, iy = (int)(gl.x >> 16), iz = (int)gl.y;
-        const float dl = a.g[((size_t)iz * N + iy) * a.ngx + ix];
+        const float dl = cell_density(a, a.g[((size_t)iz * N + iy) * a.ngx + ix], ix, iy, iz);
         const int k = settle_cell(a, stab[q], smodel[q], a.pop[0].ipop + q, q, cell, ix, iy, iz, dl, hi);
         ((uint8_t *)(sbytes + q * kLanes))[cb] =
             (uint8_t)(k < 0 ? kBytePending : (k < kBytePending ? k : kByteEscape));
@@ -450,7 +479,9 @@ __global__ __launch_bounds__(256) void k_exact(SrcArgs a) {
   const int N = a.N;
   const uint64_t rw = cell / (uint64_t)N;
   const int cx = (int)(cell - rw * N);
-  const int k = count_exact(cx, (int)(rw % N), (int)(rw / N), a.g[rw * (size_t)a.ngx + cx], cell, e.w, a.r_arr,
+  const int k = count_exact(cx, (int)(rw % N), (int)(rw / N),
+                            cell_density(a, a.g[rw * (size_t)a.ngx + cx], cx, (int)(rw % N), (int)(rw / N)), cell,
+                            e.w, a.r_arr,
                             P.amp, P.bz, P.norm, a.idr, a.r_last, P.bz0, P.norm0, P.normf, a.dx, a.ox, a.oy, a.oz,
                             a.cell_vol, a.half_box_20, P.model, P.ipop, a.seed);
   if (k > 0) atomicAdd(&P.btot[cell / kBlockCells], (uint32_t)k);
@@ -918,10 +949,13 @@ __global__ void k_src_props(const float4 *__restrict__ pos, long long n, const d
 
 }  // namespace
 
-int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc) {
+int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long long *nsrc, int fused) {
   const int nall = c->n_tr[CLR_TRACER_SRCS];
   if (ipop < -1 || ipop >= nall) return clr_fail(c, CLR_ERR_ARG, "no source population %d", ipop);
-  if (!c->density_done) return clr_fail(c, CLR_ERR_STATE, "density field not computed");
+  if (!fused && !c->density_done) return clr_fail(c, CLR_ERR_STATE, "density field not computed");
+  if (fused && (!c->have_fields || c->density_done))
+    return clr_fail(c, CLR_ERR_STATE, "fused sampling needs the Gaussian field (fields done, density not transformed)");
+  if (fused && !c->d1f) return clr_fail(c, CLR_ERR_STATE, "background tables missing");
   if (nside_base < 1 || nside_base > 64 || (nside_base & (nside_base - 1)))
     return clr_fail(c, CLR_ERR_ARG, "nside_base=%d must be a power of two <= 64", nside_base);
   const int p0 = ipop < 0 ? 0 : ipop, p1 = ipop < 0 ? nall : ipop + 1;
@@ -972,6 +1006,15 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
   a.na = na;
   a.npop = np;
   a.seed = seed;
+  a.fused = fused ? 1 : 0;
+  a.d1f = c->d1f;
+  a.s2f = (float)c->sigma2;
+  a.d1_last_f = (float)c->h_bg_last[2];
+  a.idr_f = (float)c->glob_idr;
+  a.dxf = (float)dx;
+  a.oxf = (float)c->obs[0];
+  a.oyf = (float)c->obs[1];
+  a.ozf = (float)c->obs[2];
   std::vector<long long *> bsum(np);
   for (int i = p0; i < p1; i++) {
     const TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
@@ -1017,8 +1060,12 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     CLR_HIP(c, hipMemsetAsync(a.esc_n, 0, sizeof(unsigned int), c->stream));
     CLR_HIP(c, hipMemsetAsync(a.exq_n, 0, sizeof(unsigned int), c->stream));
     CLR_HIP(c, hipEventRecord(c->ev2, c->stream));
-    hipLaunchKernelGGL(k_count, dim3((unsigned)(ncells / kCountCells)), dim3(kLanes),
-                       (size_t)np * kLanes * sizeof(uint4), c->stream, a);
+    if (fused)
+      hipLaunchKernelGGL(k_count<true>, dim3((unsigned)(ncells / kCountCells)), dim3(kLanes),
+                         (size_t)np * kLanes * sizeof(uint4), c->stream, a);
+    else
+      hipLaunchKernelGGL(k_count<false>, dim3((unsigned)(ncells / kCountCells)), dim3(kLanes),
+                         (size_t)np * kLanes * sizeof(uint4), c->stream, a);
     CLR_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_exact, dim3((a.exq_cap + 255) / 256), dim3(256), 0, c->stream, a);
     CLR_HIP(c, hipGetLastError());

@@ -1,0 +1,145 @@
+// clr_surrogate.hip -- surrogate-fused source sampling: the per-shell bias
+// normalisations of compute_density_normalization (density.c:1128-1393)
+// predicted by the learned network (clr_mlp.hip) from the shell's
+// (D, b, sigma_G, r_smooth), so that srcs_set_cartesian_single (srcs.c:
+// 120-283) runs in one pass over the Gaussian field: the lognormal transform
+// of lognormalize (density.c:1070-1103) is applied per cell inside the count
+// pass, and the shell-sum pass over the grid is skipped.  The exact two-pass
+// reduction (clr_shell_norms) stays the fallback and the judge of the
+// number-count error.
+#include "clr_internal.h"
+#include <math.h>
+#include <vector>
+
+namespace {
+
+// r of redshift z on the FP64 tables (z(r) increases with r): the inverse of
+// the linear interpolation f_of_r applies.
+double r_of_z_tab(const clr_ctx *c, double z) {
+  const std::vector<double> &zz = c->h_z, &rr = c->h_r;
+  const int na = (int)zz.size();
+  if (z <= zz[0]) return rr[0];
+  if (z >= zz[na - 1]) return rr[na - 1];
+  int lo = 0, hi = na - 1;
+  while (hi > lo + 1) {
+    const int m = (lo + hi) / 2;
+    if (zz[m] > z) hi = m; else lo = m;
+  }
+  const double t = (z - zz[lo]) / (zz[hi] - zz[lo]);
+  return rr[lo] + t * (rr[hi] - rr[lo]);
+}
+
+// Inputs of population ipop for every shell with cells (bins 1 .. nz-1 of
+// the cached geometry): at the shell's mean redshift, D, b, sigma_G = D sigma
+// (sigma^2 the measured variance of the Gaussian grid) and r_smooth.
+int shell_inputs(clr_ctx *c, int ipop, std::vector<float> &x, std::vector<int> &idx) {
+  const TracerDev &T = c->tr[CLR_TRACER_SRCS][ipop];
+  if (!T.set || T.h_bz.size() != (size_t)c->na) return clr_fail(c, CLR_ERR_STATE, "population %d has no tables", ipop);
+  const double sig = sqrt(c->sigma2);
+  x.clear();
+  idx.clear();
+  for (int s = 0; s < c->geo_nz; s++) {
+    if (!(c->geo_cnt[s] > 0)) continue;
+    const double zbar = c->geo_zsum[s] / c->geo_cnt[s];
+    const double r = r_of_z_tab(c, zbar);
+    const double D = clr::f_of_r(c->h_r.data(), c->h_d1.data(), c->glob_idr, c->r_last, r, 1.0, c->h_bg_last[2]);
+    const double b = clr::f_of_r(c->h_r.data(), T.h_bz.data(), c->glob_idr, c->r_last, r, T.bz0, 1.0);
+    x.push_back((float)D);
+    x.push_back((float)b);
+    x.push_back((float)(D * sig));
+    x.push_back((float)c->r_smooth);
+    idx.push_back(s);
+  }
+  return CLR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clr_surrogate_inputs(clr_ctx *c, int ipop, int cap, float *x, int *shell, int *n_out) {
+  if (!c) return CLR_ERR_ARG;
+  if (ipop < 0 || ipop >= c->n_tr[CLR_TRACER_SRCS]) return clr_fail(c, CLR_ERR_ARG, "no source population %d", ipop);
+  if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
+  CLR_HIP(c, hipSetDevice(c->device));
+  int rc = clr_launch_shell_geometry(c);
+  if (rc) return rc;
+  std::vector<float> xs;
+  std::vector<int> idx;
+  rc = shell_inputs(c, ipop, xs, idx);
+  if (rc) return rc;
+  const int n = (int)idx.size();
+  if (n_out) *n_out = n;
+  if ((x || shell) && cap < n) return clr_fail(c, CLR_ERR_ARG, "capacity %d < %d shells", cap, n);
+  for (int i = 0; i < n; i++) {
+    if (x)
+      for (int k = 0; k < CLR_MLP_IN; k++) x[(size_t)i * CLR_MLP_IN + k] = xs[(size_t)i * CLR_MLP_IN + k];
+    if (shell) shell[i] = idx[i];
+  }
+  return CLR_OK;
+}
+
+int clr_sample_sources_surrogate(clr_ctx *c, clr_mlp *net, int ipop, uint32_t seed, int nside_base, int precision,
+                                 long long *nsrc_out) {
+  if (!c || !net) return CLR_ERR_ARG;
+  const int nall = c->n_tr[CLR_TRACER_SRCS];
+  if (ipop < -1 || ipop >= nall) return clr_fail(c, CLR_ERR_ARG, "no source population %d", ipop);
+  if (!c->have_fields || c->density_done)
+    return clr_fail(c, CLR_ERR_STATE, "surrogate sampling needs the Gaussian field (fields done, density not transformed)");
+  CLR_HIP(c, hipSetDevice(c->device));
+  int rc = clr_launch_shell_geometry(c);
+  if (rc) return rc;
+  const int p0 = ipop < 0 ? 0 : ipop, p1 = ipop < 0 ? nall : ipop + 1, np = p1 - p0, nz = c->geo_nz;
+  CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
+  // every population's shells in one inference batch
+  std::vector<float> x, xs;
+  std::vector<int> idx, owner;
+  for (int p = p0; p < p1; p++) {
+    std::vector<int> ip;
+    rc = shell_inputs(c, p, xs, ip);
+    if (rc) return rc;
+    x.insert(x.end(), xs.begin(), xs.end());
+    idx.insert(idx.end(), ip.begin(), ip.end());
+    owner.insert(owner.end(), ip.size(), p - p0);
+  }
+  const int n = (int)idx.size();
+  std::vector<float> y((size_t)n * CLR_MLP_OUT);
+  if (n > 0) {
+    float *dx = nullptr, *dy = nullptr;
+    hipError_t e = hipMalloc((void **)&dx, (size_t)n * CLR_MLP_IN * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&dy, (size_t)n * CLR_MLP_OUT * sizeof(float));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dx, x.data(), x.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && clr_mlp_predict_device(net, dx, n, dy, precision, c->stream)) e = hipErrorUnknown;
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(y.data(), dy, y.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "surrogate inference failed");
+  }
+  // per-shell norms (y0 = ln norm); shells without cells keep 1 (the edge
+  // fill of the norm table replaces bin 0 and the last bin anyway)
+  std::vector<double> norm_s((size_t)np * nz, 1.0);
+  for (int i = 0; i < n; i++) {
+    const double v = exp((double)y[(size_t)i * CLR_MLP_OUT]);
+    if (!(v > 0) || !std::isfinite(v)) return clr_fail(c, CLR_ERR_STATE, "surrogate predicted norm %g", v);
+    norm_s[(size_t)owner[i] * nz + idx[i]] = v;
+  }
+  std::vector<int> kinds(np, CLR_TRACER_SRCS), ipops(np);
+  for (int p = 0; p < np; p++) ipops[p] = p0 + p;
+  rc = clr_launch_norm_tables_given(c, np, kinds.data(), ipops.data(), norm_s.data());
+  if (rc) return rc;
+  CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
+  CLR_HIP(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  const double t_norm = ms;
+  rc = clr_launch_sources(c, ipop, seed, nside_base, nsrc_out, 1);
+  if (rc) return rc;
+  c->stage_ms[ST_NORM] = t_norm;
+  c->norms_done = 1;
+  return CLR_OK;
+}
+
+}  // extern "C"

@@ -115,7 +115,7 @@ CONFIGS = {
 def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_format="ASCII",
                  output_density=False, with_maps=True, seed=None, imap_bands=None, imap_nside=None,
                  kappa_nside=None, lensing=(), skewers=(), custom=(), pred_dz=None, just_pred=False,
-                 output_lpt=False):
+                 output_lpt=False, sigma8=None, r_smooth=None, populations=None):
     """Writes P(k), n(z), b(z) (+T(z), nu table) and the param file of config
     `name` into outdir; returns the param-file path.  n_grid/z_max override
     the config's values (small variants for tests); `lensing` lists the
@@ -124,6 +124,13 @@ def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_form
     nside, bias) of custom maps with a Gaussian kernel K(z); pred_dz turns on
     write_pred (just_pred: just_write_pred)."""
     cfg = dict(CONFIGS[name])
+    if r_smooth is not None:
+        cfg["r_smooth"] = r_smooth
+    if populations is not None:
+        cfg["srcs"] = list(populations)
+    cosmo = dict(COSMO)
+    if sigma8 is not None:
+        cosmo["sigma_8"] = sigma8
     os.makedirs(outdir, exist_ok=True)
     n = int(n_grid or cfg["n_grid"])
     zmax = float(z_max or cfg["z_max"])
@@ -141,7 +148,7 @@ def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_form
             ["};", "field_par: {", f"  r_smooth = {cfg['r_smooth']};",
              "  smooth_potential = true;", f"  n_grid = {n};", f"  dens_type = {dens_type};",
              "  lpt_buffer_fraction = 0.6;", "  lpt_interp_type = 1;", f"  output_lpt = {int(output_lpt)};", "};",
-             "cosmo_par: {"] + [f"  {kk} = {vv};" for kk, vv in COSMO.items()] + ["};"]
+             "cosmo_par: {"] + [f"  {kk} = {vv};" for kk, vv in cosmo.items()] + ["};"]
     for i, (nzs, bzs, model) in enumerate(cfg["srcs"]):
         if nzs[0] == "gauss":
             nbar = gaussian_nbar(zs, nzs[1], nzs[2], nzs[3])
@@ -152,6 +159,8 @@ def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_form
             b = np.full_like(zs, bzs[1])
         elif bzs[0] == "1+z":
             b = 1 + zs
+        elif bzs[0] == "lin":
+            b = bzs[1] + bzs[2] * zs
         else:
             b = 0.95 / np.array([_growth(z) for z in zs])
         write_two_col(os.path.join(outdir, f"bz{i + 1}.txt"), zs, b)

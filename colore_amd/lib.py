@@ -93,6 +93,18 @@ def load_amd():
     _sig(lib, "clr_fetch_cell_counts", ci, [_vp, ci, ci, ci, _i32])
     _sig(lib, "clr_declare_grids", ci, [_vp, ctypes.c_double, ci])
     _sig(lib, "clr_stage_times", ci, [_vp, _d, ci])
+    _sig(lib, "clr_mlp_create", ci, [ctypes.POINTER(_vp), ci, ctypes.c_uint32])
+    _sig(lib, "clr_mlp_destroy", None, [_vp])
+    _sig(lib, "clr_mlp_last_error", ctypes.c_char_p, [_vp])
+    _sig(lib, "clr_mlp_set_scaling", ci, [_vp, _f, _f, _f, _f])
+    _sig(lib, "clr_mlp_set_params", ci, [_vp, _f])
+    _sig(lib, "clr_mlp_get_params", ci, [_vp, _f])
+    _sig(lib, "clr_mlp_grad", ci, [_vp, _f, _f, ci, _f, _d])
+    _sig(lib, "clr_mlp_train", ci, [_vp, _f, _f, ci, ci, ctypes.c_float, ctypes.c_float, _d])
+    _sig(lib, "clr_mlp_predict", ci, [_vp, _f, ci, _f, ci])
+    _sig(lib, "clr_surrogate_inputs", ci, [_vp, ci, ci, _f, _i32, ctypes.POINTER(ci)])
+    _sig(lib, "clr_sample_sources_surrogate", ci, [_vp, _vp, ci, ctypes.c_uint32, ci, ci,
+                                                   ctypes.POINTER(ctypes.c_longlong)])
     return lib
 
 
@@ -127,6 +139,7 @@ def load_host():
     _sig(lib, "clr_write_catalog_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
     _sig(lib, "clr_write_catalog_ascii", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, _f, _f])
     _sig(lib, "clr_append_skewers_fits", ci, [ctypes.c_char_p, ci, ctypes.c_longlong, ci, _f, _f, _f, ctypes.c_float])
+    _sig(lib, "clr_limber_cl", cd, [_vp, ci, cd, cd, cd, cd, cd, ci])
     _sig(lib, "clr_write_lpt_gadget", ci, [ctypes.c_char_p, ctypes.c_longlong, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, _f])
     return lib
@@ -296,6 +309,22 @@ class Context:
                                                      ctypes.byref(n)))
         return n.value
 
+    def surrogate_inputs(self, ipop):
+        """(x[n, 4], shell[n]): the surrogate's inputs per normalisation shell."""
+        n = ctypes.c_int()
+        self._chk(self.lib.clr_surrogate_inputs(self.h, ipop, 0, None, None, ctypes.byref(n)))
+        x = np.zeros((max(n.value, 1), 4), dtype=np.float32)
+        sh = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._chk(self.lib.clr_surrogate_inputs(self.h, ipop, n.value, ptr(x, ctypes.c_float), ptr(sh, ctypes.c_int32),
+                                                ctypes.byref(n)))
+        return x[:n.value], sh[:n.value]
+
+    def sample_sources_surrogate(self, net, ipop, seed, nside_base=2, precision=1):
+        n = ctypes.c_longlong()
+        self._chk(self.lib.clr_sample_sources_surrogate(self.h, net.h, ipop, seed, nside_base, precision,
+                                                        ctypes.byref(n)))
+        return n.value
+
     def fetch_sources(self, ipop, n):
         pos = np.zeros((n, 4), dtype=np.float32)
         ipix = np.zeros(n, dtype=np.int32)
@@ -390,3 +419,72 @@ class Context:
         ms = np.zeros(16)
         n = self.lib.clr_stage_times(self.h, ptr(ms), 16)
         return dict(zip(STAGES, ms[:n]))
+
+
+CLR_MLP_FP32, CLR_MLP_BF16_MFMA = 0, 1
+
+
+class Mlp:
+    """The surrogate network (clr_mlp): 4 -> 64 -> 64 -> 2, trained and run on
+    the device."""
+
+    def __init__(self, seed=0, device=0, amd=None):
+        self.lib = amd or load_amd()
+        h = _vp()
+        rc = self.lib.clr_mlp_create(ctypes.byref(h), device, seed)
+        self.h = h
+        if rc:
+            msg = self.lib.clr_mlp_last_error(h).decode() if h else "clr_mlp_create failed"
+            self.close()
+            raise RuntimeError(msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clr_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise RuntimeError(f"clr_mlp error {rc}: {self.lib.clr_mlp_last_error(self.h).decode()}")
+
+    def set_scaling(self, x_mean, x_scale, y_mean, y_scale):
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (x_mean, x_scale, y_mean, y_scale)]
+        self.scaling = tuple(np.asarray(v, dtype=np.float64) for v in a)
+        self._chk(self.lib.clr_mlp_set_scaling(self.h, *[ptr(v, ctypes.c_float) for v in a]))
+
+    def set_params(self, params):
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        assert P.shape == (4610,)
+        self._chk(self.lib.clr_mlp_set_params(self.h, ptr(P, ctypes.c_float)))
+
+    def params(self):
+        P = np.zeros(4610, dtype=np.float32)
+        self._chk(self.lib.clr_mlp_get_params(self.h, ptr(P, ctypes.c_float)))
+        return P
+
+    def grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(4610, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._chk(self.lib.clr_mlp_grad(self.h, ptr(x, ctypes.c_float), ptr(y, ctypes.c_float), len(x),
+                                        ptr(g, ctypes.c_float), ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, x, y, steps, lr=3e-3, lr_final=3e-4):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        losses = np.zeros(max(steps, 1))
+        self._chk(self.lib.clr_mlp_train(self.h, ptr(x, ctypes.c_float), ptr(y, ctypes.c_float), len(x), steps,
+                                         lr, lr_final, ptr(losses)))
+        return losses[:steps]
+
+    def predict(self, x, precision=CLR_MLP_BF16_MFMA):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), 2), dtype=np.float32)
+        self._chk(self.lib.clr_mlp_predict(self.h, ptr(x, ctypes.c_float), len(x), ptr(y, ctypes.c_float),
+                                           precision))
+        return y

@@ -273,6 +273,61 @@ int clr_fetch_cell_counts(clr_ctx *ctx, int ipop, int iz0, int nz, int32_t *coun
  * stage too, so that later stages can run on caller-supplied grids. */
 int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
 
+/* ---- learned surrogate of the per-shell normalisation ------------------
+ * No counterpart in the reference: a 4 -> 64 -> 64 -> 2 perceptron (tanh
+ * hidden layers) that predicts, from x = (D(z), b(z), sigma_G(z), r_smooth)
+ * of a shell, y0 = ln norm (count/sum of compute_density_normalization,
+ * density.c:1272-1355) and y1 = ln <bias_model>, the expected count per cell
+ * of the un-normalised sampler in units of n dx^3.  FP32 weights; trained
+ * on the device by its own forward/backward kernels (mean squared error of
+ * the standardised outputs, full-batch Adam); inference in FP32 or with BF16
+ * MFMA (FP32 accumulate).  Parameters: W1[64][4], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2] (W[out][in]). */
+typedef struct clr_mlp clr_mlp;
+#define CLR_MLP_IN      4
+#define CLR_MLP_HID     64
+#define CLR_MLP_OUT     2
+#define CLR_MLP_NPARAM  4610
+#define CLR_MLP_FP32      0
+#define CLR_MLP_BF16_MFMA 1
+int  clr_mlp_create(clr_mlp **net, int device, uint32_t seed);   /* Glorot-uniform init */
+void clr_mlp_destroy(clr_mlp *net);
+const char *clr_mlp_last_error(const clr_mlp *net);
+/* x_n = (x - x_mean)/x_scale on input, y = y_mean + y_scale y_n on output */
+int clr_mlp_set_scaling(clr_mlp *net, const float *x_mean, const float *x_scale,
+                        const float *y_mean, const float *y_scale);
+int clr_mlp_set_params(clr_mlp *net, const float *params);   /* resets Adam */
+int clr_mlp_get_params(clr_mlp *net, float *params);
+/* Gradient of L = (1/2n) sum |y_n - target_n|^2 over the batch (x: n x 4,
+ * y: n x 2 raw targets) at the current parameters; loss optional. */
+int clr_mlp_grad(clr_mlp *net, const float *x, const float *y, int n, float *grad, double *loss);
+/* `steps` full-batch Adam steps, learning rate decaying geometrically from lr
+ * to lr_final; loss_out (NULL or steps values) gets L before each step. */
+int clr_mlp_train(clr_mlp *net, const float *x, const float *y, int n, int steps, float lr,
+                  float lr_final, double *loss_out);
+/* y (n x 2, raw units) from x (n x 4) with CLR_MLP_FP32 or CLR_MLP_BF16_MFMA. */
+int clr_mlp_predict(clr_mlp *net, const float *x, int n, float *y, int precision);
+
+/* Inputs of the surrogate for source population ipop, one row of
+ * CLR_MLP_IN floats per normalisation shell with cells (bins 1 .. nz-1 of
+ * density.c:1233-1244): at the shell's mean redshift z = zsum/count, D(z),
+ * b(z), sigma_G(z) = D(z) sigma (sigma^2 the measured variance of the
+ * Gaussian grid, clr_gaussian_fields) and r_smooth; shell[i] is the bin.
+ * Needs clr_gaussian_fields.  *n_out = number of rows (x, shell may be NULL
+ * to query it). */
+int clr_surrogate_inputs(clr_ctx *ctx, int ipop, int cap, float *x, int *shell, int *n_out);
+/* Surrogate-fused sampling: clr_sample_sources (same catalogues, streams
+ * and order) for population ipop (-1: all) right after clr_gaussian_fields,
+ * without clr_shell_norms.  The network predicts each shell's norm (y0 = ln
+ * norm, BF16 MFMA or FP32 inference); the norm tables are built from them as
+ * compute_density_normalization builds them from the shell sums
+ * (density.c:1272-1355); the count pass reads the Gaussian field and applies
+ * the lognormal transform per cell (the same FP32 value the shell pass of
+ * clr_shell_norms(lognormal_fused) stores), so given equal norm tables the
+ * counts equal the two-pass path's.  The grid keeps the Gaussian field. */
+int clr_sample_sources_surrogate(clr_ctx *ctx, clr_mlp *net, int ipop, uint32_t seed,
+                                 int nside_base, int precision, long long *nsrc_out);
+
 /* Device time in milliseconds of the last call of each stage (0 if never),
  * in this order: mode generation + z pass, y pass, x pass, density
  * transform, shell sums (whole call), sources (whole call), kappa, ISW, imap,

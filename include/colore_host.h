@@ -98,6 +98,15 @@ void clr_xi2pk(int n, const double *r, const double *xi, double *k, double *pk);
 /* write_predictions (predictions.c:26-172): <prefix>_gbias.txt and the
  * lognormal P(k) / xi(r) predictions of every tracer every pred_dz in z. */
 int  clr_write_predictions(const clr_run *r);
+/* compute_cl (cosmo.c:114-224): the Limber angular power spectrum at
+ * multipole l of two radial windows [chi_0_a, chi_f_a] and [chi_0_b, chi_f_b]
+ * for a source plane at chi_s, of the convergence (CLR_CL_KAPPA) or of the
+ * ISW temperature (CLR_CL_ISW), from the linear P(k) table (smoothed by
+ * exp(-k^2 r_smooth^2) when the potential is smoothed). */
+#define CLR_CL_KAPPA 0
+#define CLR_CL_ISW   2
+double clr_limber_cl(const clr_run *r, int l, double chi_0_a, double chi_f_a, double chi_0_b,
+                     double chi_f_b, double chi_s, int type);
 
 /* ---- writers ------------------------------------------------------------ */
 /* write_density_grid native format (io.c:565-595) from a padded grid */

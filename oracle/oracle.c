@@ -1764,3 +1764,78 @@ void orc_pix2vec_nest(long long nside, long long pix, double *v) {
   v[1] = st * sin(phi);
   v[2] = z;
 }
+
+/* ========================================================================
+ * The surrogate network (no reference counterpart): 4 -> 64 -> 64 -> 2,
+ * tanh hidden layers, FP64.  P = W1[64][4], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2]; x_n = (x - xm)/xs, y = ym + ys out.
+ * ====================================================================== */
+enum { MI = 4, MH = 64, MO = 2, oW1 = 0, oB1 = MH * MI, oW2 = oB1 + MH, oB2 = oW2 + MH * MH, oW3 = oB2 + MH,
+       oB3 = oW3 + MO * MH, MNP = oB3 + MO };
+
+static void mlp_fwd1(const double *P, const double *xn, double *h1, double *h2, double *out) {
+  for (int o = 0; o < MH; o++) {
+    double a = P[oB1 + o];
+    for (int i = 0; i < MI; i++) a += P[oW1 + o * MI + i] * xn[i];
+    h1[o] = tanh(a);
+  }
+  for (int o = 0; o < MH; o++) {
+    double a = P[oB2 + o];
+    for (int i = 0; i < MH; i++) a += P[oW2 + o * MH + i] * h1[i];
+    h2[o] = tanh(a);
+  }
+  for (int k = 0; k < MO; k++) {
+    double a = P[oB3 + k];
+    for (int i = 0; i < MH; i++) a += P[oW3 + k * MH + i] * h2[i];
+    out[k] = a;
+  }
+}
+
+void orc_mlp_forward(const double *P, const double *xm, const double *xs, const double *ym, const double *ys,
+                     const double *x, int n, double *y) {
+  for (int s = 0; s < n; s++) {
+    double xn[MI], h1[MH], h2[MH], out[MO];
+    for (int i = 0; i < MI; i++) xn[i] = (x[s * MI + i] - xm[i]) / xs[i];
+    mlp_fwd1(P, xn, h1, h2, out);
+    for (int k = 0; k < MO; k++) y[s * MO + k] = ym[k] + ys[k] * out[k];
+  }
+}
+
+/* L = (1/2n) sum_s sum_k (out_k - t_k)^2 with t = (y - ym)/ys; grad[MNP]. */
+double orc_mlp_loss_grad(const double *P, const double *xm, const double *xs, const double *ym, const double *ys,
+                         const double *x, const double *y, int n, double *grad) {
+  memset(grad, 0, MNP * sizeof(double));
+  double loss = 0;
+  for (int s = 0; s < n; s++) {
+    double xn[MI], h1[MH], h2[MH], out[MO], g3[MO], g2[MH], g1[MH];
+    for (int i = 0; i < MI; i++) xn[i] = (x[s * MI + i] - xm[i]) / xs[i];
+    mlp_fwd1(P, xn, h1, h2, out);
+    for (int k = 0; k < MO; k++) {
+      const double e = out[k] - (y[s * MO + k] - ym[k]) / ys[k];
+      loss += 0.5 * e * e / n;
+      g3[k] = e / n;
+      grad[oB3 + k] += g3[k];
+      for (int i = 0; i < MH; i++) grad[oW3 + k * MH + i] += g3[k] * h2[i];
+    }
+    for (int i = 0; i < MH; i++) {
+      double a = 0;
+      for (int k = 0; k < MO; k++) a += P[oW3 + k * MH + i] * g3[k];
+      g2[i] = a * (1 - h2[i] * h2[i]);
+    }
+    for (int o = 0; o < MH; o++) {
+      grad[oB2 + o] += g2[o];
+      for (int i = 0; i < MH; i++) grad[oW2 + o * MH + i] += g2[o] * h1[i];
+    }
+    for (int i = 0; i < MH; i++) {
+      double a = 0;
+      for (int o = 0; o < MH; o++) a += P[oW2 + o * MH + i] * g2[o];
+      g1[i] = a * (1 - h1[i] * h1[i]);
+    }
+    for (int o = 0; o < MH; o++) {
+      grad[oB1 + o] += g1[o];
+      for (int i = 0; i < MI; i++) grad[oW1 + o * MI + i] += g1[o] * xn[i];
+    }
+  }
+  return loss;
+}
+

@@ -160,6 +160,15 @@ void orc_imap(int N, double L, const double obs[3], const orc_bg *bg, const doub
               double fgrowth_0, double hubble_0, double omega_m,
               int nside, int nr, const double *r0, const double *rf, double *data, int *nadd);
 
+/* ---- the surrogate network (no reference counterpart) -------------------- */
+/* 4 -> 64 -> 64 -> 2 tanh perceptron in FP64: P = W1[64][4], b1, W2[64][64],
+ * b2, W3[2][64], b3 (4610 values); x_n = (x - xm)/xs, y = ym + ys out. */
+void orc_mlp_forward(const double *P, const double *xm, const double *xs, const double *ym, const double *ys,
+                     const double *x, int n, double *y);
+/* L = (1/2n) sum |out - (y - ym)/ys|^2 and its gradient (4610 values). */
+double orc_mlp_loss_grad(const double *P, const double *xm, const double *xs, const double *ym, const double *ys,
+                         const double *x, const double *y, int n, double *grad);
+
 /* ---- HEALPix -------------------------------------------------------------- */
 long long orc_ang2pix_ring(long long nside, double cth, double phi);
 long long orc_vec2pix_ring(long long nside, const double *v);

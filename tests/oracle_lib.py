@@ -91,6 +91,8 @@ def load(build=True):
     _sig(lib, "orc_imap", None, [ci, cd, _d, ctypes.POINTER(Bg), _d, _d, _d, _d, cd, _d, cd, cd, ci, cd, cd, cd,
                                  ci, ci, _d, _d, _d, _i])
     _sig(lib, "orc_ang2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, cd, cd])
+    _sig(lib, "orc_mlp_forward", None, [_d, _d, _d, _d, _d, _d, ci, _d])
+    _sig(lib, "orc_mlp_loss_grad", cd, [_d, _d, _d, _d, _d, _d, _d, ci, _d])
     _sig(lib, "orc_vec2pix_ring", ctypes.c_longlong, [ctypes.c_longlong, _d])
     _sig(lib, "orc_ring2nest", ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong])
     _sig(lib, "orc_nest2ring", ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong])
@@ -329,3 +331,27 @@ def ref_streams(seed, ipop, nthr, length, n_srcs=1):
         lib.orc_mt_fill((seed + t + nthr * ipop) & 0xFFFFFFFF, length, p(row, ctypes.c_uint32))
         out[t] = row
     return out
+
+
+def mlp_forward(params, scaling, x):
+    """The surrogate network in FP64: y (n x 2) from x (n x 4); scaling =
+    (x_mean, x_scale, y_mean, y_scale)."""
+    lib = load()
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    sc = [np.ascontiguousarray(a, dtype=np.float64) for a in scaling]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros((len(x), 2))
+    lib.orc_mlp_forward(p(P), p(sc[0]), p(sc[1]), p(sc[2]), p(sc[3]), p(x), len(x), p(y))
+    return y
+
+
+def mlp_loss_grad(params, scaling, x, y):
+    """(loss, gradient[4610]) of the surrogate's mean squared error in FP64."""
+    lib = load()
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    sc = [np.ascontiguousarray(a, dtype=np.float64) for a in scaling]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    g = np.zeros(4610)
+    loss = lib.orc_mlp_loss_grad(p(P), p(sc[0]), p(sc[1]), p(sc[2]), p(sc[3]), p(x), p(y), len(x), p(g))
+    return loss, g

@@ -1,0 +1,177 @@
+"""The learned surrogate of the per-shell normalisation on the device,
+through the C ABI: its training kernels against the FP64 oracle network, its
+BF16 MFMA inference against an emulation of the same arithmetic, and its
+training.
+
+Tolerances: gradients (FP32 device, FP64 oracle) within 1e-4 of each
+parameter block's largest gradient; BF16 MFMA outputs within 2e-4 (of the
+output scale) of a numpy emulation that rounds the same operands to BF16 and
+accumulates in FP32, and within 3e-2 of the FP32 forward.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BLOCKS = [(0, 256), (256, 320), (320, 4416), (4416, 4480), (4480, 4608), (4608, 4610)]
+
+
+def _case(seed=0, n=300):
+    rng = np.random.default_rng(seed)
+    P = rng.normal(0, 0.3, 4610).astype(np.float32)
+    sc = (np.array([0.6, 1.8, 0.9, 2.5]), np.array([0.2, 0.7, 0.4, 1.1]), np.array([-0.3, 0.3]),
+          np.array([0.5, 0.4]))
+    x = np.column_stack([rng.uniform(0.3, 1, n), rng.uniform(0.5, 3.5, n), rng.uniform(0.2, 1.8, n),
+                         rng.uniform(1, 4, n)]).astype(np.float32)
+    y = rng.normal(0, 0.5, (n, 2)).astype(np.float32)
+    return P, sc, x, y
+
+
+def _net(P, sc):
+    from colore_amd import lib
+    net = lib.Mlp(seed=1)
+    net.set_scaling(*sc)
+    net.set_params(P)
+    return net
+
+
+def test_mlp_gradient_vs_oracle():
+    """300 samples: four full 64-sample blocks and a partial one."""
+    from oracle_lib import mlp_loss_grad
+    P, sc, x, y = _case()
+    net = _net(P, sc)
+    sc32 = net.scaling   # the FP32-rounded scaling the device uses
+    loss, g = net.grad(x, y)
+    oloss, og = mlp_loss_grad(P.astype(np.float64), sc32, x.astype(np.float64), y.astype(np.float64))
+    assert abs(loss / oloss - 1) < 1e-5, (loss, oloss)
+    for a, b in BLOCKS:
+        scale = np.abs(og[a:b]).max()
+        assert scale > 0
+        assert np.abs(g[a:b] - og[a:b]).max() <= 1e-4 * scale, (a, b, np.abs(g[a:b] - og[a:b]).max(), scale)
+    net.close()
+
+
+def _bf16(a):
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def test_mlp_bf16_mfma_inference():
+    """1000 samples (not a multiple of the 16-row tiles): the MFMA path against
+    the same BF16 roundings emulated in numpy (inputs after scaling, weights,
+    hidden activations; FP32 sums and biases) and against FP32."""
+    from oracle_lib import mlp_forward
+    P, sc, x, _ = _case(3, 1000)
+    net = _net(P, sc)
+    xm, xs, ym, ys = [v.astype(np.float32) for v in net.scaling]
+    yb = net.predict(x, precision=1)
+    yf = net.predict(x, precision=0)
+    W1, b1 = P[:256].reshape(64, 4), P[256:320]
+    W2, b2 = P[320:4416].reshape(64, 64), P[4416:4480]
+    W3, b3 = P[4480:4608].reshape(2, 64), P[4608:4610]
+    xn = _bf16((x - xm) / xs)
+    h1 = _bf16(np.tanh(xn @ _bf16(W1).T + b1))
+    h2 = _bf16(np.tanh(h1 @ _bf16(W2).T + b2))
+    want = ym + ys * (h2 @ _bf16(W3).T + b3)
+    scale = np.abs(want).max()
+    assert np.abs(yb - want).max() <= 2e-4 * scale, np.abs(yb - want).max() / scale
+    yo = mlp_forward(P.astype(np.float64), net.scaling, x.astype(np.float64))
+    assert np.abs(yf - yo).max() <= 2e-5 * scale
+    assert np.abs(yb - yo).max() <= 3e-2 * scale
+    net.close()
+
+
+def test_mlp_training_fits_smooth_target():
+    """The lognormal moment law ln norm = -b(b-1) sigma_G^2 / 2 (and its
+    negative): full-batch Adam on the device lowers the loss by > 1e3 and
+    fits to 1e-2 of the target range."""
+    from colore_amd import lib
+    rng = np.random.default_rng(4)
+    n = 2000
+    x = np.column_stack([rng.uniform(0.3, 1, n), rng.uniform(0.5, 3.5, n), rng.uniform(0.2, 1.5, n),
+                         rng.uniform(1, 4, n)]).astype(np.float32)
+    t = -x[:, 1] * (x[:, 1] - 1) * x[:, 2] ** 2 / 2
+    y = np.column_stack([t, -t]).astype(np.float32)
+    net = lib.Mlp(seed=7)
+    net.set_scaling(x.mean(0), x.std(0), y.mean(0), y.std(0))
+    losses = net.train(x, y, 3000, lr=3e-3, lr_final=1e-4)
+    assert losses[-1] < 1e-3 * losses[0], (losses[0], losses[-1])
+    yp = net.predict(x, precision=0)
+    assert np.abs(yp - y).max() < 1e-2 * (y.max() - y.min()), np.abs(yp - y).max()
+    net.close()
+
+
+# ------------------------------------------------ surrogate-fused sampling ----
+@pytest.fixture(scope="module")
+def trained(tmp_path_factory):
+    """A network trained on shells the method generates (128^3 realisations of
+    the three geometries, randomised sigma8 and r_smooth)."""
+    from colore_amd import surrogate
+    wd = str(tmp_path_factory.mktemp("surrogate"))
+    print("surrogate training set", flush=True)
+    x, y = surrogate.training_set([("C1", 128, 4), ("C2", 128, 6), ("C3", 128, 4)], seed=900, workdir=wd)
+    assert len(x) > 1500
+    net, losses = surrogate.train(x, y, steps=4000)
+    assert losses[-1] < 0.05 * losses[0]
+    yield net, x, y, wd
+    net.close()
+
+
+def test_surrogate_fused_counts_equal_two_pass(trained, workdir):
+    """The fused single pass (norms from the network, lognormal applied per
+    cell in the count pass) gives exactly the catalogues of the two-pass path
+    (shell pass with the fused lognormal, then sources) run with the same
+    norm tables: same density floats, same lambda, same draws."""
+    from conftest import make_run
+    net = trained[0]
+    N = 256
+    path, run = make_run("C2", N, workdir, with_maps=False)
+    from colore_amd import lib
+    ctx = lib.Context(N, run.scalar("l_box"))
+    ctx.upload_tables(run.tables())
+    npop = int(run.scalar("n_srcs"))
+    for i in range(npop):
+        amp, bz, model = run.tracer(0, i)
+        ctx.set_tracer(0, i, model, amp, bz)
+    seed = int(run.scalar("seed"))
+    s2 = ctx.gaussian_fields(seed, run.scalar("r_smooth"), True)
+    nf = ctx.sample_sources_surrogate(net, -1, seed, 2)
+    tabs = [ctx.norm_table(0, i) for i in range(npop)]
+    cats = []
+    for i in range(npop):
+        c = ctx.fetch_cell_counts(i, 0, N)
+        cats.append(c)
+    # two-pass with the surrogate's tables
+    ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=npop)
+    for i, (t, n0, nff) in enumerate(tabs):
+        ctx.set_norm_table(0, i, t, n0, nff)
+    n2 = ctx.sample_sources(-1, seed, 2)
+    assert n2 == nf and nf > 0
+    for i in range(npop):
+        assert np.array_equal(ctx.fetch_cell_counts(i, 0, N), cats[i]), i
+    ctx.close()
+
+
+def test_surrogate_gain_over_method_without_network(trained, workdir):
+    """On a held-out realisation (512^3 precision geometry, seed 2002, its two
+    populations b = 1+z and b = 2) the network's per-shell norms, against the
+    exact two-pass reduction, beat using no normalisation by > 10x (cell-
+    weighted mean of |norm/norm_exact - 1|); the errors of the network and of
+    the lognormal-moment formula are printed."""
+    from conftest import make_run
+    from colore_amd import lib, surrogate
+    net = trained[0]
+    N = 256
+    path, run = make_run("C2", N, workdir, with_maps=False)
+    ctx = lib.Context(N, run.scalar("l_box"))
+    ctx.upload_tables(run.tables())
+    npop = int(run.scalar("n_srcs"))
+    for i in range(npop):
+        amp, bz, model = run.tracer(0, i)
+        ctx.set_tracer(0, i, model, amp, bz)
+    err, s2 = surrogate.shell_errors(ctx, run, net, int(run.scalar("seed")), npop)
+    summ = surrogate.summarize(err)
+    print(summ)
+    assert summ["network"]["cell_weighted_mean"] < 0.1 * summ["none"]["cell_weighted_mean"], summ
+    ctx.close()

@@ -476,3 +476,54 @@ def test_lpt_gadget_writer(tmp_path, host):
     o += 8 + 12 * n
     assert struct.unpack("<i", raw[o:o + 4])[0] == 8 * n
     np.testing.assert_array_equal(np.frombuffer(raw[o + 4:o + 4 + 8 * n], dtype="<u8"), np.arange(n))
+
+
+@pytest.mark.parametrize("typ", [0, 2])
+def test_limber_cl_against_quadrature(tmp_path, typ):
+    """compute_cl (cosmo.c:114-224) restated: against scipy's quad of the
+    same Limber integral written from the windows' definitions on the run's
+    tables (np.interp, the oracle's P(k) interpolation), for the kappa and
+    ISW windows, a source plane inside the box, several l and window pairs."""
+    import ctypes
+    from scipy import integrate
+    from colore_amd import inputs
+    from oracle_lib import Oracle
+    p = inputs.write_config("C2", str(tmp_path), n_grid=64, with_maps=False)
+    run = lib.Run(p)
+    ta = run.table_arrays()
+    orc = Oracle(ta)
+    rs = run.scalar("r_smooth")
+    chis = run.host.clr_run_r_of_z(run.h, 1.0)
+    rmax = run.scalar("r_max")
+    pre = ta["prefac_lensing"]
+
+    def window(l, k, c0, cf):
+        chi = (l + 0.5) / k
+        if chi <= 0 or chi < c0 or chi > cf:
+            return 0.0
+        if typ == 0:
+            gf = np.interp(chi, ta["r"], ta["d1"])
+            aa = 1 / (1 + np.interp(chi, ta["r"], ta["z"]))
+            return pre * l * (l + 1.0) * gf * (chis - chi) / (k * k * chis * chi * aa)
+        return -2 * pre * np.interp(chi, ta["r"], ta["pd"]) / (k * k)
+
+    for l in (2, 10, 60, 300):
+        for (a0, af, b0, bf) in ((0, chis, 0, chis), (0, 0.5 * chis, 0.3 * chis, chis), (0, rmax, 0, rmax)):
+            got = run.host.clr_limber_cl(run.h, l, a0, af, b0, bf, chis, typ)
+
+            def f(lk):
+                k = 10 ** lk
+                pk = orc.lib.orc_pk_linear0(ctypes.byref(orc.pk), lk) * math.exp(-k * k * rs * rs)
+                return k * window(l, k, a0, af) * window(l, k, b0, bf) * pk
+
+            cmin, cmax = min(a0, af, b0, bf), max(a0, af, b0, bf)
+            lkmax = 2.0 if cmin <= 0 else math.log10((l + 0.5) / cmin)
+            lkmin = math.log10((l + 0.5) / cmax)
+            pts = [math.log10((l + 0.5) / c) for c in (a0, af, b0, bf) if c > 0]
+            pts += [lkmin + 0.5, lkmin + 1, lkmin + 2]
+            val = integrate.quad(f, lkmin, lkmax, points=sorted(q for q in pts if lkmin < q < lkmax), limit=4000,
+                                 epsrel=1e-12)[0]
+            want = math.log(10) * val / (l + 0.5)
+            assert abs(got - want) <= 2e-5 * abs(want) + 1e-30, (l, a0, af, b0, bf, got, want)
+            if typ == 0 and (a0, af) == (0, chis):
+                assert got > 0

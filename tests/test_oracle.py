@@ -1032,3 +1032,47 @@ def test_sources_ref_stream_walk():
         u = st[t][3 * k: 3 * k + 3].astype(np.float64) / 2 ** 32
         c0 = np.array([ix, iy, iz]) * dx - Lb / 2
         np.testing.assert_array_equal(pos[i, :3], (c0 + dx * (u - 0.5)).astype(np.float32))
+
+
+# ------------------------------------------------------ surrogate network ----
+def _mlp_case(seed=0, n=37):
+    rng = np.random.default_rng(seed)
+    P = rng.normal(0, 0.3, 4610)
+    sc = (np.array([0.6, 1.8, 0.9, 2.5]), np.array([0.2, 0.7, 0.4, 1.1]), np.array([-0.3, 0.3]),
+          np.array([0.5, 0.4]))
+    x = np.column_stack([rng.uniform(0.3, 1, n), rng.uniform(0.5, 3.5, n), rng.uniform(0.2, 1.8, n),
+                         rng.uniform(1, 4, n)])
+    y = rng.normal(0, 0.5, (n, 2))
+    return P, sc, x, y
+
+
+def test_mlp_forward_matches_numpy():
+    """The oracle network against numpy's matrix products (W[out][in])."""
+    from oracle_lib import mlp_forward
+    P, sc, x, _ = _mlp_case()
+    W1, b1 = P[:256].reshape(64, 4), P[256:320]
+    W2, b2 = P[320:4416].reshape(64, 64), P[4416:4480]
+    W3, b3 = P[4480:4608].reshape(2, 64), P[4608:4610]
+    xn = (x - sc[0]) / sc[1]
+    h = np.tanh(np.tanh(xn @ W1.T + b1) @ W2.T + b2)
+    want = sc[2] + sc[3] * (h @ W3.T + b3)
+    np.testing.assert_allclose(mlp_forward(P, sc, x), want, rtol=1e-13, atol=1e-13)
+
+
+def test_mlp_gradient_finite_differences():
+    """Analytic backward pass against central differences of the loss for
+    every parameter block (a transposed W2 or a dropped tanh' fails)."""
+    from oracle_lib import mlp_loss_grad
+    P, sc, x, y = _mlp_case(1)
+    loss, g = mlp_loss_grad(P, sc, x, y)
+    rng = np.random.default_rng(2)
+    idx = np.concatenate([rng.choice(np.arange(a, b), size=min(12, b - a), replace=False)
+                          for a, b in [(0, 256), (256, 320), (320, 4416), (4416, 4480), (4480, 4608), (4608, 4610)]])
+    h = 1e-6
+    for i in idx:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += h
+        Pm[i] -= h
+        fd = (mlp_loss_grad(Pp, sc, x, y)[0] - mlp_loss_grad(Pm, sc, x, y)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 * max(1.0, abs(g[i])), (i, fd, g[i])
+    assert loss > 0
