@@ -63,10 +63,11 @@ int clr_ring2nest_table(clr_ctx *c, int nside) {
 static void free_all(clr_ctx *c) {
   auto f = [](void *p) { if (p) hipFree(p); };
   f(c->dens); f(c->npot); f(c->bg); f(c->pk); f(c->tw); f(c->amp); f(c->red); f(c->red_out);
-  f(c->rowcnt); f(c->ref_cnt); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d2); f(c->d1f); f(c->r2n);
+  f(c->rowcnt); f(c->ref_cnt); f(c->los_buf); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d2); f(c->d1f); f(c->r2n);
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < CLR_NPOP_MAX; i++) {
       f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].ptab);
+      f(c->tr[k][i].edge);
     }
   for (int i = 0; i < CLR_NPOP_MAX; i++) { f(c->cat[i].pos); f(c->cat[i].ipix); }
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -169,6 +170,7 @@ int clr_upload_tables(clr_ctx *c, const clr_tables *t) {
   c->h_r.assign(t->r_arr_r2z, t->r_arr_r2z + na);
   c->h_z.assign(t->z_arr_r2z, t->z_arr_r2z + na);
   c->h_d1.assign(t->growth_d_arr, t->growth_d_arr + na);
+  c->h_pd.assign(t->growth_pd_arr, t->growth_pd_arr + na);
   c->geo_nz = 0;   // shell geometry depends on the tables
   for (int i = 0; i < 6; i++) {
     c->h_bg_first[i] = src[i][0];
@@ -305,6 +307,13 @@ int clr_fetch_sources(clr_ctx *c, int ipop, float *pos4, int32_t *ipix, long lon
   CLR_HIP(c, hipSetDevice(c->device));
   if (pos4) CLR_HIP(c, hipMemcpy(pos4, C.pos, (size_t)n * 4 * sizeof(float), hipMemcpyDeviceToHost));
   if (ipix) CLR_HIP(c, hipMemcpy(ipix, C.ipix, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return CLR_OK;
+}
+
+int clr_catalogue_size(clr_ctx *c, int ipop, long long *n) {
+  if (!c || !n) return CLR_ERR_ARG;
+  if (ipop < 0 || ipop >= CLR_NPOP_MAX) return clr_fail(c, CLR_ERR_ARG, "bad population %d", ipop);
+  *n = c->cat[ipop].n;
   return CLR_OK;
 }
 

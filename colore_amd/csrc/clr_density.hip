@@ -58,6 +58,7 @@ struct ShellArgs {
   int N, ngx, nz, ntr, fused, na;
   double dx, ox, oy, oz, idr, r_last, z_last, sigma2, idz;
   const double *r_arr, *zt;
+  const double *sigma2_dev;   // when set, sigma^2 is read here instead of `sigma2`
   const float4 *tab;   // per bin: (value, slope) of D, z, b_0 .. b_{NT-1} (k_pack_shell_table)
   float d1_last_f, z_last_f, idr_f, idz_f;
   float bz0[kMaxTr];
@@ -206,6 +207,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
   for (int i = threadIdx.x; i < kWaves * nz * NQ; i += blockDim.x) acc[i] = 0;
   __syncthreads();
   const float ox = (float)a.ox, oy = (float)a.oy, oz = (float)a.oz, dxf = (float)a.dx;
+  const float s2f = (float)(a.sigma2_dev ? *a.sigma2_dev : a.sigma2);
   const int ntx = (N + kTX - 1) / kTX, nty = N / (kTY * kSub), ntz = N / kTZ;
   const long ntiles = (long)ntx * nty * ntz;
   const long nw = (long)gridDim.x * kWaves;
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
           // the same float as lognormal_f32 of the fused count pass
           const TabPos tp{0, bn.tq, e0, e1};
           const float D = growth_at(tp, bn.C[0].x, bn.C[0].y, a.d1_last_f);
-          dl = lognormal_f32(dl, D, (float)a.sigma2);
+          dl = lognormal_f32(dl, D, s2f);
           tb[roff(rr)] = dl;
         }
         // shell index: FP32 z (error < 5e-5 shell) unless within 5e-4 shell of
@@ -385,6 +387,7 @@ __global__ __launch_bounds__(256) void k_shell_final(const double *__restrict__ 
 constexpr int kNormMaxShells = 1024;
 struct NormOuts {
   double *p[3 * CLR_NPOP_MAX];
+  double *e[3 * CLR_NPOP_MAX];   // optional per-tracer copies of (norm0, normf)
 };
 
 // Every tracer's norm table in one launch: blockIdx.y is the tracer, the x
@@ -414,6 +417,10 @@ __global__ __launch_bounds__(256) void k_norm_table(const double *__restrict__ s
     if (blockIdx.x == 0) {
       edges[2 * t] = nrm[0];
       edges[2 * t + 1] = nrm[nz - 1];
+      if (outs.e[t]) {
+        outs.e[t][0] = nrm[0];
+        outs.e[t][1] = nrm[nz - 1];
+      }
     }
   }
   __syncthreads();
@@ -567,7 +574,10 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   CLR_HIP(c, hipGetLastError());
   if (ntr > 0) {
     NormOuts outs{};
-    for (int t = 0; t < ntr; t++) outs.p[t] = c->tr[tk[t]][tp[t]].norm;
+    for (int t = 0; t < ntr; t++) {
+      outs.p[t] = c->tr[tk[t]][tp[t]].norm;
+      outs.e[t] = c->tr[tk[t]][tp[t]].edge;
+    }
     hipLaunchKernelGGL(k_norm_table, dim3((unsigned)((na + 255) / 256), (unsigned)ntr), dim3(256), 0, c->stream,
                        fin, nz, nq, zmax, r, zt, c->glob_idr, c->r_last, z_last, na, outs, work + 2 * nz);
     CLR_HIP(c, hipGetLastError());
@@ -695,6 +705,7 @@ int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const in
     TracerDev &T = c->tr[kinds[t]][ipops[t]];
     if (!T.set || !T.norm) return clr_fail(c, CLR_ERR_STATE, "tracer %d/%d has no tables", kinds[t], ipops[t]);
     outs.p[t] = T.norm;
+    outs.e[t] = T.edge;
   }
   hipLaunchKernelGGL(k_norm_table, dim3((unsigned)((na + 255) / 256), (unsigned)ntr), dim3(256), 0, c->stream,
                      dfin, nz, nq, c->geo_zmax, c->bg, c->bg + na, c->glob_idr, c->r_last, c->h_bg_last[1], na,
@@ -709,6 +720,105 @@ int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const in
     T.normf = e[2 * t + 1];
     T.norm_set = 1;
   }
+  return CLR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The shell-sum stage for graph capture: buffers first (clr_prepare_norms),
+// then kernel launches only (clr_enqueue_norms): the packed table, the shell
+// pass with the fused lognormal (sigma^2 read from the device), the final
+// sums and every tracer's norm table, its edge values left in each tracer's
+// device copy.  The same kernels and order as clr_launch_norms.
+// ---------------------------------------------------------------------------
+static int norms_tracers(clr_ctx *c, int *tk, int *tp) {
+  int ntr = 0;
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < c->n_tr[k]; i++) {
+      tk[ntr] = k;
+      tp[ntr] = i;
+      ntr++;
+    }
+  return ntr;
+}
+
+int clr_prepare_norms(clr_ctx *c) {
+  int tk[3 * CLR_NPOP_MAX], tp[3 * CLR_NPOP_MAX];
+  const int ntr = norms_tracers(c, tk, tp);
+  if (ntr < 1 || ntr > kMaxTr) return clr_fail(c, CLR_ERR_ARG, "%d tracers (1 .. %d)", ntr, kMaxTr);
+  const int nz = (int)(c->zmax_half_box / 0.05) + 2;
+  const size_t nvals = (size_t)nz * (2 + ntr);
+  int rc = clr_ensure(c, (void **)&c->red, &c->red_n,
+                      ((size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr) * sizeof(double));
+  if (rc) return rc;
+  rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)c->na * shell_tab_w(ntr) * sizeof(float4));
+  if (rc) return rc;
+  for (int t = 0; t < ntr; t++) {
+    TracerDev &T = c->tr[tk[t]][tp[t]];
+    if (!T.edge) CLR_HIP(c, hipMalloc((void **)&T.edge, 2 * sizeof(double)));
+  }
+  return CLR_OK;
+}
+
+int clr_enqueue_norms(clr_ctx *c, hipStream_t st, const double *sigma2_dev) {
+  int tk[3 * CLR_NPOP_MAX], tp[3 * CLR_NPOP_MAX];
+  const int ntr = norms_tracers(c, tk, tp);
+  const int N = c->N, na = c->na;
+  const double zmax = c->zmax_half_box;
+  const int nz = (int)(zmax / 0.05) + 2;
+  const int nq = 2 + ntr;
+  const size_t nvals = (size_t)nz * nq;
+  if (ntr < 1 || ntr > kMaxTr || c->red_n < ((size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr) * sizeof(double))
+    return clr_fail(c, CLR_ERR_STATE, "normalisation buffers not prepared");
+  ShellArgs a{};
+  ShellTabSrc tsrc{};
+  bool all0 = true;
+  NormOuts outs{};
+  for (int t = 0; t < ntr; t++) {
+    TracerDev &T = c->tr[tk[t]][tp[t]];
+    if (!T.set || !T.bzf || !T.norm || !T.edge) return clr_fail(c, CLR_ERR_STATE, "tracer tables missing");
+    tsrc.bzf[t] = T.bzf;
+    a.bz0[t] = (float)T.bz0;
+    a.model[t] = T.model;
+    all0 = all0 && T.model == CLR_BIAS_EXPONENTIAL;
+    outs.p[t] = T.norm;
+    outs.e[t] = T.edge;
+  }
+  a.g = c->dens;
+  a.N = N;
+  a.ngx = c->ngx;
+  a.nz = nz;
+  a.ntr = ntr;
+  a.fused = 1;
+  a.na = na;
+  a.dx = c->dx;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.z_last = c->h_bg_last[1];
+  a.sigma2 = 0;
+  a.sigma2_dev = sigma2_dev;
+  a.idz = (nz - 2) / zmax;
+  a.r_arr = c->bg;
+  a.zt = c->bg + na;
+  tsrc.d1f = c->d1f;
+  tsrc.zt = c->bg + na;
+  a.z_last_f = (float)c->h_bg_last[1];
+  a.idz_f = (float)a.idz;
+  a.d1_last_f = (float)c->h_bg_last[2];
+  a.idr_f = (float)c->glob_idr;
+  a.tab = (const float4 *)c->misc;
+  static size_t (*const kLds[kMaxTr + 1])(int) = {shell_lds_bytes<0>, shell_lds_bytes<1>, shell_lds_bytes<2>,
+                                                   shell_lds_bytes<3>, shell_lds_bytes<4>, shell_lds_bytes<5>,
+                                                   shell_lds_bytes<6>, shell_lds_bytes<7>, shell_lds_bytes<8>};
+  double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals, *work = fin + nvals;
+  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, st, tsrc, ntr, na, (float4 *)c->misc);
+  kShellFns[all0 ? 1 : 0][ntr](dim3(kShellBlocks), dim3(kShellThreads), kLds[ntr](nz), st, a, partial);
+  hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, st, partial, kShellBlocks, (int)nvals, fin);
+  hipLaunchKernelGGL(k_norm_table, dim3((unsigned)((na + 255) / 256), (unsigned)ntr), dim3(256), 0, st, fin, nz, nq,
+                     zmax, c->bg, c->bg + na, c->glob_idr, c->r_last, c->h_bg_last[1], na, outs, work + 2 * nz);
+  CLR_HIP(c, hipGetLastError());
   return CLR_OK;
 }
 

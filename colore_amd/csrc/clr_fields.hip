@@ -21,6 +21,9 @@
 #include "clr_internal.h"
 #include "clr_fft.h"
 #include <math.h>
+#include <algorithm>
+#include <mutex>
+#include <vector>
 
 using namespace clr;
 
@@ -293,6 +296,15 @@ __global__ __launch_bounds__(P *RPB) void k_c2r_x(float *__restrict__ g0, float 
   }
 }
 
+// sigma^2 = <d^2> - <d>^2 from the reduced (sum, sumsq), on the device (the
+// same FP64 expression clr_launch_fields evaluates on the host).
+__global__ void k_sigma2_finish(const double *__restrict__ s, double ntot, double *__restrict__ sigma2) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double mean = s[0] / ntot;
+  *sigma2 = s[1] / ntot - mean * mean;
+}
+
 // Fixed-order reduction of n (sum, sumsq) pairs: deterministic result.
 __global__ __launch_bounds__(1024) void k_reduce_pairs(const double *__restrict__ in, size_t n,
                                                         double *__restrict__ out) {
@@ -343,19 +355,45 @@ template <int N, int P, int RPB, int... Rs> struct XKernels {
   ColKernels<N, P, T, __VA_ARGS__>::gen, ColKernels<N, P, T, __VA_ARGS__>::coly
 #define XCFG(N, P, RPB, ...) XKernels<N, P, RPB, __VA_ARGS__>::c2r
 
-bool field_launch_for(int N, FieldLaunch *L) {
+// Tile shapes per N: columns per block T and threads per column P of the z
+// and y passes, rows per block and threads per row of the x pass, and the
+// radix lists.  Entry 0 is the default; clr_fft_autotune times the others on
+// the device and selects the fastest for the process.
+std::vector<FieldLaunch> field_candidates(int N) {
   switch (N) {
-    case 32:   *L = {COLCFG(32, 4, 16, 4, 8), XCFG(32, 4, 16, 4, 4), 16, 64, 16, 64}; return true;
-    case 64:   *L = {COLCFG(64, 8, 16, 8, 8), XCFG(64, 4, 16, 4, 8), 16, 128, 16, 64}; return true;
-    case 128:  *L = {COLCFG(128, 8, 16, 16, 8), XCFG(128, 8, 16, 8, 8), 16, 128, 16, 128}; return true;
-    case 192:  *L = {COLCFG(192, 16, 16, 4, 4, 12), XCFG(192, 8, 16, 4, 4, 6), 16, 256, 16, 128}; return true;
-    case 256:  *L = {COLCFG(256, 16, 16, 16, 16), XCFG(256, 8, 16, 16, 8), 16, 256, 16, 128}; return true;
-    case 384:  *L = {COLCFG(384, 16, 16, 8, 8, 6), XCFG(384, 16, 16, 4, 4, 12), 16, 256, 16, 256}; return true;
-    case 512:  *L = {COLCFG(512, 64, 16, 8, 8, 8), XCFG(512, 16, 16, 16, 16), 16, 1024, 16, 256}; return true;
-    case 1024: *L = {COLCFG(1024, 32, 8, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256}; return true;
-    case 2048: *L = {COLCFG(2048, 128, 8, 16, 16, 8), XCFG(2048, 64, 4, 16, 16, 4), 8, 1024, 4, 256}; return true;
-    default: return false;
+    case 32:   return {{COLCFG(32, 4, 16, 4, 8), XCFG(32, 4, 16, 4, 4), 16, 64, 16, 64}};
+    case 64:   return {{COLCFG(64, 8, 16, 8, 8), XCFG(64, 4, 16, 4, 8), 16, 128, 16, 64}};
+    case 128:  return {{COLCFG(128, 8, 16, 16, 8), XCFG(128, 8, 16, 8, 8), 16, 128, 16, 128},
+                       {COLCFG(128, 16, 16, 8, 4, 4), XCFG(128, 16, 8, 4, 4, 4), 16, 256, 8, 128},
+                       {COLCFG(128, 4, 16, 16, 8), XCFG(128, 4, 16, 16, 4), 16, 64, 16, 64}};
+    case 192:  return {{COLCFG(192, 16, 16, 4, 4, 12), XCFG(192, 8, 16, 4, 4, 6), 16, 256, 16, 128},
+                       {COLCFG(192, 8, 16, 8, 8, 3), XCFG(192, 4, 16, 12, 8), 16, 128, 16, 64}};
+    case 256:  return {{COLCFG(256, 16, 16, 16, 16), XCFG(256, 8, 16, 16, 8), 16, 256, 16, 128},
+                       {COLCFG(256, 32, 8, 8, 8, 4), XCFG(256, 16, 16, 8, 4, 4), 8, 256, 16, 256},
+                       {COLCFG(256, 8, 16, 16, 16), XCFG(256, 8, 8, 16, 8), 16, 128, 8, 64}};
+    case 384:  return {{COLCFG(384, 16, 16, 8, 8, 6), XCFG(384, 16, 16, 4, 4, 12), 16, 256, 16, 256},
+                       {COLCFG(384, 32, 8, 4, 4, 6, 4), XCFG(384, 8, 16, 8, 8, 3), 8, 256, 16, 128}};
+    case 512:  return {{COLCFG(512, 64, 16, 8, 8, 8), XCFG(512, 16, 16, 16, 16), 16, 1024, 16, 256}};
+    case 1024: return {{COLCFG(1024, 32, 8, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256}};
+    case 2048: return {{COLCFG(2048, 128, 8, 16, 16, 8), XCFG(2048, 64, 4, 16, 16, 4), 8, 1024, 4, 256}};
+    default: return {};
   }
+}
+
+// Process-wide selection per N (0 until clr_fft_autotune picks another).
+std::mutex g_fft_mu;
+int g_fft_choice[4097];
+
+bool field_launch_for(int N, FieldLaunch *L) {
+  const std::vector<FieldLaunch> v = field_candidates(N);
+  if (v.empty()) return false;
+  int k = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    if (N >= 0 && N <= 4096) k = g_fft_choice[N];
+  }
+  *L = v[k < (int)v.size() ? k : 0];
+  return true;
 }
 
 }  // namespace
@@ -363,6 +401,116 @@ bool field_launch_for(int N, FieldLaunch *L) {
 bool clr_fft_size_supported(int N) {
   FieldLaunch L;
   return field_launch_for(N, &L);
+}
+
+// Buffers of the field stage for graph capture (clr_enqueue_fields).
+int clr_prepare_fields(clr_ctx *c) {
+  FieldLaunch L;
+  const int N = c->N;
+  if (!field_launch_for(N, &L)) return clr_fail(c, CLR_ERR_ARG, "n_grid=%d has no FFT configuration", N);
+  const int qmax = 3 * (N / 2) * (N / 2);
+  int rc = clr_ensure(c, (void **)&c->amp, &c->amp_n, (size_t)(qmax + 1) * sizeof(float2));
+  if (rc) return rc;
+  int rpb = L.xRPB;
+  for (const FieldLaunch &f : field_candidates(N)) rpb = std::min(rpb, f.xRPB);
+  const size_t nxb = ((size_t)N * N + rpb - 1) / rpb;
+  rc = clr_ensure(c, (void **)&c->red, &c->red_n, nxb * 2 * sizeof(double));
+  if (rc) return rc;
+  return clr_ensure(c, (void **)&c->red_out, &c->red_out_n, 64 * sizeof(double));
+}
+
+// FFT autotuner: every tile-shape candidate of this grid size runs the whole
+// field stage (mode generation, z, y and x passes, sigma^2) `reps` times on
+// the context's grids; the candidate with the smallest median time becomes
+// the process-wide choice for this N.  ms[i] (optional, capacity cap) gets
+// each candidate's median.
+int clr_autotune_fields(clr_ctx *c, int reps, int cap, float *ms, int *ncand, int *best) {
+  const int N = c->N;
+  const std::vector<FieldLaunch> v = field_candidates(N);
+  if (v.empty()) return clr_fail(c, CLR_ERR_ARG, "n_grid=%d has no FFT configuration", N);
+  if (reps < 1) reps = 3;
+  int rc = clr_prepare_fields(c);
+  if (rc) return rc;
+  rc = clr_phi_ready(c, 1);
+  if (rc) return rc;
+  double *s2 = nullptr;
+  CLR_HIP(c, hipMalloc((void **)&s2, sizeof(double)));
+  int prev;
+  {
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    prev = g_fft_choice[N];
+  }
+  int kbest = 0;
+  float tbest = 3.4e38f;
+  for (int k = 0; k < (int)v.size(); k++) {
+    {
+      std::lock_guard<std::mutex> lk(g_fft_mu);
+      g_fft_choice[N] = k;
+    }
+    std::vector<float> t;
+    for (int r = 0; r <= reps; r++) {
+      CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
+      rc = clr_enqueue_fields(c, c->stream, 12345u, 1.0, 1, s2);
+      if (rc) break;
+      CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
+      CLR_HIP(c, hipEventSynchronize(c->ev1));
+      float f = 0;
+      (void)hipEventElapsedTime(&f, c->ev0, c->ev1);
+      if (r > 0) t.push_back(f);   // the first run warms up
+    }
+    if (rc) break;
+    std::sort(t.begin(), t.end());
+    const float med = t[t.size() / 2];
+    if (ms && k < cap) ms[k] = med;
+    if (med < tbest) { tbest = med; kbest = k; }
+  }
+  (void)hipFree(s2);
+  {
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    g_fft_choice[N] = rc ? prev : kbest;
+  }
+  if (rc) return rc;
+  if (ncand) *ncand = (int)v.size();
+  if (best) *best = kbest;
+  c->have_fields = 0;   // the grids hold the tuner's fields
+  return CLR_OK;
+}
+
+// The field stage as kernel launches only (no allocation, no host wait), on
+// `s`: amplitudes, mode generation + z pass, both y passes in one launch,
+// both x passes in one launch (sigma^2 partials from delta), the reduction
+// and sigma^2 into sigma2_dev.  Same kernels, hence the same grids and
+// sigma^2, as clr_launch_fields.  clr_prepare_fields must have run.
+int clr_enqueue_fields(clr_ctx *c, hipStream_t s, uint32_t seed, double r_smooth, int smooth_potential,
+                       double *sigma2_dev) {
+  FieldLaunch L;
+  const int N = c->N, NH = c->NH;
+  if (!field_launch_for(N, &L)) return clr_fail(c, CLR_ERR_ARG, "n_grid=%d has no FFT configuration", N);
+  if (!c->tables_set) return clr_fail(c, CLR_ERR_STATE, "clr_upload_tables must be called first");
+  const int qmax = 3 * (N / 2) * (N / 2);
+  const size_t nrows = (size_t)N * N;
+  const size_t nxb = (nrows + L.xRPB - 1) / L.xRPB;
+  if (c->amp_n < (size_t)(qmax + 1) * sizeof(float2) || c->red_n < nxb * 2 * sizeof(double) || c->red_out_n < 16)
+    return clr_fail(c, CLR_ERR_STATE, "field buffers not prepared");
+  const double dk = 2 * M_PI / c->L;
+  const int do_smooth = r_smooth > 0;
+  const double r2 = do_smooth ? r_smooth * r_smooth : 0.0;
+  const int compact = !(do_smooth && !smooth_potential);
+  const float phi_ratio = (float)(-c->prefac_lensing / (dk * dk));
+  hipLaunchKernelGGL(k_amp, dim3((qmax + 256) / 256), dim3(256), 0, s, (void *)c->amp, compact, qmax, dk, c->pk,
+                     c->pk + c->numk, c->numk, c->logkmin, c->logkmax, c->idlogk, c->n_scal, r2, do_smooth,
+                     smooth_potential, c->prefac_lensing);
+  float2 *d = (float2 *)c->dens, *p = (float2 *)c->npot;
+  const int ntile = (NH + L.colT - 1) / L.colT;
+  const float norm = (float)pow(sqrt(2 * M_PI) / c->L, 3);
+  L.gen(dim3((unsigned)(ntile * N)), dim3(L.colThreads), s, d, p, c->amp, compact, phi_ratio, c->tw, seed, nullptr);
+  L.coly(dim3((unsigned)(ntile * N), 1, 2), dim3(L.colThreads), s, d, p, c->tw);
+  L.c2r(dim3((unsigned)nxb, 2), dim3(L.xThreads), s, c->dens, c->npot, norm, c->red, nrows, c->tw);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, c->red, nxb, c->red_out);
+  hipLaunchKernelGGL(k_sigma2_finish, dim3(1), dim3(64), 0, s, c->red_out, (double)N * (double)N * (double)N,
+                     sigma2_dev);
+  CLR_HIP(c, hipGetLastError());
+  return CLR_OK;
 }
 
 int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_potential, int rng_mode,

@@ -20,6 +20,7 @@ struct TracerDev {
   double norm0 = 1, normf = 1;
   int norm_set = 0;
   std::vector<double> h_bz; // host copy of bz (surrogate inputs)
+  double *edge = nullptr;   // device copy of (norm0, normf) (graph-captured stages)
 };
 
 struct CatalogDev {
@@ -114,11 +115,16 @@ struct clr_ctx {
 
   // host copies of r, z, D (surrogate inputs) and the shell geometry of the
   // normalisation shells (cells and sum of z per shell; geo_nz = 0: not yet)
-  std::vector<double> h_r, h_z, h_d1;
+  std::vector<double> h_r, h_z, h_d1, h_pd;
   int geo_nz = 0;
   double geo_zmax = 0;
   std::vector<double> geo_cnt, geo_zsum;
   double r_smooth = 0;      // smoothing length of the last clr_gaussian_fields
+  // graph-captured kappa: kernel arguments, radial weights + map buffer
+  std::string los_plan;
+  void *los_buf = nullptr;
+  size_t los_buf_n = 0;
+  int los_nside = 0, los_nplanes = 0;
 
   int have_fields = 0;
   int density_done = 0;
@@ -169,6 +175,23 @@ int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, c
 // (kind, ipop) as compute_density_normalization builds them from sums.
 int clr_launch_shell_geometry(clr_ctx *c);
 int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const int *ipops, const double *norm_s);
+
+// Graph-capturable stages (clr_ensemble.hip): buffers first (prepare, may
+// allocate), then kernel launches only on the given stream (enqueue: no
+// allocation, no memset, no host wait; sigma^2 and the norm edges stay on the
+// device).  Same kernels and results as the launchers above.
+int clr_prepare_fields(clr_ctx *c);
+int clr_autotune_fields(clr_ctx *c, int reps, int cap, float *ms, int *ncand, int *best);
+int clr_enqueue_fields(clr_ctx *c, hipStream_t s, uint32_t seed, double r_smooth, int smooth_potential,
+                       double *sigma2_dev);
+int clr_prepare_norms(clr_ctx *c);
+int clr_enqueue_norms(clr_ctx *c, hipStream_t s, const double *sigma2_dev);
+int clr_prepare_sources(clr_ctx *c, int nside_base, const long long *cap);
+int clr_enqueue_sources(clr_ctx *c, hipStream_t s, uint32_t seed, int nside_base, int *flag_dev);
+int clr_finish_sources(clr_ctx *c);
+int clr_prepare_kappa(clr_ctx *c, int nside, int nplanes, const double *r_src);
+int clr_enqueue_kappa(clr_ctx *c, hipStream_t s);
+int clr_fetch_kappa(clr_ctx *c, float *out);
 
 // Prediction of the surrogate network on device buffers (clr_mlp.hip).
 int clr_mlp_predict_device(clr_mlp *net, const float *dx, int n, float *dy, int precision, hipStream_t stream);

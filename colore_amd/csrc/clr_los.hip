@@ -13,6 +13,7 @@
 #include "clr_internal.h"
 #include <algorithm>
 #include <math.h>
+#include <string.h>
 
 using namespace clr;
 
@@ -104,14 +105,18 @@ __global__ __launch_bounds__(256) void k_los(LosArgs a, const double *__restrict
 
 }  // namespace
 
-int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out) {
-  if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
+// Host part of a kappa / ISW march: the kernel arguments (planes sorted as
+// kappa_beams_preproc sorts them, their radial ranges) and the radial
+// weights f1, f2 (FP64, from the host copies of the tables).
+static int los_setup(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, LosArgs &a,
+                     std::vector<double> &f12) {
   if (nside < 1 || nside > 8192 || (nside & (nside - 1))) return clr_fail(c, CLR_ERR_ARG, "bad nside %d", nside);
-  if (nplanes < 1 || nplanes > kMaxPlanes || !r_src || !out)
+  if (nplanes < 1 || nplanes > kMaxPlanes || !r_src)
     return clr_fail(c, CLR_ERR_ARG, "need 1..%d source planes", kMaxPlanes);
   const int N = c->N, na = c->na;
   const double r_max = c->r_max;
   if (!(r_max > 0)) return clr_fail(c, CLR_ERR_STATE, "tables carry no r_max");
+  if ((int)c->h_pd.size() != na) return clr_fail(c, CLR_ERR_STATE, "background tables missing");
   const int nr = N / 2;
   const double dr = r_max / nr, idr = 1. / dr;
   std::vector<double> rs(r_src, r_src + nplanes);
@@ -119,12 +124,8 @@ int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *
   for (int i = 0; i < nplanes; i++) rsf[i] = (float)rs[i];   // ind_sort sorts flouble values
   std::vector<int> order(nplanes);
   for (int i = 0; i < nplanes; i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rsf[a] < rsf[b]; });
-  LosArgs a{};
-  {
-    const int prc = clr_phi_ready(c, 0);   // phi's passes may still run on the second stream
-    if (prc) return prc;
-  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return rsf[x] < rsf[y]; });
+  a = LosArgs{};
   a.phi = c->npot;
   a.N = N;
   a.ngx = c->ngx;
@@ -143,35 +144,43 @@ int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *
   }
   a.irmin[0] = 0;
   for (int i = 1; i < nplanes; i++) a.irmin[i] = a.irmax[i - 1] + 1;
-
-  // radial kernels (FP64, host): need D, z and PD at rm -> from host copies
-  std::vector<double> rr(na), zz(na), dd(na), pd(na);
-  CLR_HIP(c, hipMemcpy(rr.data(), c->bg, na * sizeof(double), hipMemcpyDeviceToHost));
-  CLR_HIP(c, hipMemcpy(zz.data(), c->bg + na, na * sizeof(double), hipMemcpyDeviceToHost));
-  CLR_HIP(c, hipMemcpy(dd.data(), c->bg + 2 * (size_t)na, na * sizeof(double), hipMemcpyDeviceToHost));
-  CLR_HIP(c, hipMemcpy(pd.data(), c->bg + 4 * (size_t)na, na * sizeof(double), hipMemcpyDeviceToHost));
-  std::vector<double> f1(nr), f2(nr);
+  const double *rr = c->h_r.data(), *zz = c->h_z.data(), *dd = c->h_d1.data(), *pd = c->h_pd.data();
+  f12.assign(2 * (size_t)nr, 0.0);
   for (int i = 0; i < nr; i++) {
     const double rm = (i + 0.5) * dr;
     if (which == 0) {
-      const double pg = f_of_r(rr.data(), dd.data(), c->glob_idr, c->r_last, rm, 1.0, dd[na - 1]) *
-                        (1 + f_of_r(rr.data(), zz.data(), c->glob_idr, c->r_last, rm, 0.0, zz[na - 1]));
-      f1[i] = rm * pg * dr;
-      f2[i] = rm * rm * pg * dr;
+      const double pg = f_of_r(rr, dd, c->glob_idr, c->r_last, rm, 1.0, dd[na - 1]) *
+                        (1 + f_of_r(rr, zz, c->glob_idr, c->r_last, rm, 0.0, zz[na - 1]));
+      f12[i] = rm * pg * dr;
+      f12[nr + i] = rm * rm * pg * dr;
     } else {
-      const double pdg = f_of_r(rr.data(), pd.data(), c->glob_idr, c->r_last, rm, pd[0], pd[na - 1]);
-      f1[i] = 2 * pdg * dr;
-      f2[i] = 0;
+      const double pdg = f_of_r(rr, pd, c->glob_idr, c->r_last, rm, pd[0], pd[na - 1]);
+      f12[i] = 2 * pdg * dr;
+      f12[nr + i] = 0;
     }
   }
+  return CLR_OK;
+}
+
+int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *r_src, float *out) {
+  if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
+  if (!out) return clr_fail(c, CLR_ERR_ARG, "no output map");
+  LosArgs a{};
+  std::vector<double> f12;
+  int rc = los_setup(c, which, nside, nplanes, r_src, a, f12);
+  if (rc) return rc;
+  {
+    const int prc = clr_phi_ready(c, 0);   // phi's passes may still run on the second stream
+    if (prc) return prc;
+  }
+  const int nr = a.nr;
   const int64_t npix = 12LL * nside * nside;
   const size_t need = 2 * (size_t)nr * sizeof(double) + (size_t)npix * nplanes * sizeof(float);
-  int rc = clr_ensure(c, &c->misc, &c->misc_n, need);
+  rc = clr_ensure(c, &c->misc, &c->misc_n, need);
   if (rc) return rc;
   double *df = (double *)c->misc;
   float *dout = (float *)(df + 2 * nr);
-  CLR_HIP(c, hipMemcpyAsync(df, f1.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  CLR_HIP(c, hipMemcpyAsync(df + nr, f2.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  CLR_HIP(c, hipMemcpyAsync(df, f12.data(), 2 * nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
   hipLaunchKernelGGL(k_los, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, a, df, df + nr,
                      (int64_t)nside, npix, dout);
@@ -182,6 +191,53 @@ int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->stage_ms[which == 0 ? ST_KAPPA : ST_ISW] = ms;
+  return CLR_OK;
+}
+
+// Graph-captured kappa (clr_ensemble.hip): the plan and its buffers first,
+// then the march as one kernel launch into the context's map buffer.
+int clr_prepare_kappa(clr_ctx *c, int nside, int nplanes, const double *r_src) {
+  LosArgs a{};
+  std::vector<double> f12;
+  int rc = los_setup(c, 0, nside, nplanes, r_src, a, f12);
+  if (rc) return rc;
+  const int64_t npix = 12LL * nside * nside;
+  const size_t need = 2 * (size_t)a.nr * sizeof(double) + (size_t)npix * nplanes * sizeof(float);
+  if (c->los_buf_n < need) {
+    if (c->los_buf) CLR_HIP(c, hipFree(c->los_buf));
+    c->los_buf = nullptr;
+    c->los_buf_n = 0;
+    CLR_HIP(c, hipMalloc(&c->los_buf, need));
+    c->los_buf_n = need;
+  }
+  CLR_HIP(c, hipMemcpy(c->los_buf, f12.data(), f12.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->los_plan.assign((const char *)&a, (const char *)&a + sizeof(LosArgs));
+  c->los_nside = nside;
+  c->los_nplanes = nplanes;
+  return CLR_OK;
+}
+
+int clr_enqueue_kappa(clr_ctx *c, hipStream_t s) {
+  if (c->los_plan.size() != sizeof(LosArgs) || !c->los_buf) return clr_fail(c, CLR_ERR_STATE, "kappa not prepared");
+  LosArgs a;
+  memcpy(&a, c->los_plan.data(), sizeof(LosArgs));
+  a.phi = c->npot;
+  const int64_t npix = 12LL * c->los_nside * c->los_nside;
+  double *df = (double *)c->los_buf;
+  hipLaunchKernelGGL(k_los, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, a, df, df + a.nr,
+                     (int64_t)c->los_nside, npix, (float *)(df + 2 * a.nr));
+  CLR_HIP(c, hipGetLastError());
+  return CLR_OK;
+}
+
+// The last graph-captured kappa map (NEST), out[nplanes][npix].
+int clr_fetch_kappa(clr_ctx *c, float *out) {
+  if (c->los_plan.size() != sizeof(LosArgs) || !c->los_buf) return clr_fail(c, CLR_ERR_STATE, "kappa not prepared");
+  LosArgs a;
+  memcpy(&a, c->los_plan.data(), sizeof(LosArgs));
+  const int64_t npix = 12LL * c->los_nside * c->los_nside;
+  CLR_HIP(c, hipMemcpy(out, (double *)c->los_buf + 2 * a.nr, (size_t)npix * c->los_nplanes * sizeof(float),
+                       hipMemcpyDeviceToHost));
   return CLR_OK;
 }
 

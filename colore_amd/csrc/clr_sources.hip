@@ -26,6 +26,7 @@
 #include "clr_internal.h"
 #include "clr_poisson.h"
 #include <math.h>
+#include <limits.h>
 #include <algorithm>
 #include <vector>
 
@@ -46,6 +47,8 @@ struct PopTab {
   uint32_t *btot;            // one total per block
   const long long *boff;     // exclusive prefix of btot
   uint4 *desc;               // the catalogue's position buffer, holding descriptors
+  const double *edges;       // device (norm0, normf) when set (graph-captured sampling)
+  long long cap;             // catalogue capacity (descriptors beyond it are dropped)
 };
 
 struct SrcArgs {
@@ -479,10 +482,11 @@ __global__ __launch_bounds__(256) void k_exact(SrcArgs a) {
   const int N = a.N;
   const uint64_t rw = cell / (uint64_t)N;
   const int cx = (int)(cell - rw * N);
+  const double n0 = P.edges ? P.edges[0] : P.norm0, nf = P.edges ? P.edges[1] : P.normf;
   const int k = count_exact(cx, (int)(rw % N), (int)(rw / N),
                             cell_density(a, a.g[rw * (size_t)a.ngx + cx], cx, (int)(rw % N), (int)(rw / N)), cell,
                             e.w, a.r_arr,
-                            P.amp, P.bz, P.norm, a.idr, a.r_last, P.bz0, P.norm0, P.normf, a.dx, a.ox, a.oy, a.oz,
+                            P.amp, P.bz, P.norm, a.idr, a.r_last, P.bz0, n0, nf, a.dx, a.ox, a.oy, a.oz,
                             a.cell_vol, a.half_box_20, P.model, P.ipop, a.seed);
   if (k > 0) atomicAdd(&P.btot[cell / kBlockCells], (uint32_t)k);
   P.cnt8[cell] = (uint8_t)(k < kBytePending ? k : kByteEscape);
@@ -613,8 +617,9 @@ __global__ __launch_bounds__(kLanes) void k_fill(SrcArgs a) {
 #pragma unroll
       for (int c = 0; c < kCountLaneCells; c++) {
         const uint64_t cell = cell0 + c;
+        const long long lim = b1[q] < a.pop[g0 + q].cap ? b1[q] : a.pop[g0 + q].cap;
         for (int is = 0; is < k[c]; is++, pid++)
-          if (pid < b1[q]) desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
+          if (pid < lim) desc[pid] = make_uint4((uint32_t)cell, (uint32_t)(cell >> 32), (uint32_t)is, 0u);
       }
     }
   }
@@ -631,6 +636,7 @@ struct WriteArgs {
   float4 *pos;
   int32_t *ipix;
   long long n;
+  const long long *n_dev;   // when set: the source count on the device (n is the capacity)
 };
 
 __device__ inline double get_rvel(const WriteArgs &a, int ix, int iy, int iz, double x0, double y0, double z0,
@@ -688,6 +694,7 @@ __global__ __launch_bounds__(256) void k_write(WriteArgs a) {
 #pragma clang fp contract(off)
   const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= a.n) return;
+  if (a.n_dev && pid >= *a.n_dev) return;
   const uint4 ds = ((const uint4 *)a.pos)[pid];
   const int N = a.N;
   uint64_t cell = (uint64_t)ds.x | ((uint64_t)ds.y << 32);
@@ -715,6 +722,30 @@ __global__ __launch_bounds__(256) void k_write(WriteArgs a) {
   }
   a.pos[pid] = make_float4(pp[0], pp[1], pp[2], (float)dz_rsd);
   a.ipix[pid] = a.r2n[vec2pix_ring32(a.nside_base, (double)pp[0], (double)pp[1], (double)pp[2])];
+}
+
+// Zero n 32-bit words (a kernel rather than a memset node: graph-safe).
+__global__ void k_zero_words(uint32_t *__restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// Status of a graph-captured sampling pass: 1 saturated counts (escape list
+// not sorted in graph mode), 2 FP64 queue overflow, 4 a catalogue beyond
+// its capacity.
+struct GraphTotals {
+  const long long *tot[CLR_NPOP_MAX];
+  long long cap[CLR_NPOP_MAX];
+};
+__global__ void k_graph_flags(const unsigned *__restrict__ esc_n, const unsigned *__restrict__ exq_n,
+                              unsigned exq_cap, int npop, GraphTotals g, int *__restrict__ flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int f = 0;
+  if (*esc_n > 0) f |= 1;
+  if (*exq_n > exq_cap) f |= 2;
+  for (int i = 0; i < npop; i++)
+    if (*g.tot[i] > g.cap[i]) f |= 4;
+  *flag |= f;
 }
 
 // Per-cell counts (the reference's nsources array, srcs.c:125, without the
@@ -1034,6 +1065,8 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     P.normf = T.normf;
     P.model = T.model;
     P.ipop = i;
+    P.edges = nullptr;
+    P.cap = LLONG_MAX;
     if (!P.amp || !P.bz || !P.norm || !P.tab) return clr_fail(c, CLR_ERR_STATE, "population %d tables missing", i);
   }
   if (!c->r2n) return clr_fail(c, CLR_ERR_STATE, "no RING->NEST table");
@@ -1387,3 +1420,190 @@ int clr_launch_sources_stream(clr_ctx *c, int ipop, int nthr, const int *iz_begi
   if (nsrc) *nsrc = total;
   return CLR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// The sampling stage for graph capture: buffers first (clr_prepare_sources:
+// RING->NEST table, packed tables, count scratch, lists, catalogues of the
+// given capacities), then kernel launches only (clr_enqueue_sources): every
+// population in one pass, the norm edges read from the device, counters
+// cleared by kernels, the source count left on the device (k_write reads it),
+// and a status word: saturated counts (whose escape list this path does not
+// sort), an overflowing FP64 queue or a catalogue past its capacity make the
+// result invalid (the caller then re-runs the realisation with
+// clr_sample_sources).  Same kernels, streams and order as clr_launch_sources.
+// ---------------------------------------------------------------------------
+static size_t src_per_pop(size_t ncells) {
+  const size_t nblocks = ncells / kBlockCells, nsb = (nblocks + kScanB - 1) / kScanB;
+  return (ncells + nblocks * 4 + (nblocks + 1) * 8 + (nsb + 1) * 8 + 64 + 255) / 256 * 256;
+}
+
+int clr_prepare_sources(clr_ctx *c, int nside_base, const long long *cap) {
+  const int np = c->n_tr[CLR_TRACER_SRCS], N = c->N, na = c->na;
+  const size_t ncells = (size_t)N * N * N;
+  if (np < 1) return clr_fail(c, CLR_ERR_STATE, "no source population");
+  if (N % kCountLaneCells || ncells % kCountCells)
+    return clr_fail(c, CLR_ERR_ARG, "grid of %d^3 cells is not a whole number of %d-cell blocks", N, kCountCells);
+  int rc = clr_ring2nest_table(c, nside_base);
+  if (rc) return rc;
+  for (int i = 0; i < np; i++) {
+    TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
+    if (!T.set) return clr_fail(c, CLR_ERR_STATE, "population %d has no tables", i);
+    if (!T.ptab) CLR_HIP(c, hipMalloc((void **)&T.ptab, pop_table_len(na) * sizeof(float4)));
+    if (!T.edge) CLR_HIP(c, hipMalloc((void **)&T.edge, 2 * sizeof(double)));
+    CatalogDev &C = c->cat[i];
+    if (C.cap < cap[i]) {
+      if (C.pos) (void)hipFree(C.pos);
+      if (C.ipix) (void)hipFree(C.ipix);
+      C.pos = nullptr;
+      C.ipix = nullptr;
+      C.cap = 0;
+      if (hipMalloc((void **)&C.pos, (size_t)cap[i] * 4 * sizeof(float)) != hipSuccess ||
+          hipMalloc((void **)&C.ipix, (size_t)cap[i] * sizeof(int32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        return clr_fail(c, CLR_ERR_OOM, "cannot allocate a catalogue of %lld sources", cap[i]);
+      }
+      C.cap = cap[i];
+    }
+  }
+  rc = clr_ensure(c, (void **)&c->rowcnt, &c->rows_n, (size_t)np * src_per_pop(ncells));
+  if (rc) return rc;
+  rc = clr_ensure(c, &c->esc, &c->esc_bytes, c->esc_bytes > 0 ? c->esc_bytes : (size_t)16 + 16 * 65536);
+  if (rc) return rc;
+  return clr_ensure(c, &c->misc, &c->misc_n, c->misc_n >= 16 + 16 * 65536 ? c->misc_n : (size_t)16 + 16 * 65536);
+}
+
+int clr_enqueue_sources(clr_ctx *c, hipStream_t st, uint32_t seed, int nside_base, int *flag_dev) {
+  const int np = c->n_tr[CLR_TRACER_SRCS], N = c->N, na = c->na;
+  const size_t ncells = (size_t)N * N * N, per_pop = src_per_pop(ncells);
+  const size_t nblocks = ncells / kBlockCells, nsb = (nblocks + kScanB - 1) / kScanB;
+  if (c->rows_n < (size_t)np * per_pop || !c->esc || c->misc_n < 16 + 16 * 4096 || c->r2n_nside != nside_base)
+    return clr_fail(c, CLR_ERR_STATE, "sampling buffers not prepared");
+  const double dx = c->L / N, cell_vol = dx * dx * dx;
+  SrcArgs a{};
+  a.g = c->dens;
+  a.r_arr = c->bg;
+  a.idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.dx = dx;
+  a.cell_vol = cell_vol;
+  a.half_box_20 = c->L / 2 + 20.;
+  a.ox = c->obs[0];
+  a.oy = c->obs[1];
+  a.oz = c->obs[2];
+  a.N = N;
+  a.ngx = c->ngx;
+  a.na = na;
+  a.npop = np;
+  a.seed = seed;
+  a.fused = 0;
+  a.esc_n = (unsigned int *)c->esc;
+  a.esc = (ulonglong2 *)((char *)c->esc + 16);
+  a.esc_cap = (unsigned int)((c->esc_bytes - 16) / 16);
+  a.esc_count = 0;
+  a.exq_n = (unsigned int *)c->misc;
+  a.exq = (uint4 *)((char *)c->misc + 16);
+  a.exq_cap = (unsigned int)((c->misc_n - 16) / 16);
+  std::vector<long long *> bsum(np);
+  GraphTotals gt{};
+  for (int i = 0; i < np; i++) {
+    TracerDev &T = c->tr[CLR_TRACER_SRCS][i];
+    CatalogDev &C = c->cat[i];
+    if (!T.ptab || !T.edge || !C.pos || !C.ipix) return clr_fail(c, CLR_ERR_STATE, "sampling buffers not prepared");
+    PopTab &P = a.pop[i];
+    uint8_t *base = (uint8_t *)c->rowcnt + (size_t)i * per_pop;
+    P.cnt8 = base;
+    P.btot = (uint32_t *)(base + ncells);
+    long long *off = (long long *)(((uintptr_t)(P.btot + nblocks) + 7) & ~(uintptr_t)7);
+    P.boff = off;
+    bsum[i] = off + nblocks + 1;
+    P.amp = T.amp;
+    P.bz = T.bz;
+    P.norm = T.norm;
+    P.tab = (const float4 *)T.ptab;
+    P.bz0 = T.bz0;
+    P.norm0 = 1;
+    P.normf = 1;
+    P.edges = T.edge;
+    P.model = T.model;
+    P.ipop = i;
+    P.cap = C.cap;
+    P.desc = (uint4 *)C.pos;
+    gt.tot[i] = off + nblocks;
+    gt.cap[i] = C.cap;
+    // packed tables from the current norm tables
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t *)((float4 *)T.ptab + na), 4);
+    hipLaunchKernelGGL(k_pop_table, dim3((na + 255) / 256), dim3(256), 0, st, T.amp, T.bz, T.norm, cell_vol, na,
+                       (float4 *)T.ptab);
+    hipLaunchKernelGGL(k_pop_coarse, dim3((coarse_count(na) + 255) / 256), dim3(256), 0, st, T.amp, T.bz, T.norm,
+                       cell_vol, na, (float4 *)T.ptab);
+  }
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t *)a.esc_n, 1);
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t *)a.exq_n, 1);
+  hipLaunchKernelGGL(k_count<false>, dim3((unsigned)(ncells / kCountCells)), dim3(kLanes),
+                     (size_t)np * kLanes * sizeof(uint4), st, a);
+  hipLaunchKernelGGL(k_exact, dim3((a.exq_cap + 255) / 256), dim3(256), 0, st, a);
+  ScanPtrs sp{};
+  for (int i = 0; i < np; i++) {
+    sp.in[i] = a.pop[i].btot;
+    sp.bsum[i] = bsum[i];
+    sp.out[i] = (long long *)a.pop[i].boff;
+  }
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb, (unsigned)np), dim3(kScanB), 0, st, sp, nblocks);
+  hipLaunchKernelGGL(k_scan_top, dim3(1, (unsigned)np), dim3(kScanB), 0, st, sp, nsb);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsb, (unsigned)np), dim3(kScanB), 0, st, sp, nblocks);
+  hipLaunchKernelGGL(k_graph_flags, dim3(1), dim3(64), 0, st, (const unsigned *)a.esc_n,
+                     (const unsigned *)a.exq_n, a.exq_cap, np, gt, flag_dev);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)(ncells / kCountCells)), dim3(kLanes), 0, st, a);
+  for (int i = 0; i < np; i++) {
+    CatalogDev &C = c->cat[i];
+    WriteArgs w{};
+    w.phi = c->npot;
+    w.r_arr = c->bg;
+    w.v1 = c->bg + 3 * (size_t)na;
+    w.idr = c->glob_idr;
+    w.r_last = c->r_last;
+    w.v1_0 = c->h_bg_first[3];
+    w.v1_last = c->h_bg_last[3];
+    w.dx = dx;
+    w.factor_vel = -c->fgrowth_0 / (1.5 * c->hubble_0 * c->omega_m);
+    w.idx = N / c->L;
+    w.ox = c->obs[0];
+    w.oy = c->obs[1];
+    w.oz = c->obs[2];
+    w.N = N;
+    w.ngx = c->ngx;
+    w.nside_base = nside_base;
+    w.ipop = i;
+    w.seed = seed;
+    w.r2n = c->r2n;
+    w.pos = (float4 *)C.pos;
+    w.ipix = C.ipix;
+    w.n = C.cap;
+    w.n_dev = gt.tot[i];
+    hipLaunchKernelGGL(k_write, dim3((unsigned)((C.cap + 255) / 256)), dim3(256), 0, st, w);
+  }
+  CLR_HIP(c, hipGetLastError());
+  return CLR_OK;
+}
+
+// After a graph-captured pass: the host-side state the normal path keeps
+// (source counts, count layout for clr_fetch_cell_counts).
+int clr_finish_sources(clr_ctx *c) {
+  const int np = c->n_tr[CLR_TRACER_SRCS], N = c->N;
+  const size_t ncells = (size_t)N * N * N, per_pop = src_per_pop(ncells), nblocks = ncells / kBlockCells;
+  for (int i = 0; i < np; i++) {
+    uint8_t *base = (uint8_t *)c->rowcnt + (size_t)i * per_pop;
+    const uint32_t *btot = (const uint32_t *)(base + ncells);
+    const long long *off = (const long long *)(((uintptr_t)(btot + nblocks) + 7) & ~(uintptr_t)7);
+    long long n = 0;
+    CLR_HIP(c, hipMemcpy(&n, off + nblocks, sizeof(long long), hipMemcpyDeviceToHost));
+    c->cat[i].n = n;
+  }
+  c->cnt_p0 = 0;
+  c->cnt_np = np;
+  c->cnt_per_pop = per_pop;
+  c->cnt_nesc = 0;
+  c->ref_ipop = -1;
+  return CLR_OK;
+}
+

@@ -106,10 +106,32 @@ CONFIGS = {
                      (("gauss", 1.3, 0.2, 1e-4), ("const", 2.2), "truncated")],
                kappa=dict(z_out=[0.5, 1.0, 1.5], nside=512), isw=dict(z_out=[0.5, 1.0, 1.5], nside=512),
                imap=dict(nside=512, nbands=64, t0=0.2)),
+    "C4": dict(n_grid=128, z_min=0.0, z_max=0.1, r_smooth=2.0, seed=4000,
+               srcs=[(("gauss", 0.5, 0.1, 5e-4), ("1+z",), "exponential")]),
     "C5": dict(n_grid=2048, z_min=0.05, z_max=2.0, r_smooth=1.0, seed=5005,
                srcs=[(("gauss", 0.5, 0.15, 5e-4), ("1+z",), "exponential"),
                      (("flat", 0.8, 1.1, 1e-4), ("const", 2.0), "exponential")]),
 }
+
+
+def c4_realisations(n=64, seed0=4000):
+    """The batched-ensemble workload: n realisations with seeds seed0.. and grid
+    sizes drawn uniformly from {128, 192, 256, 384} (NumPy generator seeded
+    with seed0); the box is L = 4 N Mpc/h (cell 4 Mpc/h), so z_max is the
+    redshift of L/2/(1+2/N) (cosmo.c:544 inverted); one population with
+    b = 1+z and a Gaussian n(z), mean drawn in [0.3, 0.8], sigma 0.1, peak
+    5e-4 (Mpc/h)^-3.  The kappa plane of z = 0.8 lies beyond every such box
+    (z_max 0.086 .. 0.27; the reference stops there, cosmo.c:767-768): it is
+    placed at the box edge r_max instead.  Returns a list of dicts."""
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i in range(n):
+        N = int(rng.choice([128, 192, 256, 384]))
+        mean = float(rng.uniform(0.3, 0.8))
+        lbox = 4.0 * N
+        zmax = _z_of_chi(lbox / 2 / (1 + 2.0 / N))
+        out.append(dict(n_grid=N, seed=seed0 + i, z_max=zmax, nz_mean=mean, l_box=lbox))
+    return out
 
 
 def write_config(name, outdir, n_grid=None, z_max=None, dens_type=0, output_format="ASCII",

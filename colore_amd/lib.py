@@ -80,6 +80,7 @@ def load_amd():
     _sig(lib, "clr_sample_sources_stream", ci, [_vp, ci, ci, _i32, _u32, ctypes.c_longlong, ci,
                                                 ctypes.POINTER(ctypes.c_longlong)])
     _sig(lib, "clr_source_properties", ci, [_vp, ci, _f, ctypes.c_longlong])
+    _sig(lib, "clr_catalogue_size", ci, [_vp, ci, ctypes.POINTER(ctypes.c_longlong)])
     _sig(lib, "clr_kappa", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_isw", ci, [_vp, ci, ci, _d, _f])
     _sig(lib, "clr_source_beams", ci, [_vp, ci, _d, _f, _f, ctypes.c_longlong])
@@ -102,6 +103,13 @@ def load_amd():
     _sig(lib, "clr_mlp_grad", ci, [_vp, _f, _f, ci, _f, _d])
     _sig(lib, "clr_mlp_train", ci, [_vp, _f, _f, ci, ci, ctypes.c_float, ctypes.c_float, _d])
     _sig(lib, "clr_mlp_predict", ci, [_vp, _f, ci, _f, ci])
+    _sig(lib, "clr_ensemble_create", ci, [ctypes.POINTER(_vp), ci, ctypes.POINTER(_vp), ci, ci, _d,
+                                          ctypes.POINTER(ctypes.c_longlong), ci])
+    _sig(lib, "clr_ensemble_destroy", None, [_vp])
+    _sig(lib, "clr_ensemble_last_error", ctypes.c_char_p, [_vp])
+    _sig(lib, "clr_ensemble_run", ci, [_vp, _u32, _d, ci, ctypes.POINTER(ci), _f])
+    _sig(lib, "clr_ensemble_result", ci, [_vp, ci, _d, ctypes.POINTER(ci), _f])
+    _sig(lib, "clr_fft_autotune", ci, [_vp, ci, ci, _f, ctypes.POINTER(ci), ctypes.POINTER(ci)])
     _sig(lib, "clr_surrogate_inputs", ci, [_vp, ci, ci, _f, _i32, ctypes.POINTER(ci)])
     _sig(lib, "clr_sample_sources_surrogate", ci, [_vp, _vp, ci, ctypes.c_uint32, ci, ci,
                                                    ctypes.POINTER(ctypes.c_longlong)])
@@ -325,6 +333,19 @@ class Context:
                                                         ctypes.byref(n)))
         return n.value
 
+    def fft_autotune(self, reps=3):
+        """(best candidate, median ms per candidate) of the FFT tile autotuner."""
+        ms = np.zeros(16, dtype=np.float32)
+        nc, best = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.lib.clr_fft_autotune(self.h, reps, 16, ptr(ms, ctypes.c_float), ctypes.byref(nc),
+                                            ctypes.byref(best)))
+        return best.value, ms[:nc.value].copy()
+
+    def catalogue_size(self, ipop):
+        n = ctypes.c_longlong()
+        self._chk(self.lib.clr_catalogue_size(self.h, ipop, ctypes.byref(n)))
+        return n.value
+
     def fetch_sources(self, ipop, n):
         pos = np.zeros((n, 4), dtype=np.float32)
         ipix = np.zeros(n, dtype=np.int32)
@@ -488,3 +509,51 @@ class Mlp:
         self._chk(self.lib.clr_mlp_predict(self.h, ptr(x, ctypes.c_float), len(x), ptr(y, ctypes.c_float),
                                            precision))
         return y
+
+
+class Ensemble:
+    """Realisations (contexts) run as one captured HIP graph (clr_ensemble)."""
+
+    def __init__(self, ctxs, src_cap, nside_kappa=0, r_kappa=None, nside_base=2, nstreams=4):
+        self.lib = ctxs[0].lib
+        self.ctxs = list(ctxs)
+        n = len(ctxs)
+        arr = (_vp * n)(*[c.h for c in ctxs])
+        caps = np.ascontiguousarray(np.broadcast_to(np.asarray(src_cap, dtype=np.int64), (n,)))
+        rk = np.ascontiguousarray(r_kappa if r_kappa is not None else np.zeros(n), dtype=np.float64)
+        h = _vp()
+        rc = self.lib.clr_ensemble_create(ctypes.byref(h), n, arr, nside_base, nside_kappa, ptr(rk),
+                                          caps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), nstreams)
+        self.h = h
+        self.nside_kappa = nside_kappa
+        if rc:
+            msg = self.lib.clr_ensemble_last_error(h).decode() if h else "clr_ensemble_create failed"
+            self.close()
+            raise RuntimeError(msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clr_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, seeds, r_smooth, smooth_potential=True):
+        s = np.ascontiguousarray(seeds, dtype=np.uint32)
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(r_smooth, dtype=np.float64), (len(s),)))
+        bad, ms = ctypes.c_int(), ctypes.c_float()
+        rc = self.lib.clr_ensemble_run(self.h, ptr(s, ctypes.c_uint32), ptr(r), int(smooth_potential),
+                                       ctypes.byref(bad), ctypes.byref(ms))
+        if rc:
+            raise RuntimeError(f"clr_ensemble error {rc}: {self.lib.clr_ensemble_last_error(self.h).decode()}")
+        return bad.value, ms.value
+
+    def result(self, i):
+        s2, flag = ctypes.c_double(), ctypes.c_int()
+        kap = np.zeros(12 * self.nside_kappa ** 2, dtype=np.float32) if self.nside_kappa else None
+        rc = self.lib.clr_ensemble_result(self.h, i, ctypes.byref(s2), ctypes.byref(flag),
+                                          ptr(kap, ctypes.c_float) if kap is not None else None)
+        if rc:
+            raise RuntimeError(self.lib.clr_ensemble_last_error(self.h).decode())
+        return s2.value, flag.value, kap

@@ -191,6 +191,8 @@ int clr_sample_sources_stream(clr_ctx *ctx, int ipop, int nthr, const int *iz_be
  * = (x, y, z, dz_rsd) relative to the observer, ipix[n]. */
 int clr_fetch_sources(clr_ctx *ctx, int ipop, float *pos4, int32_t *ipix,
                       long long n);
+/* Number of sources in population ipop's catalogue (par->nsources_c_this). */
+int clr_catalogue_size(clr_ctx *ctx, int ipop, long long *n);
 /* srcs_get_local_properties_single (srcs.c:386-416): per source RA, Dec
  * (degrees), z0 = z(r), dz_rsd; written as float[4n]. */
 int clr_source_properties(clr_ctx *ctx, int ipop, float *radecz, long long n);
@@ -327,6 +329,41 @@ int clr_surrogate_inputs(clr_ctx *ctx, int ipop, int cap, float *x, int *shell, 
  * counts equal the two-pass path's.  The grid keeps the Gaussian field. */
 int clr_sample_sources_surrogate(clr_ctx *ctx, clr_mlp *net, int ipop, uint32_t seed,
                                  int nside_base, int precision, long long *nsrc_out);
+
+/* ---- ensembles in one captured graph ----------------------------------
+ * No counterpart in the reference (one realisation per run there).  n
+ * independent realisations, each an existing context (its own N, box,
+ * tables, populations; all on one device), run as one HIP graph: per
+ * realisation clr_gaussian_fields (Philox), clr_shell_norms with the fused
+ * lognormal, clr_sample_sources(-1) and a kappa map of one source plane at
+ * r_kappa[i] (nside_kappa; 0: no map), spread over nstreams captured streams.
+ * Buffers are set up by clr_ensemble_create (catalogue capacity src_cap[i]
+ * per population).  clr_ensemble_run captures and instantiates the graph on
+ * the first call and whenever the seeds or smoothing change, then replays
+ * it; afterwards every context is in the state the single calls leave
+ * (clr_fetch_sources, clr_fetch_grid, clr_get_norm_table work) and its
+ * results equal those calls' bit for bit.  A realisation whose counts hit a
+ * case the graph does not handle (counts of 254 or more in a cell, an
+ * overflowing FP64 queue, a catalogue past src_cap) is reported with a
+ * non-zero flag: re-run it with the single calls. */
+typedef struct clr_ensemble clr_ensemble;
+int  clr_ensemble_create(clr_ensemble **ens, int n, clr_ctx *const *ctxs, int nside_base,
+                         int nside_kappa, const double *r_kappa, const long long *src_cap,
+                         int nstreams);
+void clr_ensemble_destroy(clr_ensemble *ens);
+const char *clr_ensemble_last_error(const clr_ensemble *ens);
+/* seeds[n], r_smooth[n]; n_invalid: realisations with a flag; ms: device
+ * time of the graph launch. */
+int  clr_ensemble_run(clr_ensemble *ens, const uint32_t *seeds, const double *r_smooth,
+                      int smooth_potential, int *n_invalid, float *ms);
+/* Realisation i: sigma^2, its flag, and (kappa != NULL) its map, NEST
+ * float[12 nside_kappa^2]. */
+int  clr_ensemble_result(clr_ensemble *ens, int i, double *sigma2, int *flag, float *kappa);
+/* FFT tile autotuner: every tile-shape candidate for this context's N runs
+ * the field stage `reps` times on its grids (which it overwrites); the one
+ * with the smallest median time is used from then on by every context of
+ * that N in the process.  ms[cap]: median per candidate. */
+int  clr_fft_autotune(clr_ctx *ctx, int reps, int cap, float *ms, int *ncand, int *best);
 
 /* Device time in milliseconds of the last call of each stage (0 if never),
  * in this order: mode generation + z pass, y pass, x pass, density

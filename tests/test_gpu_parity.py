@@ -610,16 +610,23 @@ def test_sources_reference_stream(workdir, N, nthr, ipop):
     dens = ctx.fetch_grid(0).astype(np.float64)
     npot = ctx.fetch_grid(1).astype(np.float64)
     izb = static_split(N, nthr)
-    per_thread = int(4.0 * N ** 3 / nthr * 2.5) + 100000
-    st = ref_streams(seed, ipop, nthr, per_thread)
-    n = ctx.sample_sources_stream(ipop, nthr, izb, st)
-    pos, ipix = ctx.fetch_sources(ipop, n)
-    counts = ctx.fetch_cell_counts(ipop, 0, N).reshape(-1)
     amp, bz, model = run.tracer(0, ipop)
     tab, n0, nf = ctx.norm_table(0, ipop)
     orc = _oracle(run)
     pop = orc.pop(N, run.scalar("l_box"), amp, bz, tab, n0, nf, model, ipop, seed)
-    opos, oipix, ocounts = orc.sources_ref_stream(pop, dens, npot, nthr, izb, st)
+    # stream length: the oracle's walk tells whether it suffices (doubling)
+    per_thread = int(10 * N ** 3 / nthr) + 100000
+    while True:
+        st = ref_streams(seed, ipop, nthr, per_thread)
+        try:
+            opos, oipix, ocounts = orc.sources_ref_stream(pop, dens, npot, nthr, izb, st)
+            break
+        except AssertionError:
+            per_thread *= 2
+            assert per_thread < 1 << 26
+    n = ctx.sample_sources_stream(ipop, nthr, izb, st)
+    pos, ipix = ctx.fetch_sources(ipop, n)
+    counts = ctx.fetch_cell_counts(ipop, 0, N).reshape(-1)
     assert n == len(opos) and n > 0
     np.testing.assert_array_equal(counts, ocounts)
     assert np.array_equal(pos[:, :3], opos[:, :3])
