@@ -4,6 +4,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <mutex>
 
 bool clr_fft_size_supported(int N);
 
@@ -57,6 +58,29 @@ int clr_ring2nest_table(clr_ctx *c, int nside) {
   CLR_HIP(c, hipMalloc((void **)&c->r2n, npix * sizeof(int32_t)));
   CLR_HIP(c, hipMemcpy(c->r2n, t.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice));
   c->r2n_nside = nside;
+  return CLR_OK;
+}
+
+// Device address of the no-allocation scratch (clr_shell_norms_noalloc), one
+// per device: looked up once per device by clr_create with that device
+// current.  Each GPU holds its own copy of the array at its own address.
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+static int g_scratch_state[64];   // 0 not looked up, 1 found, -1 failed
+
+static int noalloc_scratch_setup(clr_ctx *c) {
+  if (c->device < 0 || c->device >= 64) return clr_fail(c, CLR_ERR_ARG, "device %d out of range", c->device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_state[c->device] == 0) {
+    void *p = nullptr;
+    if (clr_noalloc_scratch_lookup(&p) == 0 && p) {
+      g_scratch_ptr[c->device] = p;
+      g_scratch_state[c->device] = 1;
+    } else {
+      (void)hipGetLastError();
+      g_scratch_state[c->device] = -1;
+    }
+  }
   return CLR_OK;
 }
 
@@ -116,6 +140,8 @@ int clr_create(clr_ctx **out, int n_grid, double l_box, const double pos_obs[3],
   CLR_HIP(c, hipEventCreate(&c->ev1));
   CLR_HIP(c, hipEventCreate(&c->ev2));
   CLR_HIP(c, hipEventCreate(&c->ev3));
+  int rs = noalloc_scratch_setup(c);
+  if (rs) return rs;
   c->grid_floats = (size_t)n_grid * n_grid * c->ngx;
   if (hipMalloc((void **)&c->dens, c->grid_floats * sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&c->npot, c->grid_floats * sizeof(float)) != hipSuccess) {
@@ -257,8 +283,23 @@ int clr_shell_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, int
                     unsigned long long *counts, double *zsum, double *sums) {
   if (!c) return CLR_ERR_ARG;
   CLR_HIP(c, hipSetDevice(c->device));
-  return clr_launch_norms(c, lognormal_fused, sigma2, cap, nz_out, counts, zsum, sums);
+  return clr_launch_norms(c, lognormal_fused, sigma2, cap, nz_out, counts, zsum, sums, nullptr);
 }
+
+int clr_shell_norms_noalloc(clr_ctx *c, int lognormal_fused, double sigma2, int cap, int *nz_out,
+                            unsigned long long *counts, double *zsum, double *sums) {
+  if (!c) return CLR_ERR_ARG;
+  CLR_HIP(c, hipSetDevice(c->device));
+  void *p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device >= 0 && c->device < 64 && g_scratch_state[c->device] == 1) p = g_scratch_ptr[c->device];
+  }
+  if (!p) return clr_fail(c, CLR_ERR_HIP, "no static scratch on device %d (hipGetSymbolAddress failed)", c->device);
+  return clr_launch_norms(c, lognormal_fused, sigma2, cap, nz_out, counts, zsum, sums, p);
+}
+
+long long clr_noalloc_max_bytes(void) { return (long long)clr_noalloc_scratch_bytes(); }
 
 int clr_get_norm_table(clr_ctx *c, int kind, int ipop, double *norm_arr, double *norm0, double *normf) {
   if (!c) return CLR_ERR_ARG;

@@ -484,8 +484,28 @@ int clr_launch_density(clr_ctx *c, int dens_type, double sigma2) {
   return CLR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Static scratch of the no-allocation variant (clr_shell_norms_noalloc):
+// the per-block FP64 partials of k_shells, the final sums, k_norm_table's
+// edge values and the packed FP32 table, for nz (2 + ntr) <= kNoAllocVals.
+// One copy per GPU for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocVals = 320;
+constexpr size_t kNoAllocTabFloat4 = (size_t)CLR_NA * shell_tab_w(kMaxTr);
+constexpr size_t kNoAllocDoubles = (size_t)kShellBlocks * kNoAllocVals + kNoAllocVals + 2 * (size_t)kNormMaxShells +
+                                   2 * (size_t)kMaxTr;
+constexpr size_t kNoAllocBytes = (kNoAllocDoubles * sizeof(double) + 255) / 256 * 256 + kNoAllocTabFloat4 * 16;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+size_t clr_noalloc_scratch_bytes() { return kNoAllocBytes; }
+
+int clr_noalloc_scratch_lookup(void **ptr) {
+  return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)) == hipSuccess ? 0 : -1;
+}
+
 int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, int *nz_out,
-                     unsigned long long *counts, double *zsum, double *sums) {
+                     unsigned long long *counts, double *zsum, double *sums, void *scratch_base) {
   if (!c->have_fields) return clr_fail(c, CLR_ERR_STATE, "clr_gaussian_fields must run first");
   if (lognormal_fused && c->density_done)
     return clr_fail(c, CLR_ERR_STATE, "density already transformed; call with lognormal_fused=0");
@@ -553,13 +573,30 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   for (int t = 0; t < ntr; t++) all0 = all0 && a.model[t] == CLR_BIAS_EXPONENTIAL;
   const size_t nvals = (size_t)nz * nq;
   const int w = shell_tab_w(ntr);
-  const size_t need = (size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr;
-  int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
-  if (rc) return rc;
-  double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals, *work = fin + nvals;
-  rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
-  if (rc) return rc;
-  float4 *tab = (float4 *)c->misc;
+  double *partial, *fin, *work;
+  float4 *tab;
+  if (scratch_base) {
+    // no allocation: everything in the device's static scratch
+    if (nvals > kNoAllocVals)
+      return clr_fail(c, CLR_ERR_ARG, "%d shells x (2 + %d tracers) = %zu values exceed the %zu of the static scratch",
+                      nz, ntr, nvals, kNoAllocVals);
+    if ((size_t)na * w > kNoAllocTabFloat4)
+      return clr_fail(c, CLR_ERR_ARG, "r-tables of %d entries exceed the static scratch", na);
+    partial = (double *)scratch_base;
+    fin = partial + (size_t)kShellBlocks * nvals;
+    work = fin + nvals;
+    tab = (float4 *)((char *)scratch_base + (kNoAllocDoubles * sizeof(double) + 255) / 256 * 256);
+  } else {
+    const size_t need = (size_t)kShellBlocks * nvals + nvals + 2 * nz + 2 * ntr;
+    int rc = clr_ensure(c, (void **)&c->red, &c->red_n, need * sizeof(double));
+    if (rc) return rc;
+    partial = c->red;
+    fin = c->red + (size_t)kShellBlocks * nvals;
+    work = fin + nvals;
+    rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
+    if (rc) return rc;
+    tab = (float4 *)c->misc;
+  }
   a.tab = tab;
 
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
@@ -583,8 +620,15 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
     CLR_HIP(c, hipGetLastError());
   }
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
-  std::vector<double> hv(nvals + 2 * ntr);
-  double *h = hv.data();
+  // host copies of the sums and edge values (a fixed array when they fit, so
+  // that the no-allocation variant allocates nothing on the host either)
+  double hfix[kNoAllocVals + 2 * kMaxTr];
+  std::vector<double> hv;
+  double *h = hfix;
+  if (nvals > kNoAllocVals) {
+    hv.resize(nvals + 2 * ntr);
+    h = hv.data();
+  }
   CLR_HIP(c, hipMemcpyAsync(h, fin, nvals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CLR_HIP(c, hipMemcpyAsync(h + nvals, work + 2 * nz, 2 * ntr * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));

@@ -150,8 +150,14 @@ int clr_ring2nest_table(clr_ctx *c, int nside);
 int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_potential,
                       int rng_mode, const uint32_t *injected, double *sigma2_out);
 int clr_launch_density(clr_ctx *c, int dens_type, double sigma2);
+// scratch_base: NULL (buffers from the context, grown on demand) or the
+// device's static scratch (clr_shell_norms_noalloc)
 int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, int *nz_out,
-                     unsigned long long *counts, double *zsum, double *sums);
+                     unsigned long long *counts, double *zsum, double *sums, void *scratch_base);
+// The no-allocation scratch: its size, and its device address on the current
+// device (0, or -1 when hipGetSymbolAddress fails).
+size_t clr_noalloc_scratch_bytes();
+int clr_noalloc_scratch_lookup(void **ptr);
 // Makes npot safe to use: host = 0 orders c->stream after phi's passes,
 // host = 1 waits for them on the host.
 int clr_phi_ready(clr_ctx *c, int host);

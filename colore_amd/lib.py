@@ -73,6 +73,8 @@ def load_amd():
     _sig(lib, "clr_gaussian_fields", ci, [_vp, ctypes.c_uint32, ctypes.c_double, ci, ci, _u32, _d])
     _sig(lib, "clr_density", ci, [_vp, ci, ctypes.c_double])
     _sig(lib, "clr_shell_norms", ci, [_vp, ci, ctypes.c_double, ci, ctypes.POINTER(ci), _u64, _d, _d])
+    _sig(lib, "clr_shell_norms_noalloc", ci, [_vp, ci, ctypes.c_double, ci, ctypes.POINTER(ci), _u64, _d, _d])
+    _sig(lib, "clr_noalloc_max_bytes", ctypes.c_longlong, [])
     _sig(lib, "clr_get_norm_table", ci, [_vp, ci, ci, _d, _d, _d])
     _sig(lib, "clr_set_norm_table", ci, [_vp, ci, ci, _d, ctypes.c_double, ctypes.c_double])
     _sig(lib, "clr_sample_sources", ci, [_vp, ci, ctypes.c_uint32, ci, ctypes.POINTER(ctypes.c_longlong)])
@@ -280,13 +282,14 @@ class Context:
     def density(self, dens_type, sigma2):
         self._chk(self.lib.clr_density(self.h, dens_type, sigma2))
 
-    def shell_norms(self, lognormal_fused=False, sigma2=0.0, ntracers=0, cap=256):
+    def shell_norms(self, lognormal_fused=False, sigma2=0.0, ntracers=0, cap=256, noalloc=False):
         nz = ctypes.c_int()
         cnt = np.zeros(cap, dtype=np.uint64)
         zs = np.zeros(cap)
         sums = np.zeros(max(ntracers, 1) * cap)
-        self._chk(self.lib.clr_shell_norms(self.h, int(lognormal_fused), sigma2, cap, ctypes.byref(nz),
-                                           ptr(cnt, ctypes.c_ulonglong), ptr(zs), ptr(sums)))
+        fn = self.lib.clr_shell_norms_noalloc if noalloc else self.lib.clr_shell_norms
+        self._chk(fn(self.h, int(lognormal_fused), sigma2, cap, ctypes.byref(nz), ptr(cnt, ctypes.c_ulonglong),
+                     ptr(zs), ptr(sums)))
         n = nz.value
         return n, cnt[:n].copy(), zs[:n].copy(), sums[:ntracers * n].reshape(ntracers, n).copy()
 

@@ -153,6 +153,21 @@ int clr_shell_norms(clr_ctx *ctx, int lognormal_fused, double sigma2, int cap,
                     int *nz_out, unsigned long long *counts, double *zsum,
                     double *sums);
 
+/* clr_shell_norms without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The partial
+ * sums, final sums and packed tables live in one static device array per GPU
+ * (clr_noalloc_max_bytes() bytes, about 21 MB) instead of context buffers;
+ * the host copies use a fixed array.  Limit: nz (2 + ntracers) <= 320 values
+ * (e.g. 42 shells with 5 tracers), else CLR_ERR_ARG.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other host threads,
+ * other contexts on the same device) race; results equal clr_shell_norms'. */
+int clr_shell_norms_noalloc(clr_ctx *ctx, int lognormal_fused, double sigma2, int cap,
+                            int *nz_out, unsigned long long *counts, double *zsum,
+                            double *sums);
+/* Size in bytes of the static array of clr_shell_norms_noalloc. */
+long long clr_noalloc_max_bytes(void);
+
 /* Copies a tracer's NA norm table (and its edge values norm0/normf). */
 int clr_get_norm_table(clr_ctx *ctx, int kind, int ipop, double *norm_arr,
                        double *norm0, double *normf);
