@@ -22,6 +22,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -199,6 +201,122 @@ def cpu_baseline(cfg, n_cpu_grid):
                       f"{dt:.1f} s)"}
 
 
+def bench_ensemble(args):
+    """The batched-ensemble workload: 64 realisations (grids of 128^3 ..
+    384^3, 4 Mpc/h cells, one population, a kappa map at the box edge,
+    nside 128) as one captured graph; FFT tiles autotuned per size first.
+    Times `steps` replays of the graph after one launch that captures it."""
+    from colore_amd import inputs, lib
+    real = inputs.c4_realisations(64, 4000)
+    d = tempfile.mkdtemp(prefix="colore_c4_")
+    runs, ctxs = [], []
+    for i, r in enumerate(real):
+        pops = [(("gauss", r["nz_mean"], 0.1, 5e-4), ("1+z",), "exponential")]
+        run = lib.Run(inputs.write_config("C4", os.path.join(d, str(i)), n_grid=r["n_grid"], z_max=r["z_max"],
+                                          seed=r["seed"], with_maps=False, populations=pops))
+        ctx = lib.Context(r["n_grid"], run.scalar("l_box"))
+        ctx.upload_tables(run.tables())
+        amp, bz, model = run.tracer(0, 0)
+        ctx.set_tracer(0, 0, model, amp, bz)
+        runs.append(run)
+        ctxs.append(ctx)
+    tuned = {}
+    for r, ctx in zip(real, ctxs):
+        if r["n_grid"] not in tuned:
+            best, ms = ctx.fft_autotune(3)
+            tuned[r["n_grid"]] = {"choice": best, "ms": [float(x) for x in ms]}
+    ens = lib.Ensemble(ctxs, src_cap=2_000_000, nside_kappa=128,
+                       r_kappa=np.array([run.scalar("r_max") for run in runs]), nstreams=8)
+    seeds = np.array([r["seed"] for r in real], dtype=np.uint32)
+    rsm = np.array([run.scalar("r_smooth") for run in runs])
+    t0 = time.perf_counter()
+    bad, ms_first = ens.run(seeds, rsm)
+    t_capture = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        ens.run(seeds, rsm)
+    t0 = time.perf_counter()
+    dev = []
+    for _ in range(args.steps):
+        b, ms = ens.run(seeds, rsm)
+        bad = max(bad, b)
+        dev.append(ms)
+    dt = (time.perf_counter() - t0) / args.steps
+    cells = float(sum(r["n_grid"] ** 3 for r in real))
+    nsrc = sum(c.catalogue_size(0) for c in ctxs)
+    out = {"metric": "grid cells/sec (batched ensemble: fields, lognormal + shell norms, sources, kappa)",
+           "value": cells / dt, "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (EH98 no-wiggle P(k)); Philox streams",
+           "config": {"workload": "64 realisations, seeds 4000..4063, N in {128,192,256,384}, cell 4 Mpc/h, "
+                                  "b=1+z, kappa nside 128 at the box edge, one captured HIP graph (8 streams)",
+                      "grids": {str(k): sum(1 for r in real if r["n_grid"] == k) for k in (128, 192, 256, 384)},
+                      "cells": cells, "sources": nsrc, "invalid_realisations": bad},
+           "realisations_per_s": len(real) / dt, "graph_device_ms": float(np.median(dev)),
+           "first_launch_incl_capture_s": t_capture, "fft_autotune": tuned}
+    ens.close()
+    for c in ctxs:
+        c.close()
+    print(json.dumps(out))
+
+
+def bench_surrogate(args):
+    """Surrogate-fused sampling against the exact two-pass path on the learned-
+    surrogate workload (2048^3, z in [0.05, 2.0], the two populations of the
+    512^3 precision workload, seed 5005): the network is trained first (its
+    own device kernels, shells of 256^3 .. 1024^3 realisations); then per step
+    the two-pass sampling (shell sums + sources) and the fused single pass
+    (inference + norm tables + sources) are timed on the same Gaussian field,
+    and every shell's norm error against the exact reduction is reported."""
+    from colore_amd import inputs, lib, surrogate
+    t0 = time.perf_counter()
+    x, y = surrogate.training_set()
+    net, losses = surrogate.train(x, y)
+    t_train = time.perf_counter() - t0
+    d = tempfile.mkdtemp(prefix="colore_c5_")
+    run = lib.Run(inputs.write_config("C5", d, n_grid=args.n_grid, with_maps=False))
+    N, L = int(run.scalar("n_grid")), run.scalar("l_box")
+    seed, rs = int(run.scalar("seed")), run.scalar("r_smooth")
+    npop = int(run.scalar("n_srcs"))
+    ctx = lib.Context(N, L)
+    ctx.upload_tables(run.tables())
+    for i in range(npop):
+        amp, bz, model = run.tracer(0, i)
+        ctx.set_tracer(0, i, model, amp, bz)
+    if not args.no_autotune:
+        ctx.fft_autotune(3)
+    two, fused = [], []
+    for it in range(args.warmup + args.steps):
+        s2 = ctx.gaussian_fields(seed, rs, True)
+        t = time.perf_counter()
+        ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=npop)
+        n2 = ctx.sample_sources(-1, seed, 2)
+        t2 = time.perf_counter() - t
+        s2 = ctx.gaussian_fields(seed, rs, True)
+        t = time.perf_counter()
+        nf = ctx.sample_sources_surrogate(net, -1, seed, 2)
+        tf = time.perf_counter() - t
+        if it >= args.warmup:
+            two.append(t2)
+            fused.append(tf)
+    err, _ = surrogate.shell_errors(ctx, run, net, seed, npop)
+    summ = surrogate.summarize(err)
+    t2, tf = float(np.median(two)), float(np.median(fused))
+    out = {"metric": "surrogate-fused sampling speedup over the exact two-pass sampling (shell sums + sources)",
+           "value": t2 / tf, "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": tf * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (EH98 no-wiggle P(k)); Philox streams",
+           "config": {"workload": f"{N}^3, z in [0.05, 2.0], populations b=1+z (5e-4) and b=2 (1e-4), seed {seed}",
+                      "n_grid": N, "sources_two_pass": n2, "sources_fused": nf},
+           "two_pass_ms": t2 * 1e3, "fused_ms": tf * 1e3,
+           "fused_cells_per_s": N ** 3 / tf, "two_pass_cells_per_s": N ** 3 / t2,
+           "per_shell_norm_error": summ, "shells": int(len(err["network"])),
+           "training": {"shells": int(len(x)), "seconds": t_train, "loss_first": float(losses[0]),
+                        "loss_last": float(losses[-1])}}
+    ctx.close()
+    net.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,9 +330,19 @@ def main():
                     help="also time the kappa/ISW line-of-sight maps and the 21cm painting of the workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (PMC counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default FFT tile shapes (no autotuning before the timed steps)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="surrogate-fused sampling against the two-pass path (learned-surrogate workload)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.gpus == 1 and args.config == "C4":
+        bench_ensemble(args)
+        return
+    if args.gpus == 1 and args.surrogate:
+        bench_surrogate(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -232,6 +360,12 @@ def main():
     for i in range(npop):
         amp, bz, model = run.tracer(0, i)
         ctx.set_tracer(0, i, model, amp, bz)
+    # FFT tile shapes: the autotuner times every candidate for this N (setup,
+    # not timed)
+    tuned = None
+    if not args.no_autotune:
+        best, tms = ctx.fft_autotune(3)
+        tuned = {"choice": best, "candidate_ms": [float(x) for x in tms]}
 
     acc = {}
 
@@ -282,7 +416,8 @@ def main():
         "data": "synthetic (EH98 no-wiggle P(k), Gaussian n(z)); Philox streams",
         "config": {"workload": f"{args.config}: {N}^3 FP32 field->lognormal->sources, {npop} populations, "
                                f"seed {seed}, FP64 shell sums",
-                   "n_grid": N, "l_box": L, "populations": npop, "sources": nsrc_tot, "sigma2": s2},
+                   "n_grid": N, "l_box": L, "populations": npop, "sources": nsrc_tot, "sigma2": s2,
+                   "fft_autotune": tuned},
         "path_roofline": {"b_alg_per_cell": b_alg, "achieved": value * b_alg, "peak": HBM_PEAK,
                           "frac": value * b_alg / HBM_PEAK},
         "stage_ms": {k: v / args.steps for k, v in acc.items()},

@@ -20,6 +20,8 @@
 // the result for the non-Hermitian kx=0 and kx=N/2 planes the reference draws.
 #include "clr_internal.h"
 #include "clr_fft.h"
+
+
 #include <math.h>
 #include <algorithm>
 #include <mutex>
@@ -107,8 +109,11 @@ __device__ inline void xcd_tile(unsigned b, unsigned ntile, unsigned nplanes, in
 // ---------------------------------------------------------------------------
 // fused mode generation + z pass
 // ---------------------------------------------------------------------------
-template <int N, int P, int T, int... Rs>
-__global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float2 *__restrict__ npot,
+// SPLIT: one field at a time (half the registers, the draws twice), for a
+// higher occupancy where the tile's LDS allows it.
+template <int N, int P, int T, bool SPLIT, int... Rs>
+__global__ __launch_bounds__(P *T) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 1))) void k_gen_z(
+    float2 *__restrict__ dens, float2 *__restrict__ npot,
                                                  const float2 *__restrict__ amp, int compact, float phi_ratio,
                                                  const float2 *__restrict__ tw, uint32_t seed,
                                                  const uint32_t *__restrict__ inj) {
@@ -124,72 +129,96 @@ __global__ __launch_bounds__(P *T) void k_gen_z(float2 *__restrict__ dens, float
   const bool valid = kx < NH;
   const int fy = (2 * ky <= N) ? ky : N - ky;
   static_assert(E % 2 == 0, "modes are drawn in (kz, kz+N/2) pairs");
-  float2 vd[E], vp[E];
-  // thread t holds kz = t + P j, j < E; element j sits at v[(j % (E/R0)) R0 + j / (E/R0)]
+  // Modes of one field (0: delta, 1: phi, 2: both into v and v2) in the
+  // thread's z-column elements: thread t holds kz = t + P j, j < E; element j
+  // sits at v[(j % (E/R0)) R0 + j / (E/R0)].
+  auto gen = [&](int which, float2 *v, float2 *v2) {
 #pragma unroll
-  for (int j = 0; j < E / 2; j++) {
-    const int kz0 = t + P * j;   // < N/2; its partner kz0 + N/2 is element j + E/2
-    u32x4 r{0u, 0u, 0u, 0u};
-    if (!inj) {
-      const uint64_t site = (uint64_t)kx + (uint64_t)NH * ((uint64_t)ky + (uint64_t)N * (uint64_t)kz0);
-      r = draw(site, kStreamModes, 0u, seed);
-    }
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int jj = j + h * (E / 2);
-      const int kz = kz0 + h * (N / 2);
-      const int fz = (2 * kz <= N) ? kz : N - kz;
-      const int q = kx * kx + fy * fy + fz * fz;
-      float2 d = make_float2(0.f, 0.f), p = d;
-      if (valid && q > 0) {
-        uint32_t x0, x1;
-        if (inj) {
-          const uint64_t mi = (uint64_t)kx + (uint64_t)NH * ((uint64_t)ky + (uint64_t)N * (uint64_t)kz);
-          x0 = inj[2 * mi];
-          x1 = inj[2 * mi + 1];
-        } else {
-          x0 = h ? r.z : r.x;
-          x1 = h ? r.w : r.y;
-        }
-        const float w = modulus_draw(x1);
-        // angle 2 pi u: v_sin_f32 / v_cos_f32 take their argument in turns
-        const float turns = (float)x0 * 2.3283064365386963e-10f;
-        const float sn = __builtin_amdgcn_sinf(turns), cs = __builtin_amdgcn_cosf(turns);
-        // the compact table (4 bytes per |k|^2) stays in L2; phi's amplitude
-        // is delta's times -prefac/k^2 (a few ulp)
-        float2 a;
-        if (compact) {
-          a.x = ((const float *)amp)[q];
-          a.y = a.x * (phi_ratio * __builtin_amdgcn_rcpf((float)q));
-        } else {
-          a = amp[q];
-        }
-        d = make_float2(a.x * w * cs, a.x * w * sn);
-        p = make_float2(a.y * w * cs, a.y * w * sn);
+    for (int j = 0; j < E / 2; j++) {
+      const int kz0 = t + P * j;   // < N/2; its partner kz0 + N/2 is element j + E/2
+      u32x4 r{0u, 0u, 0u, 0u};
+      if (!inj) {
+        const uint64_t site = (uint64_t)kx + (uint64_t)NH * ((uint64_t)ky + (uint64_t)N * (uint64_t)kz0);
+        r = draw(site, kStreamModes, 0u, seed);
       }
-      const int vi = (jj % (E / R0)) * R0 + jj / (E / R0);
-      vd[vi] = d;
-      vp[vi] = p;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int jj = j + h * (E / 2);
+        const int kz = kz0 + h * (N / 2);
+        const int fz = (2 * kz <= N) ? kz : N - kz;
+        const int q = kx * kx + fy * fy + fz * fz;
+        float2 d = make_float2(0.f, 0.f), p = d;
+        if (valid && q > 0) {
+          uint32_t x0, x1;
+          if (inj) {
+            const uint64_t mi = (uint64_t)kx + (uint64_t)NH * ((uint64_t)ky + (uint64_t)N * (uint64_t)kz);
+            x0 = inj[2 * mi];
+            x1 = inj[2 * mi + 1];
+          } else {
+            x0 = h ? r.z : r.x;
+            x1 = h ? r.w : r.y;
+          }
+          const float w = modulus_draw(x1);
+          // angle 2 pi u: v_sin_f32 / v_cos_f32 take their argument in turns
+          const float turns = (float)x0 * 2.3283064365386963e-10f;
+          const float sn = __builtin_amdgcn_sinf(turns), cs = __builtin_amdgcn_cosf(turns);
+          // the compact table (4 bytes per |k|^2) stays in L2; phi's amplitude
+          // is delta's times -prefac/k^2 (a few ulp)
+          float2 a;
+          if (compact) {
+            a.x = ((const float *)amp)[q];
+            a.y = a.x * (phi_ratio * __builtin_amdgcn_rcpf((float)q));
+          } else {
+            a = amp[q];
+          }
+          d = make_float2(a.x * w * cs, a.x * w * sn);
+          p = make_float2(a.y * w * cs, a.y * w * sn);
+        }
+        const int vi = (jj % (E / R0)) * R0 + jj / (E / R0);
+        if (which == 2) {
+          v[vi] = d;
+          v2[vi] = p;
+        } else {
+          v[vi] = which == 0 ? d : p;
+        }
+      }
     }
-  }
+  };
   const LdsCol<T> io{lds, c};
+  auto out = [&](const float2 *v, float2 *g) {
+#pragma unroll
+    for (int b = 0; b < E / RZ; b++)
+#pragma unroll
+      for (int q = 0; q < RZ; q++) {
+        const int z = t + b * P + q * (N / RZ);
+        if (valid) g[((size_t)z * N + ky) * NHP + kx] = v[b * RZ + q];
+      }
+  };
+  if constexpr (SPLIT) {
+  // one field at a time (half the registers, twice the draws): the same
+  // values as the joint generation
+  {
+    float2 v[E];
+    gen(0, v, nullptr);
+    Stages<N, P, N, 1, Rs...>::run(v, t, io, tw);
+    out(v, dens);
+  }
+  __syncthreads();
+  {
+    float2 v[E];
+    gen(1, v, nullptr);
+    Stages<N, P, N, 1, Rs...>::run(v, t, io, tw);
+    out(v, npot);
+  }
+  } else {
+  float2 vd[E], vp[E];
+  gen(2, vd, vp);
   Stages<N, P, N, 1, Rs...>::run(vd, t, io, tw);
-#pragma unroll
-  for (int b = 0; b < E / RZ; b++)
-#pragma unroll
-    for (int q = 0; q < RZ; q++) {
-      const int z = t + b * P + q * (N / RZ);
-      if (valid) dens[((size_t)z * N + ky) * NHP + kx] = vd[b * RZ + q];
-    }
+  out(vd, dens);
   __syncthreads();
   Stages<N, P, N, 1, Rs...>::run(vp, t, io, tw);
-#pragma unroll
-  for (int b = 0; b < E / RZ; b++)
-#pragma unroll
-    for (int q = 0; q < RZ; q++) {
-      const int z = t + b * P + q * (N / RZ);
-      if (valid) npot[((size_t)z * N + ky) * NHP + kx] = vp[b * RZ + q];
-    }
+  out(vp, npot);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -335,10 +364,16 @@ struct FieldLaunch {
   int colT, colThreads, xRPB, xThreads;
 };
 
+template <int N, int P, int T, bool SPLIT, int... Rs> struct GenKernel {
+  static void gen(dim3 g, dim3 b, hipStream_t s, float2 *d, float2 *p, const float2 *amp, int compact,
+                  float phi_ratio, const float2 *tw, uint32_t seed, const uint32_t *inj) {
+    hipLaunchKernelGGL((k_gen_z<N, P, T, SPLIT, Rs...>), g, b, 0, s, d, p, amp, compact, phi_ratio, tw, seed, inj);
+  }
+};
 template <int N, int P, int T, int... Rs> struct ColKernels {
   static void gen(dim3 g, dim3 b, hipStream_t s, float2 *d, float2 *p, const float2 *amp, int compact,
                   float phi_ratio, const float2 *tw, uint32_t seed, const uint32_t *inj) {
-    hipLaunchKernelGGL((k_gen_z<N, P, T, Rs...>), g, b, 0, s, d, p, amp, compact, phi_ratio, tw, seed, inj);
+    GenKernel<N, P, T, false, Rs...>::gen(g, b, s, d, p, amp, compact, phi_ratio, tw, seed, inj);
   }
   static void coly(dim3 g, dim3 b, hipStream_t s, float2 *d, float2 *p, const float2 *tw) {
     hipLaunchKernelGGL((k_col_y<N, P, T, Rs...>), g, b, 0, s, d, p, tw);
@@ -353,6 +388,9 @@ template <int N, int P, int RPB, int... Rs> struct XKernels {
 
 #define COLCFG(N, P, T, ...)                                                                       \
   ColKernels<N, P, T, __VA_ARGS__>::gen, ColKernels<N, P, T, __VA_ARGS__>::coly
+// the same column kernels with the one-field-at-a-time generation
+#define COLCFG_SPLIT(N, P, T, ...)                                                                 \
+  GenKernel<N, P, T, true, __VA_ARGS__>::gen, ColKernels<N, P, T, __VA_ARGS__>::coly
 #define XCFG(N, P, RPB, ...) XKernels<N, P, RPB, __VA_ARGS__>::c2r
 
 // Tile shapes per N: columns per block T and threads per column P of the z
@@ -374,7 +412,11 @@ std::vector<FieldLaunch> field_candidates(int N) {
     case 384:  return {{COLCFG(384, 16, 16, 8, 8, 6), XCFG(384, 16, 16, 4, 4, 12), 16, 256, 16, 256},
                        {COLCFG(384, 32, 8, 4, 4, 6, 4), XCFG(384, 8, 16, 8, 8, 3), 8, 256, 16, 128}};
     case 512:  return {{COLCFG(512, 64, 16, 8, 8, 8), XCFG(512, 16, 16, 16, 16), 16, 1024, 16, 256}};
-    case 1024: return {{COLCFG(1024, 32, 8, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256}};
+    case 1024: return {{COLCFG(1024, 32, 8, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256},
+                       {COLCFG_SPLIT(1024, 32, 4, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 4, 128, 8, 256},
+                       {COLCFG_SPLIT(1024, 32, 8, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256},
+                       {COLCFG(1024, 64, 4, 16, 16, 4), XCFG(1024, 64, 4, 8, 8, 8), 4, 256, 4, 256},
+                       {COLCFG_SPLIT(1024, 64, 4, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 4, 256, 8, 256}};
     case 2048: return {{COLCFG(2048, 128, 8, 16, 16, 8), XCFG(2048, 64, 4, 16, 16, 4), 8, 1024, 4, 256}};
     default: return {};
   }

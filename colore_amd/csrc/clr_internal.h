@@ -199,5 +199,10 @@ int clr_prepare_kappa(clr_ctx *c, int nside, int nplanes, const double *r_src);
 int clr_enqueue_kappa(clr_ctx *c, hipStream_t s);
 int clr_fetch_kappa(clr_ctx *c, float *out);
 
+// ln norm of the lognormal moment for surrogate inputs x (D, b, sigma_G,
+// r_smooth) of a tracer with bias model `model`; the network predicts the
+// difference from it (clr_surrogate.hip).
+double clr_surrogate_base(int model, const double *x);
+
 // Prediction of the surrogate network on device buffers (clr_mlp.hip).
 int clr_mlp_predict_device(clr_mlp *net, const float *dx, int n, float *dy, int precision, hipStream_t stream);

@@ -55,6 +55,16 @@ int shell_inputs(clr_ctx *c, int ipop, std::vector<float> &x, std::vector<int> &
 
 }  // namespace
 
+// ln norm of a shell if the density were exactly lognormal with the shell's
+// Gaussian variance sigma_G^2: <(1+delta)^b> = exp(b (b-1) sigma_G^2 / 2) for
+// the exponential bias model (0 for the other models, which have no closed
+// form).  The network predicts the difference from it.
+double clr_surrogate_base(int model, const double *x) {
+  if (model != CLR_BIAS_EXPONENTIAL) return 0.0;
+  const double b = x[1], sg = x[2];
+  return -0.5 * b * (b - 1) * sg * sg;
+}
+
 extern "C" {
 
 int clr_surrogate_inputs(clr_ctx *c, int ipop, int cap, float *x, int *shell, int *n_out) {
@@ -118,11 +128,17 @@ int clr_sample_sources_surrogate(clr_ctx *c, clr_mlp *net, int ipop, uint32_t se
     if (dy) (void)hipFree(dy);
     if (e != hipSuccess) return clr_fail(c, CLR_ERR_HIP, "surrogate inference failed");
   }
-  // per-shell norms (y0 = ln norm); shells without cells keep 1 (the edge
-  // fill of the norm table replaces bin 0 and the last bin anyway)
+  // per-shell norms: ln norm = y0 + the lognormal moment of the exponential
+  // model; shells without cells keep 1 (the edge fill of the norm table
+  // replaces bin 0 and the last bin anyway)
   std::vector<double> norm_s((size_t)np * nz, 1.0);
   for (int i = 0; i < n; i++) {
-    const double v = exp((double)y[(size_t)i * CLR_MLP_OUT]);
+    const double *xi = nullptr;
+    double xd[CLR_MLP_IN];
+    for (int k = 0; k < CLR_MLP_IN; k++) xd[k] = x[(size_t)i * CLR_MLP_IN + k];
+    xi = xd;
+    const double base = clr_surrogate_base(c->tr[CLR_TRACER_SRCS][p0 + owner[i]].model, xi);
+    const double v = exp((double)y[(size_t)i * CLR_MLP_OUT] + base);
     if (!(v > 0) || !std::isfinite(v)) return clr_fail(c, CLR_ERR_STATE, "surrogate predicted norm %g", v);
     norm_s[(size_t)owner[i] * nz + idx[i]] = v;
   }
