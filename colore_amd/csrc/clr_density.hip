@@ -193,7 +193,10 @@ __device__ inline double cell_z_f64(const ShellArgs &a, int ix, int iy, int iz) 
 }
 
 // M0: every tracer uses bias model 0 (no other model's code in the loop).
-template <int NT, bool M0>
+// MOM: the Gaussian moments pass of the surrogate: instead of bias sums, the
+// per-shell sums of G, G^2, G^3 and D^2 with G = D(r) delta_G (NT = 4, the
+// table's tracer columns unused); the grid is only read.
+template <int NT, bool M0, bool MOM = false>
 __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_shells(ShellArgs a,
                                                                                       double *__restrict__ partial) {
   constexpr int NQ = 2 + NT;
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
             for (int t = 0; t < NT; t++) comp[2 + t] = e0 ? a.bz0[t] : 1.f;
           }
         }
-        if (a.fused) {
+        if (!MOM && a.fused) {
           // the same float as lognormal_f32 of the fused count pass
           const TabPos tp{0, bn.tq, e0, e1};
           const float D = growth_at(tp, bn.C[0].x, bn.C[0].y, a.d1_last_f);
@@ -313,12 +316,21 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
           s = ind;
           v[0] = 1.0;
           v[1] = zz;
-          const float l2 = __builtin_amdgcn_logf(1.f + dl);
-          const float q = M0 ? 0.f : 1.44269504f * dl * __builtin_amdgcn_rcpf(1.f + dl);
+          if constexpr (MOM) {
+            static_assert(NT == 4, "moments pass: G, G^2, G^3, D^2");
+            const double D = (double)comp[0], G = D * (double)dl;
+            v[2] = G;
+            v[3] = G * G;
+            v[4] = G * G * G;
+            v[5] = D * D;
+          } else {
+            const float l2 = __builtin_amdgcn_logf(1.f + dl);
+            const float q = M0 ? 0.f : 1.44269504f * dl * __builtin_amdgcn_rcpf(1.f + dl);
 #pragma unroll
-          for (int t = 0; t < NT; t++)
-            v[2 + t] = (double)(M0 ? (dl > -1.f ? __builtin_amdgcn_exp2f(comp[2 + t] * l2) : 0.f)
-                                   : bias_sum_term(dl, l2, q, comp[2 + t], a.model[t]));
+            for (int t = 0; t < NT; t++)
+              v[2 + t] = (double)(M0 ? (dl > -1.f ? __builtin_amdgcn_exp2f(comp[2 + t] * l2) : 0.f)
+                                     : bias_sum_term(dl, l2, q, comp[2 + t], a.model[t]));
+          }
         }
       }
       const bool change = s >= 0 && s != cur;
@@ -448,11 +460,12 @@ __global__ __launch_bounds__(256) void k_norm_table(const double *__restrict__ s
   }
 }
 
-template <int NT, bool M0>
+template <int NT, bool M0, bool MOM = false>
 void launch_shells(dim3 g, dim3 b, size_t lds, hipStream_t s, const ShellArgs &a, double *partial) {
   if (lds > 64 * 1024)
-    (void)hipFuncSetAttribute((const void *)k_shells<NT, M0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_shells<NT, M0>), g, b, lds, s, a, partial);
+    (void)hipFuncSetAttribute((const void *)k_shells<NT, M0, MOM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  hipLaunchKernelGGL((k_shells<NT, M0, MOM>), g, b, lds, s, a, partial);
 }
 using ShellFn = void (*)(dim3, dim3, size_t, hipStream_t, const ShellArgs &, double *);
 #define CLR_SHELL_FNS(M0)                                                                                  \
@@ -658,20 +671,20 @@ int clr_launch_norms(clr_ctx *c, int lognormal_fused, double sigma2, int cap, in
   return CLR_OK;
 }
 
-// Cells and z sums per normalisation shell (the count and zsum columns of
-// collect_density_normalization_from_grid, density.c:1128-1211): the shell
-// pass without tracers; they depend only on the grid geometry and tables.
-int clr_launch_shell_geometry(clr_ctx *c) {
-  if (c->geo_nz > 0) return CLR_OK;
+namespace {
+
+// Arguments of a shell pass over the geometry only (no tracer columns): the
+// shell binning of compute_density_normalization (density.c:1128-1211).
+int shell_geo_args(clr_ctx *c, ShellArgs &a, ShellTabSrc &tsrc, int &nz) {
   if (!c->tables_set || !c->d1f) return clr_fail(c, CLR_ERR_STATE, "clr_upload_tables must be called first");
-  const int N = c->N, na = c->na;
+  const int na = c->na;
   const double zmax = c->zmax_half_box;
-  const int nz = (int)(zmax / 0.05) + 2;
+  nz = (int)(zmax / 0.05) + 2;
   if (nz > kNormMaxShells) return clr_fail(c, CLR_ERR_ARG, "%d redshift shells (at most %d)", nz, kNormMaxShells);
-  ShellArgs a{};
-  ShellTabSrc tsrc{};
+  a = ShellArgs{};
+  tsrc = ShellTabSrc{};
   a.g = c->dens;
-  a.N = N;
+  a.N = c->N;
   a.ngx = c->ngx;
   a.nz = nz;
   a.ntr = 0;
@@ -693,33 +706,76 @@ int clr_launch_shell_geometry(clr_ctx *c) {
   a.idz_f = (float)a.idz;
   a.d1_last_f = (float)c->h_bg_last[2];
   a.idr_f = (float)c->glob_idr;
-  const int nq = 2;
+  return CLR_OK;
+}
+
+// One geometry-type shell pass with ntr table columns, reduced to
+// fin[nz][2 + ntr] on the host.
+template <int NT, bool MOM>
+int shell_pass_to_host(clr_ctx *c, ShellArgs &a, const ShellTabSrc &tsrc, int nz, std::vector<double> &h) {
+  const int na = c->na, nq = 2 + NT;
   const size_t nvals = (size_t)nz * nq;
-  const int w = shell_tab_w(0);
+  const int w = shell_tab_w(NT);
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, ((size_t)kShellBlocks * nvals + nvals) * sizeof(double));
   if (rc) return rc;
   double *partial = c->red, *fin = c->red + (size_t)kShellBlocks * nvals;
   rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * w * sizeof(float4));
   if (rc) return rc;
   a.tab = (const float4 *)c->misc;
-  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, 0, na,
+  hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, NT, na,
                      (float4 *)c->misc);
-  kShellFns[1][0](dim3(kShellBlocks), dim3(kShellThreads), shell_lds_bytes<0>(nz), c->stream, a, partial);
+  launch_shells<NT, true, MOM>(dim3(kShellBlocks), dim3(kShellThreads), shell_lds_bytes<NT>(nz), c->stream, a,
+                               partial);
   hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,
                      (int)nvals, fin);
   CLR_HIP(c, hipGetLastError());
-  std::vector<double> h(nvals);
+  h.resize(nvals);
   CLR_HIP(c, hipMemcpyAsync(h.data(), fin, nvals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CLR_HIP(c, hipStreamSynchronize(c->stream));
+  return CLR_OK;
+}
+
+}  // namespace
+
+// Cells and z sums per normalisation shell (the count and zsum columns of
+// collect_density_normalization_from_grid, density.c:1128-1211): the shell
+// pass without tracers; they depend only on the grid geometry and tables.
+int clr_launch_shell_geometry(clr_ctx *c) {
+  if (c->geo_nz > 0) return CLR_OK;
+  ShellArgs a;
+  ShellTabSrc tsrc;
+  int nz = 0;
+  int rc = shell_geo_args(c, a, tsrc, nz);
+  if (rc) return rc;
+  std::vector<double> h;
+  rc = shell_pass_to_host<0, false>(c, a, tsrc, nz, h);
+  if (rc) return rc;
   c->geo_cnt.resize(nz);
   c->geo_zsum.resize(nz);
   for (int iz = 0; iz < nz; iz++) {
-    c->geo_cnt[iz] = h[(size_t)iz * nq];
-    c->geo_zsum[iz] = h[(size_t)iz * nq + 1];
+    c->geo_cnt[iz] = h[(size_t)iz * 2];
+    c->geo_zsum[iz] = h[(size_t)iz * 2 + 1];
   }
-  c->geo_zmax = zmax;
+  c->geo_zmax = c->zmax_half_box;
   c->geo_nz = nz;
   return CLR_OK;
+}
+
+// Gaussian moments per normalisation shell of the current (untransformed)
+// grid: with G = D(r) delta_G per cell, the cell count, the z sum and the sums
+// of G, G^2, G^3 and D^2 (out[nz][6]), with the shell pass's cell-to-shell
+// assignment.
+int clr_launch_shell_moments(clr_ctx *c, std::vector<double> &out) {
+  if (!c->have_fields || c->density_done)
+    return clr_fail(c, CLR_ERR_STATE, "the shell moments need the Gaussian field (fields done, density not transformed)");
+  ShellArgs a;
+  ShellTabSrc tsrc;
+  int nz = 0;
+  int rc = shell_geo_args(c, a, tsrc, nz);
+  if (rc) return rc;
+  a.ntr = 4;
+  for (int t = 0; t < 4; t++) tsrc.bzf[t] = c->d1f;   // columns unused by the moments pass
+  return shell_pass_to_host<4, true>(c, a, tsrc, nz, out);
 }
 
 // Norm tables of the given tracers from per-shell norms norm_s[t][nz] on the

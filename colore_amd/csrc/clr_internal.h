@@ -180,6 +180,7 @@ int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, c
 // the context; and norm tables built from given per-shell norms of tracers
 // (kind, ipop) as compute_density_normalization builds them from sums.
 int clr_launch_shell_geometry(clr_ctx *c);
+int clr_launch_shell_moments(clr_ctx *c, std::vector<double> &out);
 int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const int *ipops, const double *norm_s);
 
 // Graph-capturable stages (clr_ensemble.hip): buffers first (prepare, may

@@ -9,6 +9,7 @@
 // number-count error.
 #include "clr_internal.h"
 #include <math.h>
+#include <algorithm>
 #include <vector>
 
 namespace {
@@ -53,19 +54,75 @@ int shell_inputs(clr_ctx *c, int ipop, std::vector<float> &x, std::vector<int> &
   return CLR_OK;
 }
 
+// 16-point Gauss-Legendre nodes and weights on [-1, 1]: roots of P_16 by
+// Newton's iteration from x = cos(pi (i - 1/4) / (n + 1/2)), weights
+// 2 / ((1 - x^2) P_16'(x)^2).
+struct GaussLegendre16 {
+  double x[16], w[16];
+  GaussLegendre16() {
+    const int n = 16;
+    for (int i = 0; i < n / 2; i++) {
+      double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0;
+      for (int it = 0; it < 100; it++) {
+        double p0 = 1, p1 = 0;
+        for (int j = 1; j <= n; j++) {
+          const double p2 = p1;
+          p1 = p0;
+          p0 = ((2 * j - 1) * z * p1 - (j - 1) * p2) / j;
+        }
+        dp = n * (z * p0 - p1) / (z * z - 1);
+        const double dz = p0 / dp;
+        z -= dz;
+        if (fabs(dz) < 1e-16) break;
+      }
+      x[i] = -z;
+      x[n - 1 - i] = z;
+      w[i] = w[n - 1 - i] = 2 / ((1 - z * z) * dp * dp);
+    }
+  }
+};
+
 }  // namespace
 
-// ln norm of a shell if the density were exactly lognormal with the shell's
-// Gaussian variance sigma_G^2: <(1+delta)^b> = exp(b (b-1) sigma_G^2 / 2) for
-// the exponential bias model (0 for the other models, which have no closed
-// form).  The network predicts the difference from it.
-double clr_surrogate_base(int model, const double *x) {
-  if (model != CLR_BIAS_EXPONENTIAL) return 0.0;
-  const double b = x[1], sg = x[2];
-  return -0.5 * b * (b - 1) * sg * sg;
-}
+// ln norm of a shell if its density were exactly lognormal with the shell's
+// Gaussian variance sigma_G^2: -ln <bias_model(delta)> over delta =
+// exp(G - sigma_G^2/2) - 1, G ~ N(0, sigma_G^2) (the distribution the
+// lognormal transform gives one cell).  The network predicts the difference
+// from it.
+double clr_surrogate_base(int model, const double *x) { return -log(clr_bias_moment(model, x[1], x[2])); }
 
 extern "C" {
+
+double clr_bias_moment(int model, double b, double sigma_g) {
+  const double s = fabs(sigma_g);
+  if (!(s > 0)) return 1.0;
+  if (model != CLR_BIAS_EXP_TRUNCATED && model != CLR_BIAS_TRUNCATED) return exp(0.5 * b * (b - 1) * s * s);
+  static const GaussLegendre16 gl;
+  // t = G / sigma_G over [-12, 12], split where bias_model has a kink: the
+  // truncation 1 + b delta = 0, or delta = 0 for the exp-truncated model
+  double cut[4] = {-12.0, 12.0, 0, 0};
+  int nc = 2;
+  if (model == CLR_BIAS_TRUNCATED && (b > 1 || b < 0)) cut[nc++] = (log(1 - 1 / b) + 0.5 * s * s) / s;
+  if (model == CLR_BIAS_EXP_TRUNCATED) cut[nc++] = 0.5 * s;
+  std::sort(cut, cut + nc);
+  const double inv_sqrt2pi = 0.3989422804014327;
+  double acc = 0;
+  for (int c = 0; c + 1 < nc; c++) {
+    const double a0 = std::max(cut[c], -12.0), a1 = std::min(cut[c + 1], 12.0);
+    if (!(a1 > a0)) continue;
+    const int m = 24;
+    const double h = (a1 - a0) / m;
+    for (int j = 0; j < m; j++) {
+      const double mid = a0 + (j + 0.5) * h;
+      for (int q = 0; q < 16; q++) {
+        const double t = mid + 0.5 * h * gl.x[q];
+        const double d = expm1(s * t - 0.5 * s * s);
+        acc += 0.5 * h * gl.w[q] * inv_sqrt2pi * exp(-0.5 * t * t) * clr::bias_model_f64(d, b, model);
+      }
+    }
+  }
+  return acc;
+}
 
 int clr_surrogate_inputs(clr_ctx *c, int ipop, int cap, float *x, int *shell, int *n_out) {
   if (!c) return CLR_ERR_ARG;

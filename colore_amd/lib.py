@@ -19,6 +19,7 @@ CLR_TRACER_SRCS, CLR_TRACER_IMAP, CLR_TRACER_CSTM = 0, 1, 2
 CLR_BIAS_EXPONENTIAL, CLR_BIAS_EXP_TRUNCATED, CLR_BIAS_TRUNCATED = 0, 1, 2
 CLR_DENS_LOGNORMAL, CLR_DENS_CLIP = 0, 3
 CLR_RNG_PHILOX, CLR_RNG_INJECTED = 0, 1
+CLR_CL_KAPPA, CLR_CL_ISW = 0, 2
 STAGES = ["gen", "fft_yz", "fft_x", "density", "norm", "srcs", "kappa", "isw", "imap", "src_pass", "src_write",
           "fft_phi", "shells", "beams", "cstm", "skewers"]
 
@@ -105,6 +106,7 @@ def load_amd():
     _sig(lib, "clr_mlp_grad", ci, [_vp, _f, _f, ci, _f, _d])
     _sig(lib, "clr_mlp_train", ci, [_vp, _f, _f, ci, ci, ctypes.c_float, ctypes.c_float, _d])
     _sig(lib, "clr_mlp_predict", ci, [_vp, _f, ci, _f, ci])
+    _sig(lib, "clr_bias_moment", ctypes.c_double, [ci, ctypes.c_double, ctypes.c_double])
     _sig(lib, "clr_ensemble_create", ci, [ctypes.POINTER(_vp), ci, ctypes.POINTER(_vp), ci, ci, _d,
                                           ctypes.POINTER(ctypes.c_longlong), ci])
     _sig(lib, "clr_ensemble_destroy", None, [_vp])

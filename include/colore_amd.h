@@ -293,9 +293,11 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
 /* ---- learned surrogate of the per-shell normalisation ------------------
  * No counterpart in the reference: a 4 -> 64 -> 64 -> 2 perceptron (tanh
  * hidden layers) that predicts, from x = (D(z), b(z), sigma_G(z), r_smooth)
- * of a shell, y0 = ln norm (count/sum of compute_density_normalization,
- * density.c:1272-1355) and y1 = ln <bias_model>, the expected count per cell
- * of the un-normalised sampler in units of n dx^3.  FP32 weights; trained
+ * of a shell, y0 = the part of ln norm (count/sum of
+ * compute_density_normalization, density.c:1272-1355) that the one-cell
+ * lognormal expectation -ln clr_bias_moment(model, b, sigma_G) misses, and
+ * y1 = -y0 (the same for ln <bias_model>, the expected count per cell of the
+ * un-normalised sampler in units of n dx^3).  FP32 weights; trained
  * on the device by its own forward/backward kernels (mean squared error of
  * the standardised outputs, full-batch Adam); inference in FP32 or with BF16
  * MFMA (FP32 accumulate).  Parameters: W1[64][4], b1[64], W2[64][64], b2[64],
@@ -324,6 +326,12 @@ int clr_mlp_train(clr_mlp *net, const float *x, const float *y, int n, int steps
                   float lr_final, double *loss_out);
 /* y (n x 2, raw units) from x (n x 4) with CLR_MLP_FP32 or CLR_MLP_BF16_MFMA. */
 int clr_mlp_predict(clr_mlp *net, const float *x, int n, float *y, int precision);
+/* <bias_model(delta)> (common.h:414-431) over the one-cell distribution of
+ * the lognormal transform, delta = exp(G - sigma_g^2/2) - 1 with G ~
+ * N(0, sigma_g^2): exp(b(b-1) sigma_g^2/2) for the exponential model, 16-point
+ * Gauss-Legendre quadrature split at the model's kink for the two truncated
+ * ones.  Host only (no device call). */
+double clr_bias_moment(int model, double b, double sigma_g);
 
 /* Inputs of the surrogate for source population ipop, one row of
  * CLR_MLP_IN floats per normalisation shell with cells (bins 1 .. nz-1 of
@@ -335,8 +343,9 @@ int clr_mlp_predict(clr_mlp *net, const float *x, int n, float *y, int precision
 int clr_surrogate_inputs(clr_ctx *ctx, int ipop, int cap, float *x, int *shell, int *n_out);
 /* Surrogate-fused sampling: clr_sample_sources (same catalogues, streams
  * and order) for population ipop (-1: all) right after clr_gaussian_fields,
- * without clr_shell_norms.  The network predicts each shell's norm (y0 = ln
- * norm, BF16 MFMA or FP32 inference); the norm tables are built from them as
+ * without clr_shell_norms.  The network predicts each shell's norm (ln norm
+ * = y0 - ln clr_bias_moment, BF16 MFMA or FP32 inference); the norm tables
+ * are built from them as
  * compute_density_normalization builds them from the shell sums
  * (density.c:1272-1355); the count pass reads the Gaussian field and applies
  * the lognormal transform per cell (the same FP32 value the shell pass of

@@ -29,3 +29,17 @@ def make_run(name, n_grid, workdir, **kw):
         path = inputs.write_config(name, d, n_grid=n_grid, **kw)
         _RUNS[key] = (path, lib.Run(path))
     return _RUNS[key]
+
+
+def report(name, obj):
+    """Writes a JSON report into $CLR_REPORT_DIR (default gpurun_out/ under the
+    repository root when that folder exists) and prints it."""
+    import json
+    d = os.environ.get("CLR_REPORT_DIR")
+    if not d:
+        d = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else None
+    text = json.dumps(obj, indent=1)
+    print(text)
+    if d:
+        with open(os.path.join(d, name), "w") as f:
+            f.write(text + "\n")

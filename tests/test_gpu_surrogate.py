@@ -16,21 +16,6 @@ pytestmark = pytest.mark.gpu
 BLOCKS = [(0, 256), (256, 320), (320, 4416), (4416, 4480), (4480, 4608), (4608, 4610)]
 
 
-def _report(name, obj):
-    """Writes a JSON report into $CLR_REPORT_DIR (default gpurun_out/ when it exists)."""
-    import json
-    import os
-    d = os.environ.get("CLR_REPORT_DIR")
-    if not d:
-        root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-        d = os.path.join(root, "gpurun_out") if os.path.isdir(os.path.join(root, "gpurun_out")) else None
-    text = json.dumps(obj, indent=1)
-    print(text)
-    if d:
-        with open(os.path.join(d, name), "w") as f:
-            f.write(text + "\n")
-
-
 def _case(seed=0, n=300):
     rng = np.random.default_rng(seed)
     P = rng.normal(0, 0.3, 4610).astype(np.float32)
@@ -181,7 +166,7 @@ def test_surrogate_gain_over_method_without_network(trained, workdir):
     exact two-pass reduction, beat using no normalisation by > 10x (cell-
     weighted mean of |norm/norm_exact - 1|) and are not worse than the
     lognormal-moment formula the network corrects; the errors are reported."""
-    from conftest import make_run
+    from conftest import make_run, report
     from colore_amd import lib, surrogate
     net = trained[0]
     N = 256
@@ -195,7 +180,7 @@ def test_surrogate_gain_over_method_without_network(trained, workdir):
     err, s2 = surrogate.shell_errors(ctx, run, net, int(run.scalar("seed")), npop)
     summ = surrogate.summarize(err)
     print(summ)
-    _report("surrogate_gain_C2_256.txt", {"workload": "held-out 256^3 realisation, z in [0.05, 1.2], seed 2002, "
+    report("surrogate_gain_C2_256.txt", {"workload": "held-out 256^3 realisation, z in [0.05, 1.2], seed 2002, "
                                           "populations b = 1+z and b = 2 (exponential model)",
                                           "per_shell_norm_error_vs_exact_two_pass": summ,
                                           "shells": int(len(err["network"]))})

@@ -527,3 +527,52 @@ def test_limber_cl_against_quadrature(tmp_path, typ):
             assert abs(got - want) <= 2e-5 * abs(want) + 1e-30, (l, a0, af, b0, bf, got, want)
             if typ == 0 and (a0, af) == (0, chis):
                 assert got > 0
+
+
+def _bias_model_py(d, b, model):
+    """bias_model (common.h:414-431) in Python floats."""
+    if d <= -1:
+        return 0.0
+    if model == 1:
+        return math.exp(b * d / (1 + d)) if d < 0 else 1 + b * d
+    if model == 2:
+        return 1 + b * d if 1 + b * d > 0 else 0.0
+    return (1 + d) ** b
+
+
+def test_bias_moment_against_quadrature_and_closed_forms():
+    """clr_bias_moment (host code of the device library, no device call): the
+    exponential model against exp(b(b-1) s^2/2) by construction and against
+    scipy's quad; both truncated models against quad of the same expectation
+    over G ~ N(0, s^2) (kink points given to quad); the truncated model also
+    against its closed form (1-b) Phi(-g0/s) + b Phi((s^2-g0)/s), g0 =
+    ln(1-1/b) + s^2/2, for b > 1, and = 1 exactly when 0 <= b <= 1 (no
+    truncation, <delta> = 0)."""
+    from scipy import stats
+    amd = lib.load_amd()
+
+    def quad(model, b, s):
+        f = lambda t: stats.norm.pdf(t) * _bias_model_py(math.expm1(s * t - 0.5 * s * s), b, model)
+        pts = []
+        if model == 2 and (b > 1 or b < 0):
+            pts = [(math.log(1 - 1 / b) + 0.5 * s * s) / s]
+        if model == 1:
+            pts = [0.5 * s]
+        return integrate.quad(f, -14, 14, points=pts or None, limit=400, epsabs=0, epsrel=1e-13)[0]
+
+    for model in (0, 1, 2):
+        for b in (-0.5, 0.3, 1.0, 1.5, 2.5, 4.0):
+            if model == 1 and b < 0:
+                continue   # exp(b delta/(1+delta)) diverges as delta -> -1
+            for s in (0.05, 0.3, 0.8, 1.5):
+                got = amd.clr_bias_moment(model, b, s)
+                assert got == pytest.approx(quad(model, b, s), rel=1e-10), (model, b, s)
+    for b in (1.2, 2.5, 4.0):
+        for s in (0.2, 0.9):
+            g0 = math.log(1 - 1 / b) + 0.5 * s * s
+            cf = (1 - b) * stats.norm.sf(g0 / s) + b * stats.norm.sf((g0 - s * s) / s)
+            assert amd.clr_bias_moment(2, b, s) == pytest.approx(cf, rel=1e-12)
+    for b in (0.0, 0.5, 1.0):
+        assert amd.clr_bias_moment(2, b, 0.7) == pytest.approx(1.0, abs=1e-13)
+    assert amd.clr_bias_moment(0, 2.0, 0.6) == pytest.approx(math.exp(0.36), rel=1e-15)
+    assert amd.clr_bias_moment(1, 2.0, 0.0) == 1.0
