@@ -16,6 +16,7 @@ against 8 TB/s.  `cpu_baseline` times the FP64 CPU oracle on the same
 configuration at 512^3 on the host cores.
 """
 import argparse
+import math
 import json
 import os
 import sys
@@ -81,6 +82,34 @@ def traffic_per_launch(kernel, config, N):
             if line.startswith(key):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def time_path_to_kappa(run, ctx, kap, step, nsteps, nsrc, npop):
+    """Gaussian -> lognormal -> sources -> kappa: the timed step followed by
+    the convergence maps of the workload's source planes (kappa_get_beam_
+    properties, the map copied to the host at the end of the call), wall time
+    per realisation over nsteps; B_alg adds one read of phi inside the
+    lightcone sphere, (pi/6) 4p bytes per cell, and the maps' bytes."""
+    import numpy as np
+    from colore_amd import lib
+    host = lib.load_host()
+    N = int(run.scalar("n_grid"))
+    zs = [z for z in kap["z_out"] if z <= run.scalar("z_max")]
+    r_src = np.array([host.clr_run_r_of_z(run.h, z) for z in zs])
+    nside = kap["nside"]
+    ctx.kappa(nside, r_src)   # warm: the tidal records' buffer
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        step(False)
+        ctx.kappa(nside, r_src)
+        kms.append(ctx.stage_times()["kappa"])
+    dt = (time.perf_counter() - t0) / nsteps
+    cells = float(N) ** 3
+    p = (N + 2.0) / N
+    b_alg = 36 * p + 20.0 * nsrc / cells + (math.pi / 6) * 4 * p + len(zs) * 12 * nside * nside * 4 / cells
+    return {"ms_per_step": dt * 1e3, "cells_per_s": cells / dt, "kappa_ms": float(np.median(kms)), "nside": nside,
+            "planes": len(zs), "b_alg_per_cell": b_alg, "achieved": cells / dt * b_alg, "frac": cells / dt * b_alg / HBM_PEAK}
 
 
 def time_maps(run, ctx, config, sigma2):
@@ -264,9 +293,10 @@ def bench_surrogate(args):
     surrogate workload (2048^3, z in [0.05, 2.0], the two populations of the
     512^3 precision workload, seed 5005): the network is trained first (its
     own device kernels, shells of 256^3 .. 1024^3 realisations); then per step
-    the two-pass sampling (shell sums + sources) and the fused single pass
-    (inference + norm tables + sources) are timed on the same Gaussian field,
-    and every shell's norm error against the exact reduction is reported."""
+    the two-pass sampling (shell sums + sources) and the fused path (the
+    read-only shell-moments pass, closure + inference, norm tables, sources)
+    are timed on the same Gaussian field, and every shell's norm error
+    against the exact reduction is reported."""
     from colore_amd import inputs, lib, surrogate
     t0 = time.perf_counter()
     x, y = surrogate.training_set()
@@ -286,12 +316,16 @@ def bench_surrogate(args):
         ctx.fft_autotune(3)
     two, fused = [], []
     for it in range(args.warmup + args.steps):
+        # each path starts from the same finished Gaussian field (phi's passes
+        # done, so neither waits for them)
         s2 = ctx.gaussian_fields(seed, rs, True)
+        ctx.synchronize()
         t = time.perf_counter()
         ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=npop)
         n2 = ctx.sample_sources(-1, seed, 2)
         t2 = time.perf_counter() - t
         s2 = ctx.gaussian_fields(seed, rs, True)
+        ctx.synchronize()
         t = time.perf_counter()
         nf = ctx.sample_sources_surrogate(net, -1, seed, 2)
         tf = time.perf_counter() - t
@@ -332,6 +366,8 @@ def main():
                     help="only run a 1 GiB device copy (PMC counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default FFT tile shapes (no autotuning before the timed steps)")
+    ap.add_argument("--kappa-steps", type=int, default=3,
+                    help="realisations timed through the kappa maps after the main steps (0: none)")
     ap.add_argument("--surrogate", action="store_true",
                     help="surrogate-fused sampling against the two-pass path (learned-surrogate workload)")
     args = ap.parse_args()
@@ -427,6 +463,9 @@ def main():
         "kernels": {k: {"ms": kt[k], "GB/s": kb[k] / (kt[k] * 1e-3) / 1e9 if kt[k] > 0 else None}
                     for k in kt},
     }
+    kap = inputs.CONFIGS[args.config].get("kappa")
+    if kap and args.kappa_steps > 0:
+        out["path_to_kappa"] = time_path_to_kappa(run, ctx, kap, step, args.kappa_steps, nsrc_tot, npop)
     if args.maps:
         out["maps"] = time_maps(run, ctx, args.config, s2)
     ctx.close()

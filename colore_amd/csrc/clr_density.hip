@@ -193,10 +193,7 @@ __device__ inline double cell_z_f64(const ShellArgs &a, int ix, int iy, int iz) 
 }
 
 // M0: every tracer uses bias model 0 (no other model's code in the loop).
-// MOM: the Gaussian moments pass of the surrogate: instead of bias sums, the
-// per-shell sums of G, G^2, G^3 and D^2 with G = D(r) delta_G (NT = 4, the
-// table's tracer columns unused); the grid is only read.
-template <int NT, bool M0, bool MOM = false>
+template <int NT, bool M0>
 __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_shells(ShellArgs a,
                                                                                       double *__restrict__ partial) {
   constexpr int NQ = 2 + NT;
@@ -294,7 +291,7 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
             for (int t = 0; t < NT; t++) comp[2 + t] = e0 ? a.bz0[t] : 1.f;
           }
         }
-        if (!MOM && a.fused) {
+        if (a.fused) {
           // the same float as lognormal_f32 of the fused count pass
           const TabPos tp{0, bn.tq, e0, e1};
           const float D = growth_at(tp, bn.C[0].x, bn.C[0].y, a.d1_last_f);
@@ -316,21 +313,12 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
           s = ind;
           v[0] = 1.0;
           v[1] = zz;
-          if constexpr (MOM) {
-            static_assert(NT == 4, "moments pass: G, G^2, G^3, D^2");
-            const double D = (double)comp[0], G = D * (double)dl;
-            v[2] = G;
-            v[3] = G * G;
-            v[4] = G * G * G;
-            v[5] = D * D;
-          } else {
-            const float l2 = __builtin_amdgcn_logf(1.f + dl);
-            const float q = M0 ? 0.f : 1.44269504f * dl * __builtin_amdgcn_rcpf(1.f + dl);
+          const float l2 = __builtin_amdgcn_logf(1.f + dl);
+          const float q = M0 ? 0.f : 1.44269504f * dl * __builtin_amdgcn_rcpf(1.f + dl);
 #pragma unroll
-            for (int t = 0; t < NT; t++)
-              v[2 + t] = (double)(M0 ? (dl > -1.f ? __builtin_amdgcn_exp2f(comp[2 + t] * l2) : 0.f)
-                                     : bias_sum_term(dl, l2, q, comp[2 + t], a.model[t]));
-          }
+          for (int t = 0; t < NT; t++)
+            v[2 + t] = (double)(M0 ? (dl > -1.f ? __builtin_amdgcn_exp2f(comp[2 + t] * l2) : 0.f)
+                                   : bias_sum_term(dl, l2, q, comp[2 + t], a.model[t]));
         }
       }
       const bool change = s >= 0 && s != cur;
@@ -374,6 +362,153 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
     double v = 0;
     for (int w = 0; w < kWaves; w++) v += acc[(size_t)w * nz * NQ + i];
     out[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_shell_moments: the surrogate's read-only pass over the Gaussian grid.
+// A wavefront takes 512 consecutive cells of a row, 8 per lane (two float4
+// loads: the wavefront reads 2 KB contiguous).  Per cell: FP32 radius, the
+// shell from the FP32 edge radii (binary search for a lane's first cell,
+// then a step at a crossing), D(r) from the packed FP32 growth table and
+// G = D delta_G.  A lane sums its cells of the current shell in FP32; when
+// any lane's shell changes the wavefront combines the lanes shell by shell
+// (FP32 DPP sums of at most 512 values) and lane 0 adds them to the
+// wavefront's FP64 accumulators in LDS.  Per block the wavefronts' sums go
+// to partial[block][nz][kMomQ], reduced in fixed order by k_shell_final.
+// ---------------------------------------------------------------------------
+constexpr int kMomQ = 6;            // count, G, G^2, G^3, G^4, D^2
+constexpr int kMomBlocks = 2048;
+constexpr int kMomMaxShells = 256;
+
+struct MomArgs {
+  const float *g;
+  const float2 *dtab;   // per r-bin (D_i, D_{i+1} - D_i)
+  const float *edge;    // nz FP32 edge radii
+  int N, ngx, nz, na;
+  float dxf, ox, oy, oz, idr_f, d_last;
+};
+
+__global__ void k_pack_growth(const float *__restrict__ d1f, int na, float2 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  const int j = i + 1 < na ? i + 1 : i;
+  out[i] = make_float2(d1f[i], d1f[j] - d1f[i]);
+}
+
+template <int CTRL, int RM>
+__device__ inline float dpp_getf(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xF, true));
+}
+
+// Sum over the 64 lanes of a wavefront (all lanes active), broadcast.
+__device__ inline float wave_sum_f(float x) {
+  x += dpp_getf<0xB1, 0xF>(x);
+  x += dpp_getf<0x4E, 0xF>(x);
+  x += dpp_getf<0x141, 0xF>(x);
+  x += dpp_getf<0x140, 0xF>(x);
+  x += dpp_getf<0x142, 0xA>(x);
+  x += dpp_getf<0x143, 0xC>(x);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+__global__ __launch_bounds__(kShellThreads) void k_shell_moments(MomArgs a, double *__restrict__ partial) {
+  extern __shared__ double macc[];   // [wave][nz][kMomQ], then nz FP32 edges
+  const int nz = a.nz, N = a.N;
+  float *ledge = (float *)(macc + (size_t)kWaves * nz * kMomQ);
+  for (int i = threadIdx.x; i < kWaves * nz * kMomQ; i += blockDim.x) macc[i] = 0;
+  for (int i = threadIdx.x; i < nz; i += blockDim.x) ledge[i] = a.edge[i];
+  __syncthreads();
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  double *myacc = macc + (size_t)wave * nz * kMomQ;
+  const int segs = (N + 511) / 512;
+  const long nwork = (long)N * N * segs;
+  const long nw = (long)gridDim.x * kWaves;
+  for (long w = (long)blockIdx.x * kWaves + wave; w < nwork; w += nw) {
+    const int seg = (int)(w % segs);
+    const long row = w / segs;
+    const int iz = (int)(row / N), iy = (int)(row % N);
+    const int ix0 = seg * 512 + lane * 8;
+    const float y = coord_f32(iy, a.dxf, a.oy), z = coord_f32(iz, a.dxf, a.oz);
+    const float rho2 = __builtin_fmaf(y, y, z * z);
+    const float *rp = a.g + (size_t)row * a.ngx;
+    float v[8];
+    if (ix0 + 8 <= N) {
+      const float4 q0 = *(const float4 *)(rp + ix0), q1 = *(const float4 *)(rp + ix0 + 4);
+      v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
+      v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] = ix0 + j < N ? rp[ix0 + j] : 0.f;
+    }
+    int key = -1, s = 0;
+    float acc[kMomQ];
+#pragma unroll
+    for (int q = 0; q < kMomQ; q++) acc[q] = 0.f;
+    auto flush = [&]() {
+      unsigned long long mask = __ballot(key >= 1 && key < nz);
+      while (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        const int sl = __builtin_amdgcn_readlane(key, leader);
+        const bool mine = key == sl;
+        const unsigned long long mm = __ballot(mine);
+#pragma unroll
+        for (int q = 0; q < kMomQ; q++) {
+          const float t = wave_sum_f(mine ? acc[q] : 0.f);
+          if (lane == 0) myacc[sl * kMomQ + q] += (double)t;
+        }
+        mask &= ~mm;
+      }
+    };
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int ix = ix0 + j;
+      const float x = coord_f32(ix, a.dxf, a.ox);
+      const float r = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, rho2));
+      if (j == 0) {
+        // largest s in [1, nz] with edge[s-1] <= r (s = nz: past the last shell)
+        int lo = 1, hi = nz;
+        while (hi > lo) {
+          const int m = (lo + hi + 1) / 2;
+          if (ledge[m - 1] <= r) lo = m; else hi = m - 1;
+        }
+        s = lo;
+      } else {
+        while (s < nz && ledge[s] <= r) s++;
+        while (s > 1 && ledge[s - 1] > r) s--;
+      }
+      const int sk = ix < N ? s : nz;
+      if (__ballot(sk != key)) {
+        flush();
+#pragma unroll
+        for (int q = 0; q < kMomQ; q++) acc[q] = 0.f;
+        key = sk;
+      }
+      if (sk < nz) {
+        const float t = r * a.idr_f;
+        const int i0 = (int)t;
+        float D = a.d_last;
+        if (i0 < a.na - 1) {
+          const float2 dd = a.dtab[i0];
+          D = __builtin_fmaf(t - (float)i0, dd.y, dd.x);
+        }
+        const float G = D * v[j], G2 = G * G;
+        acc[0] += 1.f;
+        acc[1] += G;
+        acc[2] += G2;
+        acc[3] += G2 * G;
+        acc[4] += G2 * G2;
+        acc[5] += D * D;
+      }
+    }
+    flush();
+  }
+  __syncthreads();
+  double *out = partial + (size_t)blockIdx.x * nz * kMomQ;
+  for (int i = threadIdx.x; i < nz * kMomQ; i += blockDim.x) {
+    double t = 0;
+    for (int w2 = 0; w2 < kWaves; w2++) t += macc[(size_t)w2 * nz * kMomQ + i];
+    out[i] = t;
   }
 }
 
@@ -460,12 +595,11 @@ __global__ __launch_bounds__(256) void k_norm_table(const double *__restrict__ s
   }
 }
 
-template <int NT, bool M0, bool MOM = false>
+template <int NT, bool M0>
 void launch_shells(dim3 g, dim3 b, size_t lds, hipStream_t s, const ShellArgs &a, double *partial) {
   if (lds > 64 * 1024)
-    (void)hipFuncSetAttribute((const void *)k_shells<NT, M0, MOM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-  hipLaunchKernelGGL((k_shells<NT, M0, MOM>), g, b, lds, s, a, partial);
+    (void)hipFuncSetAttribute((const void *)k_shells<NT, M0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_shells<NT, M0>), g, b, lds, s, a, partial);
 }
 using ShellFn = void (*)(dim3, dim3, size_t, hipStream_t, const ShellArgs &, double *);
 #define CLR_SHELL_FNS(M0)                                                                                  \
@@ -711,7 +845,7 @@ int shell_geo_args(clr_ctx *c, ShellArgs &a, ShellTabSrc &tsrc, int &nz) {
 
 // One geometry-type shell pass with ntr table columns, reduced to
 // fin[nz][2 + ntr] on the host.
-template <int NT, bool MOM>
+template <int NT>
 int shell_pass_to_host(clr_ctx *c, ShellArgs &a, const ShellTabSrc &tsrc, int nz, std::vector<double> &h) {
   const int na = c->na, nq = 2 + NT;
   const size_t nvals = (size_t)nz * nq;
@@ -724,8 +858,7 @@ int shell_pass_to_host(clr_ctx *c, ShellArgs &a, const ShellTabSrc &tsrc, int nz
   a.tab = (const float4 *)c->misc;
   hipLaunchKernelGGL(k_pack_shell_table, dim3((na + 255) / 256), dim3(256), 0, c->stream, tsrc, NT, na,
                      (float4 *)c->misc);
-  launch_shells<NT, true, MOM>(dim3(kShellBlocks), dim3(kShellThreads), shell_lds_bytes<NT>(nz), c->stream, a,
-                               partial);
+  launch_shells<NT, true>(dim3(kShellBlocks), dim3(kShellThreads), shell_lds_bytes<NT>(nz), c->stream, a, partial);
   hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kShellBlocks,
                      (int)nvals, fin);
   CLR_HIP(c, hipGetLastError());
@@ -748,7 +881,7 @@ int clr_launch_shell_geometry(clr_ctx *c) {
   int rc = shell_geo_args(c, a, tsrc, nz);
   if (rc) return rc;
   std::vector<double> h;
-  rc = shell_pass_to_host<0, false>(c, a, tsrc, nz, h);
+  rc = shell_pass_to_host<0>(c, a, tsrc, nz, h);
   if (rc) return rc;
   c->geo_cnt.resize(nz);
   c->geo_zsum.resize(nz);
@@ -762,20 +895,72 @@ int clr_launch_shell_geometry(clr_ctx *c) {
 }
 
 // Gaussian moments per normalisation shell of the current (untransformed)
-// grid: with G = D(r) delta_G per cell, the cell count, the z sum and the sums
-// of G, G^2, G^3 and D^2 (out[nz][6]), with the shell pass's cell-to-shell
-// assignment.
+// grid, for the surrogate: with G = D(r) delta_G per cell, out[nz][6] = the
+// cell count and the sums of G, G^2, G^3, G^4 and D^2 (k_shell_moments).
+// The shell of a cell is the one its FP32 radius falls in between the FP32
+// radii of the z edges (nz shells of compute_density_normalization,
+// density.c:1233-1244); D(r) from the FP32 growth table.
 int clr_launch_shell_moments(clr_ctx *c, std::vector<double> &out) {
   if (!c->have_fields || c->density_done)
     return clr_fail(c, CLR_ERR_STATE, "the shell moments need the Gaussian field (fields done, density not transformed)");
-  ShellArgs a;
-  ShellTabSrc tsrc;
-  int nz = 0;
-  int rc = shell_geo_args(c, a, tsrc, nz);
+  if (!c->tables_set || !c->d1f) return clr_fail(c, CLR_ERR_STATE, "clr_upload_tables must be called first");
+  const int na = c->na;
+  const double zmax = c->zmax_half_box;
+  const int nz = (int)(zmax / 0.05) + 2;
+  if (nz > kMomMaxShells) return clr_fail(c, CLR_ERR_ARG, "%d redshift shells (the moments pass takes %d)", nz, kMomMaxShells);
+  const double idz = (nz - 2) / zmax;
+  // E[k] = r(z = k / idz), k = 0 .. nz-1: shell s holds E[s-1] <= r < E[s]
+  std::vector<float> hedge(nz);
+  const std::vector<double> &zz = c->h_z, &rr = c->h_r;
+  for (int k = 0; k < nz; k++) {
+    const double z = k / idz;
+    double r;
+    if (z <= zz[0]) r = rr[0];
+    else if (z >= zz[na - 1]) r = rr[na - 1] * 4;   // past the table: no cell reaches it
+    else {
+      int lo = 0, hi = na - 1;
+      while (hi > lo + 1) {
+        const int m = (lo + hi) / 2;
+        if (zz[m] > z) hi = m; else lo = m;
+      }
+      r = rr[lo] + (z - zz[lo]) / (zz[hi] - zz[lo]) * (rr[hi] - rr[lo]);
+    }
+    hedge[k] = (float)r;
+  }
+  hedge[0] = 0.f;
+  const size_t nvals = (size_t)nz * kMomQ;
+  int rc = clr_ensure(c, (void **)&c->red, &c->red_n, ((size_t)kMomBlocks * nvals + nvals) * sizeof(double));
   if (rc) return rc;
-  a.ntr = 4;
-  for (int t = 0; t < 4; t++) tsrc.bzf[t] = c->d1f;   // columns unused by the moments pass
-  return shell_pass_to_host<4, true>(c, a, tsrc, nz, out);
+  double *partial = c->red, *fin = c->red + (size_t)kMomBlocks * nvals;
+  rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * sizeof(float2) + (size_t)nz * sizeof(float));
+  if (rc) return rc;
+  float2 *dtab = (float2 *)c->misc;
+  float *dedge = (float *)(dtab + na);
+  CLR_HIP(c, hipMemcpyAsync(dedge, hedge.data(), nz * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_pack_growth, dim3((na + 255) / 256), dim3(256), 0, c->stream, c->d1f, na, dtab);
+  MomArgs a{};
+  a.g = c->dens;
+  a.dtab = dtab;
+  a.edge = dedge;
+  a.N = c->N;
+  a.ngx = c->ngx;
+  a.nz = nz;
+  a.na = na;
+  a.dxf = (float)c->dx;
+  a.ox = (float)c->obs[0];
+  a.oy = (float)c->obs[1];
+  a.oz = (float)c->obs[2];
+  a.idr_f = (float)c->glob_idr;
+  a.d_last = (float)c->h_bg_last[2];
+  const size_t lds = (size_t)kWaves * nz * kMomQ * sizeof(double) + (size_t)nz * sizeof(float);
+  hipLaunchKernelGGL(k_shell_moments, dim3(kMomBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
+  hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kMomBlocks, (int)nvals,
+                     fin);
+  CLR_HIP(c, hipGetLastError());
+  out.resize(nvals);
+  CLR_HIP(c, hipMemcpyAsync(out.data(), fin, nvals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CLR_HIP(c, hipStreamSynchronize(c->stream));
+  return CLR_OK;
 }
 
 // Norm tables of the given tracers from per-shell norms norm_s[t][nz] on the

@@ -121,6 +121,8 @@ struct clr_ctx {
   std::vector<double> geo_cnt, geo_zsum;
   double r_smooth = 0;      // smoothing length of the last clr_gaussian_fields
   // graph-captured kappa: kernel arguments, radial weights + map buffer
+  void *tidal = nullptr;     // kappa's tidal records, 6 floats per cell (clr_los.hip)
+  size_t tidal_n = 0;
   std::string los_plan;
   void *los_buf = nullptr;
   size_t los_buf_n = 0;
@@ -203,7 +205,6 @@ int clr_fetch_kappa(clr_ctx *c, float *out);
 // ln norm of the lognormal moment for surrogate inputs x (D, b, sigma_G,
 // r_smooth) of a tracer with bias model `model`; the network predicts the
 // difference from it (clr_surrogate.hip).
-double clr_surrogate_base(int model, const double *x);
 
 // Prediction of the surrogate network on device buffers (clr_mlp.hip).
 int clr_mlp_predict_device(clr_mlp *net, const float *dx, int n, float *dy, int precision, hipStream_t stream);

@@ -103,6 +103,85 @@ __global__ __launch_bounds__(256) void k_los(LosArgs a, const double *__restrict
   }
 }
 
+// The transverse-Hessian inputs of get_element (beaming.c:79-116) for every
+// cell once: t6[cell][6] = (xx, xy, xz, yy, yz, zz) second differences of
+// phi in FP32, the same expressions k_los evaluates per sample, so a march
+// over t6 reads one 24-byte record per step instead of 19 stencil values.
+__global__ __launch_bounds__(256) void k_tidal(const float *__restrict__ phi, int N, int ngx,
+                                               float2 *__restrict__ t6) {
+#pragma clang fp contract(off)
+  const size_t row = blockIdx.x;
+  const int iz = (int)(row / N), iy = (int)(row % N);
+  const size_t plane = (size_t)N * ngx;
+  const int yh = iy == N - 1 ? 0 : iy + 1, yl = iy == 0 ? N - 1 : iy - 1;
+  const int zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
+  const size_t Z0 = (size_t)iz * plane, ZH = (size_t)zh * plane, ZL = (size_t)zl * plane;
+  const size_t Y0 = (size_t)iy * ngx, YH = (size_t)yh * ngx, YL = (size_t)yl * ngx;
+  for (int ix = threadIdx.x; ix < N; ix += blockDim.x) {
+    const int xh = ix == N - 1 ? 0 : ix + 1, xl = ix == 0 ? N - 1 : ix - 1;
+    const float c0 = phi[ix + Y0 + Z0];
+    float t[6];
+    t[0] = phi[xh + Y0 + Z0] + phi[xl + Y0 + Z0] - 2 * c0;
+    t[3] = phi[ix + YH + Z0] + phi[ix + YL + Z0] - 2 * c0;
+    t[1] = 0.25f * (phi[xh + YH + Z0] + phi[xl + YL + Z0] - phi[xh + YL + Z0] - phi[xl + YH + Z0]);
+    t[5] = phi[ix + Y0 + ZH] + phi[ix + Y0 + ZL] - 2 * c0;
+    t[2] = 0.25f * (phi[xh + Y0 + ZH] + phi[xl + Y0 + ZL] - phi[xh + Y0 + ZL] - phi[xl + Y0 + ZH]);
+    t[4] = 0.25f * (phi[ix + YH + ZH] + phi[ix + YL + ZL] - phi[ix + YH + ZL] - phi[ix + YL + ZH]);
+    float2 *o = t6 + (row * N + ix) * 3;
+    o[0] = make_float2(t[0], t[1]);
+    o[1] = make_float2(t[2], t[3]);
+    o[2] = make_float2(t[4], t[5]);
+  }
+}
+
+// The kappa march of k_los over the precomputed t6 records.
+__global__ __launch_bounds__(256) void k_los_t6(LosArgs a, const float2 *__restrict__ t6,
+                                                const double *__restrict__ fac1, const double *__restrict__ fac2,
+                                                int64_t nside, int64_t npix, float *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= npix) return;
+  double u[3];
+  pix2vec_nest(nside, ip, u);
+  const int N = a.N;
+  double rot[6];
+  {
+    double cth = u[2];
+    if (cth >= 1) cth = 1;
+    if (cth <= -1) cth = -1;
+    const double sth = sqrt((1 - cth) * (1 + cth));
+    double cph = 1, sph = 0;
+    if (sth != 0) { cph = u[0] / sth; sph = u[1] / sth; }
+    const double pre = a.idx * a.idx;
+    rot[0] = (cth * cth * cph * cph + sph * sph) * pre;
+    rot[1] = (2 * cph * sph * (cth * cth - 1)) * pre;
+    rot[2] = (-2 * cth * sth * cph) * pre;
+    rot[3] = (cth * cth * sph * sph + cph * cph) * pre;
+    rot[4] = (-2 * cth * sth * sph) * pre;
+    rot[5] = (sth * sth) * pre;
+  }
+  double k1 = 0, k2 = 0;
+  const double obs[3] = {a.ox, a.oy, a.oz};
+  for (int pl = 0; pl < a.nplanes; pl++) {
+    for (int irr = a.irmin[pl]; irr <= a.irmax[pl]; irr++) {
+      const double rm = (irr + 0.5) * a.dr;
+      int ix[3];
+      for (int ax = 0; ax < 3; ax++) {
+        const double xn = (rm * u[ax] + obs[ax]) * a.idx;
+        ix[ax] = wrapi((long)(xn + 0.5), N);
+      }
+      const float2 *r = t6 + (((size_t)ix[2] * N + ix[1]) * N + ix[0]) * 3;
+      const float2 p0 = r[0], p1 = r[1], p2 = r[2];
+      const float t[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
+      double dotp = 0;
+      for (int q = 0; q < 6; q++) dotp += rot[q] * (double)t[q];
+      k1 += dotp * fac1[irr];
+      k2 += dotp * fac2[irr];
+    }
+    out[(size_t)pl * npix + ip] = (float)(k1 - a.inv_rmax[pl] * k2);
+  }
+}
+
 }  // namespace
 
 // Host part of a kappa / ISW march: the kernel arguments (planes sorted as
@@ -181,9 +260,30 @@ int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *
   double *df = (double *)c->misc;
   float *dout = (float *)(df + 2 * nr);
   CLR_HIP(c, hipMemcpyAsync(df, f12.data(), 2 * nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // kappa over a precomputed tidal grid (24 bytes per cell) when it fits in
+  // a quarter of the free device memory; the direct 19-point march otherwise
+  const size_t t6_bytes = (size_t)c->N * c->N * c->N * 6 * sizeof(float);
+  bool use_t6 = false;
+  if (which == 0) {
+    size_t fr = 0, tot = 0;
+    if (c->tidal_n >= t6_bytes) use_t6 = true;
+    else if (hipMemGetInfo(&fr, &tot) == hipSuccess && t6_bytes <= fr / 4) use_t6 = true;
+    else (void)hipGetLastError();
+    if (use_t6) {
+      rc = clr_ensure(c, &c->tidal, &c->tidal_n, t6_bytes);
+      if (rc) return rc;
+    }
+  }
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_los, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, a, df, df + nr,
-                     (int64_t)nside, npix, dout);
+  if (use_t6) {
+    hipLaunchKernelGGL(k_tidal, dim3((unsigned)((size_t)c->N * c->N)), dim3(256), 0, c->stream, c->npot, c->N,
+                       c->ngx, (float2 *)c->tidal);
+    hipLaunchKernelGGL(k_los_t6, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, a,
+                       (const float2 *)c->tidal, df, df + nr, (int64_t)nside, npix, dout);
+  } else {
+    hipLaunchKernelGGL(k_los, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, a, df, df + nr,
+                       (int64_t)nside, npix, dout);
+  }
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
   CLR_HIP(c, hipMemcpyAsync(out, dout, (size_t)npix * nplanes * sizeof(float), hipMemcpyDeviceToHost, c->stream));

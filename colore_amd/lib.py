@@ -106,7 +106,8 @@ def load_amd():
     _sig(lib, "clr_mlp_grad", ci, [_vp, _f, _f, ci, _f, _d])
     _sig(lib, "clr_mlp_train", ci, [_vp, _f, _f, ci, ci, ctypes.c_float, ctypes.c_float, _d])
     _sig(lib, "clr_mlp_predict", ci, [_vp, _f, ci, _f, ci])
-    _sig(lib, "clr_bias_moment", ctypes.c_double, [ci, ctypes.c_double, ctypes.c_double])
+    _sig(lib, "clr_synchronize", ci, [_vp])
+    _sig(lib, "clr_bias_moment", ctypes.c_double, [ci] + [ctypes.c_double] * 5)
     _sig(lib, "clr_ensemble_create", ci, [ctypes.POINTER(_vp), ci, ctypes.POINTER(_vp), ci, ci, _d,
                                           ctypes.POINTER(ctypes.c_longlong), ci])
     _sig(lib, "clr_ensemble_destroy", None, [_vp])
@@ -114,7 +115,7 @@ def load_amd():
     _sig(lib, "clr_ensemble_run", ci, [_vp, _u32, _d, ci, ctypes.POINTER(ci), _f])
     _sig(lib, "clr_ensemble_result", ci, [_vp, ci, _d, ctypes.POINTER(ci), _f])
     _sig(lib, "clr_fft_autotune", ci, [_vp, ci, ci, _f, ctypes.POINTER(ci), ctypes.POINTER(ci)])
-    _sig(lib, "clr_surrogate_inputs", ci, [_vp, ci, ci, _f, _i32, ctypes.POINTER(ci)])
+    _sig(lib, "clr_surrogate_inputs", ci, [_vp, ci, ci, _f, _d, _i32, ctypes.POINTER(ci)])
     _sig(lib, "clr_sample_sources_surrogate", ci, [_vp, _vp, ci, ctypes.c_uint32, ci, ci,
                                                    ctypes.POINTER(ctypes.c_longlong)])
     return lib
@@ -323,18 +324,21 @@ class Context:
         return n.value
 
     def surrogate_inputs(self, ipop):
-        """(x[n, 4], shell[n]): the surrogate's inputs per normalisation shell."""
+        """(x[n, 4], base[n], shell[n]): the surrogate's inputs and the
+        cumulant closure's ln norm per normalisation shell."""
         n = ctypes.c_int()
-        self._chk(self.lib.clr_surrogate_inputs(self.h, ipop, 0, None, None, ctypes.byref(n)))
+        self._chk(self.lib.clr_surrogate_inputs(self.h, ipop, 0, None, None, None, ctypes.byref(n)))
         x = np.zeros((max(n.value, 1), 4), dtype=np.float32)
+        base = np.zeros(max(n.value, 1))
         sh = np.zeros(max(n.value, 1), dtype=np.int32)
-        self._chk(self.lib.clr_surrogate_inputs(self.h, ipop, n.value, ptr(x, ctypes.c_float), ptr(sh, ctypes.c_int32),
-                                                ctypes.byref(n)))
-        return x[:n.value], sh[:n.value]
+        self._chk(self.lib.clr_surrogate_inputs(self.h, ipop, n.value, ptr(x, ctypes.c_float), ptr(base),
+                                                ptr(sh, ctypes.c_int32), ctypes.byref(n)))
+        return x[:n.value], base[:n.value], sh[:n.value]
 
     def sample_sources_surrogate(self, net, ipop, seed, nside_base=2, precision=1):
         n = ctypes.c_longlong()
-        self._chk(self.lib.clr_sample_sources_surrogate(self.h, net.h, ipop, seed, nside_base, precision,
+        self._chk(self.lib.clr_sample_sources_surrogate(self.h, net.h if net is not None else None, ipop, seed,
+                                                        nside_base, precision,
                                                         ctypes.byref(n)))
         return n.value
 
@@ -440,6 +444,10 @@ class Context:
 
     def declare_grids(self, sigma2, density_transformed):
         self._chk(self.lib.clr_declare_grids(self.h, sigma2, int(density_transformed)))
+
+    def synchronize(self):
+        """Waits for the context's device work (phi's passes included)."""
+        self._chk(self.lib.clr_synchronize(self.h))
 
     def stage_times(self):
         ms = np.zeros(16)

@@ -292,12 +292,13 @@ int clr_declare_grids(clr_ctx *ctx, double sigma2, int density_transformed);
 
 /* ---- learned surrogate of the per-shell normalisation ------------------
  * No counterpart in the reference: a 4 -> 64 -> 64 -> 2 perceptron (tanh
- * hidden layers) that predicts, from x = (D(z), b(z), sigma_G(z), r_smooth)
- * of a shell, y0 = the part of ln norm (count/sum of
- * compute_density_normalization, density.c:1272-1355) that the one-cell
- * lognormal expectation -ln clr_bias_moment(model, b, sigma_G) misses, and
- * y1 = -y0 (the same for ln <bias_model>, the expected count per cell of the
- * un-normalised sampler in units of n dx^3).  FP32 weights; trained
+ * hidden layers) that predicts, from a shell's x = (b, sigma_c, skewness,
+ * excess kurtosis) (clr_surrogate_inputs), y0 = the part of ln norm (count/
+ * sum of compute_density_normalization, density.c:1272-1355) that the
+ * cumulant closure (clr_bias_moment of the shell's measured Gaussian
+ * cumulants) misses, and y1 = -y0 (the same for ln <bias_model>, the
+ * expected count per cell of the un-normalised sampler in units of
+ * n dx^3).  FP32 weights; trained
  * on the device by its own forward/backward kernels (mean squared error of
  * the standardised outputs, full-batch Adam); inference in FP32 or with BF16
  * MFMA (FP32 accumulate).  Parameters: W1[64][4], b1[64], W2[64][64], b2[64],
@@ -326,31 +327,38 @@ int clr_mlp_train(clr_mlp *net, const float *x, const float *y, int n, int steps
                   float lr_final, double *loss_out);
 /* y (n x 2, raw units) from x (n x 4) with CLR_MLP_FP32 or CLR_MLP_BF16_MFMA. */
 int clr_mlp_predict(clr_mlp *net, const float *x, int n, float *y, int precision);
-/* <bias_model(delta)> (common.h:414-431) over the one-cell distribution of
- * the lognormal transform, delta = exp(G - sigma_g^2/2) - 1 with G ~
- * N(0, sigma_g^2): exp(b(b-1) sigma_g^2/2) for the exponential model, 16-point
- * Gauss-Legendre quadrature split at the model's kink for the two truncated
- * ones.  Host only (no device call). */
-double clr_bias_moment(int model, double b, double sigma_g);
+/* Mean of bias_model (common.h:414-431) of delta = exp(G - sigma_c^2/2) - 1
+ * (the lognormal transform of one cell) over a variate G with mean mu,
+ * variance var and third cumulant k3: exp(b mu + b^2 var/2 + b^3 k3/6 -
+ * b sigma_c^2/2) for the exponential model; for the two truncated models a
+ * 16-point Gauss-Legendre quadrature over the Edgeworth density (normal
+ * density times 1 + k3/(6 var^1.5) He_3), split at the model's kink.  With
+ * mu = 0, var = sigma_c^2, k3 = 0 it is the one-cell expectation
+ * <bias_model> of the lognormal field.  Host only (no device call). */
+double clr_bias_moment(int model, double b, double sigma_c, double mu, double var, double k3);
 
-/* Inputs of the surrogate for source population ipop, one row of
- * CLR_MLP_IN floats per normalisation shell with cells (bins 1 .. nz-1 of
- * density.c:1233-1244): at the shell's mean redshift z = zsum/count, D(z),
- * b(z), sigma_G(z) = D(z) sigma (sigma^2 the measured variance of the
- * Gaussian grid, clr_gaussian_fields) and r_smooth; shell[i] is the bin.
- * Needs clr_gaussian_fields.  *n_out = number of rows (x, shell may be NULL
- * to query it). */
-int clr_surrogate_inputs(clr_ctx *ctx, int ipop, int cap, float *x, int *shell, int *n_out);
+/* Inputs of the surrogate for source population ipop, one row per
+ * normalisation shell with cells (bins 1 .. nz-1 of density.c:1233-1244),
+ * from a read-only pass over the Gaussian grid that measures, per shell, the
+ * sample cumulants of G = D(r) delta_G (the variate the lognormal transform
+ * exponentiates): x[i] = (b, sigma_c, skewness, excess kurtosis) with b(z)
+ * at the shell's mean redshift and sigma_c^2 = <D^2> sigma^2 (sigma^2 the
+ * measured variance of the Gaussian grid); base[i] = -ln clr_bias_moment(
+ * model, b, sigma_c, mean, variance, third cumulant), the closure's ln norm;
+ * shell[i] is the bin.  Needs clr_gaussian_fields (density not transformed).
+ * *n_out = number of rows (x, base, shell may be NULL to query it). */
+int clr_surrogate_inputs(clr_ctx *ctx, int ipop, int cap, float *x, double *base, int *shell,
+                         int *n_out);
 /* Surrogate-fused sampling: clr_sample_sources (same catalogues, streams
  * and order) for population ipop (-1: all) right after clr_gaussian_fields,
- * without clr_shell_norms.  The network predicts each shell's norm (ln norm
- * = y0 - ln clr_bias_moment, BF16 MFMA or FP32 inference); the norm tables
- * are built from them as
- * compute_density_normalization builds them from the shell sums
- * (density.c:1272-1355); the count pass reads the Gaussian field and applies
- * the lognormal transform per cell (the same FP32 value the shell pass of
- * clr_shell_norms(lognormal_fused) stores), so given equal norm tables the
- * counts equal the two-pass path's.  The grid keeps the Gaussian field. */
+ * without clr_shell_norms.  Each shell's ln norm is the closure's plus the
+ * network's y0 (BF16 MFMA or FP32 inference; net = NULL: the closure alone);
+ * the norm tables are built from them as compute_density_normalization
+ * builds them from the shell sums (density.c:1272-1355); the count pass
+ * reads the Gaussian field and applies the lognormal transform per cell (the
+ * same FP32 value the shell pass of clr_shell_norms(lognormal_fused)
+ * stores), so given equal norm tables the counts equal the two-pass path's.
+ * The grid keeps the Gaussian field. */
 int clr_sample_sources_surrogate(clr_ctx *ctx, clr_mlp *net, int ipop, uint32_t seed,
                                  int nside_base, int precision, long long *nsrc_out);
 
@@ -399,6 +407,10 @@ int  clr_fft_autotune(clr_ctx *ctx, int reps, int cap, float *ms, int *ncand, in
  * source-beam kernel, the custom-map kernel, the skewer kernels (summed over
  * their chunks).  Fills at most n values; returns the number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
+/* Waits for every device operation of the context, the potential's passes
+ * on the second stream included (no counterpart in the reference, whose
+ * stages return when done). */
+int clr_synchronize(clr_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -541,23 +541,34 @@ def _bias_model_py(d, b, model):
 
 
 def test_bias_moment_against_quadrature_and_closed_forms():
-    """clr_bias_moment (host code of the device library, no device call): the
-    exponential model against exp(b(b-1) s^2/2) by construction and against
-    scipy's quad; both truncated models against quad of the same expectation
-    over G ~ N(0, s^2) (kink points given to quad); the truncated model also
-    against its closed form (1-b) Phi(-g0/s) + b Phi((s^2-g0)/s), g0 =
-    ln(1-1/b) + s^2/2, for b > 1, and = 1 exactly when 0 <= b <= 1 (no
-    truncation, <delta> = 0)."""
+    """clr_bias_moment (host code of the device library, no device call).
+    One-cell case (mu = 0, var = s^2, k3 = 0): the exponential model equals
+    exp(b(b-1) s^2/2) and scipy's quad; both truncated models match quad of
+    the same expectation over G ~ N(0, s^2) (kink points given to quad); the
+    truncated model also matches its closed form (1-b) Phi(-g0/s) + b
+    Phi((s^2-g0)/s), g0 = ln(1-1/b) + s^2/2, for b > 1, and = 1 exactly when
+    0 <= b <= 1 (no truncation, <delta> = 0).  Shifted case (mu, var, k3 of a
+    shell): the exponential model equals exp(b mu + b^2 var/2 + b^3 k3/6 -
+    b sc^2/2); the truncated models match quad over the Edgeworth density
+    phi(t) (1 + k3/(6 var^1.5) He_3(t)), and with k3 = 0 quad over
+    N(mu, var)."""
     from scipy import stats
     amd = lib.load_amd()
 
-    def quad(model, b, s):
-        f = lambda t: stats.norm.pdf(t) * _bias_model_py(math.expm1(s * t - 0.5 * s * s), b, model)
+    def quad(model, b, sc, mu=0.0, var=None, k3=0.0):
+        var = sc * sc if var is None else var
+        sd = math.sqrt(var)
+        g1 = k3 / sd ** 3
+
+        def f(t):
+            w = stats.norm.pdf(t) * (1 + g1 / 6 * (t ** 3 - 3 * t))
+            return w * _bias_model_py(math.expm1(mu + sd * t - 0.5 * sc * sc), b, model)
         pts = []
         if model == 2 and (b > 1 or b < 0):
-            pts = [(math.log(1 - 1 / b) + 0.5 * s * s) / s]
+            pts = [(math.log(1 - 1 / b) + 0.5 * sc * sc - mu) / sd]
         if model == 1:
-            pts = [0.5 * s]
+            pts = [(0.5 * sc * sc - mu) / sd]
+        pts = [p for p in pts if -14 < p < 14]
         return integrate.quad(f, -14, 14, points=pts or None, limit=400, epsabs=0, epsrel=1e-13)[0]
 
     for model in (0, 1, 2):
@@ -565,14 +576,21 @@ def test_bias_moment_against_quadrature_and_closed_forms():
             if model == 1 and b < 0:
                 continue   # exp(b delta/(1+delta)) diverges as delta -> -1
             for s in (0.05, 0.3, 0.8, 1.5):
-                got = amd.clr_bias_moment(model, b, s)
+                got = amd.clr_bias_moment(model, b, s, 0.0, s * s, 0.0)
                 assert got == pytest.approx(quad(model, b, s), rel=1e-10), (model, b, s)
     for b in (1.2, 2.5, 4.0):
         for s in (0.2, 0.9):
             g0 = math.log(1 - 1 / b) + 0.5 * s * s
             cf = (1 - b) * stats.norm.sf(g0 / s) + b * stats.norm.sf((g0 - s * s) / s)
-            assert amd.clr_bias_moment(2, b, s) == pytest.approx(cf, rel=1e-12)
+            assert amd.clr_bias_moment(2, b, s, 0.0, s * s, 0.0) == pytest.approx(cf, rel=1e-12)
     for b in (0.0, 0.5, 1.0):
-        assert amd.clr_bias_moment(2, b, 0.7) == pytest.approx(1.0, abs=1e-13)
-    assert amd.clr_bias_moment(0, 2.0, 0.6) == pytest.approx(math.exp(0.36), rel=1e-15)
-    assert amd.clr_bias_moment(1, 2.0, 0.0) == 1.0
+        assert amd.clr_bias_moment(2, b, 0.7, 0.0, 0.49, 0.0) == pytest.approx(1.0, abs=1e-13)
+    assert amd.clr_bias_moment(0, 2.0, 0.6, 0.0, 0.36, 0.0) == pytest.approx(math.exp(0.36), rel=1e-15)
+    assert amd.clr_bias_moment(1, 2.0, 0.0, 0.0, 0.0, 0.0) == 1.0
+    # a shell's measured cumulants
+    for (b, sc, mu, var, k3) in [(1.7, 0.8, 0.03, 0.61, 0.02), (2.4, 1.1, -0.05, 1.3, -0.04), (0.8, 0.5, 0.01, 0.24, 0.0)]:
+        want0 = math.exp(b * mu + 0.5 * b * b * var + b ** 3 * k3 / 6 - 0.5 * b * sc * sc)
+        assert amd.clr_bias_moment(0, b, sc, mu, var, k3) == pytest.approx(want0, rel=1e-14)
+        for model in (1, 2):
+            got = amd.clr_bias_moment(model, b, sc, mu, var, k3)
+            assert got == pytest.approx(quad(model, b, sc, mu, var, k3), rel=1e-10), (model, b, sc, mu, var, k3)
