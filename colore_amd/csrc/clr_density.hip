@@ -367,53 +367,50 @@ __global__ __launch_bounds__(kShellThreads) __attribute__((amdgpu_waves_per_eu(6
 
 // ---------------------------------------------------------------------------
 // k_shell_moments: the surrogate's read-only pass over the Gaussian grid.
-// A wavefront takes 512 consecutive cells of a row, 8 per lane (two float4
-// loads: the wavefront reads 2 KB contiguous).  Per cell: FP32 radius, the
-// shell from the FP32 edge radii (binary search for a lane's first cell,
-// then a step at a crossing), D(r) from the packed FP32 growth table and
-// G = D delta_G.  A lane sums its cells of the current shell in FP32; when
-// any lane's shell changes the wavefront combines the lanes shell by shell
-// (FP32 DPP sums of at most 512 values) and lane 0 adds them to the
-// wavefront's FP64 accumulators in LDS.  Per block the wavefronts' sums go
-// to partial[block][nz][kMomQ], reduced in fixed order by k_shell_final.
+// A work item is 256 consecutive x cells of one y row (4 per lane, one float4
+// load) walked over zrun consecutive z planes: the wavefront reads 1 KB
+// contiguous per plane (kMomPre planes in flight; 8 cells per lane, 2 KB,
+// measured the same) and a lane's radius changes slowly,
+// so it sums its current shell in FP32 registers and, when the shell
+// changes, adds those sums to the wavefront's FP64 accumulators in LDS with
+// atomics (lanes cross shell edges at different planes, so few collide).
+// A lane's D(r) over its item is the quadratic through the FP64 table's
+// values at its nearest, middle and farthest radius; the item's z length is
+// chosen so that a lane spans at most ~160 Mpc/h, over which the quadratic
+// is within ~1e-6 of the table (these are inputs of an approximation).  Per
+// block the wavefronts' sums go to partial[block][nz][kMomQ], reduced in
+// fixed order by k_shell_final.
 // ---------------------------------------------------------------------------
 constexpr int kMomQ = 6;            // count, G, G^2, G^3, G^4, D^2
-constexpr int kMomBlocks = 2048;
+constexpr int kMomBlocks = 4096;
 constexpr int kMomMaxShells = 256;
+constexpr int kMomZMax = 32;        // z planes per item, at most
+constexpr int kMomPre = 8;          // planes loaded ahead (the item length is a multiple)
+constexpr int kMomCPL = 4;          // x cells per lane (float4 loads)
+constexpr int kMomXW = 64 * kMomCPL;  // x cells per item
 
 struct MomArgs {
   const float *g;
-  const float2 *dtab;   // per r-bin (D_i, D_{i+1} - D_i)
-  const float *edge;    // nz FP32 edge radii
-  int N, ngx, nz, na;
-  float dxf, ox, oy, oz, idr_f, d_last;
+  const double *r_arr, *d1;   // FP64 tables (f_of_r)
+  const float *edge;          // nz FP32 edge radii
+  int N, ngx, nz, zrun;
+  double idr, r_last, d_last;
+  float dxf, ox, oy, oz;
 };
 
-__global__ void k_pack_growth(const float *__restrict__ d1f, int na, float2 *__restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= na) return;
-  const int j = i + 1 < na ? i + 1 : i;
-  out[i] = make_float2(d1f[i], d1f[j] - d1f[i]);
+// largest s in [1, nz] with edge[s-1] <= r (s = nz: past the last shell)
+__device__ inline int mom_shell(const float *edge, int nz, float r) {
+  int lo = 1, hi = nz;
+  while (hi > lo) {
+    const int m = (lo + hi + 1) / 2;
+    if (edge[m - 1] <= r) lo = m; else hi = m - 1;
+  }
+  return lo;
 }
 
-template <int CTRL, int RM>
-__device__ inline float dpp_getf(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xF, true));
-}
-
-// Sum over the 64 lanes of a wavefront (all lanes active), broadcast.
-__device__ inline float wave_sum_f(float x) {
-  x += dpp_getf<0xB1, 0xF>(x);
-  x += dpp_getf<0x4E, 0xF>(x);
-  x += dpp_getf<0x141, 0xF>(x);
-  x += dpp_getf<0x140, 0xF>(x);
-  x += dpp_getf<0x142, 0xA>(x);
-  x += dpp_getf<0x143, 0xC>(x);
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
-}
-
+template <bool FULL>
 __global__ __launch_bounds__(kShellThreads) void k_shell_moments(MomArgs a, double *__restrict__ partial) {
-  extern __shared__ double macc[];   // [wave][nz][kMomQ], then nz FP32 edges
+  extern __shared__ double macc[];   // [wave][nz][kMomQ] FP64, then nz FP32 edges
   const int nz = a.nz, N = a.N;
   float *ledge = (float *)(macc + (size_t)kWaves * nz * kMomQ);
   for (int i = threadIdx.x; i < kWaves * nz * kMomQ; i += blockDim.x) macc[i] = 0;
@@ -421,87 +418,115 @@ __global__ __launch_bounds__(kShellThreads) void k_shell_moments(MomArgs a, doub
   __syncthreads();
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   double *myacc = macc + (size_t)wave * nz * kMomQ;
-  const int segs = (N + 511) / 512;
-  const long nwork = (long)N * N * segs;
+  const int zrun = a.zrun;
+  const int nxc = (N + kMomXW - 1) / kMomXW, nzc = (N + zrun - 1) / zrun;
+  const long nwork = (long)nxc * N * nzc;
   const long nw = (long)gridDim.x * kWaves;
+  const size_t plane = (size_t)N * a.ngx;
   for (long w = (long)blockIdx.x * kWaves + wave; w < nwork; w += nw) {
-    const int seg = (int)(w % segs);
-    const long row = w / segs;
-    const int iz = (int)(row / N), iy = (int)(row % N);
-    const int ix0 = seg * 512 + lane * 8;
-    const float y = coord_f32(iy, a.dxf, a.oy), z = coord_f32(iz, a.dxf, a.oz);
-    const float rho2 = __builtin_fmaf(y, y, z * z);
-    const float *rp = a.g + (size_t)row * a.ngx;
-    float v[8];
-    if (ix0 + 8 <= N) {
-      const float4 q0 = *(const float4 *)(rp + ix0), q1 = *(const float4 *)(rp + ix0 + 4);
-      v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
-      v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; j++) v[j] = ix0 + j < N ? rp[ix0 + j] : 0.f;
+    const int xc = (int)(w % nxc);
+    const long rest = w / nxc;
+    const int iy = (int)(rest % N), zc = (int)(rest / N);
+    const int ix = xc * kMomXW + lane * kMomCPL, iz0 = zc * zrun;   // the lane's cells: ix .. ix+kMomCPL-1
+    const int nzi = FULL ? zrun : min(zrun, N - iz0);
+    const bool valid = FULL || ix < N;   // N is a multiple of kMomCPL
+    // D(r) on the lane's cells: quadratic through the table at its radius
+    // range
+    double c0, c1, c2;
+    {
+      const double xa = (valid ? ix : 0) * (double)a.dxf - a.ox, xb = xa + (kMomCPL - 1) * (double)a.dxf;
+      const double yy = iy * (double)a.dxf - a.oy;
+      const double za = iz0 * (double)a.dxf - a.oz, zb = (iz0 + nzi - 1) * (double)a.dxf - a.oz;
+      const double dxn = xa > 0 ? xa : (xb < 0 ? -xb : 0), dzn = za > 0 ? za : (zb < 0 ? -zb : 0);
+      const double r0 = sqrt(dxn * dxn + yy * yy + dzn * dzn);
+      const double r2 = sqrt(fmax(xa * xa, xb * xb) + yy * yy + fmax(za * za, zb * zb));
+      const double r1 = 0.5 * (r0 + r2);
+      const double f0 = f_of_r(a.r_arr, a.d1, a.idr, a.r_last, r0, 1.0, a.d_last);
+      const double f1 = f_of_r(a.r_arr, a.d1, a.idr, a.r_last, r1, 1.0, a.d_last);
+      const double f2 = f_of_r(a.r_arr, a.d1, a.idr, a.r_last, r2, 1.0, a.d_last);
+      const double h = r1 - r0;
+      if (h > 1e-3) {
+        // D(r) = f0 + (r - r0) (d01 + (r - r1) d012)
+        const double d01 = (f1 - f0) / h, d012 = ((f2 - f1) / h - d01) / (2 * h);
+        c2 = d012;
+        c1 = d01 - d012 * (r0 + r1);
+        c0 = f0 - r0 * d01 + d012 * r0 * r1;
+      } else {
+        c0 = f0; c1 = 0; c2 = 0;
+      }
     }
-    int key = -1, s = 0;
+    const float q0 = (float)c0, q1 = (float)c1, q2 = (float)c2;
+    const float y = coord_f32(iy, a.dxf, a.oy);
+    float rho2[kMomCPL];
+#pragma unroll
+    for (int k = 0; k < kMomCPL; k++) {
+      const float x = coord_f32(valid ? ix + k : 0, a.dxf, a.ox);
+      rho2[k] = __builtin_fmaf(x, x, y * y);
+    }
+    constexpr int V = kMomCPL / 4;   // float4 per lane and plane
+    const float4 *col = (const float4 *)(a.g + ((size_t)iz0 * N + iy) * a.ngx + (valid ? ix : 0));
+    const size_t plane4 = plane / 4;
+    int key = -1;
+    float e_lo = 0.f, e_hi = 0.f;
     float acc[kMomQ];
 #pragma unroll
     for (int q = 0; q < kMomQ; q++) acc[q] = 0.f;
-    auto flush = [&]() {
-      unsigned long long mask = __ballot(key >= 1 && key < nz);
-      while (mask) {
-        const int leader = __ffsll((long long)mask) - 1;
-        const int sl = __builtin_amdgcn_readlane(key, leader);
-        const bool mine = key == sl;
-        const unsigned long long mm = __ballot(mine);
+    float4 vq[kMomPre][V];
 #pragma unroll
-        for (int q = 0; q < kMomQ; q++) {
-          const float t = wave_sum_f(mine ? acc[q] : 0.f);
-          if (lane == 0) myacc[sl * kMomQ + q] += (double)t;
-        }
-        mask &= ~mm;
-      }
-    };
+    for (int j = 0; j < kMomPre; j++)
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int ix = ix0 + j;
-      const float x = coord_f32(ix, a.dxf, a.ox);
-      const float r = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, rho2));
-      if (j == 0) {
-        // largest s in [1, nz] with edge[s-1] <= r (s = nz: past the last shell)
-        int lo = 1, hi = nz;
-        while (hi > lo) {
-          const int m = (lo + hi + 1) / 2;
-          if (ledge[m - 1] <= r) lo = m; else hi = m - 1;
-        }
-        s = lo;
-      } else {
-        while (s < nz && ledge[s] <= r) s++;
-        while (s > 1 && ledge[s - 1] > r) s--;
-      }
-      const int sk = ix < N ? s : nz;
-      if (__ballot(sk != key)) {
-        flush();
+      for (int v = 0; v < V; v++)
+        vq[j][v] = (valid && (FULL || j < nzi)) ? col[(size_t)j * plane4 + v] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int j0 = 0; j0 < nzi; j0 += kMomPre) {
 #pragma unroll
-        for (int q = 0; q < kMomQ; q++) acc[q] = 0.f;
-        key = sk;
-      }
-      if (sk < nz) {
-        const float t = r * a.idr_f;
-        const int i0 = (int)t;
-        float D = a.d_last;
-        if (i0 < a.na - 1) {
-          const float2 dd = a.dtab[i0];
-          D = __builtin_fmaf(t - (float)i0, dd.y, dd.x);
+      for (int u = 0; u < kMomPre; u++) {
+        const int j = j0 + u;
+        float dls[kMomCPL];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          dls[4 * v] = vq[u][v].x;
+          dls[4 * v + 1] = vq[u][v].y;
+          dls[4 * v + 2] = vq[u][v].z;
+          dls[4 * v + 3] = vq[u][v].w;
         }
-        const float G = D * v[j], G2 = G * G;
-        acc[0] += 1.f;
-        acc[1] += G;
-        acc[2] += G2;
-        acc[3] += G2 * G;
-        acc[4] += G2 * G2;
-        acc[5] += D * D;
+        if (valid && (FULL ? j0 + kMomPre < zrun : j + kMomPre < nzi)) {
+#pragma unroll
+          for (int v = 0; v < V; v++) vq[u][v] = col[(size_t)(j + kMomPre) * plane4 + v];
+        }
+        if (FULL || (j < nzi && valid)) {
+          const float zc2 = coord_f32(iz0 + j, a.dxf, a.oz);
+#pragma unroll
+          for (int k = 0; k < kMomCPL; k++) {
+            const float r = __builtin_amdgcn_sqrtf(__builtin_fmaf(zc2, zc2, rho2[k]));
+            if (!(r >= e_lo && r < e_hi)) {
+              if (key >= 1 && key < nz) {
+#pragma unroll
+                for (int q = 0; q < kMomQ; q++) atomicAdd(&myacc[key * kMomQ + q], (double)acc[q]);
+              }
+#pragma unroll
+              for (int q = 0; q < kMomQ; q++) acc[q] = 0.f;
+              key = mom_shell(ledge, nz, r);
+              e_lo = ledge[key - 1];
+              e_hi = key < nz ? ledge[key] : 3.0e38f;
+            }
+            if (key < nz) {
+              const float D = __builtin_fmaf(r, __builtin_fmaf(r, q2, q1), q0);
+              const float G = D * dls[k], G2 = G * G;
+              acc[0] += 1.f;
+              acc[1] += G;
+              acc[2] += G2;
+              acc[3] = __builtin_fmaf(G2, G, acc[3]);
+              acc[4] = __builtin_fmaf(G2, G2, acc[4]);
+              acc[5] = __builtin_fmaf(D, D, acc[5]);
+            }
+          }
+        }
       }
     }
-    flush();
+    if (key >= 1 && key < nz) {
+#pragma unroll
+      for (int q = 0; q < kMomQ; q++) atomicAdd(&myacc[key * kMomQ + q], (double)acc[q]);
+    }
   }
   __syncthreads();
   double *out = partial + (size_t)blockIdx.x * nz * kMomQ;
@@ -932,28 +957,34 @@ int clr_launch_shell_moments(clr_ctx *c, std::vector<double> &out) {
   int rc = clr_ensure(c, (void **)&c->red, &c->red_n, ((size_t)kMomBlocks * nvals + nvals) * sizeof(double));
   if (rc) return rc;
   double *partial = c->red, *fin = c->red + (size_t)kMomBlocks * nvals;
-  rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)na * sizeof(float2) + (size_t)nz * sizeof(float));
+  rc = clr_ensure(c, &c->misc, &c->misc_n, (size_t)nz * sizeof(float));
   if (rc) return rc;
-  float2 *dtab = (float2 *)c->misc;
-  float *dedge = (float *)(dtab + na);
+  float *dedge = (float *)c->misc;
   CLR_HIP(c, hipMemcpyAsync(dedge, hedge.data(), nz * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_pack_growth, dim3((na + 255) / 256), dim3(256), 0, c->stream, c->d1f, na, dtab);
   MomArgs a{};
   a.g = c->dens;
-  a.dtab = dtab;
+  a.r_arr = c->bg;
+  a.d1 = c->bg + 2 * (size_t)na;
   a.edge = dedge;
   a.N = c->N;
   a.ngx = c->ngx;
   a.nz = nz;
-  a.na = na;
+  a.idr = c->glob_idr;
+  a.r_last = c->r_last;
+  a.d_last = c->h_bg_last[2];
   a.dxf = (float)c->dx;
   a.ox = (float)c->obs[0];
   a.oy = (float)c->obs[1];
   a.oz = (float)c->obs[2];
-  a.idr_f = (float)c->glob_idr;
-  a.d_last = (float)c->h_bg_last[2];
+  // items of zrun planes: a multiple of kMomPre, at most kMomZMax, spanning
+  // at most ~160 Mpc/h
+  a.zrun = std::max(kMomPre, std::min(kMomZMax, (int)(160.0 / c->dx) / kMomPre * kMomPre));
   const size_t lds = (size_t)kWaves * nz * kMomQ * sizeof(double) + (size_t)nz * sizeof(float);
-  hipLaunchKernelGGL(k_shell_moments, dim3(kMomBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
+  if (c->N % kMomCPL) return clr_fail(c, CLR_ERR_ARG, "the shell moments need N divisible by %d", kMomCPL);
+  if (c->N % kMomXW == 0 && c->N % a.zrun == 0)
+    hipLaunchKernelGGL(k_shell_moments<true>, dim3(kMomBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
+  else
+    hipLaunchKernelGGL(k_shell_moments<false>, dim3(kMomBlocks), dim3(kShellThreads), lds, c->stream, a, partial);
   hipLaunchKernelGGL(k_shell_final, dim3((unsigned)nvals), dim3(256), 0, c->stream, partial, kMomBlocks, (int)nvals,
                      fin);
   CLR_HIP(c, hipGetLastError());

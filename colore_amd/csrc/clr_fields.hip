@@ -20,11 +20,6 @@
 // the result for the non-Hermitian kx=0 and kx=N/2 planes the reference draws.
 #include "clr_internal.h"
 
-// 1: phi's y and x passes run on the context's second stream, overlapping
-// the stages after delta's; 0: delta and phi share the y and x launches.
-#ifndef CLR_PHI_OWN_STREAM
-#define CLR_PHI_OWN_STREAM 1
-#endif
 #include "clr_fft.h"
 
 
@@ -614,28 +609,6 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
         inj);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
-#if CLR_PHI_OWN_STREAM == 0
-  // delta and phi in the same y and x launches (the field is the grid's
-  // second index); the sigma^2 partials come from delta's blocks
-  hipEvent_t e2c = c->ev2, e3c = c->ev3;
-  L.coly(dim3((unsigned)(ntile * N), 1, 2), dim3(L.colThreads), c->stream, d, p, c->tw);
-  CLR_HIP(c, hipGetLastError());
-  CLR_HIP(c, hipEventRecord(e2c, c->stream));
-  L.c2r(dim3((unsigned)nxb, 2), dim3(L.xThreads), c->stream, c->dens, c->npot, norm, c->red, nrows, c->tw);
-  CLR_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, c->stream, c->red, nxb, c->red_out);
-  CLR_HIP(c, hipGetLastError());
-  CLR_HIP(c, hipEventRecord(e3c, c->stream));
-  c->phi_pending = 0;
-  c->stage_ms[ST_FFT_PHI] = 0;
-  double s[2];
-  CLR_HIP(c, hipMemcpyAsync(s, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  CLR_HIP(c, hipStreamSynchronize(c->stream));
-  float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);  c->stage_ms[ST_GEN] = ms;
-  hipEventElapsedTime(&ms, c->ev1, e2c);     c->stage_ms[ST_FFT] = ms;
-  hipEventElapsedTime(&ms, e2c, e3c);        c->stage_ms[ST_XPASS] = ms;
-#else
   // delta: y and x passes with the sigma^2 partials; phi's passes start on
   // stream2 once delta's y pass is done (its y pass shares the memory system
   // with delta's x pass, then both run under the instruction-bound stages
@@ -666,7 +639,6 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   hipEventElapsedTime(&ms, c->ev0, c->ev1);  c->stage_ms[ST_GEN] = ms;
   hipEventElapsedTime(&ms, c->ev1, e2);      c->stage_ms[ST_FFT] = ms;
   hipEventElapsedTime(&ms, e2, e3);          c->stage_ms[ST_XPASS] = ms;
-#endif
 
   const double ntot = (double)N * (double)N * (double)N;
   const double mean = s[0] / ntot;

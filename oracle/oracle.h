@@ -16,10 +16,14 @@
  * source beams, skewers, custom maps, 1LPT/2LPT displacements, imap at
  * constant T), invariants and brute force (shell sums, HEALPix, deposits).
  * Parity unpinned (no library or reference build here to pin against):
- * GSL's large-lambda Poisson sampler (orc_poisson uses inversion, pinned
- * statistically), GSL's MT19937 uniform mapping, FFTW's treatment of the
- * non-Hermitian kx = 0 and N/2 planes (numpy's convention only), chealpix's
- * exact code paths (closed forms only).
+ * the BTPE branch of GSL's binomial (n p >= 14) inside orc_ref_poisson's
+ * large-mu path (inversion is used there; the multiplication method for
+ * mu <= 10, gamma_int and the small-n p binomial are restated and pinned by
+ * hand-worked streams and distribution tests), GSL's MT19937 uniform mapping
+ * (restated from its documentation; the raw stream is pinned), FFTW's
+ * treatment of the non-Hermitian kx = 0 and N/2 planes (numpy's convention
+ * only), chealpix's exact code paths (closed forms only).  orc_poisson (the
+ * Philox production sampler's checker) uses inversion, pinned statistically.
  */
 #ifndef COLORE_ORACLE_H
 #define COLORE_ORACLE_H

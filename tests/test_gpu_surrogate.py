@@ -131,9 +131,9 @@ def test_surrogate_inputs_match_numpy_cumulants(workdir):
     the FP64 radius binning (edges r(z = k/idz)), G = D(r) delta_G with D
     interpolated on the run's table; count, mean, variance, third and fourth
     cumulants of G and <D^2>.  Cells whose FP32 and FP64 radii fall on
-    different sides of an edge make the two differ slightly: b exact to FP32,
-    sigma_c to 1e-4, skewness and kurtosis to 1e-2 absolute, the closure's ln
-    norm to 1e-4."""
+    different sides of an edge, and the pass's quadratic D(r) per work item
+    (relative error ~1e-5), make the two differ slightly: sigma_c to 5e-4,
+    skewness and kurtosis to 1e-2 absolute, the closure's ln norm to 3e-4."""
     import math
     from conftest import make_run
     from colore_amd import lib
@@ -176,10 +176,10 @@ def test_surrogate_inputs_match_numpy_cumulants(workdir):
             k3 = e3 - 3 * mu * e2 + 2 * mu ** 3
             k4 = e4 - 4 * e3 * mu - 3 * e2 * e2 + 12 * e2 * mu * mu - 6 * mu ** 4
             sc = math.sqrt(md2[s] / n[s] * s2)
-            assert abs(row[1] / sc - 1) < 1e-4, (i, s, row[1], sc)
+            assert abs(row[1] / sc - 1) < 5e-4, (i, s, row[1], sc)
             assert abs(row[2] - k3 / var ** 1.5) < 1e-2 and abs(row[3] - k4 / var ** 2) < 1e-2, (i, s, row, k3, k4)
             want = -math.log(amd.clr_bias_moment(model, float(row[0]), sc, mu, var, k3))
-            assert abs(bs - want) < 1e-4, (i, s, bs, want)
+            assert abs(bs - want) < 3e-4, (i, s, bs, want)
 
 
 def test_surrogate_fused_counts_equal_two_pass(trained, workdir):
