@@ -225,7 +225,13 @@ def cpu_baseline(cfg, n_cpu_grid):
         nsrc += len(pos)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": N ** 3 / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"value": N ** 3 / dt, "unit": "cells/s", "cores": threads, "kind": "port", "cpu_model": model,
             "sample": f"FP64 OpenMP oracle, same cosmology/populations at {N}^3 (full step, {nsrc} sources, "
                       f"{dt:.1f} s)"}
 
@@ -299,8 +305,8 @@ def bench_surrogate(args):
     against the exact reduction is reported."""
     from colore_amd import inputs, lib, surrogate
     t0 = time.perf_counter()
-    x, y = surrogate.training_set()
-    net, losses = surrogate.train(x, y)
+    x, y, groups = surrogate.training_set(with_groups=True)
+    net, losses = surrogate.train(x, y, groups=groups)
     t_train = time.perf_counter() - t0
     d = tempfile.mkdtemp(prefix="colore_c5_")
     run = lib.Run(inputs.write_config("C5", d, n_grid=args.n_grid, with_maps=False))

@@ -14,7 +14,9 @@
 // fourth); the closure evaluates the mean of f for a distribution with the
 // first three (closed form for the exponential model, Edgeworth-weighted
 // quadrature for the truncated ones); the learned network (clr_mlp.hip)
-// predicts the rest of ln norm from (b, sigma_c, skewness, excess kurtosis).
+// predicts the rest of ln norm from (b, sigma_c, skewness, excess kurtosis)
+// for the exponential model, whose closure (the cumulant series cut after
+// the third term) leaves the fourth and higher cumulants' part.
 // The exact two-pass reduction (clr_shell_norms) stays the fallback and the
 // judge of the number-count error.
 #include "clr_internal.h"
@@ -226,12 +228,15 @@ int clr_sample_sources_surrogate(clr_ctx *c, clr_mlp *net, int ipop, uint32_t se
     CLR_HIP(c, hipMemcpyAsync(y.data(), dy, y.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     CLR_HIP(c, hipStreamSynchronize(c->stream));
   }
-  // per-shell norms: ln norm = closure + the network's correction y0 (the
-  // closure alone without a network); shells without cells keep 1 (the edge
-  // fill of the norm table replaces bin 0 and the last bin anyway)
+  // per-shell norms: ln norm = closure + the network's correction y0 for
+  // populations with the exponential model (the truncated models' closures
+  // take none); the closure alone without a network; shells without cells
+  // keep 1 (the edge fill of the norm table replaces bin 0 and the last bin
+  // anyway)
   std::vector<double> norm_s((size_t)np * nz, 1.0);
   for (int i = 0; i < n; i++) {
-    const double v = exp((double)y[(size_t)i * CLR_MLP_OUT] + base[i]);
+    const bool corr = c->tr[CLR_TRACER_SRCS][p0 + owner[i]].model == CLR_BIAS_EXPONENTIAL;
+    const double v = exp((corr ? (double)y[(size_t)i * CLR_MLP_OUT] : 0.0) + base[i]);
     if (!(v > 0) || !std::isfinite(v)) return clr_fail(c, CLR_ERR_STATE, "surrogate predicted norm %g", v);
     norm_s[(size_t)owner[i] * nz + idx[i]] = v;
   }

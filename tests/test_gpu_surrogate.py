@@ -117,9 +117,10 @@ def trained(tmp_path_factory):
     from colore_amd import surrogate
     wd = str(tmp_path_factory.mktemp("surrogate"))
     print("surrogate training set", flush=True)
-    x, y = surrogate.training_set([("C1", 128, 4), ("C2", 128, 6), ("C3", 128, 4)], seed=900, workdir=wd)
+    x, y, g = surrogate.training_set([("C1", 128, 4), ("C2", 128, 6), ("C3", 128, 4)], seed=900, workdir=wd,
+                                     with_groups=True)
     assert len(x) > 1000
-    net, losses = surrogate.train(x, y, steps=4000)
+    net, losses = surrogate.train(x, y, steps=4000, groups=g)
     assert losses[-1] < losses[0]
     yield net, x, y, wd
     net.close()
