@@ -87,7 +87,7 @@ static int noalloc_scratch_setup(clr_ctx *c) {
 static void free_all(clr_ctx *c) {
   auto f = [](void *p) { if (p) hipFree(p); };
   f(c->dens); f(c->npot); f(c->bg); f(c->pk); f(c->tw); f(c->amp); f(c->red); f(c->red_out);
-  f(c->rowcnt); f(c->ref_cnt); f(c->los_buf); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d2); f(c->d1f); f(c->r2n); f(c->tidal);
+  f(c->rowcnt); f(c->ref_cnt); f(c->los_buf); f(c->esc); f(c->inj); f(c->hp_vec); f(c->misc); f(c->d2); f(c->d1f); f(c->r2n); f(c->tidal); f(c->velrec);
   for (int k = 0; k < 3; k++)
     for (int i = 0; i < CLR_NPOP_MAX; i++) {
       f(c->tr[k][i].amp); f(c->tr[k][i].bz); f(c->tr[k][i].norm); f(c->tr[k][i].bzf); f(c->tr[k][i].ptab);
@@ -171,7 +171,7 @@ void clr_destroy(clr_ctx *ctx) {
 long long clr_device_bytes(const clr_ctx *c) {
   if (!c) return 0;
   long long b = 2LL * (long long)c->grid_floats * 4 + c->amp_n + c->red_n + c->red_out_n + c->rows_n + c->inj_n +
-                c->hp_n + c->misc_n + c->tidal_n;
+                c->hp_n + c->misc_n + c->tidal_n + c->velrec_n;
   for (int i = 0; i < CLR_NPOP_MAX; i++) b += c->cat[i].cap * 20;
   return b;
 }

@@ -21,6 +21,7 @@ using namespace clr;
 namespace {
 
 constexpr int kMaxShells = 1024;
+constexpr int kThreadSub = 4;   // nsub above which a cell is painted by a whole wavefront
 
 struct ImapArgs {
   const float *dens, *phi;
@@ -62,29 +63,76 @@ __device__ inline double rvel_cell(const ImapArgs &a, int ix, int iy, int iz, do
   return 0.5 * a.idx * (v0 * u0 + v1 * u1 + v2 * u2);
 }
 
+// One cell's painting inputs: temperature T(r) bias_model(delta, b(r)) norm(r)
+// and the RSD shift v_r V1(r)/H(r) (imap.c:185-200), and the sub-sampling
+// nsub of its shell (imap.c:201-209).  false when the cell is outside the
+// shells' range or has T(r) = 0.
+struct CellPaint {
+  double x0, y0, z0, dr_rsd, dxs;
+  float temp;
+  int ns;
+};
+__device__ inline bool cell_paint(const ImapArgs &a, const double *__restrict__ r0s, const double *__restrict__ rfs,
+                                  const int *__restrict__ nsub, int ix, int iy, int iz, CellPaint &cp) {
+#pragma clang fp contract(off)
+  const double x0 = ix * a.dx - a.ox, y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
+  const double r0 = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+  const double tmean = f_of_r(a.r_arr, a.tz, a.idr, a.r_last, r0, 0.0, 0.0);
+  if (!(r0 <= a.rmax && r0 >= a.rmin)) return false;
+  if (!(tmean > 0)) return false;
+  const double bias = f_of_r(a.r_arr, a.bz, a.idr, a.r_last, r0, a.bz0, 1.0);
+  const double dnorm = f_of_r(a.r_arr, a.norm, a.idr, a.r_last, r0, a.norm0, a.normf);
+  double rvel = 0;
+  if (r0 > 0) rvel = a.factor_vel * rvel_cell(a, ix, iy, iz, x0, y0, z0, r0);
+  cp.dr_rsd = rvel * f_of_r(a.r_arr, a.v1, a.idr, a.r_last, r0, a.v1_0, a.v1_last) *
+              f_of_r(a.r_arr, a.ih, a.idr, a.r_last, r0, a.ih_0, a.ih_last);
+  const double dl = a.dens[((size_t)iz * a.N + iy) * (size_t)a.ngx + ix];
+  cp.temp = (float)(tmean * bias_model_f64(dl, bias, a.model) * dnorm);
+  const int irad = shell_of(r0s, rfs, a.nr, r0);
+  cp.ns = irad < 0 ? nsub[0] : (irad >= a.nr ? nsub[a.nr - 1] : nsub[irad]);
+  cp.dxs = a.dx / cp.ns;
+  cp.x0 = x0;
+  cp.y0 = y0;
+  cp.z0 = z0;
+  return true;
+}
+
+// The (shell, RING pixel) bin of sub-point (izz, iyy, ixx) of a cell
+// (imap.c:210-240), or -1 outside the shells.
+__device__ inline int64_t subpoint_bin(const ImapArgs &a, const double *__restrict__ r0s,
+                                       const double *__restrict__ rfs, const CellPaint &cp, int izz, int iyy,
+                                       int ixx) {
+#pragma clang fp contract(off)
+  const double z = cp.z0 + (izz + 0.5) * cp.dxs;
+  const double y = cp.y0 + (iyy + 0.5) * cp.dxs;
+  const double x = cp.x0 + (ixx + 0.5) * cp.dxs;
+  const double r = sqrt(x * x + y * y + z * z);
+  double cth = 1, phi = 0;
+  if (r != 0) {
+    const double xn = x / r, yn = y / r;
+    cth = z / r;
+    phi = atan2(yn, xn);
+    if (phi < 0) phi += 2 * M_PI;
+  }
+  const int irad = shell_of(r0s, rfs, a.nr, r + cp.dr_rsd);
+  if (irad < 0 || irad >= a.nr) return -1;
+  return (int64_t)irad * a.npix + ang2pix_ring(a.nside, cth, phi);
+}
+
+// Cells whose shell has nsub <= kThreadSub: one thread per cell.  Its
+// sub-points land in a few bins: the thread counts them in a small register
+// cache and adds n T and n per bin with device atomics.
 __global__ __launch_bounds__(256) void k_imap(ImapArgs a, const double *__restrict__ r0s,
                                               const double *__restrict__ rfs, const int *__restrict__ nsub,
                                               float *__restrict__ data, int *__restrict__ nadd) {
-#pragma clang fp contract(off)
   const size_t row = blockIdx.x;
   const int iz = (int)(row / a.N), iy = (int)(row % a.N);
   for (int ix = threadIdx.x; ix < a.N; ix += blockDim.x) {
-    const double x0 = ix * a.dx - a.ox, y0 = iy * a.dx - a.oy, z0 = iz * a.dx - a.oz;
-    const double r0 = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
-    const double tmean = f_of_r(a.r_arr, a.tz, a.idr, a.r_last, r0, 0.0, 0.0);
-    if (!(r0 <= a.rmax && r0 >= a.rmin)) continue;
-    if (!(tmean > 0)) continue;
-    const double bias = f_of_r(a.r_arr, a.bz, a.idr, a.r_last, r0, a.bz0, 1.0);
-    const double dnorm = f_of_r(a.r_arr, a.norm, a.idr, a.r_last, r0, a.norm0, a.normf);
-    double rvel = 0;
-    if (r0 > 0) rvel = a.factor_vel * rvel_cell(a, ix, iy, iz, x0, y0, z0, r0);
-    const double dr_rsd = rvel * f_of_r(a.r_arr, a.v1, a.idr, a.r_last, r0, a.v1_0, a.v1_last) *
-                          f_of_r(a.r_arr, a.ih, a.idr, a.r_last, r0, a.ih_0, a.ih_last);
-    const double dl = a.dens[row * (size_t)a.ngx + ix];
-    const float temp = (float)(tmean * bias_model_f64(dl, bias, a.model) * dnorm);
-    int irad = shell_of(r0s, rfs, a.nr, r0);
-    const int ns = irad < 0 ? nsub[0] : (irad >= a.nr ? nsub[a.nr - 1] : nsub[irad]);
-    const double dxs = a.dx / ns;
+    CellPaint cp;
+    if (!cell_paint(a, r0s, rfs, nsub, ix, iy, iz, cp)) continue;
+    if (cp.ns > kThreadSub) continue;   // k_imap_wave's cell
+    const int ns = cp.ns;
+    const float temp = cp.temp;
     // (bin, hits) of the cell's sub-points; a full cache flushes its oldest
     // entry
     constexpr int kCache = 4;   // measured: 2 / 4 / 8 / 16 / 32 entries -> 610 / 512 / 526 / 550 / 651 ms
@@ -100,48 +148,61 @@ __global__ __launch_bounds__(256) void k_imap(ImapArgs a, const double *__restri
       atomicAdd(&data[o], temp * (float)n);
       atomicAdd(&nadd[o], n);
     };
-    for (int izz = 0; izz < ns; izz++) {
-      const double z = z0 + (izz + 0.5) * dxs;
-      for (int iyy = 0; iyy < ns; iyy++) {
-        const double y = y0 + (iyy + 0.5) * dxs;
+    for (int izz = 0; izz < ns; izz++)
+      for (int iyy = 0; iyy < ns; iyy++)
         for (int ixx = 0; ixx < ns; ixx++) {
-          const double x = x0 + (ixx + 0.5) * dxs;
-          const double r = sqrt(x * x + y * y + z * z);
-          double cth = 1, phi = 0;
-          if (r != 0) {
-            const double xn = x / r, yn = y / r;
-            cth = z / r;
-            phi = atan2(yn, xn);
-            if (phi < 0) phi += 2 * M_PI;
-          }
-          irad = shell_of(r0s, rfs, a.nr, r + dr_rsd);
-          if (irad >= 0 && irad < a.nr) {
-            const int64_t pix = ang2pix_ring(a.nside, cth, phi);
-            const int64_t o = (int64_t)irad * a.npix + pix;
-            bool found = false;
+          const int64_t o = subpoint_bin(a, r0s, rfs, cp, izz, iyy, ixx);
+          if (o < 0) continue;
+          bool found = false;
+#pragma unroll
+          for (int k = 0; k < kCache; k++)
+            if (key[k] == o) {
+              hits[k]++;
+              found = true;
+            }
+          if (!found) {
 #pragma unroll
             for (int k = 0; k < kCache; k++)
-              if (key[k] == o) {
-                hits[k]++;
-                found = true;
+              if (k == next) {
+                if (key[k] >= 0) flush(key[k], hits[k]);
+                key[k] = o;
+                hits[k] = 1;
               }
-            if (!found) {
-#pragma unroll
-              for (int k = 0; k < kCache; k++)
-                if (k == next) {
-                  if (key[k] >= 0) flush(key[k], hits[k]);
-                  key[k] = o;
-                  hits[k] = 1;
-                }
-              next = next == kCache - 1 ? 0 : next + 1;
-            }
+            next = next == kCache - 1 ? 0 : next + 1;
           }
         }
-      }
-    }
 #pragma unroll
     for (int k = 0; k < kCache; k++)
       if (key[k] >= 0) flush(key[k], hits[k]);
+  }
+}
+
+// Cells whose shell has nsub > kThreadSub (the shells nearest the observer,
+// up to 22^3 sub-points per cell at 1024^3): one wavefront per cell, one
+// sub-point per lane, so the lanes of one atomic instruction mostly add into
+// the same few bins.  Cells of the cube [lo, lo + side)^3 around the
+// observer, taken by the wavefronts in turn.
+__global__ __launch_bounds__(256) void k_imap_wave(ImapArgs a, const double *__restrict__ r0s,
+                                                   const double *__restrict__ rfs, const int *__restrict__ nsub,
+                                                   int lo_x, int lo_y, int lo_z, int side, float *__restrict__ data,
+                                                   int *__restrict__ nadd) {
+  const int lane = threadIdx.x % 64;
+  const long nw = (long)gridDim.x * (blockDim.x / 64);
+  const long ncell = (long)side * side * side;
+  for (long w = (long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w < ncell; w += nw) {
+    const int ix = lo_x + (int)(w % side), iy = lo_y + (int)((w / side) % side), iz = lo_z + (int)(w / ((long)side * side));
+    CellPaint cp;
+    if (!cell_paint(a, r0s, rfs, nsub, ix, iy, iz, cp)) continue;
+    if (cp.ns <= kThreadSub) continue;   // k_imap's cell
+    const int ns = cp.ns, nq = ns * ns * ns;
+    for (int q = lane; q < nq; q += 64) {
+      const int ixx = q % ns, iyy = (q / ns) % ns, izz = q / (ns * ns);
+      const int64_t o = subpoint_bin(a, r0s, rfs, cp, izz, iyy, ixx);
+      if (o >= 0) {
+        atomicAdd(&data[o], cp.temp);
+        atomicAdd(&nadd[o], 1);
+      }
+    }
   }
 }
 
@@ -234,6 +295,31 @@ int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, c
   hipLaunchKernelGGL(k_imap, dim3((unsigned)((size_t)N * N)), dim3(N >= 256 ? 256 : N), 0, c->stream, a, dr0,
                      drf, dns, dmap, dnadd);
   CLR_HIP(c, hipGetLastError());
+  // the cells of shells with nsub > kThreadSub lie within r_wave of the
+  // observer (cells below the first shell and in gaps take the first
+  // shell's nsub, cells beyond the last take the last one's)
+  double r_wave = -1;
+  for (int i = 0; i < nr; i++)
+    if (nsub[i] > kThreadSub) r_wave = std::max(r_wave, srf[i]);
+  bool gaps = false;   // cells between two shells also take nsub[0]
+  for (int i = 0; i + 1 < nr; i++) gaps = gaps || srf[i] < sr0[i + 1];
+  if (nsub[0] > kThreadSub) r_wave = std::max(r_wave, gaps ? a.rmax : sr0[0]);
+  if (nsub[nr - 1] > kThreadSub) r_wave = a.rmax;
+  if (r_wave > 0) {
+    r_wave = std::min(r_wave, a.rmax) + dx;
+    int lo[3], hi[3];
+    for (int ax = 0; ax < 3; ax++) {
+      lo[ax] = std::max(0, (int)floor((c->obs[ax] - r_wave) / dx));
+      hi[ax] = std::min(N, (int)ceil((c->obs[ax] + r_wave) / dx) + 1);
+    }
+    const int side = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    // a cube of `side` cells from lo, clamped to the grid
+    for (int ax = 0; ax < 3; ax++) lo[ax] = std::max(0, std::min(lo[ax], N - side));
+    if (side > 0 && side <= N)
+      hipLaunchKernelGGL(k_imap_wave, dim3(4096), dim3(256), 0, c->stream, a, dr0, drf, dns, lo[0], lo[1], lo[2],
+                         side, dmap, dnadd);
+    CLR_HIP(c, hipGetLastError());
+  }
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));
   CLR_HIP(c, hipMemcpyAsync(data, dmap, mapn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   CLR_HIP(c, hipMemcpyAsync(nadd, dnadd, mapn * sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -121,8 +121,10 @@ struct clr_ctx {
   std::vector<double> geo_cnt, geo_zsum;
   double r_smooth = 0;      // smoothing length of the last clr_gaussian_fields
   // graph-captured kappa: kernel arguments, radial weights + map buffer
-  void *tidal = nullptr;     // kappa's tidal records, 6 floats per cell (clr_los.hip)
+  void *tidal = nullptr;     // tidal records, 6 floats per cell (clr_los.hip)
   size_t tidal_n = 0;
+  void *velrec = nullptr;    // velocity-difference + density records, 4 floats per cell (clr_los.hip)
+  size_t velrec_n = 0;
   std::string los_plan;
   void *los_buf = nullptr;
   size_t los_buf_n = 0;
@@ -183,6 +185,7 @@ int clr_launch_imap(clr_ctx *c, int ipop, int nside, int nr, const double *r0, c
 // (kind, ipop) as compute_density_normalization builds them from sums.
 int clr_launch_shell_geometry(clr_ctx *c);
 int clr_launch_shell_moments(clr_ctx *c, std::vector<double> &out);
+bool clr_cell_records(clr_ctx *c, bool tidal);
 int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const int *ipops, const double *norm_s);
 
 // Graph-capturable stages (clr_ensemble.hip): buffers first (prepare, may

@@ -104,20 +104,26 @@ __global__ __launch_bounds__(256) void k_los(LosArgs a, const double *__restrict
 }
 
 // The transverse-Hessian inputs of get_element (beaming.c:79-116) for every
-// cell once: t6[cell][6] = (xx, xy, xz, yy, yz, zz) second differences of
-// phi in FP32, the same expressions k_los evaluates per sample, so a march
-// over t6 reads one 24-byte record per step instead of 19 stencil values.
-__global__ __launch_bounds__(256) void k_tidal(const float *__restrict__ phi, int N, int ngx,
-                                               float2 *__restrict__ t6) {
+// cell a march can reach (centre within sqrt(r2lim) of the observer) once:
+// t6[cell][6] = (xx, xy, xz, yy, yz, zz) second differences of phi in FP32,
+// the same expressions k_los evaluates per sample, so a march over t6 reads
+// one 24-byte record per step instead of 19 stencil values.
+__global__ __launch_bounds__(256) void k_tidal(const float *__restrict__ phi, int N, int ngx, float dxf, float ox,
+                                               float oy, float oz, float r2lim, float2 *__restrict__ t6) {
 #pragma clang fp contract(off)
   const size_t row = blockIdx.x;
   const int iz = (int)(row / N), iy = (int)(row % N);
+  const float yc = iy * dxf - oy, zc = iz * dxf - oz;
+  const float rho2 = yc * yc + zc * zc;
+  if (rho2 > r2lim) return;   // no march reaches this row
   const size_t plane = (size_t)N * ngx;
   const int yh = iy == N - 1 ? 0 : iy + 1, yl = iy == 0 ? N - 1 : iy - 1;
   const int zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
   const size_t Z0 = (size_t)iz * plane, ZH = (size_t)zh * plane, ZL = (size_t)zl * plane;
   const size_t Y0 = (size_t)iy * ngx, YH = (size_t)yh * ngx, YL = (size_t)yl * ngx;
   for (int ix = threadIdx.x; ix < N; ix += blockDim.x) {
+    const float xc = ix * dxf - ox;
+    if (xc * xc + rho2 > r2lim) continue;
     const int xh = ix == N - 1 ? 0 : ix + 1, xl = ix == 0 ? N - 1 : ix - 1;
     const float c0 = phi[ix + Y0 + Z0];
     float t[6];
@@ -277,7 +283,8 @@ int clr_launch_los(clr_ctx *c, int which, int nside, int nplanes, const double *
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
   if (use_t6) {
     hipLaunchKernelGGL(k_tidal, dim3((unsigned)((size_t)c->N * c->N)), dim3(256), 0, c->stream, c->npot, c->N,
-                       c->ngx, (float2 *)c->tidal);
+                       c->ngx, (float)c->dx, (float)c->obs[0], (float)c->obs[1], (float)c->obs[2],
+                       (float)((c->r_max + 2 * c->dx) * (c->r_max + 2 * c->dx)), (float2 *)c->tidal);
     hipLaunchKernelGGL(k_los_t6, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, a,
                        (const float2 *)c->tidal, df, df + nr, (int64_t)nside, npix, dout);
   } else {
@@ -359,6 +366,8 @@ struct BeamArgs {
   const double *r_arr, *z_arr, *v1_arr;   // background r-tables
   const double *a2r;                      // r(a) at a = i/(na-1) (r_of_z, cosmo.c:101-112)
   const double *fac;                      // fac_r_0, fac_r_1, fac_r_2 (nr each)
+  const float4 *vrec;                     // per cell (k_cell_records): phi's central differences, delta
+  const float2 *t6;                       // per cell (k_tidal): the tidal tensor's second differences
 };
 
 // The reference wraps once (beaming.c:129-133), which covers every cell the
@@ -411,7 +420,37 @@ __device__ inline void vel_tid_at(const float *__restrict__ phi, int N, size_t n
   t[4] = 0.25f * (phi[ix + YH + ZH] + phi[ix + YL + ZL] - phi[ix + YH + ZL] - phi[ix + YL + ZH]);
 }
 
-template <bool LENS>
+// Per cell: phi's unscaled central differences in FP32 (as vel_at takes
+// them) and the density, one 16-byte record, so that the CIC and NGP reads
+// of the source passes take one load per cell instead of six (plus one).
+__global__ __launch_bounds__(256) void k_cell_records(const float *__restrict__ phi, const float *__restrict__ dens,
+                                                      int N, int ngx, float4 *__restrict__ out) {
+#pragma clang fp contract(off)
+  const size_t row = blockIdx.x;
+  const int iz = (int)(row / N), iy = (int)(row % N);
+  const size_t plane = (size_t)N * ngx;
+  for (int ix = threadIdx.x; ix < N; ix += blockDim.x) {
+    double v[3];
+    vel_at(phi, N, (size_t)ngx, plane, ix, iy, iz, v);
+    out[row * N + ix] = make_float4((float)v[0], (float)v[1], (float)v[2], dens[row * ngx + ix]);
+  }
+}
+
+// A cell's velocity differences (and density) from its record or from phi.
+template <bool REC>
+__device__ inline void vel_cell(const float4 *__restrict__ vrec, const float *__restrict__ phi, int N, size_t ngx,
+                                size_t plane, int ix, int iy, int iz, double v[3]) {
+  if (REC) {
+    const float4 q = vrec[((size_t)iz * N + iy) * N + ix];
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+  } else {
+    vel_at(phi, N, ngx, plane, ix, iy, iz, v);
+  }
+}
+
+template <bool LENS, bool REC>
 __global__ __launch_bounds__(256) void k_src_beams(BeamArgs a, const float4 *__restrict__ pos, long long n,
                                                    float *__restrict__ dz, float *__restrict__ lens) {
 #pragma clang fp contract(off)
@@ -444,7 +483,7 @@ __global__ __launch_bounds__(256) void k_src_beams(BeamArgs a, const float4 *__r
     const int bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
     const double w = (bz ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
     double vc[3];
-    vel_at(phi, N, ngx, plane, bx ? i1[0] : i0[0], by ? i1[1] : i0[1], bz ? i1[2] : i0[2], vc);
+    vel_cell<REC>(a.vrec, phi, N, ngx, plane, bx ? i1[0] : i0[0], by ? i1[1] : i0[1], bz ? i1[2] : i0[2], vc);
     v[0] += vc[0] * w;
     v[1] += vc[1] * w;
     v[2] += vc[2] * w;
@@ -494,7 +533,22 @@ __global__ __launch_bounds__(256) void k_src_beams(BeamArgs a, const float4 *__r
     ix[1] = wrap1((int)((rm * u[1] + a.oy) * a.idx + 0.5), N);
     ix[2] = wrap1((int)((rm * u[2] + a.oz) * a.idx + 0.5), N);
     double vp[3], tp[6];
-    vel_tid_at(phi, N, ngx, plane, ix[0], ix[1], ix[2], vp, tp);
+    if (REC) {
+      const size_t o = ((size_t)ix[2] * N + ix[1]) * N + ix[0];
+      const float4 q = a.vrec[o];
+      const float2 t0 = a.t6[3 * o], t1 = a.t6[3 * o + 1], t2 = a.t6[3 * o + 2];
+      vp[0] = q.x;
+      vp[1] = q.y;
+      vp[2] = q.z;
+      tp[0] = t0.x;
+      tp[1] = t0.y;
+      tp[2] = t1.x;
+      tp[3] = t1.y;
+      tp[4] = t2.x;
+      tp[5] = t2.y;
+    } else {
+      vel_tid_at(phi, N, ngx, plane, ix[0], ix[1], ix[2], vp, tp);
+    }
     const double fr = f1[i_r] * r - f2[i_r];
     const double frm = 2 * (f0[i_r] * r - f1[i_r]);
     double de1 = 0, de2 = 0, dk = 0;
@@ -522,6 +576,33 @@ __global__ __launch_bounds__(256) void k_src_beams(BeamArgs a, const float4 *__r
 }
 
 }  // namespace
+
+// Builds the per-cell records on the context's stream: velocity differences
+// and density (16 bytes per cell), with `tidal` also the tidal tensor (24
+// bytes per cell, k_tidal), when they fit in a quarter of the free device
+// memory (buffers kept for later calls); false: the callers read phi directly.
+bool clr_cell_records(clr_ctx *c, bool tidal) {
+  const size_t cells = (size_t)c->N * c->N * c->N;
+  const size_t vb = cells * sizeof(float4), tb = tidal ? cells * 6 * sizeof(float) : 0;
+  size_t need = (c->velrec_n >= vb ? 0 : vb) + (c->tidal_n >= tb ? 0 : tb);
+  if (need > 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    if (need > fr / 4) return false;
+  }
+  if (clr_ensure(c, &c->velrec, &c->velrec_n, vb)) return false;
+  if (tidal && clr_ensure(c, &c->tidal, &c->tidal_n, tb)) return false;
+  hipLaunchKernelGGL(k_cell_records, dim3((unsigned)((size_t)c->N * c->N)), dim3(256), 0, c->stream, c->npot, c->dens,
+                     c->N, c->ngx, (float4 *)c->velrec);
+  if (tidal)
+    hipLaunchKernelGGL(k_tidal, dim3((unsigned)((size_t)c->N * c->N)), dim3(256), 0, c->stream, c->npot, c->N,
+                       c->ngx, (float)c->dx, (float)c->obs[0], (float)c->obs[1], (float)c->obs[2],
+                       (float)((c->r_max + 2 * c->dx) * (c->r_max + 2 * c->dx)), (float2 *)c->tidal);
+  return hipGetLastError() == hipSuccess;
+}
 
 int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *dz, float *lens, long long n) {
   if (ipop < 0 || ipop >= c->n_tr[CLR_TRACER_SRCS]) return clr_fail(c, CLR_ERR_ARG, "no source population %d", ipop);
@@ -588,10 +669,22 @@ int clr_launch_src_beams(clr_ctx *c, int ipop, const double *r_arr_a2r, float *d
   CLR_HIP(c, hipMalloc((void **)&tmp, (size_t)n * per * sizeof(float)));
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
   const dim3 grid((unsigned)((n + 255) / 256));
-  if (lens)
-    hipLaunchKernelGGL(k_src_beams<true>, grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp, tmp + n);
+  // per-cell records (velocity differences; with lensing also the tidal
+  // tensor) when they fit in a quarter of the free device memory
+  const bool rec = clr_cell_records(c, lens != nullptr);
+  a.vrec = (const float4 *)c->velrec;
+  a.t6 = (const float2 *)c->tidal;
+  if (lens && rec)
+    hipLaunchKernelGGL((k_src_beams<true, true>), grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp,
+                       tmp + n);
+  else if (lens)
+    hipLaunchKernelGGL((k_src_beams<true, false>), grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp,
+                       tmp + n);
+  else if (rec)
+    hipLaunchKernelGGL((k_src_beams<false, true>), grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp,
+                       (float *)nullptr);
   else
-    hipLaunchKernelGGL(k_src_beams<false>, grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp,
+    hipLaunchKernelGGL((k_src_beams<false, false>), grid, dim3(256), 0, c->stream, a, (const float4 *)C.pos, n, tmp,
                        (float *)nullptr);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
@@ -616,6 +709,7 @@ namespace {
 
 struct SkwArgs {
   const float *phi, *dens;
+  const float4 *vrec;   // per cell (k_cell_records), or null
   int N, ngx, nr, na, gaussian;
   double idx, dr, idr, ox, oy, oz, factor_vel, sigma2;
   double glob_idr, r_last, v1_0, v1_last, d1_last;
@@ -623,6 +717,7 @@ struct SkwArgs {
   const double *r_arr, *d1_arr, *v1_arr;
 };
 
+template <bool REC>
 __global__ __launch_bounds__(256) void k_src_skewers(SkwArgs a, const float4 *__restrict__ pos, long long n,
                                                      float *__restrict__ dg, float *__restrict__ vs) {
 #pragma clang fp contract(off)
@@ -661,11 +756,20 @@ __global__ __launch_bounds__(256) void k_src_skewers(SkwArgs a, const float4 *__
     const double w = (bz ? h0[2] : h1[2]) * (by ? h0[1] : h1[1]) * (bx ? h0[0] : h1[0]);
     const int cx = bx ? i1[0] : i0[0], cy = by ? i1[1] : i0[1], cz = bz ? i1[2] : i0[2];
     double vc[3];
-    vel_at(a.phi, N, ngx, plane, cx, cy, cz, vc);
+    float dc;
+    if (REC) {
+      const float4 q = a.vrec[((size_t)cz * N + cy) * N + cx];
+      vc[0] = q.x;
+      vc[1] = q.y;
+      vc[2] = q.z;
+      dc = q.w;
+    } else {
+      vel_at(a.phi, N, ngx, plane, cx, cy, cz, vc);
+      dc = a.dens[(size_t)cz * plane + (size_t)cy * ngx + cx];
+    }
     v[0] += vc[0] * w;
     v[1] += vc[1] * w;
     v[2] += vc[2] * w;
-    const float dc = a.dens[(size_t)cz * plane + (size_t)cy * ngx + cx];
     if (a.gaussian) {
       // get_element RETURN_GAUSS (beaming.c:46-57): the cell's offsets and
       // radius in float, D(r) from the FP64 table
@@ -727,6 +831,7 @@ int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, fl
   a.r_arr = c->bg;
   a.d1_arr = c->bg + 2 * (size_t)na;
   a.v1_arr = c->bg + 3 * (size_t)na;
+  a.vrec = clr_cell_records(c, false) ? (const float4 *)c->velrec : nullptr;
   // sources per launch: at most 2^26 skewer values (512 MiB of staging)
   const long long chunk = std::max(1LL, (1LL << 26) / a.nr);
   const long long cap = std::min(n, chunk);
@@ -738,9 +843,14 @@ int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, fl
     const long long m = std::min(cap, n - s0);
     const long long tot = m * a.nr;
     e = hipEventRecord(c->ev2, c->stream);
-    if (e == hipSuccess)
-      hipLaunchKernelGGL(k_src_skewers, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
-                         (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
+    if (e == hipSuccess) {
+      if (a.vrec)
+        hipLaunchKernelGGL(k_src_skewers<true>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
+                           (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
+      else
+        hipLaunchKernelGGL(k_src_skewers<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
+                           (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
+    }
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(c->ev3, c->stream);
     if (e == hipSuccess)

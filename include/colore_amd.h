@@ -351,8 +351,9 @@ int clr_surrogate_inputs(clr_ctx *ctx, int ipop, int cap, float *x, double *base
                          int *n_out);
 /* Surrogate-fused sampling: clr_sample_sources (same catalogues, streams
  * and order) for population ipop (-1: all) right after clr_gaussian_fields,
- * without clr_shell_norms.  Each shell's ln norm is the closure's plus the
- * network's y0 (BF16 MFMA or FP32 inference; net = NULL: the closure alone);
+ * without clr_shell_norms.  Each shell's ln norm is the closure's, plus the
+ * network's y0 for populations with the exponential bias model (BF16 MFMA
+ * or FP32 inference; net = NULL: the closure alone);
  * the norm tables are built from them as compute_density_normalization
  * builds them from the shell sums (density.c:1272-1355); the count pass
  * reads the Gaussian field and applies the lognormal transform per cell (the
