@@ -407,12 +407,14 @@ def test_los_maps(workdir, which):
     assert np.abs(out - o).max() < 1e-5 * rms + 1e-30, (np.abs(out - o).max(), rms)
 
 
-def test_imap_vs_oracle(workdir):
+@pytest.mark.parametrize("N,nside,bands", [(64, 16, 8), (128, 64, 16), (256, 128, 32)])
+def test_imap_vs_oracle(workdir, N, nside, bands):
     """21cm intensity shells painted on the device against the oracle's walk
     over the same grids: identical hit counts per pixel, temperatures summed
-    in FP32 atomics within 1e-5 of the FP64 sums."""
-    N = 64
-    path, run = make_run("C3", N, workdir, z_max=1.6, imap_bands=8, imap_nside=16)
+    in FP32 atomics within 1e-5 of the FP64 sums.  The larger case paints
+    up to 2e8 sub-points, so that many of them fall near a pixel or shell edge,
+    where the device's shortened pixel evaluation defers to the full one."""
+    path, run = make_run("C3", N, workdir, z_max=1.6, imap_bands=bands, imap_nside=nside)
     ctx = _ctx_for(run, N)
     s2 = ctx.gaussian_fields(3003, run.scalar("r_smooth"), True)
     ntr = int(run.scalar("n_srcs")) + int(run.scalar("n_imap"))

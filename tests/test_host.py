@@ -594,3 +594,27 @@ def test_bias_moment_against_quadrature_and_closed_forms():
         for model in (1, 2):
             got = amd.clr_bias_moment(model, b, sc, mu, var, k3)
             assert got == pytest.approx(quad(model, b, sc, mu, var, k3), rel=1e-10), (model, b, sc, mu, var, k3)
+
+
+def test_imap_azimuth_polynomial_within_its_bound():
+    """The 21cm painter accepts a pixel from its shortened evaluation only
+    when every floor lies further than the azimuth's error bound kAtanErr
+    from an integer.  The bound must hold for the polynomial the kernel
+    evaluates: read its coefficients and the bound from the source and
+    check atan on [0, 1] densely (Abramowitz & Stegun 4.4.49 quote 2e-8).
+    A slipped coefficient or sign moves the error far past the bound."""
+    src = open(os.path.join(ROOT, "colore_amd", "csrc", "clr_imap.hip")).read()
+    body = src[src.index("inline double atan01"):]
+    body = body[:body.index("return t * p;")]
+    c0 = float(re.search(r"double p = ([0-9.eE+-]+);", body).group(1))
+    steps = re.findall(r"p = p \* t2 ([+-]) ([0-9.eE+-]+);", body)
+    assert len(steps) == 8
+    bound = float(re.search(r"constexpr double kAtanErr = ([0-9.eE+-]+);", src).group(1))
+    x = np.linspace(0.0, 1.0, 2_000_001)
+    t2 = x * x
+    p = np.full_like(x, c0)
+    for sign, c in steps:
+        p = p * t2 + (float(c) if sign == "+" else -float(c))
+    err = np.abs(x * p - np.arctan(x))
+    assert err.max() < 0.75 * bound, err.max()
+    assert bound <= 2e-8
