@@ -24,7 +24,7 @@ using namespace clr;
 namespace {
 
 constexpr int kMaxShells = 1024;
-constexpr int kThreadSub = 4;   // nsub above which a cell is painted by a whole wavefront
+constexpr int kThreadSub = 6;   // nsub above which a cell is painted by a whole wavefront
 
 struct ImapArgs {
   const float *dens, *phi;
@@ -265,8 +265,11 @@ __device__ inline int64_t subpoint_bin(const ImapArgs &a, const double *__restri
 // slots claimed with LDS compare-and-swap and listed -- and adds each bin to
 // the maps with one pair of device atomics when the brick is done.  A bin
 // that finds no slot within kProbe probes goes to the maps directly.
-// Measured on 1024^3, nside 512, 64 shells: row order with device atomics
-// per cache entry 219 ms; bricks 177 ms; bricks with the LDS table 106 ms.
+// Measured on 1024^3, nside 512, 64 shells, cells with nsub <= 4: row order
+// with device atomics per cache entry 219 ms; bricks 177 ms; bricks with the
+// LDS table 106 ms.  The split at nsub 6 (k_imap_wave's per-cell costs
+// outweigh its width for smaller cells) measured 164 ms for the whole pass
+// against 180 at 4 and 174 at 8.
 // (Giving each XCD a contiguous eighth of the Morton order measured the
 // same for this kernel and much slower for k_imap_wave.)
 constexpr int kBrick = 8;
@@ -312,7 +315,9 @@ __device__ inline int brick_slot(BrickBins &b, unsigned long long key) {
   return -1;
 }
 
-__global__ __launch_bounds__(256) void k_imap(ImapArgs a, const double *__restrict__ r0s,
+// 4 waves per SIMD (128 VGPRs, a few spilled): the sub-point chain is
+// latency-bound, and this occupancy measured 141 -> 118 ms against 3 waves.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_imap(ImapArgs a, const double *__restrict__ r0s,
                                               const double *__restrict__ rfs, const int *__restrict__ nsub,
                                               int nbs, float *__restrict__ data, int *__restrict__ nadd) {
   __shared__ BrickBins bb;
@@ -414,9 +419,10 @@ __global__ __launch_bounds__(256) void k_imap(ImapArgs a, const double *__restri
 // table is three quarters full).  Sub-points whose bin finds no slot within
 // kProbe probes, and cells whose window does not hold, add to the maps
 // directly.  Cells of the cube [lo, lo + side)^3 around the observer, taken
-// by the wavefronts in turn.  Measured on 1024^3, nside 512, 64 shells:
-// device atomics per sub-point 257 ms, with the LDS tables 133 ms.
-constexpr int kWaveSlots = 1024;               // bins per wavefront table
+// by the wavefronts in turn.  Measured on 1024^3, nside 512, 64 shells,
+// cells with nsub > 4: device atomics per sub-point 257 ms, with the LDS
+// tables 133 ms.
+constexpr int kWaveSlots = 512;                // bins per wavefront table (1024: 61 ms, the LDS then limits occupancy)
 constexpr int kWaveFill = 3 * kWaveSlots / 4;  // emptied past this many bins
 constexpr int kImapWaves = 2;                  // wavefronts per block of k_imap_wave
 
@@ -430,7 +436,7 @@ struct WaveBins {
 
 // the table's slot of `key`, claimed if new, or -1 after kProbe probes
 __device__ inline int wave_slot(WaveBins &b, int key) {
-  unsigned h = ((unsigned)key * 2654435761u) >> 22;   // top 10 bits: kWaveSlots = 1024
+  unsigned h = ((unsigned)key * 2654435761u) >> 23;   // top 9 bits: kWaveSlots = 512
   for (int p = 0; p < kProbe; p++) {
     const int k = __hip_atomic_load(&b.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     if (k == key) return (int)h;
