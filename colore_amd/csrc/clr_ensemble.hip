@@ -237,4 +237,8 @@ int clr_fft_autotune(clr_ctx *c, int reps, int cap, float *ms, int *ncand, int *
   return clr_autotune_fields(c, reps, cap, ms, ncand, best);
 }
 
+int clr_fft_candidates(int n_grid) { return clr_fft_candidate_count(n_grid); }
+
+int clr_fft_select(int n_grid, int k) { return clr_fft_select_candidate(n_grid, k) ? CLR_ERR_ARG : CLR_OK; }
+
 }  // extern "C"

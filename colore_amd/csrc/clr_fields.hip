@@ -363,6 +363,12 @@ struct FieldLaunch {
   void (*coly)(dim3, dim3, hipStream_t, float2 *, float2 *, const float2 *);
   void (*c2r)(dim3, dim3, hipStream_t, float *, float *, float, double *, size_t, const float2 *);
   int colT, colThreads, xRPB, xThreads;
+  int genT, genThreads;   // the generation kernel's tile when it differs from the y pass's (0: the same)
+  unsigned gen_blocks(int N) const {
+    const int t = genT ? genT : colT;
+    return (unsigned)((N / 2 + t) / t * N);   // ceil((N/2+1)/t) tiles per plane, N planes
+  }
+  unsigned gen_threads() const { return (unsigned)(genT ? genThreads : colThreads); }
 };
 
 template <int N, int P, int T, bool SPLIT, int... Rs> struct GenKernel {
@@ -417,7 +423,10 @@ std::vector<FieldLaunch> field_candidates(int N) {
                        {COLCFG_SPLIT(1024, 32, 4, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 4, 128, 8, 256},
                        {COLCFG_SPLIT(1024, 32, 8, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256},
                        {COLCFG(1024, 64, 4, 16, 16, 4), XCFG(1024, 64, 4, 8, 8, 8), 4, 256, 4, 256},
-                       {COLCFG_SPLIT(1024, 64, 4, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 4, 256, 8, 256}};
+                       {COLCFG_SPLIT(1024, 64, 4, 16, 16, 4), XCFG(1024, 32, 8, 16, 16, 2), 4, 256, 8, 256},
+                       // 16-column generation tiles (128-byte row segments, one block per CU)
+                       {GenKernel<1024, 32, 16, false, 16, 16, 4>::gen, ColKernels<1024, 32, 8, 16, 16, 4>::coly,
+                        XCFG(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256, 16, 512}};
     case 2048: return {{COLCFG(2048, 128, 8, 16, 16, 8), XCFG(2048, 64, 4, 16, 16, 4), 8, 1024, 4, 256}};
     default: return {};
   }
@@ -444,6 +453,16 @@ bool field_launch_for(int N, FieldLaunch *L) {
 bool clr_fft_size_supported(int N) {
   FieldLaunch L;
   return field_launch_for(N, &L);
+}
+
+int clr_fft_candidate_count(int N) { return (int)field_candidates(N).size(); }
+
+int clr_fft_select_candidate(int N, int k) {
+  const int n = (int)field_candidates(N).size();
+  if (n == 0 || k < 0 || k >= n) return -1;
+  std::lock_guard<std::mutex> lk(g_fft_mu);
+  g_fft_choice[N] = k;
+  return 0;
 }
 
 // Buffers of the field stage for graph capture (clr_enqueue_fields).
@@ -546,7 +565,7 @@ int clr_enqueue_fields(clr_ctx *c, hipStream_t s, uint32_t seed, double r_smooth
   float2 *d = (float2 *)c->dens, *p = (float2 *)c->npot;
   const int ntile = (NH + L.colT - 1) / L.colT;
   const float norm = (float)pow(sqrt(2 * M_PI) / c->L, 3);
-  L.gen(dim3((unsigned)(ntile * N)), dim3(L.colThreads), s, d, p, c->amp, compact, phi_ratio, c->tw, seed, nullptr);
+  L.gen(dim3(L.gen_blocks(N)), dim3(L.gen_threads()), s, d, p, c->amp, compact, phi_ratio, c->tw, seed, nullptr);
   L.coly(dim3((unsigned)(ntile * N), 1, 2), dim3(L.colThreads), s, d, p, c->tw);
   L.c2r(dim3((unsigned)nxb, 2), dim3(L.xThreads), s, c->dens, c->npot, norm, c->red, nrows, c->tw);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, c->red, nxb, c->red_out);
@@ -605,7 +624,7 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   rc = clr_phi_ready(c, 0);
   if (rc) return rc;
   CLR_HIP(c, hipEventRecord(c->ev0, c->stream));
-  L.gen(dim3((unsigned)(ntile * N)), dim3(L.colThreads), c->stream, d, p, c->amp, compact, phi_ratio, c->tw, seed,
+  L.gen(dim3(L.gen_blocks(N)), dim3(L.gen_threads()), c->stream, d, p, c->amp, compact, phi_ratio, c->tw, seed,
         inj);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev1, c->stream));

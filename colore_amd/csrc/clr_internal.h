@@ -194,6 +194,8 @@ int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const in
 // device).  Same kernels and results as the launchers above.
 int clr_prepare_fields(clr_ctx *c);
 int clr_autotune_fields(clr_ctx *c, int reps, int cap, float *ms, int *ncand, int *best);
+int clr_fft_candidate_count(int N);
+int clr_fft_select_candidate(int N, int k);
 int clr_enqueue_fields(clr_ctx *c, hipStream_t s, uint32_t seed, double r_smooth, int smooth_potential,
                        double *sigma2_dev);
 int clr_prepare_norms(clr_ctx *c);

@@ -115,6 +115,8 @@ def load_amd():
     _sig(lib, "clr_ensemble_run", ci, [_vp, _u32, _d, ci, ctypes.POINTER(ci), _f])
     _sig(lib, "clr_ensemble_result", ci, [_vp, ci, _d, ctypes.POINTER(ci), _f])
     _sig(lib, "clr_fft_autotune", ci, [_vp, ci, ci, _f, ctypes.POINTER(ci), ctypes.POINTER(ci)])
+    _sig(lib, "clr_fft_candidates", ci, [ci])
+    _sig(lib, "clr_fft_select", ci, [ci, ci])
     _sig(lib, "clr_surrogate_inputs", ci, [_vp, ci, ci, _f, _d, _i32, ctypes.POINTER(ci)])
     _sig(lib, "clr_sample_sources_surrogate", ci, [_vp, _vp, ci, ctypes.c_uint32, ci, ci,
                                                    ctypes.POINTER(ctypes.c_longlong)])
@@ -349,6 +351,15 @@ class Context:
         self._chk(self.lib.clr_fft_autotune(self.h, reps, 16, ptr(ms, ctypes.c_float), ctypes.byref(nc),
                                             ctypes.byref(best)))
         return best.value, ms[:nc.value].copy()
+
+    def fft_candidates(self):
+        """Number of FFT tile-shape candidates for this context's grid size."""
+        return int(self.lib.clr_fft_candidates(self.N))
+
+    def fft_select(self, k):
+        """Use FFT tile-shape candidate k for this grid size (process-wide)."""
+        if self.lib.clr_fft_select(self.N, k) != 0:
+            raise ValueError(f"no FFT candidate {k} for n_grid={self.N}")
 
     def catalogue_size(self, ipop):
         n = ctypes.c_longlong()

@@ -397,6 +397,14 @@ int  clr_ensemble_result(clr_ensemble *ens, int i, double *sigma2, int *flag, fl
  * with the smallest median time is used from then on by every context of
  * that N in the process.  ms[cap]: median per candidate. */
 int  clr_fft_autotune(clr_ctx *ctx, int reps, int cap, float *ms, int *ncand, int *best);
+/* The FFT tile-shape candidates of grid size n_grid: their number (0 when
+ * the size has no FFT configuration), and the selection of candidate k for
+ * every context of that size in the process -- what clr_fft_autotune
+ * chooses by timing, here chosen by the caller (to repeat a run with the
+ * shapes of an earlier one, or to test every shape).  No counterpart in the
+ * reference (FFTW plans its own).  Returns CLR_ERR_ARG for a k out of range. */
+int  clr_fft_candidates(int n_grid);
+int  clr_fft_select(int n_grid, int k);
 
 /* Device time in milliseconds of the last call of each stage (0 if never),
  * in this order: mode generation + z pass, y pass, x pass, density

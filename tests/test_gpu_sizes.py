@@ -91,8 +91,10 @@ def _expected_plane(orc, run, N, iz, which):
     return out * norm
 
 
-@pytest.mark.parametrize("N", [256, 384, 1024, 2048])
+@pytest.mark.parametrize("N", [128, 192, 256, 384, 1024, 2048])
 def test_single_modes_each_fft_size(workdir, N):
+    """Every FFT tile-shape candidate of the size (the autotuner may pick any
+    of them for a run), one after the other."""
     path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
     nh = N // 2 + 1
     inj = np.zeros(2 * N * N * nh, dtype=np.uint32)
@@ -100,10 +102,23 @@ def test_single_modes_each_fft_size(workdir, N):
         m = kx + nh * (ky + N * kz)
         inj[2 * m], inj[2 * m + 1] = x0, x1
     ctx = _ctx_for(run, N)
-    print(f"single modes N={N}", flush=True)
-    s2 = ctx.gaussian_fields(1, run.scalar("r_smooth"), True, injected=inj)
-    del inj
     orc = _oracle(run)
+    ncand = ctx.fft_candidates()
+    assert ncand >= 1
+    try:
+        for k in range(ncand):
+            ctx.fft_select(k)
+            print(f"single modes N={N} candidate {k}/{ncand}", flush=True)
+            s2 = ctx.gaussian_fields(1, run.scalar("r_smooth"), True, injected=inj)
+            if k == ncand - 1:
+                inj = None   # 34 GB at 2048^3
+            _check_single_modes(ctx, orc, run, N, s2)
+    finally:
+        ctx.fft_select(0)
+        ctx.close()
+
+
+def _check_single_modes(ctx, orc, run, N, s2):
     for which in (0, 1):
         rms_tot, n_tot, emax, emax_rel = 0.0, 0, 0.0, 0.0
         for iz in (0, 1, N // 2, N - 3):
@@ -124,7 +139,6 @@ def test_single_modes_each_fft_size(workdir, N):
         fac = 1.0 if kx in (0, N // 2) else 2.0
         want_s2 += (fac * a) ** 2 / 2
     assert abs(s2 / want_s2 - 1) < 1e-5, (s2, want_s2)
-    ctx.close()
 
 
 # ------------------------------------------------------ 1024^3 workload checks
