@@ -326,6 +326,98 @@ __global__ __launch_bounds__(P *RPB) void k_c2r_x(float *__restrict__ g0, float 
   }
 }
 
+// ---------------------------------------------------------------------------
+// General 3-D transforms of one padded grid, in place and unnormalised, for
+// the LPT densities (the r2c and c2r of density.c:376-1031): the same
+// Stockham engine, a forward transform being the conjugate of the inverse
+// transform of the conjugate.
+//   k_col_lpt : one column pass along y (ZAX false: element y of column kx
+//               of plane z at base[(z N + y) NHP + kx]) or z (ZAX true:
+//               element z of column kx of row y at the same address), either
+//               direction.
+//   k_r2c_x   : real rows to half spectra through one length-M=N/2 complex
+//               forward DFT per row, z[n] = x[2n] + i x[2n+1]:
+//               X[k] = (Z[k] + Z*[M-k])/2 - i e^{-2 pi i k/N} (Z[k] - Z*[M-k])/2,
+//               k = 0..M (Z[M] = Z[0]); the pairs (k, M-k) meet in LDS.
+// ---------------------------------------------------------------------------
+template <int N, int P, int T, bool FWD, bool ZAX, int... Rs>
+__global__ __launch_bounds__(P *T) void k_col_lpt(float2 *__restrict__ g, const float2 *__restrict__ tw) {
+  using RL = RadixList<Rs...>;
+  constexpr int NH = N / 2 + 1, E = N / P, R0 = RL::first, RZ = RL::last, NTILE = (NH + T - 1) / T;
+  constexpr int NHP = grid_pitch(N) / 2;
+  constexpr size_t STRIDE = ZAX ? (size_t)N * NHP : (size_t)NHP;
+  __shared__ float2 lds[N * T];
+  const int c = threadIdx.x % T, t = threadIdx.x / T;
+  int tile, w;   // w: the plane z (y pass) or the row y (z pass)
+  xcd_tile(blockIdx.x, NTILE, N, tile, w);
+  const int kx = tile * T + c;
+  const bool valid = kx < NH;
+  float2 *base = g + (ZAX ? (size_t)w * NHP : (size_t)w * N * NHP) + kx;
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / R0; b++)
+#pragma unroll
+    for (int m = 0; m < R0; m++) {
+      const int e = t + b * P + m * (N / R0);
+      float2 a = valid ? base[(size_t)e * STRIDE] : make_float2(0.f, 0.f);
+      if (FWD) a.y = -a.y;
+      v[b * R0 + m] = a;
+    }
+  const LdsCol<T> io{lds, c};
+  Stages<N, P, N, 1, Rs...>::run(v, t, io, tw);
+#pragma unroll
+  for (int b = 0; b < E / RZ; b++)
+#pragma unroll
+    for (int q = 0; q < RZ; q++) {
+      const int e = t + b * P + q * (N / RZ);
+      float2 a = v[b * RZ + q];
+      if (FWD) a.y = -a.y;
+      if (valid) base[(size_t)e * STRIDE] = a;
+    }
+}
+
+template <int N, int P, int RPB, int... Rs>
+__global__ __launch_bounds__(P *RPB) void k_r2c_x(float *__restrict__ g, size_t nrows,
+                                                   const float2 *__restrict__ tw) {
+  using RL = RadixList<Rs...>;
+  constexpr int M = N / 2, E = M / P, R0 = RL::first, RZ = RL::last;
+  static_assert(RL::product == M, "radices must multiply to N/2");
+  constexpr int LR = lds_row_len<M>();
+  __shared__ float2 lds[RPB * LR];
+  const int r = threadIdx.x / P, t = threadIdx.x % P;
+  const size_t row = (size_t)blockIdx.x * RPB + r;
+  const bool valid = row < nrows;
+  float2 *X = (float2 *)(g + (valid ? row : 0) * (size_t)grid_pitch(N));
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / R0; b++)
+#pragma unroll
+    for (int m = 0; m < R0; m++) {
+      const float2 a = X[t + b * P + m * (M / R0)];
+      v[b * R0 + m] = make_float2(a.x, -a.y);   // conjugate in, conjugate out: forward
+    }
+  const LdsRow io{lds + r * LR};
+  Stages<M, P, N, 1, Rs...>::run(v, t, io, tw);
+  __syncthreads();   // every pass's LDS reads are done
+#pragma unroll
+  for (int b = 0; b < E / RZ; b++)
+#pragma unroll
+    for (int q = 0; q < RZ; q++) {
+      const float2 a = v[b * RZ + q];
+      io.st(t + b * P + q * (M / RZ), make_float2(a.x, -a.y));   // Z[n]
+    }
+  __syncthreads();
+  if (!valid) return;
+  for (int k = t; k <= M; k += P) {
+    const float2 zk = io.ld(k == M ? 0 : k), zm = io.ld(k == 0 ? 0 : M - k);
+    const float2 s = make_float2(zk.x + zm.x, zk.y - zm.y);   // Z[k] + Z*[M-k]
+    const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);   // Z[k] - Z*[M-k]
+    const float2 w = tw[k];                                   // e^{+2 pi i k/N}
+    const float2 wd = cmul(make_float2(w.x, -w.y), d);        // e^{-2 pi i k/N} d
+    X[k] = make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));   // (s - i wd)/2
+  }
+}
+
 // sigma^2 = <d^2> - <d>^2 from the reduced (sum, sumsq), on the device (the
 // same FP64 expression clr_launch_fields evaluates on the host).
 __global__ void k_sigma2_finish(const double *__restrict__ s, double ntot, double *__restrict__ sigma2) {
@@ -448,7 +540,70 @@ bool field_launch_for(int N, FieldLaunch *L) {
   return true;
 }
 
+// The general 3-D transforms (k_col_lpt, k_r2c_x, k_c2r_x) with each size's
+// default tile shapes (the first field_candidates entry).
+struct GridFft {
+  void (*col)(dim3, dim3, hipStream_t, float2 *, const float2 *, int, int);
+  void (*r2c)(dim3, dim3, hipStream_t, float *, size_t, const float2 *);
+  void (*c2r)(dim3, dim3, hipStream_t, float *, float *, float, double *, size_t, const float2 *);
+  int colT, colThreads, xRPB, xThreads;
+};
+template <int N, int P, int T, int... Rs> struct GridCol {
+  static void run(dim3 g, dim3 b, hipStream_t s, float2 *d, const float2 *tw, int fwd, int zax) {
+    if (fwd && zax) hipLaunchKernelGGL((k_col_lpt<N, P, T, true, true, Rs...>), g, b, 0, s, d, tw);
+    else if (fwd) hipLaunchKernelGGL((k_col_lpt<N, P, T, true, false, Rs...>), g, b, 0, s, d, tw);
+    else if (zax) hipLaunchKernelGGL((k_col_lpt<N, P, T, false, true, Rs...>), g, b, 0, s, d, tw);
+    else hipLaunchKernelGGL((k_col_lpt<N, P, T, false, false, Rs...>), g, b, 0, s, d, tw);
+  }
+};
+template <int N, int P, int RPB, int... Rs> struct GridR2c {
+  static void run(dim3 g, dim3 b, hipStream_t s, float *d, size_t nrows, const float2 *tw) {
+    hipLaunchKernelGGL((k_r2c_x<N, P, RPB, Rs...>), g, b, 0, s, d, nrows, tw);
+  }
+};
+#define GRIDFFT(N, P, T, ...) GridCol<N, P, T, __VA_ARGS__>::run
+#define GRIDX(N, P, RPB, ...) GridR2c<N, P, RPB, __VA_ARGS__>::run, XKernels<N, P, RPB, __VA_ARGS__>::c2r
+
+bool grid_fft_for(int N, GridFft *L) {
+  switch (N) {
+    case 32:   *L = {GRIDFFT(32, 4, 16, 4, 8), GRIDX(32, 4, 16, 4, 4), 16, 64, 16, 64}; return true;
+    case 64:   *L = {GRIDFFT(64, 8, 16, 8, 8), GRIDX(64, 4, 16, 4, 8), 16, 128, 16, 64}; return true;
+    case 128:  *L = {GRIDFFT(128, 8, 16, 16, 8), GRIDX(128, 8, 16, 8, 8), 16, 128, 16, 128}; return true;
+    case 192:  *L = {GRIDFFT(192, 16, 16, 4, 4, 12), GRIDX(192, 8, 16, 4, 4, 6), 16, 256, 16, 128}; return true;
+    case 256:  *L = {GRIDFFT(256, 16, 16, 16, 16), GRIDX(256, 8, 16, 16, 8), 16, 256, 16, 128}; return true;
+    case 384:  *L = {GRIDFFT(384, 16, 16, 8, 8, 6), GRIDX(384, 16, 16, 4, 4, 12), 16, 256, 16, 256}; return true;
+    case 512:  *L = {GRIDFFT(512, 64, 16, 8, 8, 8), GRIDX(512, 16, 16, 16, 16), 16, 1024, 16, 256}; return true;
+    case 1024: *L = {GRIDFFT(1024, 32, 8, 16, 16, 4), GRIDX(1024, 32, 8, 16, 16, 2), 8, 256, 8, 256}; return true;
+    case 2048: *L = {GRIDFFT(2048, 128, 8, 16, 16, 8), GRIDX(2048, 64, 4, 16, 16, 4), 8, 1024, 4, 256}; return true;
+    default: return false;
+  }
+}
+
 }  // namespace
+
+// In-place 3-D transform of one padded grid float[N][N][ngx], unnormalised:
+// forward r2c (x, then y and z, sign -1) or inverse c2r (z, y, then x, sign
+// +1), as FFTW's r2c/c2r plans compute them; on the context's stream.
+int clr_fft3d(clr_ctx *c, float *g, int forward) {
+  GridFft L;
+  const int N = c->N;
+  if (!grid_fft_for(N, &L)) return clr_fail(c, CLR_ERR_ARG, "n_grid=%d has no FFT configuration", N);
+  const size_t nrows = (size_t)N * N;
+  const dim3 gcol((unsigned)((N / 2 + L.colT) / L.colT * N)), bcol((unsigned)L.colThreads);
+  const dim3 gx((unsigned)((nrows + L.xRPB - 1) / L.xRPB)), bx((unsigned)L.xThreads);
+  float2 *h = (float2 *)g;
+  if (forward) {
+    L.r2c(gx, bx, c->stream, g, nrows, c->tw);
+    L.col(gcol, bcol, c->stream, h, c->tw, 1, 0);
+    L.col(gcol, bcol, c->stream, h, c->tw, 1, 1);
+  } else {
+    L.col(gcol, bcol, c->stream, h, c->tw, 0, 1);
+    L.col(gcol, bcol, c->stream, h, c->tw, 0, 0);
+    L.c2r(gx, bx, c->stream, g, g, 1.f, nullptr, nrows, c->tw);
+  }
+  CLR_HIP(c, hipGetLastError());
+  return CLR_OK;
+}
 
 bool clr_fft_size_supported(int N) {
   FieldLaunch L;

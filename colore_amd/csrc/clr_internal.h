@@ -195,6 +195,7 @@ int clr_launch_norm_tables_given(clr_ctx *c, int ntr, const int *kinds, const in
 int clr_prepare_fields(clr_ctx *c);
 int clr_autotune_fields(clr_ctx *c, int reps, int cap, float *ms, int *ncand, int *best);
 int clr_fft_candidate_count(int N);
+int clr_fft3d(clr_ctx *c, float *g, int forward);
 int clr_fft_select_candidate(int N, int k);
 int clr_enqueue_fields(clr_ctx *c, hipStream_t s, uint32_t seed, double r_smooth, int smooth_potential,
                        double *sigma2_dev);

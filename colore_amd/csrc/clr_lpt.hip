@@ -6,14 +6,14 @@
 // place on the padded rows), each displacement component i k_a delta_k /
 // (N^3 k^2) is written to its own half spectrum and transformed back (c2r, in
 // place); 2LPT adds the six digradients, the second-order source and its
-// displacement.  The FFTs are rocFFT's (through hipFFT): this is a
-// secondary path, not the per-step hot path.  One thread per grid cell then
+// displacement.  The transforms are the project's own (clr_fft3d: the
+// Stockham engine of the field stage, run forward as the conjugate of the
+// inverse transform of the conjugate).  One thread per grid cell then
 // computes the cell's particle position (FP32 as the reference stores it,
 // wrapped into the box), one thread per particle adds its NGP, CIC or TSC
 // weights to the zeroed density grid with FP32 atomics, and a last pass
 // subtracts 1.
 #include "clr_internal.h"
-#include <hipfft/hipfft.h>
 #include <math.h>
 #include <vector>
 
@@ -342,9 +342,6 @@ __global__ __launch_bounds__(256) void k_lpt_norm(float *__restrict__ g, int N, 
   g[idx] = (float)(g[idx] * 1.0f - 1.);
 }
 
-int fft_fail(clr_ctx *c, hipfftResult r, const char *what) {
-  return clr_fail(c, CLR_ERR_HIP, "hipFFT %s failed (%d)", what, (int)r);
-}
 
 }  // namespace
 
@@ -367,26 +364,16 @@ int clr_launch_lpt(clr_ctx *c, int order, int interp, float *pos_out) {
       return clr_fail(c, CLR_ERR_HIP, "out of device memory for the LPT grids");
     }
   }
-  hipfftHandle fwd = 0, inv = 0;
-  int rc = CLR_OK;
-  // in place on the padded rows: real rows of ngx floats, complex rows of
-  // ngx/2 values (>= N/2+1)
-  long long n[3] = {N, N, N}, re[3] = {N, N, ngx}, ce[3] = {N, N, nhp};
-  const long long rdist = (long long)N * N * ngx, cdist = (long long)N * N * nhp;
-  size_t ws = 0;
-  hipfftResult fr = hipfftCreate(&fwd);
-  if (fr == HIPFFT_SUCCESS) fr = hipfftCreate(&inv);
-  if (fr == HIPFFT_SUCCESS) fr = hipfftMakePlanMany64(fwd, 3, n, re, 1, rdist, ce, 1, cdist, HIPFFT_R2C, 1, &ws);
-  if (fr == HIPFFT_SUCCESS) fr = hipfftMakePlanMany64(inv, 3, n, ce, 1, cdist, re, 1, rdist, HIPFFT_C2R, 1, &ws);
-  if (fr == HIPFFT_SUCCESS) fr = hipfftSetStream(fwd, c->stream);
-  if (fr == HIPFFT_SUCCESS) fr = hipfftSetStream(inv, c->stream);
+  int rc = CLR_OK, fr = CLR_OK;   // fr: the first transform error
   hipError_t e = hipSuccess;
   const long long modes = (long long)N * N * NH, cells = (long long)N * N * N;
   const dim3 gm((unsigned)((modes + 255) / 256)), gc((unsigned)((cells + 255) / 256)), blk(256);
   const double dkf = 2 * M_PI / c->L, fftnorm = (double)N * ((double)N * N);
-  auto ok = [&]() { return fr == HIPFFT_SUCCESS && e == hipSuccess; };
-  auto r2c = [&](float *g) { if (ok()) fr = hipfftExecR2C(fwd, (hipfftReal *)g, (hipfftComplex *)g); };
-  auto c2r = [&](float *g) { if (ok()) fr = hipfftExecC2R(inv, (hipfftComplex *)g, (hipfftReal *)g); };
+  auto ok = [&]() { return fr == CLR_OK && e == hipSuccess; };
+  // in place on the padded rows: real rows of ngx floats, complex rows of
+  // ngx/2 values (>= N/2+1), unnormalised like FFTW's plans
+  auto r2c = [&](float *g) { if (ok()) fr = clr_fft3d(c, g, 1); };
+  auto c2r = [&](float *g) { if (ok()) fr = clr_fft3d(c, g, 0); };
   if (ok()) e = hipEventRecord(c->ev0, c->stream);
   r2c(c->dens);
   const float *dsp[3], *sec[3] = {nullptr, nullptr, nullptr};
@@ -491,10 +478,8 @@ int clr_launch_lpt(clr_ctx *c, int order, int interp, float *pos_out) {
   }
   if (ok()) e = hipEventRecord(c->ev1, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (fr != HIPFFT_SUCCESS) rc = fft_fail(c, fr, "plan or transform");
+  if (fr != CLR_OK) rc = fr;   // clr_fft3d has set the message
   else if (e != hipSuccess) rc = clr_fail(c, CLR_ERR_HIP, "LPT: %s", hipGetErrorString(e));
-  if (fwd) hipfftDestroy(fwd);
-  if (inv) hipfftDestroy(inv);
   for (int i = 0; i < nbuf; i++) (void)hipFree(ps[i]);
   if (rc) return rc;
   float ms = 0;

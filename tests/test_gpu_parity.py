@@ -336,14 +336,16 @@ def test_source_beams_192(workdir):
 
 
 @pytest.mark.parametrize("order,interp,N", [(1, 0, 64), (1, 1, 64), (1, 2, 64), (2, 1, 64), (2, 2, 64),
-                                            (1, 1, 128), (1, 2, 128)])
+                                            (1, 1, 128), (1, 2, 128), (2, 1, 192), (1, 1, 256)])
 def test_lpt_density(workdir, order, interp, N):
     """1LPT and 2LPT densities from a Gaussian field on the device against the
     oracle's FP64 transforms and deposit of the same field: CIC and TSC within 1e-4
-    (FP32 FFT and atomics), NGP counts equal but in at most 1e-4 of the cells
+    up to 128^3 (FP32 FFT and atomics), NGP counts equal but in at most 1e-4 of the cells
     (a particle within rounding of a cell boundary), the same mass.  The
     field is the generated one times 25, so that displacements cross cells.
-    At 128^3 the LDS-staged deposit has interior tiles as well as wrapped ones."""
+    At 128^3 the LDS-staged deposit has interior tiles as well as wrapped ones;
+    192^3 and 256^3 run other radix lists of the project's r2c and c2r (max
+    5e-4 there, RMS 4e-5 everywhere)."""
     path, run = make_run("C1", N, workdir, with_maps=False)
     ctx = _ctx_for(run, N)
     s2 = ctx.gaussian_fields(int(run.scalar("seed")), run.scalar("r_smooth"), True)
@@ -367,7 +369,14 @@ def test_lpt_density(workdir, order, interp, N):
         assert np.all((diff < 1e-6) | (np.abs(diff - np.round(diff)) < 1e-6))
         assert np.count_nonzero(diff > 0.5) <= 1e-4 * N ** 3 + 2
     else:
-        assert np.abs(d - o).max() < 1e-4, np.abs(d - o).max()
+        # FP32 transforms: the density error grows with the grid (log N
+        # passes, more particles per cell edge); a wrong index or twiddle
+        # gives errors of order the field itself
+        e = np.abs(d - o)
+        rms = float(np.sqrt(np.mean(e * e)))
+        print(f"lpt order {order} interp {interp} N {N}: max {e.max():.3g} rms {rms:.3g}", flush=True)
+        assert e.max() < (1e-4 if N <= 128 else 5e-4), e.max()
+        assert rms < 4e-5, rms
 
 
 def test_sources_all_populations_one_pass(workdir):
