@@ -323,6 +323,25 @@ def test_c_abi_library_exports_every_declared_symbol():
         assert not missing, (hdr, missing)
 
 
+def test_fft_candidate_entry_points():
+    """clr_fft_candidates / clr_fft_select (host-side bookkeeping, no device
+    call): every grid size with an FFT configuration has at least one tile
+    shape, other sizes none; selection accepts 0..n-1 and rejects the rest."""
+    import ctypes as ct
+    so = ct.CDLL(lib.AMD_SO)
+    so.clr_fft_candidates.restype = ct.c_int
+    so.clr_fft_select.restype = ct.c_int
+    for n in (32, 64, 128, 192, 256, 384, 512, 1024, 2048):
+        k = so.clr_fft_candidates(n)
+        assert k >= 1, n
+        assert so.clr_fft_select(n, k) != 0 and so.clr_fft_select(n, -1) != 0
+        assert so.clr_fft_select(n, k - 1) == 0 and so.clr_fft_select(n, 0) == 0
+    assert so.clr_fft_candidates(1024) >= 5
+    for n in (0, 100, 96, 4096):
+        assert so.clr_fft_candidates(n) == 0
+        assert so.clr_fft_select(n, 0) != 0
+
+
 def test_cli_usage_and_test_memory(tmp_path):
     exe = os.path.join(ROOT, "colore_amd", "CoLoRe")
     r = subprocess.run([exe], capture_output=True, text=True)
