@@ -416,7 +416,7 @@ def test_los_maps(workdir, which):
     assert np.abs(out - o).max() < 1e-5 * rms + 1e-30, (np.abs(out - o).max(), rms)
 
 
-@pytest.mark.parametrize("N,nside,bands", [(64, 16, 8), (128, 64, 16), (256, 128, 32)])
+@pytest.mark.parametrize("N,nside,bands", [(64, 16, 8), (128, 64, 16), (192, 64, 12), (256, 128, 32)])
 def test_imap_vs_oracle(workdir, N, nside, bands):
     """21cm intensity shells painted on the device against the oracle's walk
     over the same grids: identical hit counts per pixel, temperatures summed
