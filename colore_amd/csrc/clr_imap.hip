@@ -78,7 +78,13 @@ __device__ inline int pixel_fast(int nside, double x, double y, double cth) {
   const double ax = fabs(x), ay = fabs(y);
   const double mx = fmax(ax, ay), mn = fmin(ax, ay);
   if (!(mx > 0)) return -1;
-  double a = atan01(mn / mx);
+  // the ratio through the hardware reciprocal and two Newton steps (a few
+  // ulp; the exact quotient is not needed: the error stays far inside the
+  // margin the floors are certified with)
+  double rm = __builtin_amdgcn_rcp(mx);
+  rm = rm * (2.0 - mx * rm);
+  rm = rm * (2.0 - mx * rm);
+  double a = atan01(mn * rm);
   if (ay > ax) a = 1.5707963267948966 - a;
   if (x < 0) a = 3.141592653589793 - a;
   if (y < 0) a = -a;
@@ -188,8 +194,24 @@ __device__ inline bool cell_paint(const ImapArgs &a, const double *__restrict__ 
   if (r0 > 0) rvel = a.factor_vel * rvel_cell(a, ix, iy, iz, x0, y0, z0, r0);
   cp.dr_rsd = rvel * f_of_r(a.r_arr, a.v1, a.idr, a.r_last, r0, a.v1_0, a.v1_last) *
               f_of_r(a.r_arr, a.ih, a.idr, a.r_last, r0, a.ih_0, a.ih_last);
-  const double dl = a.dens[((size_t)iz * a.N + iy) * (size_t)a.ngx + ix];
-  cp.temp = (float)(tmean * bias_model_f64(dl, bias, a.model) * dnorm);
+  // The temperature only enters the FP32 map sums: bias_model (common.h:
+  // 414-431) in FP32 on the hardware exp2/log2 (a few 1e-7 relative; the
+  // FP64 pow cost as much as a cell's whole sub-point walk at nsub = 2).
+  const float dl = a.dens[((size_t)iz * a.N + iy) * (size_t)a.ngx + ix];
+  float bm;
+  {
+    const float b = (float)bias;
+    if (!(dl > -1.f)) {
+      bm = 0.f;
+    } else if (a.model == 1) {
+      bm = dl < 0.f ? __builtin_amdgcn_exp2f(1.44269504f * b * dl * __builtin_amdgcn_rcpf(1.f + dl)) : 1.f + b * dl;
+    } else if (a.model == 2) {
+      bm = fmaxf(1.f + b * dl, 0.f);
+    } else {
+      bm = __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(1.f + dl));
+    }
+  }
+  cp.temp = (float)(tmean * dnorm) * bm;
   const int irad = shell_of(r0s, rfs, a.nr, r0);
   cp.ns = irad < 0 ? nsub[0] : (irad >= a.nr ? nsub[a.nr - 1] : nsub[irad]);
   cp.dxs = a.dx / cp.ns;
@@ -313,8 +335,8 @@ __device__ inline int brick_slot(BrickBins &b, unsigned long long key) {
   return -1;
 }
 
-// 4 waves per SIMD (128 VGPRs, a few spilled): the sub-point chain is
-// latency-bound, and this occupancy measured 141 -> 118 ms against 3 waves.
+// At least 4 waves per SIMD (128 VGPRs at most): 141 -> 118 ms against 3
+// waves when the kernel needed 146.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_imap(ImapArgs a, const double *__restrict__ r0s,
                                               const double *__restrict__ rfs, const int *__restrict__ nsub,
                                               int nbs, float *__restrict__ data, int *__restrict__ nadd) {
