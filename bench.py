@@ -13,7 +13,7 @@ Prints one JSON line.  `value` is grid cells per second of the whole step.
 `roofline` is for the kernel with the largest device time: algorithmic HBM
 bytes per launch over its mean duration (HIP events on the library's stream),
 against 8 TB/s.  `cpu_baseline` times the FP64 CPU oracle on the same
-configuration at 512^3 on the host cores.
+configuration at 768^3 on the host cores (about 20 s of CPU work).
 """
 import argparse
 import math
@@ -364,7 +364,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="C3")
     ap.add_argument("--n-grid", type=int, default=None)
-    ap.add_argument("--cpu-grid", type=int, default=512)
+    ap.add_argument("--cpu-grid", type=int, default=768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--maps", action="store_true",
                     help="also time the kappa/ISW line-of-sight maps and the 21cm painting of the workload")
