@@ -45,6 +45,10 @@ int clr_phi_ready(clr_ctx *c, int host) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, c->ev4, c->ev5);
   c->stage_ms[ST_FFT_PHI] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev4, c->ev6);
+  c->stage_ms[ST_PHI_Y] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev6, c->ev5);
+  c->stage_ms[ST_PHI_X] = ms;
   c->phi_pending = 0;
   return CLR_OK;
 }
@@ -101,6 +105,7 @@ static void free_all(clr_ctx *c) {
   if (c->ev_phi) hipEventDestroy(c->ev_phi);
   if (c->ev4) hipEventDestroy(c->ev4);
   if (c->ev5) hipEventDestroy(c->ev5);
+  if (c->ev6) hipEventDestroy(c->ev6);
   if (c->ev3) hipEventDestroy(c->ev3);
   if (c->stream2) { hipStreamSynchronize(c->stream2); hipStreamDestroy(c->stream2); }
   if (c->stream) hipStreamDestroy(c->stream);
@@ -136,6 +141,7 @@ int clr_create(clr_ctx **out, int n_grid, double l_box, const double pos_obs[3],
   CLR_HIP(c, hipEventCreateWithFlags(&c->ev_phi, hipEventDisableTiming));
   CLR_HIP(c, hipEventCreate(&c->ev4));
   CLR_HIP(c, hipEventCreate(&c->ev5));
+  CLR_HIP(c, hipEventCreate(&c->ev6));
   CLR_HIP(c, hipEventCreate(&c->ev0));
   CLR_HIP(c, hipEventCreate(&c->ev1));
   CLR_HIP(c, hipEventCreate(&c->ev2));
@@ -483,6 +489,8 @@ int clr_stage_times(const clr_ctx *c, double *ms, int n) {
   if (c->phi_pending && ST_FFT_PHI < n && hipEventQuery(c->ev5) == hipSuccess) {
     float f = 0;
     if (hipEventElapsedTime(&f, c->ev4, c->ev5) == hipSuccess) ms[ST_FFT_PHI] = f;
+    if (ST_PHI_X < n && hipEventElapsedTime(&f, c->ev4, c->ev6) == hipSuccess) ms[ST_PHI_Y] = f;
+    if (ST_PHI_X < n && hipEventElapsedTime(&f, c->ev6, c->ev5) == hipSuccess) ms[ST_PHI_X] = f;
   }
   return ST_COUNT;
 }

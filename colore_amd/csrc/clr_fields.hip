@@ -801,6 +801,7 @@ int clr_launch_fields(clr_ctx *c, uint32_t seed, double r_smooth, int smooth_pot
   CLR_HIP(c, hipEventRecord(c->ev4, c->stream2));
   L.coly(dim3((unsigned)(ntile * N), 1, 1), dim3(L.colThreads), c->stream2, p, p, c->tw);
   CLR_HIP(c, hipGetLastError());
+  CLR_HIP(c, hipEventRecord(c->ev6, c->stream2));
   L.c2r(dim3((unsigned)nxb, 1), dim3(L.xThreads), c->stream2, c->npot, c->npot, norm, nullptr, nrows, c->tw);
   CLR_HIP(c, hipGetLastError());
   CLR_HIP(c, hipEventRecord(c->ev5, c->stream2));

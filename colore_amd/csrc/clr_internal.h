@@ -39,6 +39,8 @@ enum StageId {
   ST_BEAMS,       // k_src_beams of a clr_source_beams call
   ST_CSTM,        // k_cstm of a clr_custom_map call
   ST_SKW,         // the k_src_skewers launches of a clr_source_skewers call (kernels only)
+  ST_PHI_Y,       // phi's y pass alone (second stream)
+  ST_PHI_X,       // phi's x pass alone (second stream)
   ST_COUNT
 };
 
@@ -53,7 +55,7 @@ struct clr_ctx {
   // phi's y and x passes run on stream2 while stream goes on with the density;
   // ev_phi marks their end, every reader of npot waits on it (clr_phi_ready)
   hipStream_t stream2 = nullptr;
-  hipEvent_t ev_gen = nullptr, ev_phi = nullptr, ev4 = nullptr, ev5 = nullptr;
+  hipEvent_t ev_gen = nullptr, ev_phi = nullptr, ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
   int phi_pending = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   double stage_ms[ST_COUNT] = {0};

@@ -21,7 +21,7 @@ CLR_DENS_LOGNORMAL, CLR_DENS_CLIP = 0, 3
 CLR_RNG_PHILOX, CLR_RNG_INJECTED = 0, 1
 CLR_CL_KAPPA, CLR_CL_ISW = 0, 2
 STAGES = ["gen", "fft_yz", "fft_x", "density", "norm", "srcs", "kappa", "isw", "imap", "src_pass", "src_write",
-          "fft_phi", "shells", "beams", "cstm", "skewers"]
+          "fft_phi", "shells", "beams", "cstm", "skewers", "phi_y", "phi_x"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
@@ -461,8 +461,8 @@ class Context:
         self._chk(self.lib.clr_synchronize(self.h))
 
     def stage_times(self):
-        ms = np.zeros(16)
-        n = self.lib.clr_stage_times(self.h, ptr(ms), 16)
+        ms = np.zeros(len(STAGES))
+        n = self.lib.clr_stage_times(self.h, ptr(ms), len(STAGES))
         return dict(zip(STAGES, ms[:n]))
 
 

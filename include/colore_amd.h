@@ -414,7 +414,8 @@ int  clr_fft_select(int n_grid, int k);
  * stream, overlapping the density's passes and the stages after them; every
  * call that reads phi waits for them), the shell-sum kernel alone, the
  * source-beam kernel, the custom-map kernel, the skewer kernels (summed over
- * their chunks).  Fills at most n values; returns the number of stages. */
+ * their chunks), then phi's y pass alone and phi's x pass alone.  Fills at
+ * most n values; returns the number of stages. */
 int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
 /* Waits for every device operation of the context, the potential's passes
  * on the second stream included (no counterpart in the reference, whose
