@@ -92,9 +92,10 @@ def traffic_per_launch(kernel, config, N):
 def time_path_to_kappa(run, ctx, kap, step, nsteps, nsrc, npop):
     """Gaussian -> lognormal -> sources -> kappa: the timed step followed by
     the convergence maps of the workload's source planes (kappa_get_beam_
-    properties, the map copied to the host at the end of the call), wall time
-    per realisation over nsteps; B_alg adds one read of phi inside the
-    lightcone sphere, (pi/6) 4p bytes per cell, and the maps' bytes."""
+    properties, the maps copied at the end of the call into one page-locked
+    host buffer that every realisation reuses), wall time per realisation over
+    nsteps; B_alg adds one read of phi inside the lightcone sphere, (pi/6) 4p
+    bytes per cell, and the maps' bytes."""
     import numpy as np
     from colore_amd import lib
     host = lib.load_host()
@@ -102,12 +103,13 @@ def time_path_to_kappa(run, ctx, kap, step, nsteps, nsrc, npop):
     zs = [z for z in kap["z_out"] if z <= run.scalar("z_max")]
     r_src = np.array([host.clr_run_r_of_z(run.h, z) for z in zs])
     nside = kap["nside"]
-    ctx.kappa(nside, r_src)   # warm: the tidal records' buffer
+    maps = lib.pinned_empty((len(zs), 12 * nside * nside))
+    ctx.kappa(nside, r_src, out=maps)   # warm: the tidal records' buffer
     kms = []
     t0 = time.perf_counter()
     for _ in range(nsteps):
         step(False)
-        ctx.kappa(nside, r_src)
+        ctx.kappa(nside, r_src, out=maps)
         kms.append(ctx.stage_times()["kappa"])
     dt = (time.perf_counter() - t0) / nsteps
     cells = float(N) ** 3

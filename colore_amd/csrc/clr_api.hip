@@ -482,6 +482,17 @@ int clr_synchronize(clr_ctx *c) {
   return CLR_OK;
 }
 
+void *clr_host_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+int clr_host_free(void *p) {
+  if (!p) return CLR_OK;
+  return hipHostFree(p) == hipSuccess ? CLR_OK : CLR_ERR_HIP;
+}
+
 int clr_stage_times(const clr_ctx *c, double *ms, int n) {
   if (!c || !ms) return CLR_ERR_ARG;
   for (int i = 0; i < n && i < ST_COUNT; i++) ms[i] = c->stage_ms[i];

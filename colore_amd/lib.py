@@ -6,6 +6,7 @@ to a CPU path by accident.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -107,6 +108,8 @@ def load_amd():
     _sig(lib, "clr_mlp_train", ci, [_vp, _f, _f, ci, ci, ctypes.c_float, ctypes.c_float, _d])
     _sig(lib, "clr_mlp_predict", ci, [_vp, _f, ci, _f, ci])
     _sig(lib, "clr_synchronize", ci, [_vp])
+    _sig(lib, "clr_host_alloc", _vp, [ctypes.c_size_t])
+    _sig(lib, "clr_host_free", ci, [_vp])
     _sig(lib, "clr_bias_moment", ctypes.c_double, [ci] + [ctypes.c_double] * 5)
     _sig(lib, "clr_ensemble_create", ci, [ctypes.POINTER(_vp), ci, ctypes.POINTER(_vp), ci, ci, _d,
                                           ctypes.POINTER(ctypes.c_longlong), ci])
@@ -162,6 +165,32 @@ def load_host():
 
 def ptr(a, ct=ctypes.c_double):
     return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def pinned_empty(shape, dtype=np.float32):
+    """An uninitialised array in page-locked host memory (clr_host_alloc),
+    freed when the array and every view of it are gone.  Device-to-host
+    copies into it skip the runtime's pageable staging: pass it as `out` to
+    the map calls that are repeated (one allocation, many realisations)."""
+    lib = load_amd()
+    dt = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dt.itemsize
+    p = lib.clr_host_alloc(max(nbytes, 1))
+    if not p:
+        raise MemoryError(f"clr_host_alloc({nbytes}) failed")
+    buf = (ctypes.c_char * max(nbytes, 1)).from_address(p)
+    weakref.finalize(buf, lib.clr_host_free, p)
+    return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+
+def _map_out(out, nplanes, npix):
+    """The caller's map buffer (C-contiguous float32 of nplanes x npix) or a
+    new one."""
+    if out is None:
+        return np.zeros((nplanes, npix), dtype=np.float32)
+    if out.dtype != np.float32 or out.shape != (nplanes, npix) or not out.flags.c_contiguous:
+        raise ValueError(f"out must be a C-contiguous float32 array of shape {(nplanes, npix)}")
+    return out
 
 
 class Run:
@@ -408,15 +437,18 @@ class Context:
         self._chk(self.lib.clr_custom_map(self.h, ipop, nside, ptr(out, ctypes.c_float)))
         return out
 
-    def kappa(self, nside, r_src):
+    def kappa(self, nside, r_src, out=None):
+        """Convergence maps at the source radii r_src, one row per plane, into
+        `out` when given (e.g. a pinned_empty array reused across calls)."""
         r = np.ascontiguousarray(r_src, dtype=np.float64)
-        out = np.zeros((len(r), 12 * nside * nside), dtype=np.float32)
+        out = _map_out(out, len(r), 12 * nside * nside)
         self._chk(self.lib.clr_kappa(self.h, nside, len(r), ptr(r), ptr(out, ctypes.c_float)))
         return out
 
-    def isw(self, nside, r_src):
+    def isw(self, nside, r_src, out=None):
+        """ISW maps at the source radii r_src; `out` as for kappa."""
         r = np.ascontiguousarray(r_src, dtype=np.float64)
-        out = np.zeros((len(r), 12 * nside * nside), dtype=np.float32)
+        out = _map_out(out, len(r), 12 * nside * nside)
         self._chk(self.lib.clr_isw(self.h, nside, len(r), ptr(r), ptr(out, ctypes.c_float)))
         return out
 

@@ -422,6 +422,16 @@ int clr_stage_times(const clr_ctx *ctx, double *ms, int n);
  * stages return when done). */
 int clr_synchronize(clr_ctx *ctx);
 
+/* Page-locked host memory for the outputs a caller fills again and again
+ * (maps, catalogues): device-to-host copies into it run at the link's rate
+ * instead of through the runtime's pageable staging (a 37.7 MB set of three
+ * nside-512 maps: 0.7 ms against ~4 ms).  No counterpart in the reference
+ * (its maps and catalogues are malloc'd once per run).  Returns NULL when the
+ * allocation fails; clr_host_free returns CLR_ERR_HIP for a pointer the
+ * runtime does not know. */
+void *clr_host_alloc(size_t bytes);
+int clr_host_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -410,10 +410,18 @@ def test_los_maps(workdir, which):
     import ctypes
     r_src = np.array([run.scalar("r_max") * 0.5, run.scalar("r_max") * 0.95])
     nside = 16
-    out = ctx.kappa(nside, r_src) if which == "kappa" else ctx.isw(nside, r_src)
+    call = ctx.kappa if which == "kappa" else ctx.isw
+    out = call(nside, r_src)
     o = _oracle(run).kappa_isw(0 if which == "kappa" else 1, N, run.scalar("l_box"), npot, nside, r_src)
     rms = np.sqrt(np.mean(o ** 2))
     assert np.abs(out - o).max() < 1e-5 * rms + 1e-30, (np.abs(out - o).max(), rms)
+    # the same maps into a page-locked buffer, twice (the second call
+    # overwrites the first's values)
+    pin = lib.pinned_empty(out.shape)
+    pin[:] = np.nan
+    for _ in range(2):
+        assert call(nside, r_src, out=pin) is pin
+        assert np.array_equal(pin, out)
 
 
 @pytest.mark.parametrize("N,nside,bands", [(64, 16, 8), (128, 64, 16), (192, 64, 12), (256, 128, 32)])

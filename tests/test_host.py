@@ -637,3 +637,15 @@ def test_imap_azimuth_polynomial_within_its_bound():
     err = np.abs(x * p - np.arctan(x))
     assert err.max() < 0.75 * bound, err.max()
     assert bound <= 2e-8
+
+
+def test_map_output_buffer_checks():
+    """The map calls write into a caller's buffer only when it has the maps'
+    exact layout (C-contiguous float32, planes x pixels); no device call."""
+    assert lib._map_out(None, 2, 48).shape == (2, 48)
+    ok = np.zeros((2, 48), dtype=np.float32)
+    assert lib._map_out(ok, 2, 48) is ok
+    for bad in (np.zeros((2, 48)), np.zeros((3, 48), dtype=np.float32),
+                np.zeros((48, 2), dtype=np.float32).T):
+        with pytest.raises(ValueError):
+            lib._map_out(bad, 2, 48)
