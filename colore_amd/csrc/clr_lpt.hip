@@ -243,14 +243,16 @@ __global__ __launch_bounds__(256) void k_lpt_deposit(const float *__restrict__ p
 }
 
 // The same CIC / TSC assignment with the weights first summed in LDS: a
-// block takes a 32 x 16 x 16 tile of particles (32 per thread) and holds the
-// grid region of the tile grown by 4 cells on every side (40 x 24 x 24
-// floats, 90 KB); weights landing in the region go to LDS, the rare ones
-// further out straight to the grid, and the region is then added to the grid
-// once (about 2.8 global atomics per particle instead of 8 for CIC, 27 for
-// TSC).  Needs N >= 64 and N a multiple of 32, so that the region never wraps
-// onto itself.
-constexpr int kDTX = 32, kDTY = 16, kDTZ = 16, kDHalo = 4;
+// block takes a 16^3 tile of particles (16 per thread) and holds the grid
+// region of the tile grown by 2 cells on every side (20^3 floats, 32 KB, so
+// that several blocks share a CU); weights landing in the region go to LDS,
+// the ones further out straight to the grid, and the region is then added to
+// the grid once (about 2 global atomics per particle instead of 8 for CIC, 27
+// for TSC).  Tiles of 32 x 16 x 16 with 4-cell margins (90 KB, two blocks
+// per CU) took 8 ms more for a 1024^3 CIC deposit; tiles from 8^3 to
+// 32 x 8 x 8 with 1- to 3-cell margins came within 1 ms of this one.  Needs N >= 64 and
+// N a multiple of 16, so that the region never wraps onto itself.
+constexpr int kDTX = 16, kDTY = 16, kDTZ = 16, kDHalo = 2;
 constexpr int kDRX = kDTX + 2 * kDHalo, kDRY = kDTY + 2 * kDHalo, kDRZ = kDTZ + 2 * kDHalo;
 constexpr size_t kDepositLds = (size_t)kDRX * kDRY * kDRZ * sizeof(float);
 
