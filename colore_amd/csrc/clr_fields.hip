@@ -659,6 +659,7 @@ int clr_autotune_fields(clr_ctx *c, int reps, int cap, float *ms, int *ncand, in
   }
   int kbest = 0;
   float tbest = 3.4e38f;
+  std::vector<float> meds;
   for (int k = 0; k < (int)v.size(); k++) {
     {
       std::lock_guard<std::mutex> lk(g_fft_mu);
@@ -679,8 +680,13 @@ int clr_autotune_fields(clr_ctx *c, int reps, int cap, float *ms, int *ncand, in
     std::sort(t.begin(), t.end());
     const float med = t[t.size() / 2];
     if (ms && k < cap) ms[k] = med;
-    if (med < tbest) { tbest = med; kbest = k; }
+    meds.push_back(med);
+    tbest = std::min(tbest, med);
   }
+  // the first candidate (entry 0 is the default) within 1 % of the fastest:
+  // timing noise between near-equal shapes does not flip the choice
+  for (int k = (int)meds.size() - 1; k >= 0; k--)
+    if (meds[k] <= 1.01f * tbest) kbest = k;
   (void)hipFree(s2);
   {
     std::lock_guard<std::mutex> lk(g_fft_mu);

@@ -393,9 +393,11 @@ int  clr_ensemble_run(clr_ensemble *ens, const uint32_t *seeds, const double *r_
  * float[12 nside_kappa^2]. */
 int  clr_ensemble_result(clr_ensemble *ens, int i, double *sigma2, int *flag, float *kappa);
 /* FFT tile autotuner: every tile-shape candidate for this context's N runs
- * the field stage `reps` times on its grids (which it overwrites); the one
- * with the smallest median time is used from then on by every context of
- * that N in the process.  ms[cap]: median per candidate. */
+ * the field stage `reps` times on its grids (which it overwrites); the
+ * first candidate whose median time is within 1 % of the smallest is used
+ * from then on by every context of that N in the process (so that timing
+ * noise between near-equal shapes does not flip the choice).  ms[cap]:
+ * median per candidate. */
 int  clr_fft_autotune(clr_ctx *ctx, int reps, int cap, float *ms, int *ncand, int *best);
 /* The FFT tile-shape candidates of grid size n_grid: their number (0 when
  * the size has no FFT configuration), and the selection of candidate k for
