@@ -207,6 +207,29 @@ def copy_probe(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
 
 
+def stream_probe(nbytes=4 << 30, reps=5):
+    """What plain streaming kernels reach on this device (torch's copy, in-place
+    add, sum and fill over 4 GiB buffers; best of `reps`, HIP events): the
+    practical ceiling next to the 8 TB/s published peak."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    ops = {"copy (read + write)": (lambda: b.copy_(a), 2), "in-place add (read + write)": (lambda: a.add_(1.0), 2),
+           "sum (read)": (lambda: a.sum(), 1), "fill (write)": (lambda: b.fill_(2.0), 1)}
+    out = {}
+    for name, (fn, passes) in ops.items():
+        best = 1e30
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name] = {"ms": best, "TB/s": passes * nbytes / (best * 1e-3) / 1e12}
+    print(json.dumps({"stream_probe_bytes": nbytes, "reps": reps, "results": out}))
+
+
 def cpu_baseline(cfg, n_cpu_grid):
     """FP64 oracle, same config at n_cpu_grid^3, full step; returns cells/s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -377,6 +400,8 @@ def main():
                     help="also time the kappa/ISW line-of-sight maps and the 21cm painting of the workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (PMC counter calibration)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="time plain streaming kernels (copy, in-place add, sum, fill) over 4 GiB and exit")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default FFT tile shapes (no autotuning before the timed steps)")
     ap.add_argument("--kappa-steps", type=int, default=3,
@@ -386,6 +411,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.stream_probe:
+        stream_probe()
         return
     if args.gpus == 1 and args.config == "C4":
         bench_ensemble(args)
