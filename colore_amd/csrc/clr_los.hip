@@ -717,14 +717,25 @@ struct SkwArgs {
   const double *r_arr, *d1_arr, *v1_arr;
 };
 
+// A block computes 16 steps of 16 consecutive sources (lane = step + 16 x
+// source): neighbouring sources lie in neighbouring cells, so their lines of
+// sight cross nearly the same cells at the same step and share the CIC
+// corners' cache lines, where one thread per (source, step) in row order
+// fetched ~170 bytes of HBM per value.  The blocks of one group of sources
+// follow each other on one XCD (workgroups go to the 8 XCDs round-robin by
+// linear id, each XCD with its own L2): logical block
+// (b % 8) (gridDim.x / 8) + b / 8, gridDim.x a multiple of 8.  nr is a
+// multiple of 16.
 template <bool REC>
 __global__ __launch_bounds__(256) void k_src_skewers(SkwArgs a, const float4 *__restrict__ pos, long long n,
                                                      float *__restrict__ dg, float *__restrict__ vs) {
 #pragma clang fp contract(off)
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * a.nr) return;
-  const long long i = t / a.nr;
-  const int i_r = (int)(t - i * a.nr);
+  const long long blk = (long long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
+  const int ntile = a.nr / 16;
+  const long long i = (blk / ntile) * 16 + threadIdx.x / 16;
+  const int i_r = (int)(blk % ntile) * 16 + (int)(threadIdx.x % 16);
+  if (i >= n) return;
+  const long long t = i * a.nr + i_r;
   const float4 p = pos[i];
   const int N = a.N;
   const size_t ngx = (size_t)a.ngx, plane = (size_t)N * ngx;
@@ -812,6 +823,7 @@ int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, fl
   a.N = N;
   a.ngx = c->ngx;
   a.nr = N / 2;
+  if (a.nr % 16) return clr_fail(c, CLR_ERR_ARG, "source skewers need N/2 a multiple of 16 (N = %d)", N);
   a.na = na;
   a.gaussian = gaussian ? 1 : 0;
   a.idx = N / c->L;
@@ -842,13 +854,16 @@ int clr_launch_src_skewers(clr_ctx *c, int ipop, int gaussian, double sigma2, fl
   for (long long s0 = 0; s0 < n && e == hipSuccess; s0 += cap) {
     const long long m = std::min(cap, n - s0);
     const long long tot = m * a.nr;
+    // 16 sources x 16 steps per block, the count rounded up to the 8 XCDs
+    // (the extra blocks' sources are >= m and return)
+    const unsigned nblk = (unsigned)(((m + 15) / 16 * (a.nr / 16) + 7) / 8 * 8);
     e = hipEventRecord(c->ev2, c->stream);
     if (e == hipSuccess) {
       if (a.vrec)
-        hipLaunchKernelGGL(k_src_skewers<true>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
+        hipLaunchKernelGGL(k_src_skewers<true>, dim3(nblk), dim3(256), 0, c->stream, a,
                            (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
       else
-        hipLaunchKernelGGL(k_src_skewers<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a,
+        hipLaunchKernelGGL(k_src_skewers<false>, dim3(nblk), dim3(256), 0, c->stream, a,
                            (const float4 *)C.pos + s0, m, tmp, tmp + (size_t)cap * a.nr);
     }
     if (e == hipSuccess) e = hipGetLastError();
