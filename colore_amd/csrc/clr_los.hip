@@ -121,22 +121,36 @@ __global__ __launch_bounds__(256) void k_tidal(const float *__restrict__ phi, in
   const int zh = iz == N - 1 ? 0 : iz + 1, zl = iz == 0 ? N - 1 : iz - 1;
   const size_t Z0 = (size_t)iz * plane, ZH = (size_t)zh * plane, ZL = (size_t)zl * plane;
   const size_t Y0 = (size_t)iy * ngx, YH = (size_t)yh * ngx, YL = (size_t)yl * ngx;
-  for (int ix = threadIdx.x; ix < N; ix += blockDim.x) {
+  // 256 cells at a time: the records go through LDS so that consecutive
+  // lanes store consecutive float2 of the row (a lane's own 24-byte record
+  // would spread each store over 1.5 KB)
+  __shared__ float2 st[256 * 3];
+  __shared__ unsigned char ok[256];
+  for (int x0 = 0; x0 < N; x0 += 256) {
+    const int ix = x0 + threadIdx.x;
     const float xc = ix * dxf - ox;
-    if (xc * xc + rho2 > r2lim) continue;
-    const int xh = ix == N - 1 ? 0 : ix + 1, xl = ix == 0 ? N - 1 : ix - 1;
-    const float c0 = phi[ix + Y0 + Z0];
-    float t[6];
-    t[0] = phi[xh + Y0 + Z0] + phi[xl + Y0 + Z0] - 2 * c0;
-    t[3] = phi[ix + YH + Z0] + phi[ix + YL + Z0] - 2 * c0;
-    t[1] = 0.25f * (phi[xh + YH + Z0] + phi[xl + YL + Z0] - phi[xh + YL + Z0] - phi[xl + YH + Z0]);
-    t[5] = phi[ix + Y0 + ZH] + phi[ix + Y0 + ZL] - 2 * c0;
-    t[2] = 0.25f * (phi[xh + Y0 + ZH] + phi[xl + Y0 + ZL] - phi[xh + Y0 + ZL] - phi[xl + Y0 + ZH]);
-    t[4] = 0.25f * (phi[ix + YH + ZH] + phi[ix + YL + ZL] - phi[ix + YH + ZL] - phi[ix + YL + ZH]);
-    float2 *o = t6 + (row * N + ix) * 3;
-    o[0] = make_float2(t[0], t[1]);
-    o[1] = make_float2(t[2], t[3]);
-    o[2] = make_float2(t[4], t[5]);
+    const bool in = ix < N && !(xc * xc + rho2 > r2lim);
+    ok[threadIdx.x] = in;
+    if (in) {
+      const int xh = ix == N - 1 ? 0 : ix + 1, xl = ix == 0 ? N - 1 : ix - 1;
+      const float c0 = phi[ix + Y0 + Z0];
+      float t[6];
+      t[0] = phi[xh + Y0 + Z0] + phi[xl + Y0 + Z0] - 2 * c0;
+      t[3] = phi[ix + YH + Z0] + phi[ix + YL + Z0] - 2 * c0;
+      t[1] = 0.25f * (phi[xh + YH + Z0] + phi[xl + YL + Z0] - phi[xh + YL + Z0] - phi[xl + YH + Z0]);
+      t[5] = phi[ix + Y0 + ZH] + phi[ix + Y0 + ZL] - 2 * c0;
+      t[2] = 0.25f * (phi[xh + Y0 + ZH] + phi[xl + Y0 + ZL] - phi[xh + Y0 + ZL] - phi[xl + Y0 + ZH]);
+      t[4] = 0.25f * (phi[ix + YH + ZH] + phi[ix + YL + ZL] - phi[ix + YH + ZL] - phi[ix + YL + ZH]);
+      st[3 * threadIdx.x] = make_float2(t[0], t[1]);
+      st[3 * threadIdx.x + 1] = make_float2(t[2], t[3]);
+      st[3 * threadIdx.x + 2] = make_float2(t[4], t[5]);
+    }
+    __syncthreads();
+    float2 *o = t6 + (row * N + x0) * 3;
+    const int nk = 3 * min(256, N - x0);
+    for (int k = threadIdx.x; k < nk; k += 256)
+      if (ok[k / 3]) o[k] = st[k];
+    __syncthreads();
   }
 }
 
