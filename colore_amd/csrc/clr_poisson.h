@@ -23,6 +23,10 @@
 namespace clr {
 
 constexpr double kPoisSplit = 500.0;
+// Largest Poisson mean sampled: a mean that is not finite or above 2^20 (a
+// bias model with b < 0 near delta = -1 gives such means) has no count here;
+// poisson_exact returns -1 and the library reports the cell as an error.
+constexpr double kPoisMax = 1048576.0;
 
 __host__ __device__ inline uint32_t stream_poisson_lo(int ipop) { return 0x40000100u + (uint32_t)ipop; }
 
@@ -48,11 +52,13 @@ __host__ __device__ inline int poisson_inv_f64(double lam, double u) {
   return k;
 }
 
-// FP64 count from the full 32-bit uniform x = hi << 16 | lo.
+// FP64 count from the full 32-bit uniform x = hi << 16 | lo; -1 for a mean
+// that is not finite or above kPoisMax.
 __host__ __device__ inline int poisson_exact(double lam, uint64_t cell, uint32_t stream_extra, uint32_t seed,
                                              uint32_t x) {
 #pragma clang fp contract(off)
   if (!(lam > 0)) return 0;
+  if (!(lam <= kPoisMax)) return -1;
   if (lam <= kPoisSplit) return poisson_inv_f64(lam, u01(x));
   const int m = (int)ceil(lam / kPoisSplit);
   const double lp = lam / m;

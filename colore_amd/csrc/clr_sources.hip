@@ -37,6 +37,8 @@ namespace {
 constexpr int kLanes = 64, kCellsPerLane = 8, kBlockCells = kLanes * kCellsPerLane;
 constexpr int kByteEscape = 255;    // stored for counts >= kBytePending: the count is in the escape list
 constexpr int kBytePending = 254;   // k_count left the cell to k_exact (never seen by k_fill)
+// escape-list count of a cell whose Poisson mean has no count (see kPoisMax)
+constexpr unsigned long long kCountInvalid = ~0ull;
 
 struct PopTab {
   const double *amp, *bz, *norm;
@@ -488,6 +490,14 @@ __global__ __launch_bounds__(256) void k_exact(SrcArgs a) {
                             e.w, a.r_arr,
                             P.amp, P.bz, P.norm, a.idr, a.r_last, P.bz0, n0, nf, a.dx, a.ox, a.oy, a.oz,
                             a.cell_vol, a.half_box_20, P.model, P.ipop, a.seed);
+  if (k < 0) {
+    // no count for this mean: an escape entry the host turns into an error
+    P.cnt8[cell] = kByteEscape;
+    const unsigned q = atomicAdd(a.esc_n, 1u);
+    if (q < a.esc_cap)
+      a.esc[q] = make_ulonglong2(((unsigned long long)ip << 56) | (unsigned long long)cell, kCountInvalid);
+    return;
+  }
   if (k > 0) atomicAdd(&P.btot[cell / kBlockCells], (uint32_t)k);
   P.cnt8[cell] = (uint8_t)(k < kBytePending ? k : kByteEscape);
   if (k >= kBytePending) {
@@ -1142,6 +1152,15 @@ int clr_launch_sources(clr_ctx *c, int ipop, uint32_t seed, int nside_base, long
     std::vector<ulonglong2> h(nesc);
     CLR_HIP(c, hipMemcpy(h.data(), a.esc, (size_t)nesc * sizeof(ulonglong2), hipMemcpyDeviceToHost));
     std::sort(h.begin(), h.end(), [](const ulonglong2 &u, const ulonglong2 &v) { return u.x < v.x; });
+    for (const ulonglong2 &u : h)
+      if (u.y == kCountInvalid) {
+        c->cnt_p0 = -1;   // no counts to fetch
+        c->cnt_np = 0;
+        return clr_fail(c, CLR_ERR_ARG,
+                        "population %d, cell %llu: the Poisson mean is not finite or above 2^20 (a bias model "
+                        "with b < 0 near delta = -1?)",
+                        p0 + (int)(u.x >> 56), (unsigned long long)(u.x & ((1ull << 56) - 1)));
+      }
     CLR_HIP(c, hipMemcpy(a.esc, h.data(), (size_t)nesc * sizeof(ulonglong2), hipMemcpyHostToDevice));
   }
   float ms = 0;

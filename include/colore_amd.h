@@ -183,7 +183,10 @@ int clr_set_norm_table(clr_ctx *ctx, int kind, int ipop, const double *norm_arr,
  * the device in global z-major cell order; returns their number.  ipop = -1
  * samples every population in one pass over the grid (srcs_set_cartesian,
  * srcs.c:285-294) with the same catalogues as one call per population, and
- * returns the total. */
+ * returns the total.  A cell whose lambda is not finite or above 2^20 (a
+ * bias model with b < 0 near delta = -1 can overflow) gets no count: the
+ * call returns CLR_ERR_ARG naming the population and cell, and no catalogue
+ * is written (the reference would pass such a lambda to gsl_ran_poisson). */
 int clr_sample_sources(clr_ctx *ctx, int ipop, uint32_t seed, int nside_base,
                        long long *nsrc_out);
 /* clr_sample_sources for population ipop fed the reference's own uniform

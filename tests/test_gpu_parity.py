@@ -209,6 +209,32 @@ def test_sources_negative_bias(workdir, model):
     _sources_vs_oracle(run, 64, 0, 3003, edit)
 
 
+def test_sources_mean_out_of_range_is_an_error(workdir):
+    """A cell whose Poisson mean is not finite or above 2^20 gets no count:
+    the call fails with CLR_ERR_ARG naming the cell (here bias model 1 with
+    b = -0.6 and one cell's delta just above -1, so exp(b d/(1+d)) overflows),
+    and the context samples normally once the grid is sane again."""
+    N = 64
+    path, run = make_run("C3", N, workdir, z_max=1.6, with_maps=False)
+    ctx = _ctx_for(run, N)
+    amp, bz, model = run.tracer(0, 0)
+    ctx.set_tracer(0, 0, 1, amp, np.full_like(bz, -0.6))
+    s2 = ctx.gaussian_fields(3003, run.scalar("r_smooth"), True)
+    ctx.shell_norms(lognormal_fused=True, sigma2=s2, ntracers=int(run.scalar("n_srcs")))
+    g = ctx.fetch_grid(0)
+    cell = (N // 2, N // 2, N // 2 + 11)   # z, y, x: r = 11 cells ~ 1.1 Gpc/h, inside the n(z) of population 0
+    orig = g[cell]
+    g[cell] = np.float32(-0.9999999)
+    ctx.upload_grid(0, g)
+    print("sampling with one cell at delta = -0.9999999", flush=True)
+    with pytest.raises(RuntimeError, match="Poisson mean"):
+        ctx.sample_sources(0, 3003, 2)
+    g[cell] = orig
+    ctx.upload_grid(0, g)
+    assert ctx.sample_sources(0, 3003, 2) > 0
+    ctx.close()
+
+
 def test_sources_gaps_in_nz(workdir):
     """n(z) with empty stretches, a single nonzero node and a negative node:
     zero bins, bins the FP64 path decides and coarse bounds of 0 next to
